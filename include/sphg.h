@@ -1,0 +1,230 @@
+/*
+ * sphg.h -- C ABI of the MI355X-native per-step particle loop of the two-phase
+ * (water-soil) weakly compressible SPH solver.
+ *
+ * The reference solver is Fortran.  Its per-step loop is
+ *   time_integration (declarations_sph.f90:525)
+ *     -> single_step (declarations_sph.f90:1131)
+ *        -> direct_find / link_list (m_particles.f90:938, :1146)
+ *        -> con_density, delta_sph, shear_strain_rate, pressure, ... (m_particles.f90)
+ *        -> art_visc (m_particles.f90:2504)
+ *        -> direct_find_2, darcy_law, pore_water_pressure, volume fractions
+ * and it operates on `type(particles)` (m_particles.f90:62-384), which is not
+ * C-interoperable.  This header is the boundary that replaces those call
+ * sites: a `bind(C)` Fortran wrapper (sources_amd/fortran/sphg_mod.f90) packs
+ * the particle arrays of one phase into `sphg_phase_view` and calls the entry
+ * points below, which run hand-written HIP kernels on one GPU.
+ *
+ * Conventions
+ *  - Host arrays are owned by the caller and keep the reference layout:
+ *    vectors are Fortran x(2,maxn), i.e. interleaved [x1,y1,x2,y2,...];
+ *    scalars are length maxn; pair arrays are length max_interaction and
+ *    dwdx is dwdx(2,max_interaction).  Pair indices are 1-based particle
+ *    numbers, as in the reference.
+ *  - The library owns device memory.  It never frees or resizes caller memory.
+ *    A view stays bound until the next sphg_bind_phase on the same phase.
+ *  - Every entry point returns 0 (SPHG_OK) or a negative error code; the
+ *    codes mirror the reference's `stop` statements.
+ *  - One host thread per context.  Calls are synchronous at the boundary:
+ *    on return, every array the entry point documents as output has been
+ *    copied back into the bound host view.  The exceptions are
+ *    sphg_time_integration(..., sync=0) and sphg_single_step(..., sync=0),
+ *    which leave the state on the device until sphg_sync().
+ */
+#ifndef SPHG_H
+#define SPHG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define SPHG_OK                      0
+#define SPHG_ERR_TOO_MANY_INTERACTIONS (-1) /* m_particles.f90:991-994, 1052-1055, 1251-1253 */
+#define SPHG_ERR_OUT_OF_RANGE        (-2)  /* grid_geom, m_particles.f90:1131-1136 */
+#define SPHG_ERR_BAD_KERNEL          (-3)  /* get_scale_k, m_particles.f90:932 */
+#define SPHG_ERR_INVALID             (-4)  /* bad argument / unbound phase */
+#define SPHG_ERR_HIP                 (-5)  /* HIP runtime error (see sphg_last_hip_error) */
+#define SPHG_ERR_UNSUPPORTED         (-6)  /* switch combination outside the ported path */
+#define SPHG_ERR_NONFINITE           (-7)  /* NaN/Inf detected in a checked field */
+#define SPHG_ERR_OUT_OF_MEMORY       (-8)
+
+/* ---- phases ------------------------------------------------------------ */
+#define SPHG_WATER 1   /* `parts`, itype +-2 (m_particles.f90:896-913) */
+#define SPHG_SOIL  2   /* `soil`,  itype +-3 */
+
+/* ---- precision modes --------------------------------------------------- */
+#define SPHG_FP64  0   /* every operation in FP64 (the reference's real(dp)) */
+#define SPHG_MIXED 1   /* pair sums in FP32 with Kahan compensation; FP64 leapfrog,
+                          density and Drucker-Prager projection */
+
+/* ---- field masks for sphg_sync --------------------------------------- */
+#define SPHG_F_X        (1ull << 0)
+#define SPHG_F_VX       (1ull << 1)
+#define SPHG_F_RHO      (1ull << 2)
+#define SPHG_F_P        (1ull << 3)
+#define SPHG_F_C        (1ull << 4)
+#define SPHG_F_VOF      (1ull << 5)
+#define SPHG_F_STRESS   (1ull << 6)   /* sxx, sxy, syy */
+#define SPHG_F_EPSP     (1ull << 7)   /* epsilon_p */
+#define SPHG_F_DRHO     (1ull << 8)
+#define SPHG_F_DVX      (1ull << 9)
+#define SPHG_F_DVOF     (1ull << 10)
+#define SPHG_F_DSTRESS  (1ull << 11)  /* dsxx, dsxy, dsyy, dp */
+#define SPHG_F_AV       (1ull << 12)
+#define SPHG_F_STRAIN   (1ull << 13)  /* txx, txy, tyy (+ vcc, wxy for soil) */
+#define SPHG_F_VOF2     (1ull << 14)
+#define SPHG_F_FAIL     (1ull << 15)  /* fail(), nfail */
+#define SPHG_F_COUNTIAC (1ull << 16)
+#define SPHG_F_MIN      (1ull << 17)  /* rho_min, v_min, vof_min */
+#define SPHG_F_ALL      (~0ull)
+
+/*
+ * Simulation parameters.  Field meanings and defaults follow the reference:
+ *   material  -> `type material`   (m_particles.f90:19-29), set from the
+ *                steering keywords WATER PROPERTY / SOIL PROPERTY /
+ *                DARCY PARAMETER / SOIL FAILURE CRITERIA
+ *                (declarations_sph.f90:280-312)
+ *   numerical -> `type numerical`  (m_particles.f90:31-55)
+ *   switches  -> module param      (param.f90:78-103)
+ *   grid box  -> `type particles`  (m_particles.f90:89-102), used by link_list
+ */
+typedef struct sphg_params {
+    /* water: rho0, b, gamma, c, viscosity */
+    double water_rho0, water_b, water_gamma, water_c, water_viscosity;
+    /* soil: rho0, k (bulk modulus), E, niu, c, porosity, permeability,
+       cohesion, phi (radians) */
+    double soil_rho0, soil_k, soil_E, soil_niu, soil_c;
+    double soil_porosity, soil_permeability, soil_cohesion, soil_phi;
+    /* numerical: nnps (1 direct_find, 2 link_list), skf (1..4) */
+    int32_t nnps, skf;
+    double numeric_gravity;           /* numerical%gravity, used by darcy_law */
+    double alpha, beta, etq;          /* artificial viscosity */
+    double dd, p1, p2;                /* wall repulsion */
+    double delta;                     /* delta-SPH */
+    double epsilon;                   /* XSPH */
+    double gravity;                   /* declarations_sph.f90:13, added to dvx(2,:) */
+    /* switches (0/1 unless stated) */
+    int32_t single_phase, summation_density, average_velocity, visc_artificial;
+    int32_t artificial_density, soil_artificial_stress, volume_fraction;
+    int32_t water_artificial_volume, volume_fraction_renorm;
+    int32_t stress_integration;       /* 1 or 2 */
+    int32_t yield_criterion;          /* 1 Mohr-Coulomb, 2 Drucker-Prager */
+    int32_t plasticity;               /* 0..3 */
+    /* link_list sorting grid */
+    double x_mingeom, x_maxgeom, y_mingeom, y_maxgeom;
+    int32_t maxngx, maxngy;
+    double dt;
+    int32_t precision;                /* SPHG_FP64 or SPHG_MIXED */
+    int32_t reserved[7];
+} sphg_params;
+
+/*
+ * Host view of one phase: the subset of `type(particles)` that the per-step
+ * loop reads or writes.  Pointers may be NULL for arrays the phase does not
+ * have (e.g. stress for water); an entry point that needs a NULL array
+ * returns SPHG_ERR_INVALID.
+ */
+typedef struct sphg_phase_view {
+    int32_t ntotal, nvirt, maxn, max_interaction;
+    int32_t material;                 /* SPHG_WATER or SPHG_SOIL */
+    int32_t niac, nfail, reserved0;   /* outputs: pair count, failed points */
+    double  dspp;                     /* initial particle spacing (repulsion r0) */
+    int32_t *itype, *fail, *countiac, *pair_i, *pair_j;
+    double *x, *vx, *mass, *hsml;
+    double *rho, *p, *c, *vof, *vof2, *dvof, *vof_min;
+    double *rho_min, *v_min, *drho, *dvx, *av;
+    double *sxx, *syy, *sxy, *txx, *tyy, *txy, *wxy, *vcc, *epsilon_p;
+    double *dsxx, *dsyy, *dsxy, *dp;
+    double *w, *dwdx;
+} sphg_phase_view;
+
+typedef struct sphg_ctx sphg_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Create a context on the current HIP device; replaces allocate_sph
+   (declarations_sph.f90:338) for the device side. */
+int sphg_create(const sphg_params *params, sphg_ctx **out);
+void sphg_destroy(sphg_ctx *ctx);
+/* Replace the parameters (e.g. dt) of a live context. */
+int sphg_set_params(sphg_ctx *ctx, const sphg_params *params);
+/* Bind a phase view and upload its state (x, vx, mass, hsml, itype, rho, p,
+   c, vof, stress, epsilon_p and the rate arrays) to the device. */
+int sphg_bind_phase(sphg_ctx *ctx, int phase, const sphg_phase_view *view);
+/* Copy fields between device and the bound view: to_host=1 downloads. */
+int sphg_sync(sphg_ctx *ctx, int phase, uint64_t field_mask, int to_host);
+/* Time step counter (declarations_sph.f90:51 itimestep). */
+int sphg_set_itimestep(sphg_ctx *ctx, int itimestep);
+int sphg_get_itimestep(const sphg_ctx *ctx);
+
+/* ---- neighbour search (materialised pair lists) ------------------------ */
+
+/* direct_find(parts), m_particles.f90:938-1001: all i<j pairs with
+   sqrt(dx^2+dy^2) < scale_k*(h_i+h_j)/2, in i-major / j-ascending order,
+   with w, dwdx and countiac.  Writes pair_i, pair_j, w, dwdx, countiac and
+   niac of the view. */
+int sphg_direct_find(sphg_ctx *ctx, int phase);
+/* link_list(parts), m_particles.f90:1146-1265: the same predicate with h(1),
+   restricted to the cells the reference's sorting grid reaches (init_grid /
+   grid_geom, m_particles.f90:1069-1143).  Same pair SET as the reference;
+   pairs are written in i-major / j-ascending order. */
+int sphg_link_list(sphg_ctx *ctx, int phase);
+/* direct_find_2(parts, soil), m_particles.f90:1004-1062: water i x soil j
+   cross pairs (including virtual x virtual and r = 0 pairs) into the WATER
+   view's pair arrays; countiac of both phases. */
+int sphg_direct_find_2(sphg_ctx *ctx);
+
+/* ---- granular physics entry points (state taken from the device) ------- */
+
+/* con_density(parts), m_particles.f90:2402-2431: drho of the phase. */
+int sphg_con_density(sphg_ctx *ctx, int phase);
+/* sum_density(parts), m_particles.f90:2243-2298: Shepard summation density. */
+int sphg_sum_density(sphg_ctx *ctx, int phase);
+/* delta_sph on the density: adds the delta-SPH term to drho
+   (m_particles.f90:2658-2695 with f = rho, df = drho). */
+int sphg_delta_sph_density(sphg_ctx *ctx, int phase);
+/* The internal-force group of single_step for one phase
+   (declarations_sph.f90:1222-1308): shear_strain_rate, velocity_divergence,
+   pressure, newtonian_fluid / drucker_prager, the df/df3 momentum, plastic
+   flow rule 3 and Jaumann rate.  Overwrites dvx (and for soil ds*, dp,
+   epsilon_p, fail); txx/txy/tyy/vcc/wxy become valid. */
+int sphg_internal_force(sphg_ctx *ctx, int phase);
+/* art_visc(parts), m_particles.f90:2504-2564: adds to dvx. */
+int sphg_art_visc(sphg_ctx *ctx, int phase);
+
+/* ---- fused step -------------------------------------------------------- */
+
+/* One right-hand-side evaluation: single_step (two-phase,
+   declarations_sph.f90:1131-1435) or single_step_for_water
+   (declarations_sph.f90:1439-1526) depending on params->single_phase.
+   Uses the context's itimestep.  sync=1 copies every rate/diagnostic field
+   back into the views. */
+int sphg_single_step(sphg_ctx *ctx, int sync);
+/* nsteps leapfrog steps: time_integration (declarations_sph.f90:525-791) or
+   time_integration_for_water (:794-886).  Advances itimestep.  sync=1
+   downloads the whole state at the end. */
+int sphg_time_integration(sphg_ctx *ctx, int nsteps, int sync);
+
+/* ---- diagnostics / measurement ---------------------------------------- */
+const char *sphg_strerror(int code);
+int sphg_last_hip_error(const sphg_ctx *ctx);
+/* HIP stream (hipStream_t) all work of the context is issued on. */
+void *sphg_stream(sphg_ctx *ctx);
+/* Per-kernel timing with HIP events on the context stream.  When enabled,
+   every launch of a named kernel is bracketed by events; stats accumulate
+   until reset. */
+int sphg_profile_enable(sphg_ctx *ctx, int enable);
+int sphg_profile_reset(sphg_ctx *ctx);
+/* Fills launches and total milliseconds for kernel `name`; returns
+   SPHG_ERR_INVALID if the name is unknown. */
+int sphg_profile_get(sphg_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
+/* Number of same-phase / cross pairs found by the most recent step. */
+int sphg_pair_counts(sphg_ctx *ctx, int64_t *water_pairs, int64_t *soil_pairs, int64_t *cross_pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHG_H */

@@ -1,0 +1,1025 @@
+// sphg_api.hip -- host runtime behind include/sphg.h: context, device
+// buffers, cell-list build, step orchestration, granular entry points,
+// host<->device sync and per-kernel event timing.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sphg.h"
+#include "sph_kernels.h"
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline int nblk(int n, int b = TPB) { return (n + b - 1) / b; }
+
+struct ProfStat {
+    int64_t launches = 0;
+    double ms = 0.0;
+};
+
+struct PendingEvent {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct PhaseBuf {
+    bool bound = false;
+    sphg_phase_view view{};
+    int n = 0;
+    DPhase d{};
+    std::vector<void*> allocs;
+    bool cells_valid = false;
+    // materialised pairs
+    int *fwd = nullptr, *off = nullptr;
+    int *pi = nullptr, *pj = nullptr;
+    double *w = nullptr, *dwdx = nullptr;
+    int pair_cap = 0;
+    // staging for interleaved host vectors
+    double* stage = nullptr;
+};
+
+}  // namespace
+
+struct sphg_ctx {
+    sphg_params P{};
+    KParams K{};
+    Grid g{};          // cell grid (link fields filled per phase)
+    Grid gl[2]{};      // per phase copy with the link_list sorting-grid fields
+    int ncell = 0;
+    int itimestep = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    PhaseBuf ph[2];
+    int* d_err = nullptr;       // [0] out of range
+    int* d_scan = nullptr;      // tile sums
+    int scan_cap = 0;
+    int last_hip = 0;
+    bool prof = false;
+    std::map<std::string, ProfStat> stats;
+    std::vector<PendingEvent> pending;
+    std::vector<hipEvent_t> event_pool;
+    int64_t pairs[3] = {0, 0, 0};
+};
+
+namespace {
+
+#define HIPCHK(ctx, expr)                                  \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) {                            \
+            (ctx)->last_hip = (int)e_;                     \
+            return SPHG_ERR_HIP;                           \
+        }                                                  \
+    } while (0)
+
+#define RC(expr)                  \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != SPHG_OK) return rc_; \
+    } while (0)
+
+hipEvent_t get_event(sphg_ctx* c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void drain_events(sphg_ctx* c)
+{
+    if (c->pending.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto& pe : c->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, pe.a, pe.b);
+        auto& s = c->stats[pe.name];
+        s.launches += 1;
+        s.ms += ms;
+        c->event_pool.push_back(pe.a);
+        c->event_pool.push_back(pe.b);
+    }
+    c->pending.clear();
+}
+
+// Launch helper: brackets the launch with HIP events when profiling is on.
+template <typename... KArgs, typename... Args>
+int launch(sphg_ctx* c, const char* name, void (*kern)(KArgs...), int grid, int block, Args... args)
+{
+    if (grid <= 0) return SPHG_OK;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (c->prof) {
+        ea = get_event(c);
+        eb = get_event(c);
+        hipEventRecord(ea, c->stream);
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, args...);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        c->last_hip = (int)e;
+        return SPHG_ERR_HIP;
+    }
+    if (c->prof) {
+        hipEventRecord(eb, c->stream);
+        c->pending.push_back({name, ea, eb});
+        if (c->pending.size() > 4096) drain_events(c);
+    }
+    return SPHG_OK;
+}
+
+template <class T>
+int dalloc(sphg_ctx* c, PhaseBuf& b, T** p, size_t count)
+{
+    void* q = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        c->last_hip = (int)e;
+        return SPHG_ERR_OUT_OF_MEMORY;
+    }
+    b.allocs.push_back(q);
+    *p = (T*)q;
+    return SPHG_OK;
+}
+
+void free_phase(PhaseBuf& b)
+{
+    for (void* p : b.allocs) hipFree(p);
+    b.allocs.clear();
+    b = PhaseBuf{};
+}
+
+int fill_dbl(sphg_ctx* c, double* a, int n, double v)
+{
+    return launch(c, "fill", k_fill_dbl, nblk(n), TPB, a, n, v);
+}
+
+int fill_int(sphg_ctx* c, int* a, int n, int v)
+{
+    return launch(c, "fill", k_fill_int, nblk(n), TPB, a, n, v);
+}
+
+// Exclusive scan of n ints into out[0..n], total at out[n].
+int exclusive_scan(sphg_ctx* c, const int* in, int* out, int n)
+{
+    const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (ntiles > SCAN_TILE) return SPHG_ERR_UNSUPPORTED;
+    RC(launch(c, "scan", k_scan_tile, ntiles > 0 ? ntiles : 1, SCAN_BLOCK, in, out, n, c->d_scan));
+    RC(launch(c, "scan", k_scan_sums, 1, SCAN_BLOCK, c->d_scan, ntiles > 0 ? ntiles : 1));
+    RC(launch(c, "scan", k_scan_add, nblk(n > 0 ? n : 1), TPB, out, n, (const int*)c->d_scan, ntiles > 0 ? ntiles : 1));
+    return SPHG_OK;
+}
+
+int phase_index(int phase) { return phase == SPHG_WATER ? 0 : (phase == SPHG_SOIL ? 1 : -1); }
+
+// Cell list of one phase for the current positions.
+int build_cells(sphg_ctx* c, int k)
+{
+    PhaseBuf& b = c->ph[k];
+    if (b.cells_valid) return SPHG_OK;
+    RC(fill_int(c, b.d.cell_count, c->ncell, 0));
+    RC(launch(c, "cell_key", k_cell_key, nblk(b.n), TPB, b.d, c->gl[k], c->d_err));
+    RC(exclusive_scan(c, b.d.cell_count, b.d.cell_start, c->ncell));
+    RC(launch(c, "cell_scatter", k_cell_scatter, nblk(b.n), TPB, b.d));
+    RC(launch(c, "cell_sort", k_cell_sort, nblk(c->ncell), TPB, b.d, c->ncell));
+    b.cells_valid = true;
+    return SPHG_OK;
+}
+
+int check_flags(sphg_ctx* c)
+{
+    int err = 0;
+    HIPCHK(c, hipMemcpyAsync(&err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (err & 1) {
+        int z = 0;
+        HIPCHK(c, hipMemcpy(c->d_err, &z, sizeof(int), hipMemcpyHostToDevice));
+        return SPHG_ERR_OUT_OF_RANGE;
+    }
+    return SPHG_OK;
+}
+
+void set_kparams(sphg_ctx* c)
+{
+    const sphg_params& P = c->P;
+    KParams& K = c->K;
+    K.skf = P.skf;
+    K.nnps = P.nnps;
+    const int sk = (P.skf == 1 || P.skf == 4) ? 2 : ((P.skf == 2 || P.skf == 3) ? 3 : 0);
+    K.scale_k = (double)sk;
+    K.alpha = P.alpha; K.beta = P.beta; K.etq = P.etq;
+    K.dd = P.dd; K.p1 = P.p1; K.p2 = P.p2;
+    K.delta = P.delta; K.epsilon = P.epsilon;
+    K.gravity = P.gravity; K.numeric_gravity = P.numeric_gravity;
+    K.w_rho0 = P.water_rho0; K.w_b = P.water_b; K.w_gamma = P.water_gamma; K.w_c = P.water_c;
+    K.w_mu = P.water_viscosity;
+    K.s_rho0 = P.soil_rho0; K.s_k = P.soil_k; K.s_c = P.soil_c;
+    K.s_G = P.soil_E / (2.0 * (1.0 + P.soil_niu));                       // m_particles.f90:3061
+    const double t = std::tan(P.soil_phi);
+    K.s_alpha1 = t / std::sqrt(9.0 + 12.0 * (t * t));                   // m_particles.f90:3337
+    K.s_perm = P.soil_permeability;
+    K.dt = P.dt;
+    K.artificial_density = P.artificial_density;
+    K.visc_artificial = P.visc_artificial;
+    K.average_velocity = P.average_velocity;
+    K.soil_artificial_stress = P.soil_artificial_stress;
+    K.water_artificial_volume = P.water_artificial_volume;
+    K.volume_fraction = P.volume_fraction;
+    K.plasticity = P.plasticity;
+    K.yield_criterion = P.yield_criterion;
+    K.two_phase = P.single_phase ? 0 : 1;
+}
+
+int check_params(const sphg_params& P)
+{
+    if (!(P.skf >= 1 && P.skf <= 4)) return SPHG_ERR_BAD_KERNEL;
+    if (P.nnps != 1 && P.nnps != 2) return SPHG_ERR_UNSUPPORTED;
+    if (!(P.x_maxgeom > P.x_mingeom && P.y_maxgeom > P.y_mingeom)) return SPHG_ERR_INVALID;
+    return SPHG_OK;
+}
+
+// Cell grid over the geometry box, cell size >= the largest support radius.
+int setup_grid(sphg_ctx* c)
+{
+    const sphg_params& P = c->P;
+    double hmax = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        PhaseBuf& b = c->ph[k];
+        if (!b.bound) continue;
+        for (int i = 0; i < b.n; ++i) hmax = std::fmax(hmax, b.view.hsml[i]);
+    }
+    if (!(hmax > 0.0)) return SPHG_ERR_INVALID;
+    const double cs = c->K.scale_k * hmax * (1.0 + 1.0e-9);
+    Grid g{};
+    g.x0 = P.x_mingeom; g.y0 = P.y_mingeom; g.x1 = P.x_maxgeom; g.y1 = P.y_maxgeom;
+    g.inv_cs = 1.0 / cs;
+    const double fx = std::floor((P.x_maxgeom - P.x_mingeom) / cs) + 1.0;
+    const double fy = std::floor((P.y_maxgeom - P.y_mingeom) / cs) + 1.0;
+    if (fx * fy > 6.0e7) return SPHG_ERR_UNSUPPORTED;
+    g.ncx = (int)fx;
+    g.ncy = (int)fy;
+    c->g = g;
+    const int ncell = g.ncx * g.ncy;
+    // per phase: reference link_list sorting grid (init_grid, m_particles.f90:1069-1113)
+    for (int k = 0; k < 2; ++k) {
+        Grid gk = g;
+        PhaseBuf& b = c->ph[k];
+        gk.link_list = (P.nnps == 2) ? 1 : 0;
+        if (b.bound && b.n > 0) {
+            const double dgx = P.x_maxgeom - P.x_mingeom, dgy = P.y_maxgeom - P.y_mingeom;
+            int ngx = (int)std::sqrt(b.n * dgx / (dgy * 3.0)) + 1;
+            if (ngx > P.maxngx) ngx = P.maxngx;
+            int ngy = (int)(ngx * dgy / dgx) + 1;
+            if (ngy > P.maxngy) ngy = P.maxngy;
+            const double h1 = b.view.hsml[0];
+            gk.lreach_x = (int)((double)(float)ngx * h1 / dgx) + 1;
+            gk.lreach_y = (int)((double)(float)ngy * h1 / dgy) + 1;
+            gk.lmin_x = P.x_mingeom;
+            gk.lmin_y = P.y_mingeom;
+            gk.lscale_x = (double)(float)ngx / dgx;
+            gk.lscale_y = (double)(float)ngy / dgy;
+            if (k == 0) c->K.h1_water = h1; else c->K.h1_soil = h1;
+        }
+        c->gl[k] = gk;
+    }
+    // (re)allocate cell arrays
+    if (ncell != c->ncell) {
+        c->ncell = ncell;
+        for (int k = 0; k < 2; ++k) {
+            PhaseBuf& b = c->ph[k];
+            if (!b.bound) continue;
+            RC(dalloc(c, b, &b.d.cell_count, (size_t)ncell));
+            RC(dalloc(c, b, &b.d.cell_start, (size_t)ncell + 1));
+            b.cells_valid = false;
+        }
+        const int ntiles = (ncell + SCAN_TILE - 1) / SCAN_TILE;
+        int need = ntiles + 1;
+        for (int k = 0; k < 2; ++k) need = std::max(need, (c->ph[k].n + SCAN_TILE - 1) / SCAN_TILE + 1);
+        if (need > c->scan_cap) {
+            if (c->d_scan) hipFree(c->d_scan);
+            HIPCHK(c, hipMalloc(&c->d_scan, sizeof(int) * (size_t)(need + 1)));
+            c->scan_cap = need;
+        }
+    }
+    return SPHG_OK;
+}
+
+// ---- host <-> device field transfer ----------------------------------------
+
+__global__ void k_split2(const double* in, double* a, double* b, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { a[i] = in[2 * i]; b[i] = in[2 * i + 1]; }
+}
+
+__global__ void k_join2(double* out, const double* a, const double* b, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { out[2 * i] = a[i]; out[2 * i + 1] = b[i]; }
+}
+
+int up_scalar(sphg_ctx* c, PhaseBuf& b, const double* h, double* d)
+{
+    if (!h) return SPHG_OK;
+    HIPCHK(c, hipMemcpyAsync(d, h, sizeof(double) * (size_t)b.n, hipMemcpyHostToDevice, c->stream));
+    return SPHG_OK;
+}
+
+int up_int(sphg_ctx* c, PhaseBuf& b, const int32_t* h, int* d)
+{
+    if (!h) return SPHG_OK;
+    HIPCHK(c, hipMemcpyAsync(d, h, sizeof(int) * (size_t)b.n, hipMemcpyHostToDevice, c->stream));
+    return SPHG_OK;
+}
+
+int up_vec(sphg_ctx* c, PhaseBuf& b, const double* h, double* dx, double* dy)
+{
+    if (!h) return SPHG_OK;
+    HIPCHK(c, hipMemcpyAsync(b.stage, h, sizeof(double) * 2 * (size_t)b.n, hipMemcpyHostToDevice, c->stream));
+    return launch(c, "split", k_split2, nblk(b.n), TPB, (const double*)b.stage, dx, dy, b.n);
+}
+
+int down_scalar(sphg_ctx* c, PhaseBuf& b, double* h, const double* d)
+{
+    if (!h) return SPHG_OK;
+    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(double) * (size_t)b.n, hipMemcpyDeviceToHost, c->stream));
+    return SPHG_OK;
+}
+
+int down_int(sphg_ctx* c, PhaseBuf& b, int32_t* h, const int* d)
+{
+    if (!h) return SPHG_OK;
+    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(int) * (size_t)b.n, hipMemcpyDeviceToHost, c->stream));
+    return SPHG_OK;
+}
+
+int down_vec(sphg_ctx* c, PhaseBuf& b, double* h, const double* dx, const double* dy)
+{
+    if (!h) return SPHG_OK;
+    RC(launch(c, "join", k_join2, nblk(b.n), TPB, b.stage, dx, dy, b.n));
+    HIPCHK(c, hipMemcpyAsync(h, b.stage, sizeof(double) * 2 * (size_t)b.n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SPHG_OK;
+}
+
+int upload(sphg_ctx* c, int k, uint64_t m)
+{
+    PhaseBuf& b = c->ph[k];
+    const sphg_phase_view& v = b.view;
+    DPhase& d = b.d;
+    if (m & SPHG_F_X) { RC(up_vec(c, b, v.x, d.x, d.y)); b.cells_valid = false; }
+    if (m & SPHG_F_VX) RC(up_vec(c, b, v.vx, d.vx, d.vy));
+    if (m & SPHG_F_RHO) RC(up_scalar(c, b, v.rho, d.rho));
+    if (m & SPHG_F_P) RC(up_scalar(c, b, v.p, d.p));
+    if (m & SPHG_F_C) RC(up_scalar(c, b, v.c, d.c));
+    if (m & SPHG_F_VOF) RC(up_scalar(c, b, v.vof, d.vof));
+    if (m & SPHG_F_STRESS) {
+        if (b.d.material == SPHG_SOIL) {
+            RC(up_scalar(c, b, v.sxx, d.sxx));
+            RC(up_scalar(c, b, v.sxy, d.sxy));
+            RC(up_scalar(c, b, v.syy, d.syy));
+        }
+    }
+    if (m & SPHG_F_EPSP) RC(up_scalar(c, b, v.epsilon_p, d.eps_p));
+    if (m & SPHG_F_DRHO) RC(up_scalar(c, b, v.drho, d.drho));
+    if (m & SPHG_F_DVX) RC(up_vec(c, b, v.dvx, d.ax, d.ay));
+    if (m & SPHG_F_DVOF) RC(up_scalar(c, b, v.dvof, d.dvof));
+    if (m & SPHG_F_DSTRESS) {
+        RC(up_scalar(c, b, v.dsxx, d.dsxx));
+        RC(up_scalar(c, b, v.dsxy, d.dsxy));
+        RC(up_scalar(c, b, v.dsyy, d.dsyy));
+        RC(up_scalar(c, b, v.dp, d.dp));
+    }
+    if (m & SPHG_F_AV) RC(up_vec(c, b, v.av, d.avx, d.avy));
+    if (m & SPHG_F_STRAIN) {
+        RC(up_scalar(c, b, v.txx, d.txx));
+        RC(up_scalar(c, b, v.txy, d.txy));
+        RC(up_scalar(c, b, v.tyy, d.tyy));
+        RC(up_scalar(c, b, v.vcc, d.vcc));
+        RC(up_scalar(c, b, v.wxy, d.wxy));
+    }
+    if (m & SPHG_F_VOF2) RC(up_scalar(c, b, v.vof2, d.vof2));
+    if (m & SPHG_F_FAIL) RC(up_int(c, b, v.fail, d.fail));
+    if (m & SPHG_F_MIN) {
+        RC(up_scalar(c, b, v.rho_min, d.rho_min));
+        RC(up_vec(c, b, v.v_min, d.vx_min, d.vy_min));
+        RC(up_scalar(c, b, v.vof_min, d.vof_min));
+    }
+    return SPHG_OK;
+}
+
+int download(sphg_ctx* c, int k, uint64_t m)
+{
+    PhaseBuf& b = c->ph[k];
+    sphg_phase_view& v = b.view;
+    DPhase& d = b.d;
+    if (m & SPHG_F_X) RC(down_vec(c, b, v.x, d.x, d.y));
+    if (m & SPHG_F_VX) RC(down_vec(c, b, v.vx, d.vx, d.vy));
+    if (m & SPHG_F_RHO) RC(down_scalar(c, b, v.rho, d.rho));
+    if (m & SPHG_F_P) RC(down_scalar(c, b, v.p, d.p));
+    if (m & SPHG_F_C) RC(down_scalar(c, b, v.c, d.c));
+    if (m & SPHG_F_VOF) RC(down_scalar(c, b, v.vof, d.vof));
+    if ((m & SPHG_F_STRESS) && b.d.material == SPHG_SOIL) {
+        RC(down_scalar(c, b, v.sxx, d.sxx));
+        RC(down_scalar(c, b, v.sxy, d.sxy));
+        RC(down_scalar(c, b, v.syy, d.syy));
+    }
+    if (m & SPHG_F_EPSP) RC(down_scalar(c, b, v.epsilon_p, d.eps_p));
+    if (m & SPHG_F_DRHO) RC(down_scalar(c, b, v.drho, d.drho));
+    if (m & SPHG_F_DVX) RC(down_vec(c, b, v.dvx, d.ax, d.ay));
+    if (m & SPHG_F_DVOF) RC(down_scalar(c, b, v.dvof, d.dvof));
+    if (m & SPHG_F_DSTRESS) {
+        RC(down_scalar(c, b, v.dsxx, d.dsxx));
+        RC(down_scalar(c, b, v.dsxy, d.dsxy));
+        RC(down_scalar(c, b, v.dsyy, d.dsyy));
+        RC(down_scalar(c, b, v.dp, d.dp));
+    }
+    if (m & SPHG_F_AV) RC(down_vec(c, b, v.av, d.avx, d.avy));
+    if (m & (SPHG_F_STRAIN | SPHG_F_STRESS)) {
+        RC(down_scalar(c, b, v.txx, d.txx));
+        RC(down_scalar(c, b, v.txy, d.txy));
+        RC(down_scalar(c, b, v.tyy, d.tyy));
+        if (m & SPHG_F_STRAIN) {
+            RC(down_scalar(c, b, v.vcc, d.vcc));
+            RC(down_scalar(c, b, v.wxy, d.wxy));
+        }
+    }
+    if (m & SPHG_F_VOF2) RC(down_scalar(c, b, v.vof2, d.vof2));
+    if (m & SPHG_F_FAIL) RC(down_int(c, b, v.fail, d.fail));
+    if (m & SPHG_F_COUNTIAC) RC(down_int(c, b, v.countiac, d.countiac));
+    if (m & SPHG_F_MIN) {
+        RC(down_scalar(c, b, v.rho_min, d.rho_min));
+        RC(down_vec(c, b, v.v_min, d.vx_min, d.vy_min));
+        RC(down_scalar(c, b, v.vof_min, d.vof_min));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // Water stress is sigma = mu * t (newtonian_fluid, m_particles.f90:2496-2498).
+    if ((m & SPHG_F_STRESS) && b.d.material == SPHG_WATER && v.txx) {
+        const double mu = c->P.water_viscosity;
+        for (int i = 0; i < b.n; ++i) {
+            if (v.sxx) v.sxx[i] = mu * v.txx[i];
+            if (v.syy) v.syy[i] = mu * v.tyy[i];
+            if (v.sxy) v.sxy[i] = mu * v.txy[i];
+        }
+    }
+    if ((m & SPHG_F_FAIL) && v.fail) {
+        int nf = 0;
+        for (int i = 0; i < b.n; ++i) nf += v.fail[i] == 1;
+        v.nfail = nf;
+    }
+    return SPHG_OK;
+}
+
+int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
+{
+    PhaseBuf& b = c->ph[k];
+    free_phase(b);
+    b.view = *v;
+    b.n = v->ntotal + v->nvirt;
+    b.bound = true;
+    DPhase& d = b.d;
+    d.n = b.n;
+    d.ntotal = v->ntotal;
+    d.material = v->material;
+    d.dspp = v->dspp;
+    const size_t n = (size_t)b.n;
+    RC(dalloc(c, b, &d.itype, n));
+    RC(dalloc(c, b, &d.fail, n));
+    RC(dalloc(c, b, &d.countiac, n));
+    RC(dalloc(c, b, &d.key, n));
+    RC(dalloc(c, b, &d.slot, n));
+    RC(dalloc(c, b, &d.order, n));
+    double** dbls[] = {&d.x, &d.y, &d.vx, &d.vy, &d.mass, &d.hsml, &d.rho, &d.rho_min, &d.rho_next,
+                       &d.vof, &d.vof_min, &d.vof2, &d.dvof, &d.dvof_x, &d.p, &d.c, &d.c_next,
+                       &d.drho, &d.ax, &d.ay, &d.avx, &d.avy, &d.vx_min, &d.vy_min,
+                       &d.txx, &d.txy, &d.tyy, &d.vcc, &d.wxy, &d.sxx, &d.sxy, &d.syy, &d.eps_p,
+                       &d.dsxx, &d.dsxy, &d.dsyy, &d.dp, &d.repx, &d.repy, &d.drgx, &d.drgy,
+                       &d.porx, &d.pory};
+    for (double** p : dbls) {
+        RC(dalloc(c, b, p, n));
+        RC(fill_dbl(c, *p, b.n, 0.0));
+    }
+    RC(dalloc(c, b, &b.stage, 2 * n));
+    RC(fill_int(c, d.itype, b.n, 0));
+    RC(fill_int(c, d.fail, b.n, 0));
+    RC(fill_int(c, d.countiac, b.n, 0));
+    // pair-count scratch
+    RC(dalloc(c, b, &b.fwd, n));
+    RC(dalloc(c, b, &b.off, n + 1));
+    b.cells_valid = false;
+    return SPHG_OK;
+}
+
+int ensure_pairs(sphg_ctx* c, PhaseBuf& b, int cap)
+{
+    if (cap <= b.pair_cap) return SPHG_OK;
+    RC(dalloc(c, b, &b.pi, (size_t)cap));
+    RC(dalloc(c, b, &b.pj, (size_t)cap));
+    RC(dalloc(c, b, &b.w, (size_t)cap));
+    RC(dalloc(c, b, &b.dwdx, 2 * (size_t)cap));
+    b.pair_cap = cap;
+    return SPHG_OK;
+}
+
+void swap_c(PhaseBuf& b) { std::swap(b.d.c, b.d.c_next); }
+
+// ---- fused passes -------------------------------------------------------
+
+constexpr int MASK_A_W = M_CON | M_DRHO_DELTA | M_STRAIN | M_EOS | M_XSPH | M_REP | M_CROSS;
+constexpr int MASK_A_S = M_CON | M_DRHO_DELTA | M_STRAIN | M_EOS | M_XSPH | M_REP | M_CROSS | M_PLASTIC;
+constexpr int MASK_B = M_MOMENTUM | M_ART | M_DELTA_FIELD | M_REP | M_GRAV | M_CROSS;
+
+int mask_a(const sphg_ctx* c, int base)
+{
+    int m = base;
+    if (!c->P.artificial_density) m &= ~M_DRHO_DELTA;
+    if (!c->P.average_velocity) m &= ~M_XSPH;
+    if (c->P.single_phase) m &= ~M_CROSS;
+    return m;
+}
+
+int mask_b(const sphg_ctx* c, int phase)
+{
+    int m = MASK_B;
+    if (!c->P.visc_artificial) m &= ~M_ART;
+    if (phase == 0 && !c->P.water_artificial_volume) m &= ~M_DELTA_FIELD;
+    if (phase == 1 && !c->P.soil_artificial_stress) m &= ~M_DELTA_FIELD;
+    if (c->P.single_phase) m &= ~M_CROSS;
+    return m;
+}
+
+// single_step (declarations_sph.f90:1131-1435) after the soil Drucker-Prager
+// projection of :1244 has been applied.
+int rhs_two_phase(sphg_ctx* c)
+{
+    PhaseBuf& W = c->ph[0];
+    PhaseBuf& S = c->ph[1];
+    RC(build_cells(c, 0));
+    RC(build_cells(c, 1));
+    RC(launch(c, "water_a", k_water_a, nblk(W.n), TPB, W.d, S.d, c->gl[0], c->K, mask_a(c, MASK_A_W)));
+    RC(launch(c, "soil_a", k_soil_a, nblk(S.n), TPB, S.d, W.d, c->gl[1], c->K, mask_a(c, MASK_A_S)));
+    swap_c(W);
+    swap_c(S);
+    RC(launch(c, "water_b", k_water_b, nblk(W.n), TPB, W.d, c->gl[0], c->K, mask_b(c, 0)));
+    RC(launch(c, "soil_b", k_soil_b, nblk(S.n), TPB, S.d, c->gl[1], c->K, mask_b(c, 1)));
+    return SPHG_OK;
+}
+
+// single_step_for_water (declarations_sph.f90:1439-1526).
+int rhs_water(sphg_ctx* c)
+{
+    PhaseBuf& W = c->ph[0];
+    RC(build_cells(c, 0));
+    RC(launch(c, "sum_density", k_sum_density, nblk(W.n), TPB, W.d, c->gl[0], c->K));
+    std::swap(W.d.rho, W.d.rho_next);
+    int ma = M_DRHO_DELTA | M_XSPH;
+    if (!c->P.artificial_density) ma &= ~M_DRHO_DELTA;
+    if (!c->P.average_velocity) ma &= ~M_XSPH;
+    RC(launch(c, "water1_a", k_water1_a, nblk(W.n), TPB, W.d, c->gl[0], c->K, ma));
+    int mb = c->P.visc_artificial ? M_ART : 0;
+    RC(launch(c, "water1_b", k_water1_b, nblk(W.n), TPB, W.d, c->gl[0], c->K, mb));
+    return SPHG_OK;
+}
+
+int check_fused_switches(const sphg_ctx* c)
+{
+    const sphg_params& P = c->P;
+    if (P.summation_density || P.stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
+    if (!P.single_phase) {
+        if (!c->ph[1].bound) return SPHG_ERR_INVALID;
+        if (P.yield_criterion != 2) return SPHG_ERR_UNSUPPORTED;
+        if (P.plasticity != 3 && P.plasticity != 0) return SPHG_ERR_UNSUPPORTED;
+    }
+    if (!c->ph[0].bound) return SPHG_ERR_INVALID;
+    return SPHG_OK;
+}
+
+int one_step(sphg_ctx* c)
+{
+    const int step = ++c->itimestep;
+    PhaseBuf& W = c->ph[0];
+    PhaseBuf& S = c->ph[1];
+    if (c->P.single_phase) {
+        if (step != 1) RC(launch(c, "first_half", k_first_half_water, nblk(W.n), TPB, W.d, c->K));
+        RC(rhs_water(c));
+        RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, 0));
+        W.cells_valid = false;
+        return SPHG_OK;
+    }
+    if (step != 1) {
+        RC(launch(c, "first_half", k_first_half_water, nblk(W.n), TPB, W.d, c->K));
+        RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 1, 2));
+    } else {
+        RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 0, 1));
+    }
+    RC(rhs_two_phase(c));
+    const int vf = c->P.volume_fraction ? 1 : 0;
+    RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, vf));
+    RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, step == 1 ? 1 : 0));
+    W.cells_valid = false;
+    S.cells_valid = false;
+    return SPHG_OK;
+}
+
+// ---- materialised pair lists -------------------------------------------
+
+int materialise(sphg_ctx* c, int ka, int kb, int same, int link, sphg_phase_view* out_view,
+                int* out_niac)
+{
+    PhaseBuf& A = c->ph[ka];
+    PhaseBuf& B = c->ph[kb];
+    RC(build_cells(c, kb));
+    const double h1 = (ka == 0) ? c->K.h1_water : c->K.h1_soil;
+    RC(launch(c, "pair_count", k_pair_count, nblk(A.n), TPB, A.d, B.d, c->gl[ka], c->K, same, link, h1,
+              A.fwd, A.d.countiac));
+    RC(exclusive_scan(c, A.fwd, A.off, A.n));
+    int total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, A.off + A.n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    RC(check_flags(c));
+    *out_niac = total;
+    if (total > out_view->max_interaction) return SPHG_ERR_TOO_MANY_INTERACTIONS;
+    if (!out_view->pair_i || !out_view->pair_j || !out_view->w || !out_view->dwdx) return SPHG_ERR_INVALID;
+    RC(ensure_pairs(c, A, total > 0 ? total : 1));
+    RC(launch(c, "pair_write", k_pair_write, nblk(A.n), TPB, A.d, B.d, c->gl[ka], c->K, same, link, h1,
+              (const int*)A.off, A.pi, A.pj, A.w, A.dwdx));
+    if (total > 0) {
+        HIPCHK(c, hipMemcpyAsync(out_view->pair_i, A.pi, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out_view->pair_j, A.pj, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out_view->w, A.w, sizeof(double) * (size_t)total, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out_view->dwdx, A.dwdx, sizeof(double) * 2 * (size_t)total, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SPHG_OK;
+}
+
+int bound_index(sphg_ctx* c, int phase)
+{
+    const int k = phase_index(phase);
+    if (k < 0 || !c->ph[k].bound) return -1;
+    return k;
+}
+
+}  // namespace
+
+// ===========================================================================
+// public entry points
+// ===========================================================================
+extern "C" {
+
+int sphg_create(const sphg_params* params, sphg_ctx** out)
+{
+    if (!params || !out) return SPHG_ERR_INVALID;
+    RC(check_params(*params));
+    sphg_ctx* c = new sphg_ctx();
+    c->P = *params;
+    set_kparams(c);
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipMalloc(&c->d_err, sizeof(int) * 4) != hipSuccess) {
+        delete c;
+        return SPHG_ERR_HIP;
+    }
+    const int zeros[4] = {0, 0, 0, 0};
+    if (hipMemcpy(c->d_err, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess) {
+        sphg_destroy(c);
+        return SPHG_ERR_HIP;
+    }
+    *out = c;
+    return SPHG_OK;
+}
+
+void sphg_destroy(sphg_ctx* c)
+{
+    if (!c) return;
+    hipStreamSynchronize(c->stream);
+    drain_events(c);
+    for (auto& pb : c->ph) free_phase(pb);
+    for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
+    if (c->d_err) hipFree(c->d_err);
+    if (c->d_scan) hipFree(c->d_scan);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sphg_set_params(sphg_ctx* c, const sphg_params* params)
+{
+    if (!c || !params) return SPHG_ERR_INVALID;
+    RC(check_params(*params));
+    const bool regrid = params->x_mingeom != c->P.x_mingeom || params->x_maxgeom != c->P.x_maxgeom ||
+                        params->y_mingeom != c->P.y_mingeom || params->y_maxgeom != c->P.y_maxgeom ||
+                        params->skf != c->P.skf || params->nnps != c->P.nnps ||
+                        params->maxngx != c->P.maxngx || params->maxngy != c->P.maxngy;
+    c->P = *params;
+    set_kparams(c);
+    if (regrid && (c->ph[0].bound || c->ph[1].bound)) {
+        c->ncell = 0;
+        RC(setup_grid(c));
+        for (auto& pb : c->ph) pb.cells_valid = false;
+    }
+    return SPHG_OK;
+}
+
+int sphg_bind_phase(sphg_ctx* c, int phase, const sphg_phase_view* v)
+{
+    if (!c || !v) return SPHG_ERR_INVALID;
+    const int k = phase_index(phase);
+    if (k < 0 || v->material != phase) return SPHG_ERR_INVALID;
+    if (v->ntotal < 0 || v->nvirt < 0 || v->ntotal + v->nvirt > v->maxn) return SPHG_ERR_INVALID;
+    if (!v->x || !v->vx || !v->mass || !v->hsml || !v->itype || !v->rho) return SPHG_ERR_INVALID;
+    RC(alloc_phase(c, k, v));
+    PhaseBuf& b = c->ph[k];
+    RC(up_int(c, b, v->itype, b.d.itype));
+    RC(up_scalar(c, b, v->mass, b.d.mass));
+    RC(up_scalar(c, b, v->hsml, b.d.hsml));
+    RC(upload(c, k, SPHG_F_ALL & ~SPHG_F_COUNTIAC));
+    if (!v->c) RC(fill_dbl(c, b.d.c, b.n, phase == SPHG_WATER ? c->P.water_c : c->P.soil_c));
+    if (!v->vof) RC(fill_dbl(c, b.d.vof, b.n, 1.0));
+    HIPCHK(c, hipMemcpyAsync(b.d.c_next, b.d.c, sizeof(double) * (size_t)b.n, hipMemcpyDeviceToDevice, c->stream));
+    c->ncell = 0;
+    RC(setup_grid(c));
+    for (auto& pb : c->ph) pb.cells_valid = false;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SPHG_OK;
+}
+
+int sphg_sync(sphg_ctx* c, int phase, uint64_t mask, int to_host)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    const int k = bound_index(c, phase);
+    if (k < 0) return SPHG_ERR_INVALID;
+    if (to_host) {
+        RC(download(c, k, mask));
+        RC(check_flags(c));
+    } else {
+        RC(upload(c, k, mask));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return SPHG_OK;
+}
+
+int sphg_set_itimestep(sphg_ctx* c, int it)
+{
+    if (!c || it < 0) return SPHG_ERR_INVALID;
+    c->itimestep = it;
+    return SPHG_OK;
+}
+
+int sphg_get_itimestep(const sphg_ctx* c) { return c ? c->itimestep : -1; }
+
+int sphg_direct_find(sphg_ctx* c, int phase)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    const int k = bound_index(c, phase);
+    if (k < 0) return SPHG_ERR_INVALID;
+    RC(upload(c, k, SPHG_F_X));
+    int niac = 0;
+    const int rc = materialise(c, k, k, 1, 0, &c->ph[k].view, &niac);
+    c->ph[k].view.niac = niac;
+    if (rc == SPHG_OK) RC(download(c, k, SPHG_F_COUNTIAC));
+    return rc;
+}
+
+int sphg_link_list(sphg_ctx* c, int phase)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    const int k = bound_index(c, phase);
+    if (k < 0) return SPHG_ERR_INVALID;
+    RC(upload(c, k, SPHG_F_X));
+    int niac = 0;
+    const int rc = materialise(c, k, k, 1, 1, &c->ph[k].view, &niac);
+    c->ph[k].view.niac = niac;
+    if (rc == SPHG_OK) RC(download(c, k, SPHG_F_COUNTIAC));
+    return rc;
+}
+
+int sphg_direct_find_2(sphg_ctx* c)
+{
+    if (!c || !c->ph[0].bound || !c->ph[1].bound) return SPHG_ERR_INVALID;
+    RC(upload(c, 0, SPHG_F_X));
+    RC(upload(c, 1, SPHG_F_X));
+    int niac = 0;
+    const int rc = materialise(c, 0, 1, 0, 0, &c->ph[0].view, &niac);
+    c->ph[0].view.niac = niac;
+    if (rc != SPHG_OK) return rc;
+    // countiac of the soil phase: number of water partners
+    PhaseBuf& S = c->ph[1];
+    RC(build_cells(c, 0));
+    RC(launch(c, "pair_count", k_pair_count, nblk(S.n), TPB, S.d, c->ph[0].d, c->gl[1], c->K, 0, 0, 0.0,
+              S.fwd, S.d.countiac));
+    RC(download(c, 0, SPHG_F_COUNTIAC));
+    RC(download(c, 1, SPHG_F_COUNTIAC));
+    return SPHG_OK;
+}
+
+int sphg_con_density(sphg_ctx* c, int phase)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    const int k = bound_index(c, phase);
+    if (k < 0) return SPHG_ERR_INVALID;
+    RC(upload(c, k, SPHG_F_X | SPHG_F_VX | SPHG_F_RHO));
+    RC(build_cells(c, k));
+    PhaseBuf& b = c->ph[k];
+    if (k == 0)
+        RC(launch(c, "water_a", k_water_a, nblk(b.n), TPB, b.d, c->ph[1].d, c->gl[0], c->K, (int)M_CON));
+    else
+        RC(launch(c, "soil_a", k_soil_a, nblk(b.n), TPB, b.d, c->ph[0].d, c->gl[1], c->K, (int)M_CON));
+    RC(download(c, k, SPHG_F_DRHO));
+    return check_flags(c);
+}
+
+int sphg_delta_sph_density(sphg_ctx* c, int phase)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    const int k = bound_index(c, phase);
+    if (k < 0) return SPHG_ERR_INVALID;
+    RC(upload(c, k, SPHG_F_X | SPHG_F_VX | SPHG_F_RHO | SPHG_F_C | SPHG_F_DRHO));
+    RC(build_cells(c, k));
+    PhaseBuf& b = c->ph[k];
+    const int m = M_DRHO_DELTA | M_DRHO_ACCUM;
+    if (k == 0)
+        RC(launch(c, "water_a", k_water_a, nblk(b.n), TPB, b.d, c->ph[1].d, c->gl[0], c->K, m));
+    else
+        RC(launch(c, "soil_a", k_soil_a, nblk(b.n), TPB, b.d, c->ph[0].d, c->gl[1], c->K, m));
+    RC(download(c, k, SPHG_F_DRHO));
+    return check_flags(c);
+}
+
+int sphg_sum_density(sphg_ctx* c, int phase)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    const int k = bound_index(c, phase);
+    if (k < 0) return SPHG_ERR_INVALID;
+    RC(upload(c, k, SPHG_F_X | SPHG_F_RHO));
+    RC(build_cells(c, k));
+    PhaseBuf& b = c->ph[k];
+    RC(launch(c, "sum_density", k_sum_density, nblk(b.n), TPB, b.d, c->gl[k], c->K));
+    std::swap(b.d.rho, b.d.rho_next);
+    RC(download(c, k, SPHG_F_RHO));
+    return check_flags(c);
+}
+
+int sphg_internal_force(sphg_ctx* c, int phase)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    const int k = bound_index(c, phase);
+    if (k < 0) return SPHG_ERR_INVALID;
+    PhaseBuf& b = c->ph[k];
+    if (k == 0) {
+        RC(upload(c, k, SPHG_F_X | SPHG_F_VX | SPHG_F_RHO | SPHG_F_VOF | SPHG_F_C));
+        RC(build_cells(c, k));
+        RC(launch(c, "water_a", k_water_a, nblk(b.n), TPB, b.d, c->ph[1].d, c->gl[0], c->K, (int)(M_STRAIN | M_EOS)));
+        swap_c(b);
+        RC(launch(c, "water_b", k_water_b, nblk(b.n), TPB, b.d, c->gl[0], c->K, (int)M_MOMENTUM));
+        RC(download(c, k, SPHG_F_DVX | SPHG_F_P | SPHG_F_C | SPHG_F_STRAIN | SPHG_F_STRESS));
+    } else {
+        if (c->P.yield_criterion != 2 || (c->P.plasticity != 3 && c->P.plasticity != 0)) return SPHG_ERR_UNSUPPORTED;
+        RC(upload(c, k, SPHG_F_X | SPHG_F_VX | SPHG_F_RHO | SPHG_F_VOF | SPHG_F_C | SPHG_F_P | SPHG_F_STRESS |
+                            SPHG_F_DSTRESS | SPHG_F_EPSP));
+        RC(build_cells(c, k));
+        RC(launch(c, "first_half", k_first_half_soil, nblk(b.n), TPB, b.d, c->K, 0, 1));
+        RC(launch(c, "soil_a", k_soil_a, nblk(b.n), TPB, b.d, c->ph[0].d, c->gl[1], c->K,
+                  (int)(M_STRAIN | M_EOS | M_PLASTIC | M_DS_ACCUM)));
+        swap_c(b);
+        RC(launch(c, "soil_b", k_soil_b, nblk(b.n), TPB, b.d, c->gl[1], c->K, (int)M_MOMENTUM));
+        RC(download(c, k, SPHG_F_DVX | SPHG_F_P | SPHG_F_C | SPHG_F_STRAIN | SPHG_F_STRESS | SPHG_F_DSTRESS |
+                              SPHG_F_EPSP | SPHG_F_FAIL));
+    }
+    return check_flags(c);
+}
+
+int sphg_art_visc(sphg_ctx* c, int phase)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    const int k = bound_index(c, phase);
+    if (k < 0) return SPHG_ERR_INVALID;
+    RC(upload(c, k, SPHG_F_X | SPHG_F_VX | SPHG_F_RHO | SPHG_F_C | SPHG_F_DVX));
+    RC(build_cells(c, k));
+    PhaseBuf& b = c->ph[k];
+    if (c->P.single_phase && k == 0)
+        RC(launch(c, "water1_b", k_water1_b, nblk(b.n), TPB, b.d, c->gl[0], c->K, (int)M_ART));
+    else if (k == 0)
+        RC(launch(c, "water_b", k_water_b, nblk(b.n), TPB, b.d, c->gl[0], c->K, (int)M_ART));
+    else
+        RC(launch(c, "soil_b", k_soil_b, nblk(b.n), TPB, b.d, c->gl[1], c->K, (int)M_ART));
+    RC(download(c, k, SPHG_F_DVX));
+    return check_flags(c);
+}
+
+int sphg_single_step(sphg_ctx* c, int sync)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    RC(check_fused_switches(c));
+    if (c->P.single_phase) {
+        RC(rhs_water(c));
+    } else {
+        PhaseBuf& S = c->ph[1];
+        RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 0, 1));
+        RC(rhs_two_phase(c));
+    }
+    if (sync) {
+        const uint64_t m = SPHG_F_ALL & ~SPHG_F_X & ~SPHG_F_VX & ~SPHG_F_MIN;
+        RC(download(c, 0, m));
+        if (!c->P.single_phase) RC(download(c, 1, m));
+    }
+    return check_flags(c);
+}
+
+int sphg_time_integration(sphg_ctx* c, int nsteps, int sync)
+{
+    if (!c || nsteps < 0) return SPHG_ERR_INVALID;
+    RC(check_fused_switches(c));
+    for (int it = 0; it < nsteps; ++it) RC(one_step(c));
+    if (sync) {
+        RC(download(c, 0, SPHG_F_ALL));
+        if (!c->P.single_phase) RC(download(c, 1, SPHG_F_ALL));
+        return check_flags(c);
+    }
+    HIPCHK(c, hipGetLastError());
+    return SPHG_OK;
+}
+
+const char* sphg_strerror(int code)
+{
+    switch (code) {
+    case SPHG_OK: return "ok";
+    case SPHG_ERR_TOO_MANY_INTERACTIONS: return ">>> ERROR <<< : Too many interactions";
+    case SPHG_ERR_OUT_OF_RANGE: return ">>> ERROR <<< : Particle out of range";
+    case SPHG_ERR_BAD_KERNEL: return "get_scale_k: No such kernel function!";
+    case SPHG_ERR_INVALID: return "invalid argument or unbound phase";
+    case SPHG_ERR_HIP: return "HIP runtime error";
+    case SPHG_ERR_UNSUPPORTED: return "switch combination not supported by this path";
+    case SPHG_ERR_NONFINITE: return "non-finite value detected";
+    case SPHG_ERR_OUT_OF_MEMORY: return "out of device memory";
+    default: return "unknown error";
+    }
+}
+
+int sphg_last_hip_error(const sphg_ctx* c) { return c ? c->last_hip : 0; }
+
+void* sphg_stream(sphg_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int sphg_profile_enable(sphg_ctx* c, int enable)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    if (!enable) drain_events(c);
+    c->prof = enable != 0;
+    return SPHG_OK;
+}
+
+int sphg_profile_reset(sphg_ctx* c)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    drain_events(c);
+    c->stats.clear();
+    return SPHG_OK;
+}
+
+int sphg_profile_get(sphg_ctx* c, const char* name, int64_t* launches, double* total_ms)
+{
+    if (!c || !name) return SPHG_ERR_INVALID;
+    drain_events(c);
+    auto it = c->stats.find(name);
+    if (it == c->stats.end()) {
+        if (launches) *launches = 0;
+        if (total_ms) *total_ms = 0.0;
+        return SPHG_ERR_INVALID;
+    }
+    if (launches) *launches = it->second.launches;
+    if (total_ms) *total_ms = it->second.ms;
+    return SPHG_OK;
+}
+
+int sphg_pair_counts(sphg_ctx* c, int64_t* wp, int64_t* sp, int64_t* xp)
+{
+    if (!c) return SPHG_ERR_INVALID;
+    // Pairs seen by the most recent fused step: half the same-phase neighbour
+    // counts (countiac holds them in single-phase mode) or the cross counts.
+    int64_t out[3] = {0, 0, 0};
+    for (int k = 0; k < 2; ++k) {
+        PhaseBuf& b = c->ph[k];
+        if (!b.bound) continue;
+        std::vector<int> cnt((size_t)b.n);
+        HIPCHK(c, hipMemcpy(cnt.data(), b.d.countiac, sizeof(int) * (size_t)b.n, hipMemcpyDeviceToHost));
+        int64_t s = 0;
+        for (int v : cnt) s += v;
+        if (c->P.single_phase) out[k] = s / 2;
+        else if (k == 0) out[2] = s;
+    }
+    if (wp) *wp = out[0];
+    if (sp) *sp = out[1];
+    if (xp) *xp = out[2];
+    return SPHG_OK;
+}
+
+}  // extern "C"

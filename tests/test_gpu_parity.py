@@ -1,0 +1,250 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs.  Pair lists bit-exact; FP64 fields within 1e-12
+norm-wise (max|gpu-ref| / max|ref|) after one step."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import assert_fields, canonical_pairs, field_errors
+from sources_amd import abi, scenarios
+from sources_amd.sph import Solver
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL_1STEP = 1e-12
+
+
+def small_cases():
+    return {
+        "savage_skf4": lambda: scenarios.savage_column(),
+        "savage_skf1": lambda: scenarios.savage_column(skf=1),
+        "column_jitter": lambda: scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3,
+                                                            tank_h=0.12, jitter=0.05, seed=7),
+        "slope_small": lambda: scenarios.slope(dx=0.01, scale=0.25, jitter=0.1, seed=3),
+    }
+
+
+def _both(make):
+    sc = make()
+    return sc, sc.copy()
+
+
+@pytest.mark.parametrize("case", list(small_cases()))
+@pytest.mark.parametrize("phase", ["water", "soil"])
+def test_direct_find_bit_exact(case, phase):
+    print(f"case={case} phase={phase}", flush=True)
+    sc, ref = _both(small_cases()[case])
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.direct_find(phase)
+    g = getattr(sc, phase)
+    r = getattr(ref, phase)
+    oracle.call_phase("direct_find", ref.params, r)
+    assert g.niac == r.niac > 0
+    k = r.niac
+    np.testing.assert_array_equal(g.pair_i[:k], r.pair_i[:k])
+    np.testing.assert_array_equal(g.pair_j[:k], r.pair_j[:k])
+    np.testing.assert_array_equal(g.countiac[:r.n], r.countiac[:r.n])
+    assert np.array_equal(g.w[:k], r.w[:k]), np.max(np.abs(g.w[:k] - r.w[:k]))
+    assert np.array_equal(g.dwdx[:k], r.dwdx[:k]), np.max(np.abs(g.dwdx[:k] - r.dwdx[:k]))
+    s.close()
+
+
+@pytest.mark.parametrize("case", list(small_cases()))
+def test_direct_find_2_bit_exact(case):
+    print(f"case={case}", flush=True)
+    sc, ref = _both(small_cases()[case])
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.direct_find_2()
+    oracle.call_cross("direct_find_2", ref.params, ref.water, ref.soil)
+    k = ref.water.niac
+    assert sc.water.niac == k > 0
+    np.testing.assert_array_equal(sc.water.pair_i[:k], ref.water.pair_i[:k])
+    np.testing.assert_array_equal(sc.water.pair_j[:k], ref.water.pair_j[:k])
+    np.testing.assert_array_equal(sc.water.countiac[:ref.water.n], ref.water.countiac[:ref.water.n])
+    np.testing.assert_array_equal(sc.soil.countiac[:ref.soil.n], ref.soil.countiac[:ref.soil.n])
+    assert np.array_equal(sc.water.w[:k], ref.water.w[:k])
+    assert np.array_equal(sc.water.dwdx[:k], ref.water.dwdx[:k])
+    s.close()
+
+
+def _tight_box_column(**kw):
+    # Box so tight that the reference sorting grid's cells are smaller than
+    # the 2h support: link_list then misses pairs (m_particles.f90:1107-1109).
+    return scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3, tank_h=0.12,
+                                      jitter=0.05, seed=11, x_mingeom=-0.02, x_maxgeom=0.32,
+                                      y_mingeom=-0.02, y_maxgeom=0.15, **kw)
+
+
+@pytest.mark.parametrize("make", [scenarios.savage_column, _tight_box_column], ids=["savage", "tight_box"])
+def test_link_list_same_set(make):
+    sc, ref = _both(make)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.link_list("water")
+    oracle.call_phase("link_list", ref.params, ref.water)
+    k = ref.water.niac
+    assert sc.water.niac == k
+    a = canonical_pairs(sc.water.pair_i[:k], sc.water.pair_j[:k])
+    b = canonical_pairs(ref.water.pair_i[:k], ref.water.pair_j[:k])
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(sc.water.countiac[:ref.water.n], ref.water.countiac[:ref.water.n])
+    s.close()
+
+
+def test_link_list_reproduces_reference_misses():
+    sc, ref = _both(_tight_box_column)
+    oracle.call_phase("link_list", ref.params, ref.water)
+    full = sc.copy()
+    oracle.call_phase("direct_find", full.params, full.water)
+    assert ref.water.niac < full.water.niac  # the reference's reach really is short here
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.link_list("water")
+    assert sc.water.niac == ref.water.niac
+    s.close()
+
+
+@pytest.mark.parametrize("case", list(small_cases()))
+def test_one_step_two_phase(case):
+    print(f"case={case}", flush=True)
+    sc, ref = _both(small_cases()[case])
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(1, sync=True)
+    oracle.time_integration(ref.params, ref.water, ref.soil, 0, 1)
+    ew = assert_fields(sc.water, ref.water, FP64_TOL_1STEP, label=f"{case} water")
+    es = assert_fields(sc.soil, ref.soil, FP64_TOL_1STEP, label=f"{case} soil")
+    print("water", max(ew.values()), "soil", max(es.values()))
+    np.testing.assert_array_equal(sc.soil.fail[:ref.soil.n], ref.soil.fail[:ref.soil.n])
+    s.close()
+
+
+@pytest.mark.parametrize("nsteps,tol", [(2, 1e-12), (10, 1e-10)])
+def test_steps_two_phase(nsteps, tol):
+    sc, ref = _both(small_cases()["column_jitter"])
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(nsteps, sync=True)
+    oracle.time_integration(ref.params, ref.water, ref.soil, 0, nsteps)
+    assert s.itimestep == nsteps
+    ew = assert_fields(sc.water, ref.water, tol, label="water")
+    es = assert_fields(sc.soil, ref.soil, tol, label="soil")
+    print(nsteps, "water", max(ew.values()), "soil", max(es.values()))
+    s.close()
+
+
+@pytest.mark.parametrize("skf", [1, 4])
+def test_single_phase_dam_break(skf):
+    sc, ref = _both(lambda: scenarios.dam_break(dx=0.01, skf=skf, jitter=0.05, seed=5))
+    s = Solver(sc.params, sc.water)
+    s.time_integration(1, sync=True)
+    oracle.time_integration(ref.params, ref.water, None, 0, 1)
+    fields = ("x", "vx", "rho", "p", "drho", "dvx", "av", "txx", "txy", "tyy")
+    e1 = assert_fields(sc.water, ref.water, FP64_TOL_1STEP, fields, label="1 step")
+    s.time_integration(4, sync=True)
+    oracle.time_integration(ref.params, ref.water, None, 1, 4)
+    e5 = assert_fields(sc.water, ref.water, 1e-11, fields, label="5 steps")
+    print(max(e1.values()), max(e5.values()))
+    s.close()
+
+
+def test_nnps2_fused_step_matches_link_list_oracle():
+    sc, ref = _both(lambda: _tight_box_column(nnps=2))
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(1, sync=True)
+    oracle.time_integration(ref.params, ref.water, ref.soil, 0, 1)
+    assert_fields(sc.water, ref.water, FP64_TOL_1STEP, label="water")
+    assert_fields(sc.soil, ref.soil, FP64_TOL_1STEP, label="soil")
+    s.close()
+
+
+def _advanced_state():
+    """A two-phase state a few steps in, so velocities and rates are non-zero."""
+    sc = small_cases()["column_jitter"]()
+    oracle.time_integration(sc.params, sc.water, sc.soil, 0, 3)
+    return sc
+
+
+@pytest.mark.parametrize("phase", ["water", "soil"])
+def test_granular_con_density_and_delta(phase):
+    sc = _advanced_state()
+    ref = sc.copy()
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.con_density(phase)
+    s.sync(phase, abi.F_DRHO)
+    r = getattr(ref, phase)
+    oracle.call_phase("find_pairs", ref.params, r)
+    oracle.call_phase("con_density", ref.params, r)
+    g = getattr(sc, phase)
+    assert field_errors(g, r, ("drho",))["drho"] <= 1e-12
+    s.delta_sph_density(phase)
+    s.sync(phase, abi.F_DRHO)
+    oracle.delta_sph(ref.params, r, r.rho, r.drho)
+    assert field_errors(g, r, ("drho",))["drho"] <= 1e-12
+    s.close()
+
+
+@pytest.mark.parametrize("phase", ["water", "soil"])
+def test_granular_internal_force_and_art_visc(phase):
+    sc = _advanced_state()
+    ref = sc.copy()
+    s = Solver(sc.params, sc.water, sc.soil)
+    g = getattr(sc, phase)
+    r = getattr(ref, phase)
+    # reference: zero rates, pairs, then the internal-force group (declarations_sph.f90:1222-1308)
+    for ph in (g, r):
+        ph.dvx[:] = 0.0
+        if ph.material == abi.SPHG_SOIL:
+            for f in ("dsxx", "dsxy", "dsyy", "dp"):
+                getattr(ph, f)[:] = 0.0
+    s.internal_force(phase)
+    s.sync(phase, abi.F_ALL)
+    oracle.call_phase("find_pairs", ref.params, r)
+    P = ref.params
+    oracle.call_phase("shear_strain_rate", P, r)
+    if phase == "soil":
+        oracle.call_phase("velocity_divergence", P, r)
+    oracle.call_phase("pressure", P, r)
+    n = r.n
+    if phase == "water":
+        oracle.call_phase("newtonian_fluid", P, r)
+        t1 = oracle.df(r, r.p, 0); t2 = oracle.df(r, r.vof * r.sxx, 0); t3 = oracle.df(r, r.vof * r.sxy, 1)
+        ax = -(r.vof * t1) + t2 + t3
+        t1 = oracle.df(r, r.p, 1); t2 = oracle.df(r, r.vof * r.sxy, 0); t3 = oracle.df(r, r.vof * r.syy, 1)
+        ay = -(r.vof * t1) + t2 + t3
+    else:
+        oracle.call_phase("drucker_prager", P, r)
+        fp = r.vof * r.p
+        ax = -oracle.df(r, fp, 0, "df3") + oracle.df(r, r.vof * r.sxx, 0, "df3") + oracle.df(r, r.vof * r.sxy, 1, "df3")
+        ay = -oracle.df(r, fp, 1, "df3") + oracle.df(r, r.vof * r.sxy, 0, "df3") + oracle.df(r, r.vof * r.syy, 1, "df3")
+    pos = r.rho > 0
+    r.dvx[:, 0] = np.where(pos, ax / np.where(pos, r.rho, 1.0), ax)
+    r.dvx[:, 1] = np.where(pos, ay / np.where(pos, r.rho, 1.0), ay)
+    fields = ["dvx", "p", "c", "txx", "txy", "tyy"]
+    if phase == "soil":
+        oracle.call_phase("plastic_or_not", P, r)
+        oracle.call_phase("plastic_flow_rule3", P, r)
+        oracle.call_phase("jaumann_rate", P, r)
+        fields += ["dsxx", "dsxy", "dsyy", "dp", "epsilon_p", "sxx", "sxy", "syy", "vcc", "wxy"]
+    assert_fields(g, r, 1e-12, fields, label=f"internal_force {phase}")
+    s.art_visc(phase)
+    s.sync(phase, abi.F_DVX)
+    oracle.call_phase("art_visc", P, r)
+    assert_fields(g, r, 1e-12, ["dvx"], label=f"art_visc {phase}")
+    assert n == g.n
+    s.close()
+
+
+def test_out_of_range_is_reported():
+    sc = scenarios.savage_column()
+    sc.water.x[0, 0] = 10.0  # outside the geometry box
+    s = Solver(sc.params, sc.water, sc.soil)
+    with pytest.raises(abi.SphgError) as e:
+        s.time_integration(1, sync=True)
+    assert e.value.code == abi.SPHG_ERR_OUT_OF_RANGE
+    s.close()
+
+
+def test_too_many_interactions_is_reported():
+    sc = scenarios.savage_column(max_interaction=100)
+    s = Solver(sc.params, sc.water, sc.soil)
+    with pytest.raises(abi.SphgError) as e:
+        s.direct_find("water")
+    assert e.value.code == abi.SPHG_ERR_TOO_MANY_INTERACTIONS
+    s.close()
