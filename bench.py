@@ -1,0 +1,201 @@
+"""Benchmark: particle-steps/s of the two-phase per-step loop at 1M particles.
+
+Workload: the partially submerged 30-degree slope (sources_amd.scenarios.slope
+at dx = 0.001, ~1.0M real + wall particles), FP64, full leapfrog steps
+(time_integration) with the state resident in HBM.  Prints ONE JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S] [--no-cpu]
+
+The timed region is K steps bracketed by device synchronisation, timed with
+HIP events recorded on the library's stream.  A second, shorter pass with
+per-launch HIP events gives each kernel's average duration for the roofline.
+The CPU baseline times the oracle restatement of the reference algorithm
+(pair list, O(N^2) direct_find, O(N_w N_s) direct_find_2, scatter passes) on
+a bounded sample of the same scenario on one host core.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1M two-phase SPH; learned-vs-plain speedup at ≤1% ρ err"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per particle of each fused pass, FP64: the
+# two-neighbour-stage model (state read once per stage, final state written
+# once, stage-A -> stage-B intermediates written and read once), split by stage.
+#   water: A = 100 state + 40 intermediates = 140, B = 100 + 40 + 88 = 228 (368 total)
+#   soil:  A = 124 + 48 = 172,                    B = 124 + 48 + 112 = 284 (456 total)
+#   cell list: 28 per particle and step
+ALGO_BYTES = {
+    "water_a": ("water", 140), "water_b": ("water", 228),
+    "soil_a": ("soil", 172), "soil_b": ("soil", 284),
+}
+STEP_BYTES = {"water": 368 + 28, "soil": 456 + 28}
+
+
+class Hip:
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+    def sync(self):
+        assert self.h.hipDeviceSynchronize() == 0
+
+
+def cpu_baseline(scale, steps):
+    """Oracle restatement of the reference loop on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from sources_amd import scenarios
+    sc = scenarios.slope(dx=0.005, scale=scale)
+    n = sc.n_particles
+    it = oracle.time_integration(sc.params, sc.water, sc.soil, 0, 1)  # warm-up step (step 1 path)
+    t0 = time.perf_counter()
+    oracle.time_integration(sc.params, sc.water, sc.soil, it, steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+        "sample": f"oracle (serial FP64 restatement, nnps=1 direct_find + direct_find_2) on the same slope "
+                  f"scenario at dx=0.005 ({n} particles), {steps} steps after 1 warm-up step, "
+                  f"{dt:.1f} s wall",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--dx", type=float, default=0.001)
+    ap.add_argument("--prof-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from sources_amd import scenarios
+    from sources_amd.sph import Solver
+
+    t_gen = time.perf_counter()
+    sc = scenarios.slope(dx=args.dx, scale=args.scale, max_interaction=0)
+    nw, ns = sc.water.n, sc.soil.n
+    ntot = nw + ns
+    print(f"# generated slope: water {sc.water.ntotal}+{sc.water.nvirt}, soil {sc.soil.ntotal}+{sc.soil.nvirt}, "
+          f"total {ntot} in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    s = Solver(sc.params, sc.water, sc.soil)
+    hip = Hip()
+    stream = s.stream
+
+    s.time_integration(args.warmup, sync=False)
+    hip.sync()
+    e0, e1 = hip.event(), hip.event()
+    w0 = time.perf_counter()
+    hip.record(e0, stream)
+    s.time_integration(args.steps, sync=False)
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1)
+    hip.sync()
+    wall = time.perf_counter() - w0
+    s.sync_all()  # checks error flags and pulls the state
+    print(f"# timed {args.steps} steps: {ms:.3f} ms (events), {wall * 1e3:.3f} ms (wall)", file=sys.stderr, flush=True)
+    import numpy as np
+    finite = all(bool(np.isfinite(getattr(p, f)[:p.n]).all()) for p in (sc.water, sc.soil)
+                 for f in ("x", "vx", "rho", "p"))
+    ms_step = ms / args.steps
+    value = ntot * args.steps / (ms * 1e-3)
+
+    # per-kernel durations (separate pass: per-launch events add overhead)
+    s.profile(True)
+    s.profile_reset()
+    s.time_integration(args.prof_steps, sync=False)
+    hip.sync()
+    kern = {}
+    for name in ("cell_key", "cell_scatter", "cell_sort", "scan", "fill", "first_half",
+                 "water_a", "soil_a", "water_b", "soil_b", "final_water", "final_soil"):
+        n, t = s.profile_get(name)
+        if n:
+            kern[name] = {"launches": n, "avg_us": 1e3 * t / n, "total_ms_per_step": t / args.prof_steps}
+    s.profile(False)
+    pairs_w, pairs_s, pairs_x = s.pair_counts()
+    dom = max((k for k in kern if k in ALGO_BYTES), key=lambda k: kern[k]["total_ms_per_step"])
+    ph, bpp = ALGO_BYTES[dom]
+    npart = nw if ph == "water" else ns
+    algo = bpp * npart
+    avg_s = kern[dom]["avg_us"] * 1e-6
+    achieved = algo / avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_slope_1m_fp64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+    step_bytes = STEP_BYTES["water"] * nw + STEP_BYTES["soil"] * ns
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded slope scenario)",
+        "config": {
+            "workload": "two-phase 30-degree slope, partially submerged, dx=0.001, FP64 leapfrog time_integration",
+            "particles": ntot, "water": nw, "soil": ns,
+            "water_real": sc.water.ntotal, "soil_real": sc.soil.ntotal,
+            "cross_pairs": pairs_x, "skf": sc.params.skf, "precision": "fp64",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "algo_bytes_per_launch": algo, "avg_launch_us": kern[dom]["avg_us"],
+        },
+        "step_roofline": {
+            "algo_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
+            "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "kernels": kern,
+        "finite_state": finite,
+        "wall_ms_timed": wall * 1e3,
+    }
+    s.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(1.0, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -152,8 +152,10 @@ void sphg_destroy(sphg_ctx *ctx);
 /* Replace the parameters (e.g. dt) of a live context. */
 int sphg_set_params(sphg_ctx *ctx, const sphg_params *params);
 /* Bind a phase view and upload its state (x, vx, mass, hsml, itype, rho, p,
-   c, vof, stress, epsilon_p and the rate arrays) to the device. */
-int sphg_bind_phase(sphg_ctx *ctx, int phase, const sphg_phase_view *view);
+   c, vof, stress, epsilon_p and the rate arrays) to the device.  The view
+   struct itself stays in use while bound: the library writes the scalar
+   outputs niac and nfail into it. */
+int sphg_bind_phase(sphg_ctx *ctx, int phase, sphg_phase_view *view);
 /* Copy fields between device and the bound view: to_host=1 downloads. */
 int sphg_sync(sphg_ctx *ctx, int phase, uint64_t field_mask, int to_host);
 /* Time step counter (declarations_sph.f90:51 itimestep). */

@@ -32,6 +32,7 @@ struct PendingEvent {
 struct PhaseBuf {
     bool bound = false;
     sphg_phase_view view{};
+    sphg_phase_view* user = nullptr;   // the caller's bound view: receives niac / nfail
     int n = 0;
     DPhase d{};
     std::vector<void*> allocs;
@@ -477,6 +478,7 @@ int download(sphg_ctx* c, int k, uint64_t m)
         int nf = 0;
         for (int i = 0; i < b.n; ++i) nf += v.fail[i] == 1;
         v.nfail = nf;
+        if (b.user) b.user->nfail = nf;
     }
     return SPHG_OK;
 }
@@ -730,7 +732,7 @@ int sphg_set_params(sphg_ctx* c, const sphg_params* params)
     return SPHG_OK;
 }
 
-int sphg_bind_phase(sphg_ctx* c, int phase, const sphg_phase_view* v)
+int sphg_bind_phase(sphg_ctx* c, int phase, sphg_phase_view* v)
 {
     if (!c || !v) return SPHG_ERR_INVALID;
     const int k = phase_index(phase);
@@ -739,6 +741,7 @@ int sphg_bind_phase(sphg_ctx* c, int phase, const sphg_phase_view* v)
     if (!v->x || !v->vx || !v->mass || !v->hsml || !v->itype || !v->rho) return SPHG_ERR_INVALID;
     RC(alloc_phase(c, k, v));
     PhaseBuf& b = c->ph[k];
+    b.user = v;
     RC(up_int(c, b, v->itype, b.d.itype));
     RC(up_scalar(c, b, v->mass, b.d.mass));
     RC(up_scalar(c, b, v->hsml, b.d.hsml));
@@ -786,6 +789,7 @@ int sphg_direct_find(sphg_ctx* c, int phase)
     int niac = 0;
     const int rc = materialise(c, k, k, 1, 0, &c->ph[k].view, &niac);
     c->ph[k].view.niac = niac;
+    c->ph[k].user->niac = niac;
     if (rc == SPHG_OK) RC(download(c, k, SPHG_F_COUNTIAC));
     return rc;
 }
@@ -799,6 +803,7 @@ int sphg_link_list(sphg_ctx* c, int phase)
     int niac = 0;
     const int rc = materialise(c, k, k, 1, 1, &c->ph[k].view, &niac);
     c->ph[k].view.niac = niac;
+    c->ph[k].user->niac = niac;
     if (rc == SPHG_OK) RC(download(c, k, SPHG_F_COUNTIAC));
     return rc;
 }
@@ -811,6 +816,7 @@ int sphg_direct_find_2(sphg_ctx* c)
     int niac = 0;
     const int rc = materialise(c, 0, 1, 0, 0, &c->ph[0].view, &niac);
     c->ph[0].view.niac = niac;
+    c->ph[0].user->niac = niac;
     if (rc != SPHG_OK) return rc;
     // countiac of the soil phase: number of water partners
     PhaseBuf& S = c->ph[1];

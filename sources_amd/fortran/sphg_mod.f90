@@ -81,7 +81,7 @@ interface
       import :: c_int, c_ptr, sphg_phase_view_t
       type(c_ptr), value :: c
       integer(c_int), value :: phase
-      type(sphg_phase_view_t), intent(in) :: v
+      type(sphg_phase_view_t), intent(inout) :: v
    end function
    integer(c_int) function sphg_sync(c, phase, mask, to_host) bind(C, name='sphg_sync')
       import :: c_int, c_ptr, c_int64_t
@@ -97,11 +97,6 @@ interface
    integer(c_int) function sphg_get_itimestep(c) bind(C, name='sphg_get_itimestep')
       import :: c_int, c_ptr
       type(c_ptr), value :: c
-   end function
-   integer(c_int) function sphg_phase_call(c, phase) bind(C)
-      import :: c_int, c_ptr
-      type(c_ptr), value :: c
-      integer(c_int), value :: phase
    end function
    integer(c_int) function sphg_direct_find(c, phase) bind(C, name='sphg_direct_find')
       import :: c_int, c_ptr
@@ -242,23 +237,22 @@ end subroutine
 ! direct_find(parts), m_particles.f90:938-1001
 subroutine direct_find(pl)
    type(particles), intent(inout), target :: pl
-   call make_view(pl)
-   call check(sphg_bind_phase(ctx, int(pl%material, c_int), pl%view), 'sphg_bind_phase')
    call check(sphg_direct_find(ctx, int(pl%material, c_int)), 'direct_find')
+   pl%niac = pl%view%niac
 end subroutine
 
 ! link_list(parts), m_particles.f90:1146-1265
 subroutine link_list(pl)
    type(particles), intent(inout), target :: pl
-   call make_view(pl)
-   call check(sphg_bind_phase(ctx, int(pl%material, c_int), pl%view), 'sphg_bind_phase')
    call check(sphg_link_list(ctx, int(pl%material, c_int)), 'link_list')
+   pl%niac = pl%view%niac
 end subroutine
 
 ! direct_find_2(parts, soil), m_particles.f90:1004-1062
 subroutine direct_find_2(water, soil)
    type(particles), intent(inout), target :: water, soil
    call check(sphg_direct_find_2(ctx), 'direct_find_2')
+   water%niac = water%view%niac
 end subroutine
 
 ! con_density(parts), m_particles.f90:2402-2431
@@ -271,6 +265,7 @@ end subroutine
 subroutine internal_force(pl)
    type(particles), intent(inout), target :: pl
    call check(sphg_internal_force(ctx, int(pl%material, c_int)), 'internal_force')
+   pl%nfail = pl%view%nfail
 end subroutine
 
 ! art_visc(parts), m_particles.f90:2504-2564

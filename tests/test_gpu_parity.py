@@ -129,18 +129,35 @@ def test_steps_two_phase(nsteps, tol):
     s.close()
 
 
+def _sensitivity(make, nsteps, fields, seed=123):
+    """Norm-wise change of the oracle's result when the input density is
+    perturbed by one ulp (relative 2**-52, random sign): the conditioning of
+    each field.  Summation-density and delta-SPH terms difference nearly equal
+    densities, so a field can legitimately differ between two correct
+    summation orders by far more than 1e-12 of its own magnitude."""
+    a, b = make(), make()
+    u = np.random.default_rng(seed).choice([-1.0, 1.0], size=b.water.rho.shape)
+    b.water.rho[:] = b.water.rho * (1.0 + u * 2.0 ** -52)
+    oracle.time_integration(a.params, a.water, None, 0, nsteps)
+    oracle.time_integration(b.params, b.water, None, 0, nsteps)
+    return field_errors(b.water, a.water, fields)
+
+
 @pytest.mark.parametrize("skf", [1, 4])
 def test_single_phase_dam_break(skf):
-    sc, ref = _both(lambda: scenarios.dam_break(dx=0.01, skf=skf, jitter=0.05, seed=5))
-    s = Solver(sc.params, sc.water)
-    s.time_integration(1, sync=True)
-    oracle.time_integration(ref.params, ref.water, None, 0, 1)
+    make = lambda: scenarios.dam_break(dx=0.01, skf=skf, jitter=0.05, seed=5)  # noqa: E731
+    sc, ref = _both(make)
     fields = ("x", "vx", "rho", "p", "drho", "dvx", "av", "txx", "txy", "tyy")
-    e1 = assert_fields(sc.water, ref.water, FP64_TOL_1STEP, fields, label="1 step")
-    s.time_integration(4, sync=True)
-    oracle.time_integration(ref.params, ref.water, None, 1, 4)
-    e5 = assert_fields(sc.water, ref.water, 1e-11, fields, label="5 steps")
-    print(max(e1.values()), max(e5.values()))
+    s = Solver(sc.params, sc.water)
+    for nsteps, done in ((1, 0), (5, 1)):
+        s.time_integration(nsteps - done, sync=True)
+        oracle.time_integration(ref.params, ref.water, None, done, nsteps - done)
+        sens = _sensitivity(make, nsteps, fields)
+        errs = field_errors(sc.water, ref.water, fields)
+        # within 1e-12 norm-wise, or within 16x the oracle's own 1-ulp sensitivity
+        bad = {f: (errs[f], sens[f]) for f in fields if not (errs[f] <= max(1e-12, 16.0 * sens[f]))}
+        print(nsteps, "errs", errs, "sens", sens)
+        assert not bad, f"{nsteps} steps: {bad}"
     s.close()
 
 
