@@ -1,0 +1,868 @@
+// sph_fused.h -- the fused two-phase step (FP64), tuned for CDNA4.
+//
+// Per phase and step:
+//   pack      thread t of the cell order: half-step update of particle order[t]
+//             and its 128-byte neighbour record rec[t] (cell-sorted, so the
+//             records of a cell row are contiguous and a neighbour costs one
+//             cache line instead of one line per SoA field);
+//   stage A   candidate scan of the 3 cell rows with a sqrt-free exact
+//             predicate (dr < T, T precomputed so that dr < T <=> sqrt(dr) < R),
+//             accepted partners kept as one 64-bit mask per row; then a dense
+//             loop over the mask bits evaluates the pair terms.  The masks are
+//             stored for stage B, so each pair is tested once per step;
+//   stage B   dense loop over the stored masks.
+// Lanes of a wavefront hold consecutive cell-sorted particles; the dense loops
+// diverge only by the lanes' neighbour-count spread.
+#pragma once
+#include "sph_kernels.h"
+
+// record slots (doubles)
+enum : int {
+    R_X = 0, R_Y, R_VX, R_VY, R_RHO, R_M, R_H, R_VOF, R_P, R_C, R_S1, R_S2, R_S3, R_IT, R_IRHO, R_V,
+    R_STRIDE = 16
+};
+
+#ifndef SPHG_STAGE_WAVES
+#define SPHG_STAGE_WAVES 0   // 0: compiler's choice; n: ask for n waves per SIMD
+#endif
+#if SPHG_STAGE_WAVES > 0
+#define STAGE_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPHG_STAGE_WAVES, 8)))
+#else
+#define STAGE_ATTR __launch_bounds__(256)
+#endif
+
+struct FusedK {
+    // dr thresholds and uniform-h flags for same-phase (water, soil) and cross pairs
+    double thr_w, thr_s, thr_x;
+    int uni_w, uni_s, uni_x;
+};
+
+// 1/d and 1/sqrt(d) from the hardware estimates plus two Newton steps
+// (each step doubles the correct bits), i.e. to about one ulp; far shorter
+// dependency chains than IEEE-exact FP64 division and square root.
+__device__ __forceinline__ double fast_rcp(double d)
+{
+    double x = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, x, 1.0);
+    x = fma(x, e, x);
+    e = fma(-d, x, 1.0);
+    return fma(x, e, x);
+}
+
+__device__ __forceinline__ double fast_rsq(double d)
+{
+    double y = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    y = y * fma(-h, y * y, 1.5);
+    y = y * fma(-h, y * y, 1.5);
+    return y;
+}
+
+// r, 1/r and 1/r^2 of a squared distance dr > 0.
+struct Dist {
+    double r, ir, idr;
+};
+
+__device__ __forceinline__ Dist dist_of(double dr)
+{
+    const double ir = fast_rsq(dr);
+    return Dist{dr * ir, ir, ir * ir};
+}
+
+// Kernel constants for one smoothing length (uniform-h fast path).
+struct KC {
+    double inv_h, f, fh1, fh2;
+};
+
+__device__ __forceinline__ KC make_kc(int skf, double h)
+{
+    KC k;
+    k.inv_h = 1.0 / h;
+    double f;
+    if (skf == 1) f = 15.0 / (7.0 * SPH_PI_F * h * h);
+    else if (skf == 2) f = 1.0 / ((h * h) * SPH_PI_F);
+    else if (skf == 3) f = 7.0 / (478.0 * SPH_PI_F * h * h);
+    else f = 7.0 / (4.0 * SPH_PI_F * h * h);
+    k.f = f;
+    k.fh1 = f * k.inv_h;
+    k.fh2 = f * k.inv_h * k.inv_h;
+    return k;
+}
+
+// W and grad W (m_particles.f90:1469-1602) with precomputed constants;
+// inv_r = 1/r is only used where the reference divides by r.
+template <int SKF, bool NEED_W>
+__device__ __forceinline__ void kern_fast(const KC& k, double r, double inv_r, double dx, double dy,
+                                          double& w, double& gx, double& gy)
+{
+    const double q = r * k.inv_h;
+    w = 0.0; gx = 0.0; gy = 0.0;
+    if constexpr (SKF == 1) {
+        if (q <= 1.0) {
+            if (NEED_W) w = k.f * (SPH_C23_F - q * q + ((q * q) * q) * 0.5);
+            const double s = k.fh2 * (-2.0 + 1.5 * q);
+            gx = s * dx; gy = s * dy;
+        } else if (q <= 2.0) {
+            const double t = 2.0 - q;
+            if (NEED_W) w = k.f * (1.0 / 6.0) * ((t * t) * t);
+            const double s = -(0.5 * k.fh1) * (t * t) * inv_r;
+            gx = s * dx; gy = s * dy;
+        }
+    } else if constexpr (SKF == 4) {
+        if (q <= 2.0) {
+            const double t = 1.0 - 0.5 * q;
+            const double t2 = t * t;
+            if (NEED_W) w = k.f * ((t2 * t2) * (1.0 + 2.0 * q));
+            const double s = k.fh2 * (-5.0 + 7.5 * q - 3.75 * (q * q) + 0.625 * ((q * q) * q));
+            gx = s * dx; gy = s * dy;
+        }
+    } else if constexpr (SKF == 2) {
+        if (q <= 3.0) {
+            const double e = k.f * exp(-q * q);
+            if (NEED_W) w = e;
+            const double s = -2.0 * e * (k.inv_h * k.inv_h);
+            gx = s * dx; gy = s * dy;
+        }
+    } else {
+        const double a = 3.0 - q, b = 2.0 - q, c = 1.0 - q;
+        const double a2 = a * a, b2 = b * b, c2 = c * c;
+        const double a4 = a2 * a2, b4 = b2 * b2, c4 = c2 * c2;
+        if (q <= 1.0) {
+            if (NEED_W) w = k.f * (a4 * a - 6.0 * (b4 * b) + 15.0 * (c4 * c));
+            const double s = k.fh2 * (-120.0 + 120.0 * q - 50.0 * (q * q));  // reference polynomial
+            gx = s * dx; gy = s * dy;
+        } else if (q <= 2.0) {
+            if (NEED_W) w = k.f * (a4 * a - 6.0 * (b4 * b));
+            const double s = k.fh1 * (-(5.0 * a4) + 30.0 * b4) * inv_r;
+            gx = s * dx; gy = s * dy;
+        } else if (q <= 3.0) {
+            if (NEED_W) w = k.f * (a4 * a);
+            const double s = k.fh1 * (-(5.0 * a4)) * inv_r;
+            gx = s * dx; gy = s * dy;
+        }
+    }
+}
+
+__device__ __forceinline__ const double2* recp(const double* rec, int k)
+{
+    return reinterpret_cast<const double2*>(rec + (size_t)R_STRIDE * k);
+}
+
+// Exact pair predicate: sqrt(dx^2+dy^2) < scale_k * mh  (direct_find) or
+// < scale_k * h(1) within the reference grid reach (link_list).
+__device__ __forceinline__ bool pred(bool uni, double thr, double dr, double ha, double hb, const KParams& K)
+{
+    if (uni) return dr < thr;
+    return __builtin_sqrt(dr) < K.scale_k * sph_mhsml(ha, hb);
+}
+
+// Cell-row ranges of the 3 x 3 block around (x, y) in the cell order.
+struct Rows {
+    int k0[3], k1[3];
+};
+
+__device__ __forceinline__ Rows block_rows(const Grid& g, const int* cell_start, double x, double y)
+{
+    Rows R;
+    int cx, cy;
+    grid_cell(g, x, y, cx, cy);
+    cx = min(max(cx, 0), g.ncx - 1);
+    cy = min(max(cy, 0), g.ncy - 1);
+    const int c0 = max(cx - 1, 0), c1 = min(cx + 1, g.ncx - 1);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int row = cy - 1 + r;
+        if (row >= 0 && row < g.ncy) {
+            R.k0[r] = cell_start[row * g.ncx + c0];
+            R.k1[r] = cell_start[row * g.ncx + c1 + 1];
+        } else {
+            R.k0[r] = 0;
+            R.k1[r] = 0;
+        }
+    }
+    return R;
+}
+
+// ------------------------------------------------------------------------
+// pack: half-step update (first_half, declarations_sph.f90:643-684, and the
+// soil Drucker-Prager projections) fused with writing the neighbour record.
+// ------------------------------------------------------------------------
+__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const int i = P.order[t];
+    double rho = P.rho[i], vof = P.vof[i], vx = P.vx[i], vy = P.vy[i];
+    const int it = P.itype[i];
+    if (do_half) {
+        const double h = K.dt / 2.0;
+        P.rho_min[i] = rho;
+        rho = rho + h * P.drho[i];
+        P.rho[i] = rho;
+        if (K.volume_fraction && K.two_phase) {
+            P.vof_min[i] = vof;
+            vof = vof + h * P.dvof[i];
+            P.vof[i] = vof;
+        }
+        if (it > 0) {
+            P.vx_min[i] = vx;
+            P.vy_min[i] = vy;
+            vx = vx + h * P.ax[i];
+            vy = vy + h * P.ay[i];
+            P.vx[i] = vx;
+            P.vy[i] = vy;
+        }
+    }
+    const double m = P.mass[i];
+    const double irho = 1.0 / rho;
+    double2* R = reinterpret_cast<double2*>(rec + (size_t)R_STRIDE * t);
+    R[0] = make_double2(P.x[i], P.y[i]);
+    R[1] = make_double2(vx, vy);
+    R[2] = make_double2(rho, m);
+    R[3] = make_double2(P.hsml[i], vof);
+    R[6] = make_double2(0.0, (double)it);
+    R[7] = make_double2(irho, m * irho);
+}
+
+__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const int i = P.order[t];
+    double rho = P.rho[i], vx = P.vx[i], vy = P.vy[i];
+    const int it = P.itype[i];
+    if (do_half) {
+        const double h = K.dt / 2.0;
+        P.rho_min[i] = rho;
+        rho = rho + h * P.drho[i];
+        P.rho[i] = rho;
+        if (it > 0) {
+            P.vx_min[i] = vx;
+            P.vy_min[i] = vy;
+            vx = vx + h * P.ax[i];
+            vy = vy + h * P.ay[i];
+            P.vx[i] = vx;
+            P.vy[i] = vy;
+        }
+    }
+    double p = P.p[i], sxx = P.sxx[i], sxy = P.sxy[i], syy = P.syy[i];
+    if (ndp > 0) {
+        int f = 0;
+        for (int k = 0; k < ndp; ++k) f = dp_project(K.s_alpha1, p, sxx, sxy, syy);
+        P.p[i] = p; P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy;
+        P.fail[i] = f;
+    }
+    const double m = P.mass[i];
+    const double irho = 1.0 / rho;
+    double2* R = reinterpret_cast<double2*>(rec + (size_t)R_STRIDE * t);
+    R[0] = make_double2(P.x[i], P.y[i]);
+    R[1] = make_double2(vx, vy);
+    R[2] = make_double2(rho, m);
+    R[3] = make_double2(P.hsml[i], P.vof[i]);
+    R[4] = make_double2(p, P.c[i]);
+    R[5] = make_double2(sxx, sxy);
+    R[6] = make_double2(syy, (double)it);
+    R[7] = make_double2(irho, m * irho);
+}
+
+
+
+// ------------------------------------------------------------------------
+// Partner masks.  For particle t, bit j of row mask r marks partner
+// k0[r] + j of the 3 x 3 block's row r (cell order).  A row with more than 64
+// candidates sets bit r of `ovf`; such rows are walked directly.
+// ------------------------------------------------------------------------
+struct Masks {
+    uint64_t* m;   // [3][n]
+    int* ovf;      // [n]
+};
+
+struct MaskIter {
+    uint64_t m0, m1, m2;
+    int b0, b1, b2;
+    __device__ __forceinline__ int next()
+    {
+        if (m0) { const int j = __builtin_ctzll(m0); m0 &= m0 - 1; return b0 + j; }
+        if (m1) { const int j = __builtin_ctzll(m1); m1 &= m1 - 1; return b1 + j; }
+        if (m2) { const int j = __builtin_ctzll(m2); m2 &= m2 - 1; return b2 + j; }
+        return -1;
+    }
+};
+
+__device__ __forceinline__ MaskIter mask_iter(const Masks& M, int n, int t, const Rows& R)
+{
+    MaskIter it;
+    it.m0 = M.m[t];
+    it.m1 = M.m[n + t];
+    it.m2 = M.m[2 * (size_t)n + t];
+    it.b0 = R.k0[0]; it.b1 = R.k0[1]; it.b2 = R.k0[2];
+    return it;
+}
+
+// Batched scan of one candidate row: eight independent record loads are in
+// flight before the compares.
+template <class P>
+__device__ __forceinline__ uint64_t scan_row(const double* rec, int k0, int k1, int self, P&& accept)
+{
+    uint64_t m = 0;
+#pragma unroll 1
+    for (int k = k0; k < k1; k += 8) {
+        double2 xy[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xy[u] = recp(rec, min(k + u, k1 - 1))[0];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = k + u;
+            const bool ok = kk < k1 && kk != self && accept(kk, xy[u]);
+            m |= (uint64_t)ok << (kk - k0);
+        }
+    }
+    return m;
+}
+
+// Same-phase masks of phase A and cross masks A -> B.  Low register count:
+// this is the latency-bound part of the neighbour search.
+__global__ __launch_bounds__(256) void k_masks(const double* recA, const int* startA, int nA, const double* recB,
+                                               const int* startB, Masks MS, Masks MX, Grid g, KParams K,
+                                               int uni_s, double thr_s, int link, int do_cross, int uni_x,
+                                               double thr_x)
+{
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= nA) return;
+    const double2 xy = recp(recA, t)[0];
+    const double xa = xy.x, ya = xy.y;
+    const double ha = recp(recA, t)[3].x;
+    {
+        const Rows R = block_rows(g, startA, xa, ya);
+        int ovf = 0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            uint64_t m = 0;
+            if (R.k1[r] - R.k0[r] > 64) ovf |= 1 << r;
+            else
+                m = scan_row(recA, R.k0[r], R.k1[r], t, [&](int k, double2 b) {
+                    const double dx = xa - b.x, dy = ya - b.y;
+                    double dr;
+                    {
+#pragma clang fp contract(off)
+                        dr = dx * dx + dy * dy;
+                    }
+                    const double hb = uni_s ? ha : recp(recA, k)[3].x;
+                    if (!pred(uni_s, thr_s, dr, ha, hb, K)) return false;
+                    return !link || ll_reach(g, xa, ya, b.x, b.y);
+                });
+            MS.m[(size_t)r * nA + t] = m;
+        }
+        MS.ovf[t] = ovf;
+    }
+    if (do_cross) {
+        const Rows R = block_rows(g, startB, xa, ya);
+        int ovf = 0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            uint64_t m = 0;
+            if (R.k1[r] - R.k0[r] > 64) ovf |= 1 << r;
+            else
+                m = scan_row(recB, R.k0[r], R.k1[r], -1, [&](int k, double2 b) {
+                    const double dx = xa - b.x, dy = ya - b.y;
+                    double dr;
+                    {
+#pragma clang fp contract(off)
+                        dr = dx * dx + dy * dy;
+                    }
+                    const double hb = uni_x ? ha : recp(recB, k)[3].x;
+                    return pred(uni_x, thr_x, dr, ha, hb, K);
+                });
+            MX.m[(size_t)r * nA + t] = m;
+        }
+        MX.ovf[t] = ovf;
+    }
+}
+
+// Visit every partner of t: the mask bits with the next partner's record
+// loads issued before the current partner's arithmetic, then the direct walk
+// of overflowed rows.  LOAD(k) returns the partner data, BODY(k, data).
+template <class L, class B, class P>
+__device__ __forceinline__ void visit_pf(MaskIter it, const Rows& R, int ovf, int self, L&& load, P&& accept,
+                                         B&& body)
+{
+    int k = it.next();
+#pragma unroll 1
+    while (k >= 0) {
+        const int k2 = it.next();
+        const auto d1 = load(k);
+        if (k2 >= 0) {
+            const auto d2 = load(k2);
+            body(k, d1);
+            body(k2, d2);
+            k = it.next();
+        } else {
+            body(k, d1);
+            break;
+        }
+    }
+    if (ovf) {
+#pragma unroll 1
+        for (int r = 0; r < 3; ++r) {
+            if (!(ovf & (1 << r))) continue;
+#pragma unroll 1
+            for (int kk = R.k0[r]; kk < R.k1[r]; ++kk) {
+                if (kk == self || !accept(kk)) continue;
+                body(kk, load(kk));
+            }
+        }
+    }
+}
+
+struct NbS {   // same-phase partner, stage A
+    double2 xy, v, rm, hv, it, iv;
+};
+struct NbX {   // cross partner
+    double2 xy, v, rm, hv, pc, iv;
+};
+struct NbB {   // stage B partner: full record
+    double2 xy, v, rm, hv, pc, s12, s3, iv;
+};
+
+__device__ __forceinline__ NbS load_s(const double* rec, int k)
+{
+    const double2* R = recp(rec, k);
+    return NbS{R[0], R[1], R[2], R[3], R[6], R[7]};
+}
+__device__ __forceinline__ NbX load_x(const double* rec, int k)
+{
+    const double2* R = recp(rec, k);
+    return NbX{R[0], R[1], R[2], R[3], R[4], R[7]};
+}
+__device__ __forceinline__ NbB load_b(const double* rec, int k)
+{
+    const double2* R = recp(rec, k);
+    return NbB{R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7]};
+}
+
+// Exact predicate for the direct walk of overflowed rows (same as k_masks).
+struct Accept {
+    const double* rec;
+    double xa, ya, ha, thr;
+    int uni, link;
+    const Grid* g;
+    const KParams* K;
+    __device__ __forceinline__ bool operator()(int k) const
+    {
+        const double2 b = recp(rec, k)[0];
+        const double dx = xa - b.x, dy = ya - b.y;
+        double dr;
+        {
+#pragma clang fp contract(off)
+            dr = dx * dx + dy * dy;
+        }
+        const double hb = uni ? ha : recp(rec, k)[3].x;
+        if (!pred(uni, thr, dr, ha, hb, *K)) return false;
+        return !link || ll_reach(*g, xa, ya, b.x, b.y);
+    }
+};
+
+// ------------------------------------------------------------------------
+// Stage A, water: same-phase terms (con_density, delta_sph(rho),
+// shear_strain_rate, av_vel, repulsive_force) and cross terms from soil
+// partners (darcy_law water side, volume_fraction_water2 / _water), then
+// pressure() per particle.  Writes a-space outputs and record slots
+// p, c_new, vof*sigma.
+// ------------------------------------------------------------------------
+template <int SKF>
+__global__ STAGE_ATTR void k_water_a3(DPhase W, DPhase S, const double* recW, double* recWw,
+                                                  const double* recS, Masks MW, Masks MX, Grid g, KParams K,
+                                                  FusedK F, int cross)
+{
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= W.n) return;
+    const int a = W.order[t];
+    const double2* RA = recp(recW, t);
+    const double2 xy = RA[0], va = RA[1], rm = RA[2], hv = RA[3];
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hv.x, vofa = hv.y;
+    const double irhoa = RA[7].x;
+    const int ita = (int)RA[6].y;
+    const double ca = W.c[a];
+    const bool link = K.nnps == 2;
+    const bool uni = link || F.uni_w;
+    const KC kc = make_kc(SKF, link ? K.h1_water : ha);
+    const Rows rows = block_rows(g, W.cell_start, xa, ya);
+    const Accept acc{recW, xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0, &g, &K};
+
+    double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
+    int cnt = 0;
+    visit_pf(mask_iter(MW, W.n, t, rows), rows, MW.ovf[t], t, [&](int k) { return load_s(recW, k); }, acc,
+             [&](int, const NbS& b) {
+        const double dx = xa - b.xy.x, dy = ya - b.xy.y;
+        const double dr = dx * dx + dy * dy;
+        const Dist D = dist_of(dr);
+        const double r = D.r, idr = D.idr;
+        double w, gx, gy;
+        if (uni) kern_fast<SKF, true>(kc, r, D.ir, dx, dy, w, gx, gy);
+        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, D.ir, dx, dy, w, gx, gy);
+        ++cnt;
+        const double mb = b.rm.y, Vb = b.iv.y;
+        const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
+        con += mb * (dvx * gx + dvy * gy);
+        drd += mb * (2.0 * (rhoa - b.rm.x) * idr * (dx * gx + dy * gy)) * b.iv.x;
+        txx += Vb * (-2.0 * dvx * gx + dvy * gy);
+        txy += Vb * (-(dvx * gy) - dvy * gx);
+        tyy += Vb * (-2.0 * dvy * gy + dvx * gx);
+        const double f = mb * w * (2.0 * fast_rcp(rhoa + b.rm.x));
+        avx -= dvx * f;
+        avy -= dvy * f;
+        if (ita > 0 && !(r > W.dspp) && (int)b.it.y < 0) {
+            const double q = W.dspp / r;
+            const double fr = (pow(q, K.p1) - pow(q, K.p2)) * idr;
+            repx += K.dd * dx * fr;
+            repy += K.dd * dy * fr;
+        }
+    });
+    double dg_x = 0.0, dg_y = 0.0, dvof_x = 0.0, vof2 = 0.0;
+    int cntx = 0;
+    if (cross) {
+        const Rows rs = block_rows(g, S.cell_start, xa, ya);
+        const Accept accx{recS, xa, ya, ha, F.thr_x, F.uni_x, 0, &g, &K};
+        const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
+        const KC kx = make_kc(SKF, sph_mhsml(ha, F.uni_x ? recp(recS, 0)[3].x : ha));
+        visit_pf(mask_iter(MX, W.n, t, rs), rs, MX.ovf[t], -1, [&](int k) { return load_x(recS, k); }, accx,
+                 [&](int, const NbX& b) {
+            const double dx = xa - b.xy.x, dy = ya - b.xy.y;
+            const double dr = dx * dx + dy * dy;
+            const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
+            const double r = dr * ir;
+            double w, gx, gy;
+            if (F.uni_x) kern_fast<SKF, true>(kx, r, ir, dx, dy, w, gx, gy);
+            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, ir, dx, dy, w, gx, gy);
+            ++cntx;
+            const double mb = b.rm.y;
+            const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
+            const double sc = mb * (cfa * b.hv.y) * (w * irhoa * b.iv.x);
+            dg_x += sc * dvx;
+            dg_y += sc * dvy;
+            dvof_x += mb * (dvx * gx + dvy * gy);
+            vof2 += mb * w;
+        });
+    }
+    const double rr = rhoa / (K.w_rho0 * vofa);
+    const double p = K.w_b * (pow(rr, K.w_gamma) - 1.0);
+    const double cn = K.w_c * pow(rr, 3.0);
+    const double mu = K.w_mu;
+    txx *= SPH_C23_F;
+    tyy *= SPH_C23_F;
+    W.drho[a] = con + (K.delta * ha * ca) * drd;
+    W.txx[a] = txx; W.txy[a] = txy; W.tyy[a] = tyy;
+    if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
+    W.avx[a] = avx; W.avy[a] = avy;
+    W.repx[a] = repx; W.repy[a] = repy;
+    W.p[a] = p;
+    W.c_next[a] = cn;
+    if (cross) {
+        W.drgx[a] = dg_x; W.drgy[a] = dg_y; W.dvof_x[a] = dvof_x * (1.0 / K.s_rho0);
+        W.vof2[a] = 1.0 - vof2 / K.s_rho0;
+        W.countiac[a] = cntx;
+    } else {
+        W.countiac[a] = cnt;
+    }
+    double* RW = recWw + (size_t)R_STRIDE * t;
+    reinterpret_cast<double2*>(RW)[4] = make_double2(p, cn);
+    reinterpret_cast<double2*>(RW)[5] = make_double2(vofa * (mu * txx), vofa * (mu * txy));
+    RW[R_S3] = vofa * (mu * tyy);
+}
+
+// ------------------------------------------------------------------------
+// Stage A, soil: same-phase terms (con_density, delta_sph(rho),
+// shear_strain_rate, velocity_divergence, Jaumann spin, av_vel,
+// repulsive_force), Darcy drag and pore-water pressure from water partners
+// (new water p), then plastic_or_not + plastic_flow_rule3 + Jaumann_rate.
+// ------------------------------------------------------------------------
+template <int SKF>
+__global__ STAGE_ATTR void k_soil_a3(DPhase S, DPhase W, const double* recS, double* recSw,
+                                                 const double* recW, Masks MS, Masks MX, Grid g, KParams K,
+                                                 FusedK F, int cross)
+{
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= S.n) return;
+    const int a = S.order[t];
+    const double2* RA = recp(recS, t);
+    const double2 xy = RA[0], va = RA[1], rm = RA[2], hv = RA[3];
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hv.x, vofa = hv.y;
+    const double irhoa = RA[7].x;
+    const int ita = (int)RA[6].y;
+    const double ca = RA[4].y;
+    const bool link = K.nnps == 2;
+    const bool uni = link || F.uni_s;
+    const KC kc = make_kc(SKF, link ? K.h1_soil : ha);
+    const Rows rows = block_rows(g, S.cell_start, xa, ya);
+    const Accept acc{recS, xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0, &g, &K};
+
+    double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;
+    double avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
+    int cnt = 0;
+    visit_pf(mask_iter(MS, S.n, t, rows), rows, MS.ovf[t], t, [&](int k) { return load_s(recS, k); }, acc,
+             [&](int, const NbS& b) {
+        const double dx = xa - b.xy.x, dy = ya - b.xy.y;
+        const double dr = dx * dx + dy * dy;
+        const Dist D = dist_of(dr);
+        const double r = D.r, idr = D.idr;
+        double w, gx, gy;
+        if (uni) kern_fast<SKF, true>(kc, r, D.ir, dx, dy, w, gx, gy);
+        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, D.ir, dx, dy, w, gx, gy);
+        ++cnt;
+        const double mb = b.rm.y, Vb = b.iv.y;
+        const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
+        const double dvg = dvx * gx + dvy * gy;
+        con += mb * dvg;
+        drd += mb * (2.0 * (rhoa - b.rm.x) * idr * (dx * gx + dy * gy)) * b.iv.x;
+        txx += Vb * (-2.0 * dvx * gx + dvy * gy);
+        txy += Vb * (-(dvx * gy) - dvy * gx);
+        tyy += Vb * (-2.0 * dvy * gy + dvx * gx);
+        vcc -= Vb * dvg;
+        wxy += Vb * (-(dvx * gy) + dvy * gx);
+        const double f = mb * w * (2.0 * fast_rcp(rhoa + b.rm.x));
+        avx -= dvx * f;
+        avy -= dvy * f;
+        if (ita > 0 && !(r > S.dspp) && (int)b.it.y < 0) {
+            const double q = S.dspp / r;
+            const double fr = (pow(q, K.p1) - pow(q, K.p2)) * idr;
+            repx += K.dd * dx * fr;
+            repy += K.dd * dy * fr;
+        }
+    });
+    double dg_x = 0.0, dg_y = 0.0, po_x = 0.0, po_y = 0.0;
+    int cntx = 0;
+    if (cross) {
+        const Rows rw = block_rows(g, W.cell_start, xa, ya);
+        const Accept accx{recW, xa, ya, ha, F.thr_x, F.uni_x, 0, &g, &K};
+        const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
+        const KC kx = make_kc(SKF, sph_mhsml(F.uni_x ? recp(recW, 0)[3].x : ha, ha));
+        visit_pf(mask_iter(MX, S.n, t, rw), rw, MX.ovf[t], -1, [&](int k) { return load_x(recW, k); }, accx,
+                 [&](int, const NbX& b) {
+            const double dx = b.xy.x - xa, dy = b.xy.y - ya;   // water - soil
+            const double dr = dx * dx + dy * dy;
+            const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
+            const double r = dr * ir;
+            double w, gx, gy;
+            if (F.uni_x) kern_fast<SKF, true>(kx, r, ir, dx, dy, w, gx, gy);
+            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(b.hv.x, ha)), r, ir, dx, dy, w, gx, gy);
+            ++cntx;
+            const double mw = b.rm.y, rhow = b.rm.x;
+            const double sc = mw * (b.hv.y * rhow * cfs) * (w * b.iv.x * irhoa);
+            dg_x += sc * (b.v.x - vxa);
+            dg_y += sc * (b.v.y - vya);
+            const double mprr = mw * b.pc.x * (b.iv.x * irhoa);
+            po_x += mprr * gx;
+            po_y += mprr * gy;
+        });
+        po_x *= vofa;
+        po_y *= vofa;
+    }
+    txx *= SPH_C23_F;
+    tyy *= SPH_C23_F;
+    wxy *= 0.5;
+    S.drho[a] = con + (K.delta * ha * ca) * drd;
+    S.txx[a] = txx; S.txy[a] = txy; S.tyy[a] = tyy; S.vcc[a] = vcc; S.wxy[a] = wxy;
+    if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
+    S.avx[a] = avx; S.avy[a] = avy;
+    S.repx[a] = repx; S.repy[a] = repy;
+    if (cross) {
+        S.drgx[a] = dg_x; S.drgy[a] = dg_y; S.porx[a] = po_x; S.pory[a] = po_y;
+        S.countiac[a] = cntx;
+    } else {
+        S.countiac[a] = cnt;
+    }
+    S.c_next[a] = K.s_c;
+    recSw[(size_t)R_STRIDE * t + R_C] = K.s_c;
+    {
+        const double2 ps = RA[4], s12 = RA[5], s3 = RA[6];
+        const double p = ps.x, sxx = s12.x, sxy = s12.y, syy = s3.x;
+        const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;
+        double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
+        if (K.plasticity == 3) {
+            const double I1 = 3.0 * p;
+            const double j2r = sxx * sxx + 2.0 * (sxy * sxy) + syy * syy + (sxx + syy) * (sxx + syy);
+            const double J2 = __builtin_sqrt(j2r / 2.0) + 1.0e-6;
+            const int fail = fabs(J2 - alpha * I1) < SPH_YIELD_TOL_F ? 1 : 0;
+            if (fail) {
+                const double exx = txx * 0.5 + vcc * (1.0 / 3.0);
+                const double exy = txy * 0.5;
+                const double eyy = tyy * 0.5 + vcc * (1.0 / 3.0);
+                const double sde = sxx * exx + 2.0 * sxy * exy + syy * eyy;
+                const double sj = __builtin_sqrt(j2r / 2.0 + 1.0e-10);
+                const double gs = G / sj;
+                const double dl = (3.0 * alpha * Kb * vcc + gs * sde) / (9.0 * (alpha * alpha) * Kb + G);
+                if (!(dl < 0.0)) {
+                    dsxx = -dl * (gs * sxx);
+                    dsxy = -dl * (gs * sxy);
+                    dsyy = -dl * (gs * syy);
+                    dpv = 3.0 * Kb * alpha * dl;
+                    S.eps_p[a] = S.eps_p[a] + dl * sxy / (2.0 * sj) * SPH_EPSP_F;
+                }
+            }
+            S.fail[a] = fail;
+        }
+        dsxx = dsxx + G * txx + 2.0 * sxy * wxy;
+        dsxy = dsxy + G * txy - (sxx - syy) * wxy;
+        dsyy = dsyy + G * tyy - 2.0 * sxy * wxy;
+        dpv = dpv - Kb * vcc;
+        S.dsxx[a] = dsxx; S.dsxy[a] = dsxy; S.dsyy[a] = dsyy; S.dp[a] = dpv;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Stage B, water: -vof*df(p) + df(vof*sigma) (declarations_sph.f90:1257-1265),
+// art_visc, delta_sph(vof, dvof); /rho, + repulsion, + gravity, - Darcy drag.
+// ------------------------------------------------------------------------
+template <int SKF>
+__global__ STAGE_ATTR void k_water_b3(DPhase W, const double* recW, Masks MW, Grid g, KParams K,
+                                                  FusedK F, int mask)
+{
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= W.n) return;
+    const int a = W.order[t];
+    const NbB A = load_b(recW, t);
+    const double xa = A.xy.x, ya = A.xy.y, vxa = A.v.x, vya = A.v.y, rhoa = A.rm.x, ha = A.hv.x, vofa = A.hv.y;
+    const double pa = A.pc.x, ca = A.pc.y, fxx_a = A.s12.x, fxy_a = A.s12.y, fyy_a = A.s3.x;
+    const bool link = K.nnps == 2;
+    const bool uni = link || F.uni_w;
+    const KC kc = make_kc(SKF, link ? K.h1_water : ha);
+    const Rows rows = block_rows(g, W.cell_start, xa, ya);
+    const Accept acc{recW, xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0, &g, &K};
+    const double et2 = K.etq * K.etq;
+    double Px = 0.0, Py = 0.0, Sxx = 0.0, Sxyy = 0.0, Sxyx = 0.0, Syy = 0.0;
+    double artx = 0.0, arty = 0.0, dvd = 0.0;
+    visit_pf(mask_iter(MW, W.n, t, rows), rows, MW.ovf[t], t, [&](int k) { return load_b(recW, k); }, acc,
+             [&](int, const NbB& b) {
+        const double dx = xa - b.xy.x, dy = ya - b.xy.y;
+        const double dr = dx * dx + dy * dy;
+        const Dist D = dist_of(dr);
+        const double r = D.r, idr = D.idr;
+        double w, gx, gy;
+        const double amh = sph_mhsml(ha, b.hv.x);
+        if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, w, gx, gy);
+        else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, w, gx, gy);
+        const double Vb = b.iv.y, mb = b.rm.y;
+        const double ps = pa + b.pc.x;
+        Px += Vb * (ps * gx);
+        Py += Vb * (ps * gy);
+        const double sxy = fxy_a + b.s12.y;
+        Sxx += Vb * ((fxx_a + b.s12.x) * gx);
+        Sxyy += Vb * (sxy * gy);
+        Sxyx += Vb * (sxy * gx);
+        Syy += Vb * ((fyy_a + b.s3.x) * gy);
+        const double vr = (vxa - b.v.x) * dx + (vya - b.v.y) * dy;
+        if (vr < 0.0) {
+            const double muv = amh * vr * fast_rcp(dr + amh * amh * et2);
+            const double mc = 0.5 * (ca + b.pc.y);
+            const double piv = (K.beta * muv - K.alpha * mc) * muv * (2.0 * fast_rcp(rhoa + b.rm.x));
+            artx -= mb * piv * gx;
+            arty -= mb * piv * gy;
+        }
+        dvd += mb * (2.0 * (vofa - b.hv.y) * idr * (dx * gx + dy * gy)) * b.iv.x;
+    });
+    double ax = -(vofa * Px) + Sxx + Sxyy;
+    double ay = -(vofa * Py) + Sxyx + Syy;
+    if (rhoa > 0.0) {
+        ax = ax * A.iv.x;
+        ay = ay * A.iv.x;
+    }
+    if (mask & M_ART) { ax = ax + artx; ay = ay + arty; }
+    ax = ax + W.repx[a];
+    ay = ay + W.repy[a];
+    ay = ay + K.gravity;
+    double dvof = (mask & M_DELTA_FIELD) ? (K.delta * ha * ca) * dvd : 0.0;
+    if (mask & M_CROSS) {
+        ax = ax - W.drgx[a];
+        ay = ay - W.drgy[a];
+        dvof = dvof - W.dvof_x[a];
+    }
+    W.ax[a] = ax;
+    W.ay[a] = ay;
+    W.dvof[a] = dvof;
+}
+
+// ------------------------------------------------------------------------
+// Stage B, soil: -df3(vof*p) + df3(vof*sigma) (:1270-1277), art_visc,
+// delta_sph on p, sxx, sxy, syy; /rho, + repulsion, + gravity, + Darcy drag,
+// + pore-water pressure force.
+// ------------------------------------------------------------------------
+template <int SKF>
+__global__ STAGE_ATTR void k_soil_b3(DPhase S, const double* recS, Masks MS, Grid g, KParams K,
+                                                 FusedK F, int mask)
+{
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= S.n) return;
+    const int a = S.order[t];
+    const NbB A = load_b(recS, t);
+    const double xa = A.xy.x, ya = A.xy.y, vxa = A.v.x, vya = A.v.y, rhoa = A.rm.x, ha = A.hv.x, vofa = A.hv.y;
+    const double pa = A.pc.x, ca = A.pc.y, sxxa = A.s12.x, sxya = A.s12.y, syya = A.s3.x, irhoa = A.iv.x;
+    const double ir2a = irhoa * irhoa;
+    const double fpa = vofa * pa * ir2a, fxxa = vofa * sxxa * ir2a, fxya = vofa * sxya * ir2a,
+                 fyya = vofa * syya * ir2a;
+    const bool link = K.nnps == 2;
+    const bool uni = link || F.uni_s;
+    const KC kc = make_kc(SKF, link ? K.h1_soil : ha);
+    const Rows rows = block_rows(g, S.cell_start, xa, ya);
+    const Accept acc{recS, xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0, &g, &K};
+    const double et2 = K.etq * K.etq;
+    double Ax = 0.0, Ay = 0.0, Bxx = 0.0, Cxyy = 0.0, Cxyx = 0.0, Dyy = 0.0;
+    double artx = 0.0, arty = 0.0, dP = 0.0, dXX = 0.0, dXY = 0.0, dYY = 0.0;
+    visit_pf(mask_iter(MS, S.n, t, rows), rows, MS.ovf[t], t, [&](int k) { return load_b(recS, k); }, acc,
+             [&](int, const NbB& b) {
+        const double dx = xa - b.xy.x, dy = ya - b.xy.y;
+        const double dr = dx * dx + dy * dy;
+        const Dist D = dist_of(dr);
+        const double r = D.r, idr = D.idr;
+        double w, gx, gy;
+        const double amh = sph_mhsml(ha, b.hv.x);
+        if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, w, gx, gy);
+        else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, w, gx, gy);
+        const double mb = b.rm.y, vofb = b.hv.y, irb = b.iv.x;
+        const double ir2b = irb * irb;
+        const double fp = fpa + vofb * b.pc.x * ir2b;
+        const double fxx = fxxa + vofb * b.s12.x * ir2b;
+        const double fxy = fxya + vofb * b.s12.y * ir2b;
+        const double fyy = fyya + vofb * b.s3.x * ir2b;
+        Ax += mb * (fp * gx);
+        Ay += mb * (fp * gy);
+        Bxx += mb * (fxx * gx);
+        Cxyy += mb * (fxy * gy);
+        Cxyx += mb * (fxy * gx);
+        Dyy += mb * (fyy * gy);
+        const double vr = (vxa - b.v.x) * dx + (vya - b.v.y) * dy;
+        if (vr < 0.0) {
+            const double muv = amh * vr * fast_rcp(dr + amh * amh * et2);
+            const double mc = 0.5 * (ca + b.pc.y);
+            const double piv = (K.beta * muv - K.alpha * mc) * muv * (2.0 * fast_rcp(rhoa + b.rm.x));
+            artx -= mb * piv * gx;
+            arty -= mb * piv * gy;
+        }
+        const double hh = 2.0 * idr * (dx * gx + dy * gy) * mb * irb;
+        dP += hh * (pa - b.pc.x);
+        dXX += hh * (sxxa - b.s12.x);
+        dXY += hh * (sxya - b.s12.y);
+        dYY += hh * (syya - b.s3.x);
+    });
+    double ax = -(Ax * rhoa) + Bxx * rhoa + Cxyy * rhoa;
+    double ay = -(Ay * rhoa) + Cxyx * rhoa + Dyy * rhoa;
+    if (rhoa > 0.0) { ax = ax * irhoa; ay = ay * irhoa; }
+    if (mask & M_ART) { ax = ax + artx; ay = ay + arty; }
+    ax = ax + S.repx[a];
+    ay = ay + S.repy[a];
+    ay = ay + K.gravity;
+    if (mask & M_CROSS) {
+        ax = ax + S.drgx[a]; ay = ay + S.drgy[a];
+        ax = ax + S.porx[a]; ay = ay + S.pory[a];
+    }
+    S.ax[a] = ax;
+    S.ay[a] = ay;
+    if (mask & M_DELTA_FIELD) {
+        const double cf = K.delta * ha * ca;
+        S.dp[a] = S.dp[a] + cf * dP;
+        S.dsxx[a] = S.dsxx[a] + cf * dXX;
+        S.dsxy[a] = S.dsxy[a] + cf * dXY;
+        S.dsyy[a] = S.dsyy[a] + cf * dYY;
+    }
+}

@@ -17,6 +17,7 @@
 struct DPhase {
     int n, ntotal, material;
     double dspp;
+    int *id;   // reference particle number (0-based) of each device slot
     int *itype, *fail, *countiac, *key, *slot;
     double *x, *y, *vx, *vy, *mass, *hsml;
     double *rho, *rho_min, *rho_next;
@@ -84,6 +85,29 @@ __global__ void k_fill_dbl(double* a, int n, double v)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
+}
+
+__global__ void k_iota(int* a, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// dst[t] = src[idx[t]]: permutation into cell order, and host -> device
+// transfer by reference particle number.
+template <class T>
+__global__ void k_gather(const T* src, T* dst, const int* idx, int n)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[idx[t]];
+}
+
+// dst[idx[t]] = src[t]: device -> host order.
+template <class T>
+__global__ void k_scatter(const T* src, T* dst, const int* idx, int n)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[idx[t]] = src[t];
 }
 
 // Exclusive scan of n ints in tiles of SCAN_TILE; out has n+1 entries.
@@ -1011,15 +1035,16 @@ __global__ void k_pair_count(DPhase A, DPhase B, Grid g, KParams K, int same, in
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= A.n) return;
     const CellList cl{B.cell_start, B.order};
+    const int ida = A.id[a];
     int nf = 0, nall = 0;
     for_each_candidate(g, cl, A.x[a], A.y[a], [&](int b) {
         if (same && b == a) return;
         double dx, dy, r, mh;
         if (!pair_pred(A, a, B, b, g, K, link, h1, dx, dy, r, mh)) return;
         ++nall;
-        if (!same || b > a) ++nf;
+        if (!same || B.id[b] > ida) ++nf;
     });
-    fwd[a] = nf;
+    fwd[ida] = nf;
     if (countiac) countiac[a] = nall;
 }
 
@@ -1029,21 +1054,24 @@ __global__ void k_pair_write(DPhase A, DPhase B, Grid g, KParams K, int same, in
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= A.n) return;
     const CellList cl{B.cell_start, B.order};
-    int o = off[a];
-    const int cnt = off[a + 1] - o;
-    int prev = same ? a : -1;
+    const int ida = A.id[a];
+    int o = off[ida];
+    const int cnt = off[ida + 1] - o;
+    int prev = same ? ida : -1;
     for (int s = 0; s < cnt; ++s) {
-        int best = 0x7fffffff;
+        int best = 0x7fffffff, bestb = 0;
         for_each_candidate(g, cl, A.x[a], A.y[a], [&](int b) {
-            if (b <= prev || b >= best) return;
+            const int idb = B.id[b];
+            if (idb <= prev || idb >= best) return;
             double dx, dy, r, mh;
             if (!pair_pred(A, a, B, b, g, K, link, h1, dx, dy, r, mh)) return;
-            best = b;
+            best = idb;
+            bestb = b;
         });
         double dx, dy, r, mh, ww, gx, gy;
-        pair_pred(A, a, B, best, g, K, link, h1, dx, dy, r, mh);
+        pair_pred(A, a, B, bestb, g, K, link, h1, dx, dy, r, mh);
         sph_kernel(K.skf, r, dx, dy, mh, ww, gx, gy);
-        pi[o] = a + 1;
+        pi[o] = ida + 1;
         pj[o] = best + 1;
         w[o] = ww;
         dwdx[2 * o] = gx;

@@ -5,13 +5,14 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/sphg.h"
-#include "sph_kernels.h"
+#include "sph_fused.h"
 
 namespace {
 
@@ -44,6 +45,13 @@ struct PhaseBuf {
     int pair_cap = 0;
     // staging for interleaved host vectors
     double* stage = nullptr;
+    // fused path: cell-sorted neighbour records and same-phase masks
+    double* rec = nullptr;
+    Masks masks{};               // same-phase partners
+    Masks xmasks{};              // partners in the other phase
+    int* spare_i = nullptr;      // permutation scratch
+    double* spare_d = nullptr;
+    double hmin = 0.0, hmax = 0.0;
 };
 
 }  // namespace
@@ -67,6 +75,10 @@ struct sphg_ctx {
     std::vector<PendingEvent> pending;
     std::vector<hipEvent_t> event_pool;
     int64_t pairs[3] = {0, 0, 0};
+    FusedK F{};
+    bool legacy = false;        // SPHG_LEGACY=1: unfused reference-order kernels
+    int reorder_every = 16;     // steps between permutations of the state into cell order
+    int since_reorder = 0;
 };
 
 namespace {
@@ -198,6 +210,32 @@ int build_cells(sphg_ctx* c, int k)
     return SPHG_OK;
 }
 
+// Permute the persistent state of phase k into the current cell order, so
+// that slot t holds the t-th particle of the order (order becomes the
+// identity and the cell list stays valid).
+int reorder(sphg_ctx* c, int k)
+{
+    PhaseBuf& b = c->ph[k];
+    DPhase& d = b.d;
+    const int* ord = d.order;
+    auto pd = [&](double*& arr) -> int {
+        RC(launch(c, "permute", k_gather<double>, nblk(b.n), TPB, (const double*)arr, b.spare_d, ord, b.n));
+        std::swap(arr, b.spare_d);
+        return SPHG_OK;
+    };
+    auto pi = [&](int*& arr) -> int {
+        RC(launch(c, "permute", k_gather<int>, nblk(b.n), TPB, (const int*)arr, b.spare_i, ord, b.n));
+        std::swap(arr, b.spare_i);
+        return SPHG_OK;
+    };
+    RC(pi(d.id)); RC(pi(d.itype)); RC(pi(d.fail));
+    double** f[] = {&d.x, &d.y, &d.vx, &d.vy, &d.mass, &d.hsml, &d.rho, &d.vof, &d.p, &d.c,
+                    &d.drho, &d.ax, &d.ay, &d.dvof, &d.sxx, &d.sxy, &d.syy, &d.eps_p};
+    for (double** a : f) RC(pd(*a));
+    RC(launch(c, "iota", k_iota, nblk(b.n), TPB, d.order, b.n));
+    return SPHG_OK;
+}
+
 int check_flags(sphg_ctx* c)
 {
     int err = 0;
@@ -250,6 +288,21 @@ int check_params(const sphg_params& P)
     return SPHG_OK;
 }
 
+// Smallest T with sqrt(T) >= R: for every dr >= 0, dr < T <=> sqrt(dr) < R,
+// because the correctly rounded sqrt is monotone.  The pair predicate
+// sqrt(dx*dx+dy*dy) < scale_k*h then costs one compare.
+double dr_threshold(double R)
+{
+    double T = R * R;
+    while (std::sqrt(T) < R) T = std::nextafter(T, INFINITY);
+    for (;;) {
+        const double lo = std::nextafter(T, 0.0);
+        if (std::sqrt(lo) >= R) T = lo;
+        else break;
+    }
+    return T;
+}
+
 // Cell grid over the geometry box, cell size >= the largest support radius.
 int setup_grid(sphg_ctx* c)
 {
@@ -294,6 +347,19 @@ int setup_grid(sphg_ctx* c)
         }
         c->gl[k] = gk;
     }
+    // sqrt-free pair predicate thresholds (uniform smoothing length)
+    {
+        FusedK& F = c->F;
+        const PhaseBuf& W = c->ph[0];
+        const PhaseBuf& S = c->ph[1];
+        const double sk = c->K.scale_k;
+        F.uni_w = (P.nnps == 2) || (W.bound && W.hmin == W.hmax);
+        F.uni_s = (P.nnps == 2) || (S.bound && S.hmin == S.hmax);
+        F.uni_x = W.bound && S.bound && W.hmin == W.hmax && S.hmin == S.hmax;
+        F.thr_w = W.bound ? dr_threshold(sk * W.view.hsml[0]) : 0.0;
+        F.thr_s = S.bound ? dr_threshold(sk * S.view.hsml[0]) : 0.0;
+        F.thr_x = (W.bound && S.bound) ? dr_threshold(sk * ((W.view.hsml[0] + S.view.hsml[0]) / 2.0)) : 0.0;
+    }
     // (re)allocate cell arrays
     if (ncell != c->ncell) {
         c->ncell = ncell;
@@ -318,57 +384,65 @@ int setup_grid(sphg_ctx* c)
 
 // ---- host <-> device field transfer ----------------------------------------
 
-__global__ void k_split2(const double* in, double* a, double* b, int n)
+// Host arrays are in reference particle order; device slot t holds particle
+// id[t].  Uploads gather by id, downloads scatter by id.
+__global__ void k_split2(const double* in, double* a, double* b, const int* id, int n)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) { a[i] = in[2 * i]; b[i] = in[2 * i + 1]; }
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) { const int i = id[t]; a[t] = in[2 * i]; b[t] = in[2 * i + 1]; }
 }
 
-__global__ void k_join2(double* out, const double* a, const double* b, int n)
+__global__ void k_join2(double* out, const double* a, const double* b, const int* id, int n)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) { out[2 * i] = a[i]; out[2 * i + 1] = b[i]; }
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) { const int i = id[t]; out[2 * i] = a[t]; out[2 * i + 1] = b[t]; }
 }
 
 int up_scalar(sphg_ctx* c, PhaseBuf& b, const double* h, double* d)
 {
     if (!h) return SPHG_OK;
-    HIPCHK(c, hipMemcpyAsync(d, h, sizeof(double) * (size_t)b.n, hipMemcpyHostToDevice, c->stream));
-    return SPHG_OK;
+    HIPCHK(c, hipMemcpyAsync(b.stage, h, sizeof(double) * (size_t)b.n, hipMemcpyHostToDevice, c->stream));
+    return launch(c, "gather", k_gather<double>, nblk(b.n), TPB, (const double*)b.stage, d, (const int*)b.d.id, b.n);
 }
 
 int up_int(sphg_ctx* c, PhaseBuf& b, const int32_t* h, int* d)
 {
     if (!h) return SPHG_OK;
-    HIPCHK(c, hipMemcpyAsync(d, h, sizeof(int) * (size_t)b.n, hipMemcpyHostToDevice, c->stream));
-    return SPHG_OK;
+    int* st = reinterpret_cast<int*>(b.stage);
+    HIPCHK(c, hipMemcpyAsync(st, h, sizeof(int) * (size_t)b.n, hipMemcpyHostToDevice, c->stream));
+    return launch(c, "gather", k_gather<int>, nblk(b.n), TPB, (const int*)st, d, (const int*)b.d.id, b.n);
 }
 
 int up_vec(sphg_ctx* c, PhaseBuf& b, const double* h, double* dx, double* dy)
 {
     if (!h) return SPHG_OK;
     HIPCHK(c, hipMemcpyAsync(b.stage, h, sizeof(double) * 2 * (size_t)b.n, hipMemcpyHostToDevice, c->stream));
-    return launch(c, "split", k_split2, nblk(b.n), TPB, (const double*)b.stage, dx, dy, b.n);
+    return launch(c, "split", k_split2, nblk(b.n), TPB, (const double*)b.stage, dx, dy, (const int*)b.d.id, b.n);
 }
 
 int down_scalar(sphg_ctx* c, PhaseBuf& b, double* h, const double* d)
 {
     if (!h) return SPHG_OK;
-    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(double) * (size_t)b.n, hipMemcpyDeviceToHost, c->stream));
+    RC(launch(c, "scatter", k_scatter<double>, nblk(b.n), TPB, d, b.stage, (const int*)b.d.id, b.n));
+    HIPCHK(c, hipMemcpyAsync(h, b.stage, sizeof(double) * (size_t)b.n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return SPHG_OK;
 }
 
 int down_int(sphg_ctx* c, PhaseBuf& b, int32_t* h, const int* d)
 {
     if (!h) return SPHG_OK;
-    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(int) * (size_t)b.n, hipMemcpyDeviceToHost, c->stream));
+    int* st = reinterpret_cast<int*>(b.stage);
+    RC(launch(c, "scatter", k_scatter<int>, nblk(b.n), TPB, d, st, (const int*)b.d.id, b.n));
+    HIPCHK(c, hipMemcpyAsync(h, st, sizeof(int) * (size_t)b.n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return SPHG_OK;
 }
 
 int down_vec(sphg_ctx* c, PhaseBuf& b, double* h, const double* dx, const double* dy)
 {
     if (!h) return SPHG_OK;
-    RC(launch(c, "join", k_join2, nblk(b.n), TPB, b.stage, dx, dy, b.n));
+    RC(launch(c, "join", k_join2, nblk(b.n), TPB, b.stage, dx, dy, (const int*)b.d.id, b.n));
     HIPCHK(c, hipMemcpyAsync(h, b.stage, sizeof(double) * 2 * (size_t)b.n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SPHG_OK;
@@ -496,6 +570,10 @@ int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
     d.material = v->material;
     d.dspp = v->dspp;
     const size_t n = (size_t)b.n;
+    RC(dalloc(c, b, &d.id, n));
+    RC(launch(c, "iota", k_iota, nblk(b.n), TPB, d.id, b.n));
+    RC(dalloc(c, b, &b.spare_i, n));
+    RC(dalloc(c, b, &b.spare_d, n));
     RC(dalloc(c, b, &d.itype, n));
     RC(dalloc(c, b, &d.fail, n));
     RC(dalloc(c, b, &d.countiac, n));
@@ -516,6 +594,17 @@ int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
     RC(fill_int(c, d.itype, b.n, 0));
     RC(fill_int(c, d.fail, b.n, 0));
     RC(fill_int(c, d.countiac, b.n, 0));
+    // fused-path buffers
+    RC(dalloc(c, b, &b.rec, (size_t)R_STRIDE * n));
+    RC(dalloc(c, b, &b.masks.m, 3 * n));
+    RC(dalloc(c, b, &b.masks.ovf, n));
+    RC(dalloc(c, b, &b.xmasks.m, 3 * n));
+    RC(dalloc(c, b, &b.xmasks.ovf, n));
+    b.hmin = b.hmax = v->hsml[0];
+    for (int i = 1; i < b.n; ++i) {
+        b.hmin = std::fmin(b.hmin, v->hsml[i]);
+        b.hmax = std::fmax(b.hmax, v->hsml[i]);
+    }
     // pair-count scratch
     RC(dalloc(c, b, &b.fwd, n));
     RC(dalloc(c, b, &b.off, n + 1));
@@ -562,8 +651,8 @@ int mask_b(const sphg_ctx* c, int phase)
 }
 
 // single_step (declarations_sph.f90:1131-1435) after the soil Drucker-Prager
-// projection of :1244 has been applied.
-int rhs_two_phase(sphg_ctx* c)
+// projection of :1244 has been applied (legacy unfused kernels).
+int rhs_two_phase_legacy(sphg_ctx* c)
 {
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
@@ -576,6 +665,53 @@ int rhs_two_phase(sphg_ctx* c)
     RC(launch(c, "water_b", k_water_b, nblk(W.n), TPB, W.d, c->gl[0], c->K, mask_b(c, 0)));
     RC(launch(c, "soil_b", k_soil_b, nblk(S.n), TPB, S.d, c->gl[1], c->K, mask_b(c, 1)));
     return SPHG_OK;
+}
+
+// The fused step: pack (first_half + soil Drucker-Prager projections +
+// records), partner masks, stage A, stage B.  water_half / soil_half /
+// soil_ndp select the leapfrog variant (step 1, later steps, or a bare
+// single_step).
+template <int SKF>
+int stages_fused(sphg_ctx* c)
+{
+    PhaseBuf& W = c->ph[0];
+    PhaseBuf& S = c->ph[1];
+    const int cross = c->P.single_phase ? 0 : 1;
+    const FusedK& F = c->F;
+    const int link = c->P.nnps == 2 ? 1 : 0;
+    RC(launch(c, "masks", k_masks, nblk(W.n), TPB, (const double*)W.rec, (const int*)W.d.cell_start, W.n,
+              (const double*)S.rec, (const int*)S.d.cell_start, W.masks, W.xmasks, c->gl[0], c->K,
+              (link || F.uni_w) ? 1 : 0, F.thr_w, link, cross, F.uni_x, F.thr_x));
+    RC(launch(c, "masks", k_masks, nblk(S.n), TPB, (const double*)S.rec, (const int*)S.d.cell_start, S.n,
+              (const double*)W.rec, (const int*)W.d.cell_start, S.masks, S.xmasks, c->gl[1], c->K,
+              (link || F.uni_s) ? 1 : 0, F.thr_s, link, cross, F.uni_x, F.thr_x));
+    RC(launch(c, "water_a", k_water_a3<SKF>, nblk(W.n), TPB, W.d, S.d, (const double*)W.rec, W.rec,
+              (const double*)S.rec, W.masks, W.xmasks, c->gl[0], c->K, c->F, cross));
+    RC(launch(c, "soil_a", k_soil_a3<SKF>, nblk(S.n), TPB, S.d, W.d, (const double*)S.rec, S.rec,
+              (const double*)W.rec, S.masks, S.xmasks, c->gl[1], c->K, c->F, cross));
+    swap_c(W);
+    swap_c(S);
+    RC(launch(c, "water_b", k_water_b3<SKF>, nblk(W.n), TPB, W.d, (const double*)W.rec, W.masks, c->gl[0], c->K,
+              c->F, mask_b(c, 0)));
+    RC(launch(c, "soil_b", k_soil_b3<SKF>, nblk(S.n), TPB, S.d, (const double*)S.rec, S.masks, c->gl[1], c->K,
+              c->F, mask_b(c, 1)));
+    return SPHG_OK;
+}
+
+int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
+{
+    PhaseBuf& W = c->ph[0];
+    PhaseBuf& S = c->ph[1];
+    RC(build_cells(c, 0));
+    RC(build_cells(c, 1));
+    RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half));
+    RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp));
+    switch (c->P.skf) {
+    case 1: return stages_fused<1>(c);
+    case 2: return stages_fused<2>(c);
+    case 3: return stages_fused<3>(c);
+    default: return stages_fused<4>(c);
+    }
 }
 
 // single_step_for_water (declarations_sph.f90:1439-1526).
@@ -619,13 +755,25 @@ int one_step(sphg_ctx* c)
         W.cells_valid = false;
         return SPHG_OK;
     }
-    if (step != 1) {
-        RC(launch(c, "first_half", k_first_half_water, nblk(W.n), TPB, W.d, c->K));
-        RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 1, 2));
+    if (c->legacy) {
+        if (step != 1) {
+            RC(launch(c, "first_half", k_first_half_water, nblk(W.n), TPB, W.d, c->K));
+            RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 1, 2));
+        } else {
+            RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 0, 1));
+        }
+        RC(rhs_two_phase_legacy(c));
     } else {
-        RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 0, 1));
+        if (++c->since_reorder >= c->reorder_every) {
+            RC(build_cells(c, 0));
+            RC(build_cells(c, 1));
+            RC(reorder(c, 0));
+            RC(reorder(c, 1));
+            c->since_reorder = 0;
+        }
+        if (step != 1) RC(rhs_two_phase_fused(c, 1, 1, 2));
+        else RC(rhs_two_phase_fused(c, 0, 0, 1));
     }
-    RC(rhs_two_phase(c));
     const int vf = c->P.volume_fraction ? 1 : 0;
     RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, vf));
     RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, step == 1 ? 1 : 0));
@@ -686,6 +834,13 @@ int sphg_create(const sphg_params* params, sphg_ctx** out)
     RC(check_params(*params));
     sphg_ctx* c = new sphg_ctx();
     c->P = *params;
+    {
+        const char* e = std::getenv("SPHG_LEGACY");
+        c->legacy = e && e[0] == '1';
+        const char* r = std::getenv("SPHG_REORDER_EVERY");
+        if (r && std::atoi(r) > 0) c->reorder_every = std::atoi(r);
+        c->since_reorder = c->reorder_every;   // first fused step sorts the state
+    }
     set_kparams(c);
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipMalloc(&c->d_err, sizeof(int) * 4) != hipSuccess) {
@@ -930,8 +1085,12 @@ int sphg_single_step(sphg_ctx* c, int sync)
         RC(rhs_water(c));
     } else {
         PhaseBuf& S = c->ph[1];
-        RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 0, 1));
-        RC(rhs_two_phase(c));
+        if (c->legacy) {
+            RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 0, 1));
+            RC(rhs_two_phase_legacy(c));
+        } else {
+            RC(rhs_two_phase_fused(c, 0, 0, 1));
+        }
     }
     if (sync) {
         const uint64_t m = SPHG_F_ALL & ~SPHG_F_X & ~SPHG_F_VX & ~SPHG_F_MIN;

@@ -129,6 +129,26 @@ def test_steps_two_phase(nsteps, tol):
     s.close()
 
 
+@pytest.mark.parametrize("every", ["1", "5"])
+def test_state_reorder_is_transparent(monkeypatch, every):
+    # The device keeps particles in cell order, permuting the state every N
+    # steps; host views, pair lists and results stay in reference order.
+    monkeypatch.setenv("SPHG_REORDER_EVERY", every)
+    sc, ref = _both(small_cases()["slope_small"])
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(7, sync=True)
+    oracle.time_integration(ref.params, ref.water, ref.soil, 0, 7)
+    assert_fields(sc.water, ref.water, 1e-11, label="water")
+    assert_fields(sc.soil, ref.soil, 1e-11, label="soil")
+    s.direct_find("soil")
+    oracle.call_phase("direct_find", ref.params, ref.soil)
+    k = ref.soil.niac
+    assert sc.soil.niac == k
+    np.testing.assert_array_equal(sc.soil.pair_i[:k], ref.soil.pair_i[:k])
+    np.testing.assert_array_equal(sc.soil.pair_j[:k], ref.soil.pair_j[:k])
+    s.close()
+
+
 def _sensitivity(make, nsteps, fields, seed=123):
     """Norm-wise change of the oracle's result when the input density is
     perturbed by one ulp (relative 2**-52, random sign): the conditioning of
