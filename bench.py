@@ -95,7 +95,22 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="fp64: reference arithmetic; mixed: FP32 Kahan pair sums, FP64 integration")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
+                         "the known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, see DESIGN.md)", file=sys.stderr)
@@ -106,6 +121,8 @@ def main():
 
     t_gen = time.perf_counter()
     sc = scenarios.slope(dx=args.dx, scale=args.scale, max_interaction=0)
+    from sources_amd import abi
+    sc.params.precision = abi.SPHG_MIXED if args.precision == "mixed" else abi.SPHG_FP64
     nw, ns = sc.water.n, sc.soil.n
     ntot = nw + ns
     print(f"# generated slope: water {sc.water.ntotal}+{sc.water.nvirt}, soil {sc.soil.ntotal}+{sc.soil.nvirt}, "
@@ -168,13 +185,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (seeded slope scenario)",
         "config": {
             "workload": "two-phase 30-degree slope, partially submerged, dx=0.001, FP64 leapfrog time_integration",
             "particles": ntot, "water": nw, "soil": ns,
             "water_real": sc.water.ntotal, "soil_real": sc.soil.ntotal,
-            "cross_pairs": pairs_x, "skf": sc.params.skf, "precision": "fp64",
+            "cross_pairs": pairs_x, "skf": sc.params.skf, "precision": args.precision,
         },
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -22,8 +22,22 @@ enum : int {
     R_STRIDE = 16
 };
 
+// Record chunks staged in the per-wave LDS window (bit c = chunk c) for the
+// same-phase (LA) and cross (LX) loops of stage A and for stage B (LB).
+#ifndef SPHG_LA
+#define SPHG_LA 0
+#endif
+#ifndef SPHG_LX
+#define SPHG_LX 0
+#endif
+#ifndef SPHG_LB
+#define SPHG_LB 0
+#endif
 #ifndef SPHG_STAGE_WAVES
 #define SPHG_STAGE_WAVES 0   // 0: compiler's choice; n: ask for n waves per SIMD
+#endif
+#ifndef SPHG_PIPELINE
+#define SPHG_PIPELINE 1
 #endif
 #if SPHG_STAGE_WAVES > 0
 #define STAGE_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPHG_STAGE_WAVES, 8)))
@@ -98,24 +112,26 @@ __device__ __forceinline__ void kern_fast(const KC& k, double r, double inv_r, d
     const double q = r * k.inv_h;
     w = 0.0; gx = 0.0; gy = 0.0;
     if constexpr (SKF == 1) {
-        if (q <= 1.0) {
-            if (NEED_W) w = k.f * (SPH_C23_F - q * q + ((q * q) * q) * 0.5);
-            const double s = k.fh2 * (-2.0 + 1.5 * q);
-            gx = s * dx; gy = s * dy;
-        } else if (q <= 2.0) {
-            const double t = 2.0 - q;
-            if (NEED_W) w = k.f * (1.0 / 6.0) * ((t * t) * t);
-            const double s = -(0.5 * k.fh1) * (t * t) * inv_r;
-            gx = s * dx; gy = s * dy;
+        // both pieces, then a select: no divergent branch in the pair loop
+        const bool in1 = q <= 1.0, in2 = q <= 2.0;
+        const double t = 2.0 - q;
+        const double s1 = k.fh2 * (-2.0 + 1.5 * q);
+        const double s2 = -(0.5 * k.fh1) * (t * t) * inv_r;
+        const double s = in1 ? s1 : (in2 ? s2 : 0.0);
+        if (NEED_W) {
+            const double w1 = k.f * (SPH_C23_F - q * q + ((q * q) * q) * 0.5);
+            const double w2 = k.f * (1.0 / 6.0) * ((t * t) * t);
+            w = in1 ? w1 : (in2 ? w2 : 0.0);
         }
+        gx = s * dx; gy = s * dy;
     } else if constexpr (SKF == 4) {
-        if (q <= 2.0) {
-            const double t = 1.0 - 0.5 * q;
-            const double t2 = t * t;
-            if (NEED_W) w = k.f * ((t2 * t2) * (1.0 + 2.0 * q));
-            const double s = k.fh2 * (-5.0 + 7.5 * q - 3.75 * (q * q) + 0.625 * ((q * q) * q));
-            gx = s * dx; gy = s * dy;
-        }
+        const bool in2 = q <= 2.0;
+        const double t = 1.0 - 0.5 * q;
+        const double t2 = t * t;
+        if (NEED_W) w = in2 ? k.f * ((t2 * t2) * (1.0 + 2.0 * q)) : 0.0;
+        double s = k.fh2 * (-5.0 + 7.5 * q - 3.75 * (q * q) + 0.625 * ((q * q) * q));
+        s = in2 ? s : 0.0;
+        gx = s * dx; gy = s * dy;
     } else if constexpr (SKF == 2) {
         if (q <= 3.0) {
             const double e = k.f * exp(-q * q);
@@ -183,11 +199,36 @@ __device__ __forceinline__ Rows block_rows(const Grid& g, const int* cell_start,
     return R;
 }
 
+constexpr int Q_STRIDE = 16;   // floats per record
+
+__device__ __forceinline__ int cell_tag(int cx, int cy, bool is_virtual)
+{
+    return (is_virtual ? (int)0x80000000 : 0) | (cx << 15) | cy;
+}
+__device__ __forceinline__ int tag_cx(int tag) { return (tag >> 15) & 0x7fff; }
+__device__ __forceinline__ int tag_cy(int tag) { return tag & 0x7fff; }
+
+// FP32 neighbour record of the mixed-precision form (16 floats; the layout
+// is described in sph_mixed.h): chunks q0 and q1.
+__device__ __forceinline__ void write_q01(float* Q, const Grid& g, double cs, double x, double y, double h, bool virt,
+                                          double vx, double vy, double drho, double m)
+{
+    int cx, cy;
+    grid_cell(g, x, y, cx, cy);
+    cx = min(max(cx, 0), g.ncx - 1);
+    cy = min(max(cy, 0), g.ncy - 1);
+    const double xc = g.x0 + ((double)cx + 0.5) * cs, yc = g.y0 + ((double)cy + 0.5) * cs;
+    float4* R = reinterpret_cast<float4*>(Q);
+    R[0] = make_float4((float)(x - xc), (float)(y - yc), __int_as_float(cell_tag(cx, cy, virt)), (float)h);
+    R[1] = make_float4((float)vx, (float)vy, (float)drho, (float)m);
+}
+
 // ------------------------------------------------------------------------
 // pack: half-step update (first_half, declarations_sph.f90:643-684, and the
 // soil Drucker-Prager projections) fused with writing the neighbour record.
 // ------------------------------------------------------------------------
-__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half)
+// q != nullptr selects the FP32 record of the mixed-precision form.
+__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,6 +257,12 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half)
     }
     const double m = P.mass[i];
     const double irho = 1.0 / rho;
+    if (q) {
+        float* Q = q + (size_t)Q_STRIDE * t;
+        write_q01(Q, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.w_rho0, m);
+        reinterpret_cast<float4*>(Q)[2] = make_float4((float)vof, (float)irho, (float)(m * irho), 0.0f);
+        return;
+    }
     double2* R = reinterpret_cast<double2*>(rec + (size_t)R_STRIDE * t);
     R[0] = make_double2(P.x[i], P.y[i]);
     R[1] = make_double2(vx, vy);
@@ -225,7 +272,7 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half)
     R[7] = make_double2(irho, m * irho);
 }
 
-__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp)
+__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,6 +303,13 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
     }
     const double m = P.mass[i];
     const double irho = 1.0 / rho;
+    if (q) {
+        float* Q = q + (size_t)Q_STRIDE * t;
+        write_q01(Q, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.s_rho0, m);
+        reinterpret_cast<float4*>(Q)[2] = make_float4((float)P.vof[i], (float)irho, (float)(m * irho), (float)p);
+        reinterpret_cast<float4*>(Q)[3] = make_float4((float)P.c[i], (float)sxx, (float)sxy, (float)syy);
+        return;
+    }
     double2* R = reinterpret_cast<double2*>(rec + (size_t)R_STRIDE * t);
     R[0] = make_double2(P.x[i], P.y[i]);
     R[1] = make_double2(vx, vy);
@@ -282,11 +336,20 @@ struct Masks {
 struct MaskIter {
     uint64_t m0, m1, m2;
     int b0, b1, b2;
+    int d0, d1, d2;   // window index - record index, per row
     __device__ __forceinline__ int next()
     {
         if (m0) { const int j = __builtin_ctzll(m0); m0 &= m0 - 1; return b0 + j; }
         if (m1) { const int j = __builtin_ctzll(m1); m1 &= m1 - 1; return b1 + j; }
         if (m2) { const int j = __builtin_ctzll(m2); m2 &= m2 - 1; return b2 + j; }
+        return -1;
+    }
+    // next record index k; i receives its window index
+    __device__ __forceinline__ int next(int& i)
+    {
+        if (m0) { const int k = b0 + __builtin_ctzll(m0); m0 &= m0 - 1; i = k + d0; return k; }
+        if (m1) { const int k = b1 + __builtin_ctzll(m1); m1 &= m1 - 1; i = k + d1; return k; }
+        if (m2) { const int k = b2 + __builtin_ctzll(m2); m2 &= m2 - 1; i = k + d2; return k; }
         return -1;
     }
 };
@@ -298,6 +361,7 @@ __device__ __forceinline__ MaskIter mask_iter(const Masks& M, int n, int t, cons
     it.m1 = M.m[n + t];
     it.m2 = M.m[2 * (size_t)n + t];
     it.b0 = R.k0[0]; it.b1 = R.k0[1]; it.b2 = R.k0[2];
+    it.d0 = it.d1 = it.d2 = 0;
     return it;
 }
 
@@ -389,6 +453,21 @@ __device__ __forceinline__ void visit_pf(MaskIter it, const Rows& R, int ovf, in
                                          B&& body)
 {
     int k = it.next();
+#if SPHG_PIPELINE
+    // Software pipeline: the next partner's loads are issued before the
+    // current partner's arithmetic; the scheduling barrier keeps the
+    // compiler from sinking them back below it.
+    auto cur = load(k < 0 ? 0 : k);
+#pragma unroll 1
+    while (k >= 0) {
+        const int kn = it.next();
+        const auto nxt = load(kn < 0 ? k : kn);
+        __builtin_amdgcn_sched_barrier(0);
+        body(k, cur);
+        cur = nxt;
+        k = kn;
+    }
+#else
 #pragma unroll 1
     while (k >= 0) {
         const int k2 = it.next();
@@ -403,6 +482,7 @@ __device__ __forceinline__ void visit_pf(MaskIter it, const Rows& R, int ovf, in
             break;
         }
     }
+#endif
     if (ovf) {
 #pragma unroll 1
         for (int r = 0; r < 3; ++r) {
@@ -465,43 +545,437 @@ struct Accept {
 };
 
 // ------------------------------------------------------------------------
+// Wave-cooperative LDS windows.
+//
+// A workgroup is one wavefront holding 64 consecutive cell-sorted
+// particles.  The union of their 3 x 3 cell blocks is at most six contiguous
+// ranges of the cell order (one or two particle rows, each needing three
+// neighbour rows).  The wave copies those records into LDS with coalesced
+// loads -- eight lanes per 128-byte record, so one load instruction covers
+// 1 KB of consecutive memory -- and every candidate test and partner read
+// afterwards is an LDS access.  A per-lane gather would instead touch 64
+// cache lines per load instruction, which is what bounds a naive gather.
+// Waves whose window does not fit (more than two particle rows or more than
+// WIN_N records) use the same code with global loads.
+// ------------------------------------------------------------------------
+constexpr int WIN_N = 256;
+
+struct Seg {
+    int row, klo, khi, off;
+};
+
+struct Win {
+    Seg s[6];
+    int ns, total, ok;
+};
+
+__device__ __forceinline__ void win_add(Win& w, const Grid& g, const int* start, int row, int c0, int c1)
+{
+    if (row < 0 || row >= g.ncy) return;
+    c0 = max(c0, 0);
+    c1 = min(c1, g.ncx - 1);
+    Seg& sg = w.s[w.ns++];
+    sg.row = row;
+    sg.klo = start[row * g.ncx + c0];
+    sg.khi = start[row * g.ncx + c1 + 1];
+    sg.off = w.total;
+    w.total += sg.khi - sg.klo;
+}
+
+// Window of the wave; cx, cy are the lane's (clamped) cell; `last` is the
+// highest valid lane.  All lanes compute the same window.
+__device__ __forceinline__ Win make_window(const Grid& g, const int* start, int cx, int cy, int last, bool valid,
+                                           bool use = true)
+{
+    Win w;
+    w.ns = 0;
+    w.total = 0;
+    w.ok = 0;
+    if (!use) return w;
+    const int cyf = __shfl(cy, 0), cyl = __shfl(cy, last);
+    const int cxf = __shfl(cx, 0), cxl = __shfl(cx, last);
+    if (cyl == cyf) {
+        for (int r = cyf - 1; r <= cyf + 1; ++r) win_add(w, g, start, r, cxf - 1, cxl + 1);
+    } else if (cyl == cyf + 1) {
+        const uint64_t mb = __ballot(valid && cy == cyl);
+        const int fb = __builtin_ctzll(mb);
+        const int cxa = __shfl(cx, fb - 1), cxb = __shfl(cx, fb);
+        // row cyf holds the wave's first lanes (high columns a0..a1), row cyl
+        // its last lanes (low columns b0..b1)
+        const int a0 = cxf - 1, a1 = cxa + 1, b0 = cxb - 1, b1 = cxl + 1;
+        const bool merge = b1 >= a0 - 1;   // the two column ranges touch or overlap
+        win_add(w, g, start, cyf - 1, a0, a1);
+        if (merge) {
+            win_add(w, g, start, cyf, min(a0, b0), max(a1, b1));
+            win_add(w, g, start, cyl, min(a0, b0), max(a1, b1));
+        } else {
+            win_add(w, g, start, cyf, b0, b1);
+            win_add(w, g, start, cyf, a0, a1);
+            win_add(w, g, start, cyl, b0, b1);
+            win_add(w, g, start, cyl, a0, a1);
+        }
+        win_add(w, g, start, cyl + 1, b0, b1);
+    } else {
+        return w;
+    }
+    w.ok = w.total <= WIN_N;
+    return w;
+}
+
+// LDS index base of a lane's neighbour row: the segment of that row which
+// contains the lane's candidate range [k0, k1).
+__device__ __forceinline__ int win_base(const Win& w, int row, int k0, int k1)
+{
+    int b = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        if (i < w.ns && w.s[i].row == row && w.s[i].klo <= k0 && k1 <= w.s[i].khi) b = w.s[i].off - w.s[i].klo;
+    }
+    return b;
+}
+
+// Number of LDS chunks for the chunk mask LM, and the LDS slot of chunk c.
+template <int LM>
+struct Lds {
+    static constexpr int n = __builtin_popcount(LM);
+    static constexpr int slot(int c) { return __builtin_popcount(LM & ((1 << c) - 1)); }
+};
+
+// Copy the chunks (16-byte pairs of record slots) selected by LM of every
+// window record into lds[slot][i].
+template <int LM>
+__device__ __forceinline__ void win_load(const Win& w, const double* rec, double2 (*lds)[WIN_N], int lane)
+{
+    constexpr int NC = Lds<LM>::n;
+    if constexpr (NC == 0) return;
+    int CH[NC > 0 ? NC : 1];
+    {
+        int q = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (LM & (1 << c)) CH[q++] = c;
+    }
+    constexpr int NCD = NC > 0 ? NC : 1;
+    const int n = w.total * NC;
+#pragma unroll 2
+    for (int idx = lane; idx < n; idx += 64) {
+        const int i = idx / NCD, c = idx - i * NCD;
+        int sg = 0;
+#pragma unroll
+        for (int q = 1; q < 6; ++q)
+            if (q < w.ns && i >= w.s[q].off) sg = q;
+        const int k = w.s[sg].klo + (i - w.s[sg].off);
+        int ch = CH[0];
+#pragma unroll
+        for (int q = 1; q < NC; ++q)
+            if (c == q) ch = CH[q];
+        lds[c][i] = reinterpret_cast<const double2*>(rec + (size_t)R_STRIDE * k)[ch];
+    }
+    __syncthreads();
+}
+
+// The lane's three candidate rows with LDS (window) or global (fallback) bases.
+struct LaneRows {
+    int k0[3], k1[3], base[3];
+};
+
+__device__ __forceinline__ LaneRows lane_rows(const Rows& R, const Win& w, int cy)
+{
+    LaneRows L;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        L.k0[r] = R.k0[r];
+        L.k1[r] = R.k1[r];
+        L.base[r] = w.ok ? win_base(w, cy - 1 + r, R.k0[r], R.k1[r]) : 0;
+    }
+    return L;
+}
+
+__device__ __forceinline__ MaskIter iter_of(uint64_t m0, uint64_t m1, uint64_t m2, const LaneRows& L)
+{
+    MaskIter it;
+    it.m0 = m0; it.m1 = m1; it.m2 = m2;
+    it.b0 = L.k0[0];
+    it.b1 = L.k0[1];
+    it.b2 = L.k0[2];
+    it.d0 = L.base[0];
+    it.d1 = L.base[1];
+    it.d2 = L.base[2];
+    return it;
+}
+
+// Scan of one lane row: exact predicate on the candidates' positions
+// (LDS chunk 0 of the window, or the record).  accept(i, k, xy) gets the
+// window index i and the record index k.
+template <class P>
+__device__ __forceinline__ uint64_t scan_lane_row(int k0, int k1, int base, int self, bool win, const double2* ldsxy,
+                                                  const double* rec, P&& accept)
+{
+    uint64_t m = 0;
+    if (k1 - k0 > 64) return 0;
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) {
+        const double2 b = win ? ldsxy[k + base] : recp(rec, k)[0];
+        const bool ok = k != self && accept(k + base, k, b);
+        m |= (uint64_t)ok << (k - k0);
+    }
+    return m;
+}
+
+__device__ __forceinline__ int ovf_of(const LaneRows& L)
+{
+    int o = 0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        if (L.k1[r] - L.k0[r] > 64) o |= 1 << r;
+    return o;
+}
+
+// Dense visit of the masked partners, then the direct walk of overflowed rows.
+template <class IT, class LD, class B, class P>
+__device__ __forceinline__ void visit_win(IT it, const LaneRows& L, int ovf, int self, LD&& load, P&& accept,
+                                          B&& body)
+{
+    int i = 0;
+#pragma unroll 1
+    for (int k = it.next(i); k >= 0; k = it.next(i)) body(load(i, k));
+    if (ovf) {
+#pragma unroll 1
+        for (int r = 0; r < 3; ++r) {
+            if (!(ovf & (1 << r))) continue;
+#pragma unroll 1
+            for (int kk = L.k0[r]; kk < L.k1[r]; ++kk) {
+                const int i = kk + L.base[r];
+                if (kk == self || !accept(i, kk)) continue;
+                body(load(i, kk));
+            }
+        }
+    }
+}
+
+// The three row masks of a lane concatenated into one 128-bit mask over
+// the candidate list row0 ++ row1 ++ row2 (at most 128 candidates); a lane
+// with more, or with a row over 64, walks its rows directly (ovf = 7).
+struct CatMask {
+    uint64_t lo, hi;
+    int ovf;
+};
+
+__device__ __forceinline__ CatMask cat_masks(const LaneRows& L, uint64_t m0, uint64_t m1, uint64_t m2)
+{
+    const int l0 = L.k1[0] - L.k0[0], l1 = L.k1[1] - L.k0[1], l2 = L.k1[2] - L.k0[2];
+    CatMask c;
+    c.ovf = (ovf_of(L) || l0 + l1 + l2 > 128) ? 7 : 0;
+    if (c.ovf) {
+        c.lo = c.hi = 0;
+        return c;
+    }
+    unsigned __int128 v = (unsigned __int128)m2;
+    v = (v << l1) | m1;
+    v = (v << l0) | m0;
+    c.lo = (uint64_t)v;
+    c.hi = (uint64_t)(v >> 64);
+    return c;
+}
+
+// Iterator over the set bits of a CatMask; branch-free apart from the loop.
+struct CatIter {
+    uint64_t lo, hi;
+    int s1, s2;        // start of row 1 and row 2 in the concatenated list
+    int e0, e1, e2;    // record index - list index, per row
+    int d0, d1, d2;    // window index - record index, per row
+    __device__ __forceinline__ int next(int& i)
+    {
+        const bool in_lo = lo != 0;
+        const uint64_t m = in_lo ? lo : hi;
+        if (m == 0) return -1;
+        const int c = __builtin_ctzll(m) + (in_lo ? 0 : 64);
+        const uint64_t rest = m & (m - 1);
+        lo = in_lo ? rest : lo;
+        hi = in_lo ? hi : rest;
+        const bool a = c < s1, b = c < s2;
+        const int k = c + (a ? e0 : (b ? e1 : e2));
+        i = k + (a ? d0 : (b ? d1 : d2));
+        return k;
+    }
+};
+
+__device__ __forceinline__ CatIter cat_iter(const CatMask& c, const LaneRows& L)
+{
+    CatIter it;
+    it.lo = c.lo;
+    it.hi = c.hi;
+    const int l0 = L.k1[0] - L.k0[0], l1 = L.k1[1] - L.k0[1];
+    it.s1 = l0;
+    it.s2 = l0 + l1;
+    it.e0 = L.k0[0];
+    it.e1 = L.k0[1] - l0;
+    it.e2 = L.k0[2] - l0 - l1;
+    it.d0 = L.base[0];
+    it.d1 = L.base[1];
+    it.d2 = L.base[2];
+    return it;
+}
+
+// Partner data of the stage kernels: the eight record chunks
+// (x,y | vx,vy | rho,m | h,vof | p,c | s1,s2 | s3,itype | 1/rho,m/rho).
+struct Nb {
+    double2 xy, v, rm, hv, pc, s12, s3, iv;
+    __device__ __forceinline__ void set(int c, double2 q)
+    {
+        switch (c) {
+        case 0: xy = q; break;
+        case 1: v = q; break;
+        case 2: rm = q; break;
+        case 3: hv = q; break;
+        case 4: pc = q; break;
+        case 5: s12 = q; break;
+        case 6: s3 = q; break;
+        default: iv = q; break;
+        }
+    }
+};
+
+// Loads the chunks NEED of partner record k (window index i): chunks in LM
+// come from the LDS window when it is in use, the others from the record.
+template <int LM, int NEED>
+__device__ __forceinline__ Nb load_nb(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k)
+{
+    Nb b{};
+    const double2* R = recp(rec, k);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        if (!(NEED & (1 << c))) continue;
+        if ((LM & (1 << c)) && win) b.set(c, lds[Lds<LM>::slot(c)][i]);
+        else b.set(c, R[c]);
+    }
+    return b;
+}
+
+// Exact pair predicate on a candidate at window/global index i.
+struct Pred {
+    double xa, ya, ha, thr;
+    int uni, link;
+    __device__ __forceinline__ bool operator()(double2 b, double hb, const Grid& g, const KParams& K) const
+    {
+        const double dx = xa - b.x, dy = ya - b.y;
+        double dr;
+        {
+#pragma clang fp contract(off)
+            dr = dx * dx + dy * dy;
+        }
+        if (!pred(uni != 0, thr, dr, ha, hb, K)) return false;
+        return !link || ll_reach(g, xa, ya, b.x, b.y);
+    }
+};
+
+// Wave-level set-up shared by the stage kernels.
+struct WaveCtx {
+    int lane, t, tt, last;
+    bool valid;
+};
+
+__device__ __forceinline__ WaveCtx wave_ctx(int n)
+{
+    WaveCtx c;
+    c.lane = threadIdx.x;
+    const int t0 = xcd_block() * 64;
+    c.t = t0 + c.lane;
+    c.valid = c.t < n;
+    c.last = min(63, n - 1 - t0);
+    c.tt = c.valid ? c.t : n - 1;
+    return c;
+}
+
+__device__ __forceinline__ void lane_cell(const Grid& g, double x, double y, int& cx, int& cy)
+{
+    grid_cell(g, x, y, cx, cy);
+    cx = min(max(cx, 0), g.ncx - 1);
+    cy = min(max(cy, 0), g.ncy - 1);
+}
+
+
+template <int LM>
+__device__ __forceinline__ double2 xy_of(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k)
+{
+    if ((LM & 1) && win) return lds[0][i];
+    return recp(rec, k)[0];
+}
+
+template <int LM>
+__device__ __forceinline__ double h_of(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k)
+{
+    if ((LM & 8) && win) return lds[Lds<LM>::slot(3)][i].x;
+    return recp(rec, k)[3].x;
+}
+
+constexpr int lds_rows(int a, int b) { return (a > b ? a : b) > 0 ? (a > b ? a : b) : 1; }
+
+// Chunks read per partner by each loop.
+constexpr int NEED_SAME_A = 0xCF;    // x,y v rho,m h,vof s3,itype 1/rho,V
+constexpr int NEED_WX_A = 0x8F;      // soil partners of water: x,y v rho,m h,vof 1/rho,V
+constexpr int NEED_SX_A = 0x9F;      // water partners of soil: + p,c
+constexpr int NEED_B = 0xFF;
+
+// ------------------------------------------------------------------------
 // Stage A, water: same-phase terms (con_density, delta_sph(rho),
 // shear_strain_rate, av_vel, repulsive_force) and cross terms from soil
 // partners (darcy_law water side, volume_fraction_water2 / _water), then
-// pressure() per particle.  Writes a-space outputs and record slots
-// p, c_new, vof*sigma.
+// pressure() per particle.  Writes a-space outputs, the same-phase masks
+// and the record slots p, c_new, vof*sigma.  LA / LX select the record
+// chunks staged in the LDS window for the same-phase and cross loops.
 // ------------------------------------------------------------------------
-template <int SKF>
-__global__ STAGE_ATTR void k_water_a3(DPhase W, DPhase S, const double* recW, double* recWw,
-                                                  const double* recS, Masks MW, Masks MX, Grid g, KParams K,
-                                                  FusedK F, int cross)
+template <int SKF, int LA, int LX>
+__global__ __launch_bounds__(64) void k_water_a4(DPhase W, DPhase S, const double* recW, double* recWw,
+                                                 const double* recS, Masks MW, Grid g, KParams K, FusedK F,
+                                                 int cross)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= W.n) return;
-    const int a = W.order[t];
-    const double2* RA = recp(recW, t);
+    static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
+    __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
+    const WaveCtx wc = wave_ctx(W.n);
+    const int t = wc.t;
+    const double2* RA = recp(recW, wc.tt);
     const double2 xy = RA[0], va = RA[1], rm = RA[2], hv = RA[3];
     const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hv.x, vofa = hv.y;
     const double irhoa = RA[7].x;
     const int ita = (int)RA[6].y;
+    const int a = W.order[wc.tt];
     const double ca = W.c[a];
     const bool link = K.nnps == 2;
     const bool uni = link || F.uni_w;
     const KC kc = make_kc(SKF, link ? K.h1_water : ha);
-    const Rows rows = block_rows(g, W.cell_start, xa, ya);
-    const Accept acc{recW, xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0, &g, &K};
+    int cx, cy;
+    lane_cell(g, xa, ya, cx, cy);
 
+    // ---- same-phase window, scan, masks
+    const Win w = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
+    if (w.ok) win_load<LA>(w, recW, lds, wc.lane);
+    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya), w, cy);
+    const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
+    auto acc_b = [&](int i, int k, double2 b) {
+        return P(b, uni ? ha : h_of<LA>(lds, recW, w.ok, i, k), g, K);
+    };
+    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, recW, w.ok, i, k)); };
+    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], recW, acc_b);
+    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], recW, acc_b);
+    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], recW, acc_b);
+    const CatMask cm = cat_masks(L, m0, m1, m2);
+    if (wc.valid) {
+        MW.m[t] = cm.lo;
+        MW.m[W.n + t] = cm.hi;
+        MW.ovf[t] = cm.ovf;
+    }
+    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, recW, w.ok, i, k); };
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
     int cnt = 0;
-    visit_pf(mask_iter(MW, W.n, t, rows), rows, MW.ovf[t], t, [&](int k) { return load_s(recW, k); }, acc,
-             [&](int, const NbS& b) {
+    visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
         const double dx = xa - b.xy.x, dy = ya - b.xy.y;
         const double dr = dx * dx + dy * dy;
         const Dist D = dist_of(dr);
         const double r = D.r, idr = D.idr;
-        double w, gx, gy;
-        if (uni) kern_fast<SKF, true>(kc, r, D.ir, dx, dy, w, gx, gy);
-        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, D.ir, dx, dy, w, gx, gy);
+        double ww, gx, gy;
+        if (uni) kern_fast<SKF, true>(kc, r, D.ir, dx, dy, ww, gx, gy);
+        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, D.ir, dx, dy, ww, gx, gy);
         ++cnt;
         const double mb = b.rm.y, Vb = b.iv.y;
         const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
@@ -510,42 +984,56 @@ __global__ STAGE_ATTR void k_water_a3(DPhase W, DPhase S, const double* recW, do
         txx += Vb * (-2.0 * dvx * gx + dvy * gy);
         txy += Vb * (-(dvx * gy) - dvy * gx);
         tyy += Vb * (-2.0 * dvy * gy + dvx * gx);
-        const double f = mb * w * (2.0 * fast_rcp(rhoa + b.rm.x));
+        const double f = mb * ww * (2.0 * fast_rcp(rhoa + b.rm.x));
         avx -= dvx * f;
         avy -= dvy * f;
-        if (ita > 0 && !(r > W.dspp) && (int)b.it.y < 0) {
+        if (ita > 0 && !(r > W.dspp) && (int)b.s3.y < 0) {
             const double q = W.dspp / r;
             const double fr = (pow(q, K.p1) - pow(q, K.p2)) * idr;
             repx += K.dd * dx * fr;
             repy += K.dd * dy * fr;
         }
     });
+
+    // ---- cross window (soil partners)
     double dg_x = 0.0, dg_y = 0.0, dvof_x = 0.0, vof2 = 0.0;
     int cntx = 0;
     if (cross) {
-        const Rows rs = block_rows(g, S.cell_start, xa, ya);
-        const Accept accx{recS, xa, ya, ha, F.thr_x, F.uni_x, 0, &g, &K};
+        if (LA != 0) __syncthreads();
+        const Win wx = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
+        if (wx.ok) win_load<LX>(wx, recS, lds, wc.lane);
+        const LaneRows LXr = lane_rows(block_rows(g, S.cell_start, xa, ya), wx, cy);
+        const Pred PX{xa, ya, ha, F.thr_x, F.uni_x, 0};
+        auto accx_b = [&](int i, int k, double2 b) {
+            return PX(b, F.uni_x ? ha : h_of<LX>(lds, recS, wx.ok, i, k), g, K);
+        };
+        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, recS, wx.ok, i, k)); };
+        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recS, accx_b);
+        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recS, accx_b);
+        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recS, accx_b);
+        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_WX_A>(lds, recS, wx.ok, i, k); };
         const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
         const KC kx = make_kc(SKF, sph_mhsml(ha, F.uni_x ? recp(recS, 0)[3].x : ha));
-        visit_pf(mask_iter(MX, W.n, t, rs), rs, MX.ovf[t], -1, [&](int k) { return load_x(recS, k); }, accx,
-                 [&](int, const NbX& b) {
+        const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
+        visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
             const double dx = xa - b.xy.x, dy = ya - b.xy.y;
             const double dr = dx * dx + dy * dy;
             const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
             const double r = dr * ir;
-            double w, gx, gy;
-            if (F.uni_x) kern_fast<SKF, true>(kx, r, ir, dx, dy, w, gx, gy);
-            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, ir, dx, dy, w, gx, gy);
+            double ww, gx, gy;
+            if (F.uni_x) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, ir, dx, dy, ww, gx, gy);
             ++cntx;
             const double mb = b.rm.y;
             const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
-            const double sc = mb * (cfa * b.hv.y) * (w * irhoa * b.iv.x);
+            const double sc = mb * (cfa * b.hv.y) * (ww * irhoa * b.iv.x);
             dg_x += sc * dvx;
             dg_y += sc * dvy;
             dvof_x += mb * (dvx * gx + dvy * gy);
-            vof2 += mb * w;
+            vof2 += mb * ww;
         });
     }
+    if (!wc.valid) return;
     const double rr = rhoa / (K.w_rho0 * vofa);
     const double p = K.w_b * (pow(rr, K.w_gamma) - 1.0);
     const double cn = K.w_c * pow(rr, 3.0);
@@ -578,38 +1066,57 @@ __global__ STAGE_ATTR void k_water_a3(DPhase W, DPhase S, const double* recW, do
 // repulsive_force), Darcy drag and pore-water pressure from water partners
 // (new water p), then plastic_or_not + plastic_flow_rule3 + Jaumann_rate.
 // ------------------------------------------------------------------------
-template <int SKF>
-__global__ STAGE_ATTR void k_soil_a3(DPhase S, DPhase W, const double* recS, double* recSw,
-                                                 const double* recW, Masks MS, Masks MX, Grid g, KParams K,
-                                                 FusedK F, int cross)
+template <int SKF, int LA, int LX>
+__global__ __launch_bounds__(64) void k_soil_a4(DPhase S, DPhase W, const double* recS, double* recSw,
+                                                const double* recW, Masks MS, Grid g, KParams K, FusedK F,
+                                                int cross)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= S.n) return;
-    const int a = S.order[t];
-    const double2* RA = recp(recS, t);
+    static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
+    __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
+    const WaveCtx wc = wave_ctx(S.n);
+    const int t = wc.t;
+    const double2* RA = recp(recS, wc.tt);
     const double2 xy = RA[0], va = RA[1], rm = RA[2], hv = RA[3];
     const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hv.x, vofa = hv.y;
     const double irhoa = RA[7].x;
     const int ita = (int)RA[6].y;
     const double ca = RA[4].y;
+    const int a = S.order[wc.tt];
     const bool link = K.nnps == 2;
     const bool uni = link || F.uni_s;
     const KC kc = make_kc(SKF, link ? K.h1_soil : ha);
-    const Rows rows = block_rows(g, S.cell_start, xa, ya);
-    const Accept acc{recS, xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0, &g, &K};
+    int cx, cy;
+    lane_cell(g, xa, ya, cx, cy);
 
+    const Win w = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
+    if (w.ok) win_load<LA>(w, recS, lds, wc.lane);
+    const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya), w, cy);
+    const Pred P{xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0};
+    auto acc_b = [&](int i, int k, double2 b) {
+        return P(b, uni ? ha : h_of<LA>(lds, recS, w.ok, i, k), g, K);
+    };
+    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, recS, w.ok, i, k)); };
+    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], recS, acc_b);
+    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], recS, acc_b);
+    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], recS, acc_b);
+    const CatMask cm = cat_masks(L, m0, m1, m2);
+    if (wc.valid) {
+        MS.m[t] = cm.lo;
+        MS.m[S.n + t] = cm.hi;
+        MS.ovf[t] = cm.ovf;
+    }
+    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, recS, w.ok, i, k); };
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;
     double avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
     int cnt = 0;
-    visit_pf(mask_iter(MS, S.n, t, rows), rows, MS.ovf[t], t, [&](int k) { return load_s(recS, k); }, acc,
-             [&](int, const NbS& b) {
+    visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
         const double dx = xa - b.xy.x, dy = ya - b.xy.y;
         const double dr = dx * dx + dy * dy;
         const Dist D = dist_of(dr);
         const double r = D.r, idr = D.idr;
-        double w, gx, gy;
-        if (uni) kern_fast<SKF, true>(kc, r, D.ir, dx, dy, w, gx, gy);
-        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, D.ir, dx, dy, w, gx, gy);
+        double ww, gx, gy;
+        if (uni) kern_fast<SKF, true>(kc, r, D.ir, dx, dy, ww, gx, gy);
+        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, D.ir, dx, dy, ww, gx, gy);
         ++cnt;
         const double mb = b.rm.y, Vb = b.iv.y;
         const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
@@ -621,35 +1128,47 @@ __global__ STAGE_ATTR void k_soil_a3(DPhase S, DPhase W, const double* recS, dou
         tyy += Vb * (-2.0 * dvy * gy + dvx * gx);
         vcc -= Vb * dvg;
         wxy += Vb * (-(dvx * gy) + dvy * gx);
-        const double f = mb * w * (2.0 * fast_rcp(rhoa + b.rm.x));
+        const double f = mb * ww * (2.0 * fast_rcp(rhoa + b.rm.x));
         avx -= dvx * f;
         avy -= dvy * f;
-        if (ita > 0 && !(r > S.dspp) && (int)b.it.y < 0) {
+        if (ita > 0 && !(r > S.dspp) && (int)b.s3.y < 0) {
             const double q = S.dspp / r;
             const double fr = (pow(q, K.p1) - pow(q, K.p2)) * idr;
             repx += K.dd * dx * fr;
             repy += K.dd * dy * fr;
         }
     });
+
     double dg_x = 0.0, dg_y = 0.0, po_x = 0.0, po_y = 0.0;
     int cntx = 0;
     if (cross) {
-        const Rows rw = block_rows(g, W.cell_start, xa, ya);
-        const Accept accx{recW, xa, ya, ha, F.thr_x, F.uni_x, 0, &g, &K};
+        if (LA != 0) __syncthreads();
+        const Win wx = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
+        if (wx.ok) win_load<LX>(wx, recW, lds, wc.lane);
+        const LaneRows LXr = lane_rows(block_rows(g, W.cell_start, xa, ya), wx, cy);
+        const Pred PX{xa, ya, ha, F.thr_x, F.uni_x, 0};
+        auto accx_b = [&](int i, int k, double2 b) {
+            return PX(b, F.uni_x ? ha : h_of<LX>(lds, recW, wx.ok, i, k), g, K);
+        };
+        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, recW, wx.ok, i, k)); };
+        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recW, accx_b);
+        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recW, accx_b);
+        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recW, accx_b);
+        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_SX_A>(lds, recW, wx.ok, i, k); };
         const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
         const KC kx = make_kc(SKF, sph_mhsml(F.uni_x ? recp(recW, 0)[3].x : ha, ha));
-        visit_pf(mask_iter(MX, S.n, t, rw), rw, MX.ovf[t], -1, [&](int k) { return load_x(recW, k); }, accx,
-                 [&](int, const NbX& b) {
+        const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
+        visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
             const double dx = b.xy.x - xa, dy = b.xy.y - ya;   // water - soil
             const double dr = dx * dx + dy * dy;
             const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
             const double r = dr * ir;
-            double w, gx, gy;
-            if (F.uni_x) kern_fast<SKF, true>(kx, r, ir, dx, dy, w, gx, gy);
-            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(b.hv.x, ha)), r, ir, dx, dy, w, gx, gy);
+            double ww, gx, gy;
+            if (F.uni_x) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(b.hv.x, ha)), r, ir, dx, dy, ww, gx, gy);
             ++cntx;
             const double mw = b.rm.y, rhow = b.rm.x;
-            const double sc = mw * (b.hv.y * rhow * cfs) * (w * b.iv.x * irhoa);
+            const double sc = mw * (b.hv.y * rhow * cfs) * (ww * b.iv.x * irhoa);
             dg_x += sc * (b.v.x - vxa);
             dg_y += sc * (b.v.y - vya);
             const double mprr = mw * b.pc.x * (b.iv.x * irhoa);
@@ -659,6 +1178,7 @@ __global__ STAGE_ATTR void k_soil_a3(DPhase S, DPhase W, const double* recS, dou
         po_x *= vofa;
         po_y *= vofa;
     }
+    if (!wc.valid) return;
     txx *= SPH_C23_F;
     tyy *= SPH_C23_F;
     wxy *= 0.5;
@@ -714,35 +1234,53 @@ __global__ STAGE_ATTR void k_soil_a3(DPhase S, DPhase W, const double* recS, dou
 // ------------------------------------------------------------------------
 // Stage B, water: -vof*df(p) + df(vof*sigma) (declarations_sph.f90:1257-1265),
 // art_visc, delta_sph(vof, dvof); /rho, + repulsion, + gravity, - Darcy drag.
+// The partner masks come from stage A.
 // ------------------------------------------------------------------------
-template <int SKF>
-__global__ STAGE_ATTR void k_water_b3(DPhase W, const double* recW, Masks MW, Grid g, KParams K,
-                                                  FusedK F, int mask)
+template <int SKF, int LB>
+__global__ __launch_bounds__(64) void k_water_b4(DPhase W, const double* recW, Masks MW, Grid g, KParams K,
+                                                 FusedK F, int mask)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= W.n) return;
-    const int a = W.order[t];
-    const NbB A = load_b(recW, t);
-    const double xa = A.xy.x, ya = A.xy.y, vxa = A.v.x, vya = A.v.y, rhoa = A.rm.x, ha = A.hv.x, vofa = A.hv.y;
-    const double pa = A.pc.x, ca = A.pc.y, fxx_a = A.s12.x, fxy_a = A.s12.y, fyy_a = A.s3.x;
+    static_assert(LB == 0 || (LB & 1), "window needs positions");
+    __shared__ double2 lds[lds_rows(Lds<LB>::n, 0)][WIN_N];
+    const WaveCtx wc = wave_ctx(W.n);
+    const int t = wc.t;
+    const double2* RA = recp(recW, wc.tt);
+    const double2 axy = RA[0], av = RA[1], arm = RA[2], ahv = RA[3], apc = RA[4], a12 = RA[5], a3 = RA[6],
+                  aiv = RA[7];
+    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = ahv.x, vofa = ahv.y;
+    const double pa = apc.x, ca = apc.y, fxx_a = a12.x, fxy_a = a12.y, fyy_a = a3.x;
+    const int a = W.order[wc.tt];
     const bool link = K.nnps == 2;
     const bool uni = link || F.uni_w;
     const KC kc = make_kc(SKF, link ? K.h1_water : ha);
-    const Rows rows = block_rows(g, W.cell_start, xa, ya);
-    const Accept acc{recW, xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0, &g, &K};
+    int cx, cy;
+    lane_cell(g, xa, ya, cx, cy);
+    const Win w = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LB != 0);
+    if (w.ok) win_load<LB>(w, recW, lds, wc.lane);
+    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya), w, cy);
+    const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
+    auto acc_i = [&](int i, int k) {
+        return P(xy_of<LB>(lds, recW, w.ok, i, k), uni ? ha : h_of<LB>(lds, recW, w.ok, i, k), g, K);
+    };
+    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, recW, w.ok, i, k); };
+    CatMask cm{0, 0, 0};
+    if (wc.valid) {
+        cm.lo = MW.m[t];
+        cm.hi = MW.m[W.n + t];
+        cm.ovf = MW.ovf[t];
+    }
     const double et2 = K.etq * K.etq;
     double Px = 0.0, Py = 0.0, Sxx = 0.0, Sxyy = 0.0, Sxyx = 0.0, Syy = 0.0;
     double artx = 0.0, arty = 0.0, dvd = 0.0;
-    visit_pf(mask_iter(MW, W.n, t, rows), rows, MW.ovf[t], t, [&](int k) { return load_b(recW, k); }, acc,
-             [&](int, const NbB& b) {
+    visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
         const double dx = xa - b.xy.x, dy = ya - b.xy.y;
         const double dr = dx * dx + dy * dy;
         const Dist D = dist_of(dr);
         const double r = D.r, idr = D.idr;
-        double w, gx, gy;
+        double ww, gx, gy;
         const double amh = sph_mhsml(ha, b.hv.x);
-        if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, w, gx, gy);
-        else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, w, gx, gy);
+        if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, ww, gx, gy);
+        else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, ww, gx, gy);
         const double Vb = b.iv.y, mb = b.rm.y;
         const double ps = pa + b.pc.x;
         Px += Vb * (ps * gx);
@@ -753,20 +1291,22 @@ __global__ STAGE_ATTR void k_water_b3(DPhase W, const double* recW, Masks MW, Gr
         Sxyx += Vb * (sxy * gx);
         Syy += Vb * ((fyy_a + b.s3.x) * gy);
         const double vr = (vxa - b.v.x) * dx + (vya - b.v.y) * dy;
-        if (vr < 0.0) {
+        {   // art_visc, branch-free: approaching pairs only (vr < 0)
             const double muv = amh * vr * fast_rcp(dr + amh * amh * et2);
             const double mc = 0.5 * (ca + b.pc.y);
-            const double piv = (K.beta * muv - K.alpha * mc) * muv * (2.0 * fast_rcp(rhoa + b.rm.x));
+            double piv = (K.beta * muv - K.alpha * mc) * muv * (2.0 * fast_rcp(rhoa + b.rm.x));
+            piv = vr < 0.0 ? piv : 0.0;
             artx -= mb * piv * gx;
             arty -= mb * piv * gy;
         }
         dvd += mb * (2.0 * (vofa - b.hv.y) * idr * (dx * gx + dy * gy)) * b.iv.x;
     });
+    if (!wc.valid) return;
     double ax = -(vofa * Px) + Sxx + Sxyy;
     double ay = -(vofa * Py) + Sxyx + Syy;
     if (rhoa > 0.0) {
-        ax = ax * A.iv.x;
-        ay = ay * A.iv.x;
+        ax = ax * aiv.x;
+        ay = ay * aiv.x;
     }
     if (mask & M_ART) { ax = ax + artx; ay = ay + arty; }
     ax = ax + W.repx[a];
@@ -788,37 +1328,54 @@ __global__ STAGE_ATTR void k_water_b3(DPhase W, const double* recW, Masks MW, Gr
 // delta_sph on p, sxx, sxy, syy; /rho, + repulsion, + gravity, + Darcy drag,
 // + pore-water pressure force.
 // ------------------------------------------------------------------------
-template <int SKF>
-__global__ STAGE_ATTR void k_soil_b3(DPhase S, const double* recS, Masks MS, Grid g, KParams K,
-                                                 FusedK F, int mask)
+template <int SKF, int LB>
+__global__ __launch_bounds__(64) void k_soil_b4(DPhase S, const double* recS, Masks MS, Grid g, KParams K,
+                                                FusedK F, int mask)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= S.n) return;
-    const int a = S.order[t];
-    const NbB A = load_b(recS, t);
-    const double xa = A.xy.x, ya = A.xy.y, vxa = A.v.x, vya = A.v.y, rhoa = A.rm.x, ha = A.hv.x, vofa = A.hv.y;
-    const double pa = A.pc.x, ca = A.pc.y, sxxa = A.s12.x, sxya = A.s12.y, syya = A.s3.x, irhoa = A.iv.x;
+    static_assert(LB == 0 || (LB & 1), "window needs positions");
+    __shared__ double2 lds[lds_rows(Lds<LB>::n, 0)][WIN_N];
+    const WaveCtx wc = wave_ctx(S.n);
+    const int t = wc.t;
+    const double2* RA = recp(recS, wc.tt);
+    const double2 axy = RA[0], av = RA[1], arm = RA[2], ahv = RA[3], apc = RA[4], a12 = RA[5], a3 = RA[6],
+                  aiv = RA[7];
+    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = ahv.x, vofa = ahv.y;
+    const double pa = apc.x, ca = apc.y, sxxa = a12.x, sxya = a12.y, syya = a3.x, irhoa = aiv.x;
     const double ir2a = irhoa * irhoa;
     const double fpa = vofa * pa * ir2a, fxxa = vofa * sxxa * ir2a, fxya = vofa * sxya * ir2a,
                  fyya = vofa * syya * ir2a;
+    const int a = S.order[wc.tt];
     const bool link = K.nnps == 2;
     const bool uni = link || F.uni_s;
     const KC kc = make_kc(SKF, link ? K.h1_soil : ha);
-    const Rows rows = block_rows(g, S.cell_start, xa, ya);
-    const Accept acc{recS, xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0, &g, &K};
+    int cx, cy;
+    lane_cell(g, xa, ya, cx, cy);
+    const Win w = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LB != 0);
+    if (w.ok) win_load<LB>(w, recS, lds, wc.lane);
+    const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya), w, cy);
+    const Pred P{xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0};
+    auto acc_i = [&](int i, int k) {
+        return P(xy_of<LB>(lds, recS, w.ok, i, k), uni ? ha : h_of<LB>(lds, recS, w.ok, i, k), g, K);
+    };
+    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, recS, w.ok, i, k); };
+    CatMask cm{0, 0, 0};
+    if (wc.valid) {
+        cm.lo = MS.m[t];
+        cm.hi = MS.m[S.n + t];
+        cm.ovf = MS.ovf[t];
+    }
     const double et2 = K.etq * K.etq;
     double Ax = 0.0, Ay = 0.0, Bxx = 0.0, Cxyy = 0.0, Cxyx = 0.0, Dyy = 0.0;
     double artx = 0.0, arty = 0.0, dP = 0.0, dXX = 0.0, dXY = 0.0, dYY = 0.0;
-    visit_pf(mask_iter(MS, S.n, t, rows), rows, MS.ovf[t], t, [&](int k) { return load_b(recS, k); }, acc,
-             [&](int, const NbB& b) {
+    visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
         const double dx = xa - b.xy.x, dy = ya - b.xy.y;
         const double dr = dx * dx + dy * dy;
         const Dist D = dist_of(dr);
         const double r = D.r, idr = D.idr;
-        double w, gx, gy;
+        double ww, gx, gy;
         const double amh = sph_mhsml(ha, b.hv.x);
-        if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, w, gx, gy);
-        else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, w, gx, gy);
+        if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, ww, gx, gy);
+        else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, ww, gx, gy);
         const double mb = b.rm.y, vofb = b.hv.y, irb = b.iv.x;
         const double ir2b = irb * irb;
         const double fp = fpa + vofb * b.pc.x * ir2b;
@@ -832,10 +1389,11 @@ __global__ STAGE_ATTR void k_soil_b3(DPhase S, const double* recS, Masks MS, Gri
         Cxyx += mb * (fxy * gx);
         Dyy += mb * (fyy * gy);
         const double vr = (vxa - b.v.x) * dx + (vya - b.v.y) * dy;
-        if (vr < 0.0) {
+        {   // art_visc, branch-free: approaching pairs only (vr < 0)
             const double muv = amh * vr * fast_rcp(dr + amh * amh * et2);
             const double mc = 0.5 * (ca + b.pc.y);
-            const double piv = (K.beta * muv - K.alpha * mc) * muv * (2.0 * fast_rcp(rhoa + b.rm.x));
+            double piv = (K.beta * muv - K.alpha * mc) * muv * (2.0 * fast_rcp(rhoa + b.rm.x));
+            piv = vr < 0.0 ? piv : 0.0;
             artx -= mb * piv * gx;
             arty -= mb * piv * gy;
         }
@@ -845,6 +1403,7 @@ __global__ STAGE_ATTR void k_soil_b3(DPhase S, const double* recS, Masks MS, Gri
         dXY += hh * (sxya - b.s12.y);
         dYY += hh * (syya - b.s3.x);
     });
+    if (!wc.valid) return;
     double ax = -(Ax * rhoa) + Bxx * rhoa + Cxyy * rhoa;
     double ay = -(Ay * rhoa) + Cxyx * rhoa + Dyy * rhoa;
     if (rhoa > 0.0) { ax = ax * irhoa; ay = ay * irhoa; }

@@ -1,0 +1,660 @@
+// sph_mixed.h -- mixed-precision form of the fused two-phase step.
+//
+// The state, the leapfrog update, the Drucker-Prager projections, the EOS,
+// the flow rule and the Jaumann rate stay FP64 (pack and final kernels and
+// the per-particle tails of the stage kernels).  The pair sums run in FP32
+// with compensated (Kahan) accumulation over 64-byte FP32 neighbour records:
+//   q0: xl, yl, tag, h         xl, yl: position relative to the centre of the
+//                              particle's cell; tag: cx << 15 | cy, bit 31 set
+//                              for a virtual particle (int bits)
+//   q1: vx, vy, rho - rho0, m  (rho0 of the phase: differences keep precision)
+//   q2: vof, 1/rho, m/rho, p
+//   q3: c, s1, s2, s3          water: c, vof*sigma_xx, vof*sigma_xy, vof*sigma_yy
+//                              soil:  c, sxx, sxy, syy
+// A pair's separation is (xl_a - xl_b) + (cx_a - cx_b) * cs, exact up to the
+// FP32 rounding of the cell-local coordinates (~1e-7 relative).  Pair
+// selection uses the FP32 distance; pairs at r ~ 2h, where W and grad W
+// vanish, may differ from the FP64 selection.
+#pragma once
+#include "sph_fused.h"
+
+__device__ __forceinline__ const float4* recq(const float* rec, int k)
+{
+    return reinterpret_cast<const float4*>(rec + (size_t)Q_STRIDE * k);
+}
+
+// Compensated FP32 sum (Kahan): value() = s - c.
+struct Kahan {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float v)
+    {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+struct KCf {
+    float inv_h, f, fh1, fh2;
+};
+
+__device__ __forceinline__ KCf make_kcf(int skf, double h)
+{
+    const KC k = make_kc(skf, h);
+    return KCf{(float)k.inv_h, (float)k.f, (float)k.fh1, (float)k.fh2};
+}
+
+// W and grad W in FP32 (the forms of kern_fast).
+template <int SKF, bool NEED_W>
+__device__ __forceinline__ void kern_f(const KCf& k, float r, float inv_r, float dx, float dy, float& w, float& gx,
+                                       float& gy)
+{
+    const float q = r * k.inv_h;
+    w = 0.0f;
+    float s = 0.0f;
+    if constexpr (SKF == 1) {
+        const bool in1 = q <= 1.0f, in2 = q <= 2.0f;
+        const float t = 2.0f - q;
+        const float s1 = k.fh2 * (-2.0f + 1.5f * q);
+        const float s2 = -(0.5f * k.fh1) * (t * t) * inv_r;
+        s = in1 ? s1 : (in2 ? s2 : 0.0f);
+        if (NEED_W) {
+            const float w1 = k.f * ((float)SPH_C23_F - q * q + ((q * q) * q) * 0.5f);
+            const float w2 = k.f * (1.0f / 6.0f) * ((t * t) * t);
+            w = in1 ? w1 : (in2 ? w2 : 0.0f);
+        }
+    } else if constexpr (SKF == 4) {
+        const bool in2 = q <= 2.0f;
+        const float t = 1.0f - 0.5f * q;
+        const float t2 = t * t;
+        if (NEED_W) w = in2 ? k.f * ((t2 * t2) * (1.0f + 2.0f * q)) : 0.0f;
+        s = k.fh2 * (-5.0f + 7.5f * q - 3.75f * (q * q) + 0.625f * ((q * q) * q));
+        s = in2 ? s : 0.0f;
+    } else if constexpr (SKF == 2) {
+        if (q <= 3.0f) {
+            const float e = k.f * __expf(-q * q);
+            if (NEED_W) w = e;
+            s = -2.0f * e * (k.inv_h * k.inv_h);
+        }
+    } else {
+        const float a = 3.0f - q, b = 2.0f - q, c = 1.0f - q;
+        const float a2 = a * a, b2 = b * b, c2 = c * c;
+        const float a4 = a2 * a2, b4 = b2 * b2, c4 = c2 * c2;
+        if (q <= 1.0f) {
+            if (NEED_W) w = k.f * (a4 * a - 6.0f * (b4 * b) + 15.0f * (c4 * c));
+            s = k.fh2 * (-120.0f + 120.0f * q - 50.0f * (q * q));  // reference polynomial
+        } else if (q <= 2.0f) {
+            if (NEED_W) w = k.f * (a4 * a - 6.0f * (b4 * b));
+            s = k.fh1 * (-(5.0f * a4) + 30.0f * b4) * inv_r;
+        } else if (q <= 3.0f) {
+            if (NEED_W) w = k.f * (a4 * a);
+            s = k.fh1 * (-(5.0f * a4)) * inv_r;
+        }
+    }
+    gx = s * dx;
+    gy = s * dy;
+}
+
+// Mixed-precision constants of a step.
+struct MixK {
+    float cs;                  // cell size
+    float thr_w, thr_s, thr_x; // (scale_k h)^2 of uniform-h phases / cross pairs
+    float scale_k;
+    float rho0_w, rho0_s;
+};
+
+// Candidate rows of the 3 x 3 block around cell (cx, cy).
+__device__ __forceinline__ LaneRows rows_of_cell(const Grid& g, const int* start, int cx, int cy)
+{
+    LaneRows L;
+    const int c0 = max(cx - 1, 0), c1 = min(cx + 1, g.ncx - 1);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int row = cy - 1 + r;
+        const bool ok = row >= 0 && row < g.ncy;
+        L.k0[r] = ok ? start[row * g.ncx + c0] : 0;
+        L.k1[r] = ok ? start[row * g.ncx + c1 + 1] : 0;
+        L.base[r] = 0;
+    }
+    return L;
+}
+
+// Self view of a particle for the FP32 pair sums.
+struct SelfF {
+    float x, y, h;
+    int cx, cy;
+};
+
+__device__ __forceinline__ SelfF self_of(float4 q0)
+{
+    const int tag = __float_as_int(q0.z);
+    return SelfF{q0.x, q0.y, q0.w, tag_cx(tag), tag_cy(tag)};
+}
+
+// Separation a - b of a pair from cell-local coordinates.
+__device__ __forceinline__ void sep(const SelfF& a, float4 b0, float cs, float& dx, float& dy)
+{
+    const int tb = __float_as_int(b0.z);
+    dx = (a.x - b0.x) + (float)(a.cx - tag_cx(tb)) * cs;
+    dy = (a.y - b0.y) + (float)(a.cy - tag_cy(tb)) * cs;
+}
+
+// Scan of one candidate row (FP32 predicate).
+template <class P>
+__device__ __forceinline__ uint64_t scan_row_f(const float* rec, int k0, int k1, int self, P&& accept)
+{
+    uint64_t m = 0;
+    if (k1 - k0 > 64) return 0;
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) {
+        const bool ok = k != self && accept(recq(rec, k)[0]);
+        m |= (uint64_t)ok << (k - k0);
+    }
+    return m;
+}
+
+__device__ __forceinline__ bool pred_f(bool uni, float thr, float dr, float ha, float hb, float scale_k)
+{
+    if (uni) return dr < thr;
+    return __builtin_sqrtf(dr) < scale_k * (0.5f * (ha + hb));
+}
+
+// Partner iteration over a lane's CatMask, then the direct walk of overflowed rows.
+template <class B, class P>
+__device__ __forceinline__ void visit_f(const CatMask& cm, const LaneRows& L, int self, P&& accept, B&& body)
+{
+    CatIter it = cat_iter(cm, L);
+    int i = 0;
+#pragma unroll 1
+    for (int k = it.next(i); k >= 0; k = it.next(i)) body(k);
+    if (cm.ovf) {
+#pragma unroll 1
+        for (int r = 0; r < 3; ++r) {
+#pragma unroll 1
+            for (int k = L.k0[r]; k < L.k1[r]; ++k)
+                if (k != self && accept(k)) body(k);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Stage A, water (mixed): the terms of k_water_a4 with FP32 pair sums.
+// ------------------------------------------------------------------------
+template <int SKF>
+__global__ __launch_bounds__(64) void k_water_a_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS,
+                                                   Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
+{
+    const WaveCtx wc = wave_ctx(W.n);
+    const int t = wc.t;
+    const float4* QA = recq(qW, wc.tt);
+    const float4 a0 = QA[0], a1 = QA[1], a2 = QA[2];
+    const SelfF me = self_of(a0);
+    const float vxa = a1.x, vya = a1.y, dra = a1.z;
+    const float rhoa = dra + X.rho0_w;
+    const int a = W.order[wc.tt];
+    const bool uni = F.uni_w;
+    const KCf kc = make_kcf(SKF, me.h);
+    const LaneRows L = rows_of_cell(g, W.cell_start, me.cx, me.cy);
+    auto acc_q = [&](float4 b0) {
+        float dx, dy;
+        sep(me, b0, X.cs, dx, dy);
+        return pred_f(uni, X.thr_w, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
+    };
+    const uint64_t m0 = scan_row_f(qW, L.k0[0], L.k1[0], t, acc_q);
+    const uint64_t m1 = scan_row_f(qW, L.k0[1], L.k1[1], t, acc_q);
+    const uint64_t m2 = scan_row_f(qW, L.k0[2], L.k1[2], t, acc_q);
+    const CatMask cm = cat_masks(L, m0, m1, m2);
+    if (wc.valid) {
+        MW.m[t] = cm.lo;
+        MW.m[W.n + t] = cm.hi;
+        MW.ovf[t] = cm.ovf;
+    }
+    const bool real_a = W.itype[a] > 0;
+    const float dspp = (float)W.dspp;
+    Kahan con, drd, txx, txy, tyy, avx, avy;
+    float repx = 0.0f, repy = 0.0f;
+    int cnt = 0;
+    visit_f(cm, L, t, [&](int k) { return acc_q(recq(qW, k)[0]); }, [&](int k) {
+        const float4* Q = recq(qW, k);
+        const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2];
+        float dx, dy;
+        sep(me, b0, X.cs, dx, dy);
+        const float dr = dx * dx + dy * dy;
+        const float ir = __builtin_amdgcn_rsqf(dr);
+        const float r = dr * ir, idr = ir * ir;
+        float ww, gx, gy;
+        if (uni) kern_f<SKF, true>(kc, r, ir, dx, dy, ww, gx, gy);
+        else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
+        ++cnt;
+        const float mb = b1.w, Vb = b2.z, irb = b2.y;
+        const float dvx = vxa - b1.x, dvy = vya - b1.y;
+        con.add(mb * (dvx * gx + dvy * gy));
+        drd.add(mb * (2.0f * (dra - b1.z) * idr * (dx * gx + dy * gy)) * irb);
+        txx.add(Vb * (-2.0f * dvx * gx + dvy * gy));
+        txy.add(Vb * (-(dvx * gy) - dvy * gx));
+        tyy.add(Vb * (-2.0f * dvy * gy + dvx * gx));
+        const float f = mb * ww * (2.0f * __builtin_amdgcn_rcpf(rhoa + b1.z + X.rho0_w));
+        avx.add(-(dvx * f));
+        avy.add(-(dvy * f));
+        if (real_a && !(r > dspp) && __float_as_int(b0.z) < 0) {
+            const float q = dspp / r;
+            const float fr = (__powf(q, (float)K.p1) - __powf(q, (float)K.p2)) * idr;
+            repx += (float)K.dd * dx * fr;
+            repy += (float)K.dd * dy * fr;
+        }
+    });
+
+    Kahan dg_x, dg_y, dvof_x, vof2;
+    int cntx = 0;
+    if (cross) {
+        const LaneRows LX = rows_of_cell(g, S.cell_start, me.cx, me.cy);
+        auto accx_q = [&](float4 b0) {
+            float dx, dy;
+            sep(me, b0, X.cs, dx, dy);
+            return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
+        };
+        const uint64_t c0 = scan_row_f(qS, LX.k0[0], LX.k1[0], -1, accx_q);
+        const uint64_t c1 = scan_row_f(qS, LX.k0[1], LX.k1[1], -1, accx_q);
+        const uint64_t c2 = scan_row_f(qS, LX.k0[2], LX.k1[2], -1, accx_q);
+        const CatMask cx3 = cat_masks(LX, c0, c1, c2);
+        const double vofa_d = W.vof[a], rhoa_d = W.rho[a];
+        const float cfa = (float)(vofa_d * rhoa_d * (-K.numeric_gravity) / K.s_perm);
+        const float irhoa = a2.y;
+        const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)me.h + (double)recq(qS, 0)[0].w) : (double)me.h);
+        visit_f(cx3, LX, -1, [&](int k) { return accx_q(recq(qS, k)[0]); }, [&](int k) {
+            const float4* Q = recq(qS, k);
+            const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2];
+            float dx, dy;
+            sep(me, b0, X.cs, dx, dy);
+            const float dr = dx * dx + dy * dy;
+            const float ir = dr > 0.0f ? __builtin_amdgcn_rsqf(dr) : 0.0f;
+            const float r = dr * ir;
+            float ww, gx, gy;
+            if (F.uni_x) kern_f<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+            else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
+            ++cntx;
+            const float mb = b1.w;
+            const float dvx = vxa - b1.x, dvy = vya - b1.y;
+            const float sc = mb * (cfa * b2.x) * (ww * irhoa * b2.y);
+            dg_x.add(sc * dvx);
+            dg_y.add(sc * dvy);
+            dvof_x.add(mb * (dvx * gx + dvy * gy));
+            vof2.add(mb * ww);
+        });
+    }
+    if (!wc.valid) return;
+    const double rhoa_d = W.rho[a], vofa_d = W.vof[a], ha_d = W.hsml[a], ca_d = W.c[a];
+    const double rr = rhoa_d / (K.w_rho0 * vofa_d);
+    const double p = K.w_b * (pow(rr, K.w_gamma) - 1.0);
+    const double cn = K.w_c * pow(rr, 3.0);
+    const double mu = K.w_mu;
+    const double txx_d = txx.value() * SPH_C23_F, tyy_d = tyy.value() * SPH_C23_F, txy_d = txy.value();
+    W.drho[a] = con.value() + (K.delta * ha_d * ca_d) * drd.value();
+    W.txx[a] = txx_d; W.txy[a] = txy_d; W.tyy[a] = tyy_d;
+    double avx_d = avx.value(), avy_d = avy.value();
+    if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
+    W.avx[a] = avx_d; W.avy[a] = avy_d;
+    W.repx[a] = repx; W.repy[a] = repy;
+    W.p[a] = p;
+    W.c_next[a] = cn;
+    if (cross) {
+        W.drgx[a] = dg_x.value(); W.drgy[a] = dg_y.value(); W.dvof_x[a] = dvof_x.value() * (1.0 / K.s_rho0);
+        W.vof2[a] = 1.0 - vof2.value() / K.s_rho0;
+        W.countiac[a] = cntx;
+    } else {
+        W.countiac[a] = cnt;
+    }
+    float* QW = qWw + (size_t)Q_STRIDE * t;
+    QW[11] = (float)p;
+    reinterpret_cast<float4*>(QW)[3] =
+        make_float4((float)cn, (float)(vofa_d * (mu * txx_d)), (float)(vofa_d * (mu * txy_d)),
+                    (float)(vofa_d * (mu * tyy_d)));
+}
+
+// ------------------------------------------------------------------------
+// Stage A, soil (mixed).
+// ------------------------------------------------------------------------
+template <int SKF>
+__global__ __launch_bounds__(64) void k_soil_a_mx(DPhase S, DPhase W, const float* qS, float* qSw, const float* qW,
+                                                  Masks MS, Grid g, KParams K, FusedK F, MixK X, int cross)
+{
+    const WaveCtx wc = wave_ctx(S.n);
+    const int t = wc.t;
+    const float4* QA = recq(qS, wc.tt);
+    const float4 a0 = QA[0], a1 = QA[1], a2 = QA[2];
+    const SelfF me = self_of(a0);
+    const float vxa = a1.x, vya = a1.y, dra = a1.z;
+    const float rhoa = dra + X.rho0_s;
+    const int a = S.order[wc.tt];
+    const bool uni = F.uni_s;
+    const KCf kc = make_kcf(SKF, me.h);
+    const LaneRows L = rows_of_cell(g, S.cell_start, me.cx, me.cy);
+    auto acc_q = [&](float4 b0) {
+        float dx, dy;
+        sep(me, b0, X.cs, dx, dy);
+        return pred_f(uni, X.thr_s, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
+    };
+    const uint64_t m0 = scan_row_f(qS, L.k0[0], L.k1[0], t, acc_q);
+    const uint64_t m1 = scan_row_f(qS, L.k0[1], L.k1[1], t, acc_q);
+    const uint64_t m2 = scan_row_f(qS, L.k0[2], L.k1[2], t, acc_q);
+    const CatMask cm = cat_masks(L, m0, m1, m2);
+    if (wc.valid) {
+        MS.m[t] = cm.lo;
+        MS.m[S.n + t] = cm.hi;
+        MS.ovf[t] = cm.ovf;
+    }
+    const bool real_a = S.itype[a] > 0;
+    const float dspp = (float)S.dspp;
+    Kahan con, drd, txx, txy, tyy, vcc, wxy, avx, avy;
+    float repx = 0.0f, repy = 0.0f;
+    int cnt = 0;
+    visit_f(cm, L, t, [&](int k) { return acc_q(recq(qS, k)[0]); }, [&](int k) {
+        const float4* Q = recq(qS, k);
+        const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2];
+        float dx, dy;
+        sep(me, b0, X.cs, dx, dy);
+        const float dr = dx * dx + dy * dy;
+        const float ir = __builtin_amdgcn_rsqf(dr);
+        const float r = dr * ir, idr = ir * ir;
+        float ww, gx, gy;
+        if (uni) kern_f<SKF, true>(kc, r, ir, dx, dy, ww, gx, gy);
+        else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
+        ++cnt;
+        const float mb = b1.w, Vb = b2.z, irb = b2.y;
+        const float dvx = vxa - b1.x, dvy = vya - b1.y;
+        const float dvg = dvx * gx + dvy * gy;
+        con.add(mb * dvg);
+        drd.add(mb * (2.0f * (dra - b1.z) * idr * (dx * gx + dy * gy)) * irb);
+        txx.add(Vb * (-2.0f * dvx * gx + dvy * gy));
+        txy.add(Vb * (-(dvx * gy) - dvy * gx));
+        tyy.add(Vb * (-2.0f * dvy * gy + dvx * gx));
+        vcc.add(-(Vb * dvg));
+        wxy.add(Vb * (-(dvx * gy) + dvy * gx));
+        const float f = mb * ww * (2.0f * __builtin_amdgcn_rcpf(rhoa + b1.z + X.rho0_s));
+        avx.add(-(dvx * f));
+        avy.add(-(dvy * f));
+        if (real_a && !(r > dspp) && __float_as_int(b0.z) < 0) {
+            const float q = dspp / r;
+            const float fr = (__powf(q, (float)K.p1) - __powf(q, (float)K.p2)) * idr;
+            repx += (float)K.dd * dx * fr;
+            repy += (float)K.dd * dy * fr;
+        }
+    });
+
+    Kahan dg_x, dg_y, po_x, po_y;
+    int cntx = 0;
+    const double vofa_d = S.vof[a], rhoa_d = S.rho[a];
+    if (cross) {
+        const LaneRows LX = rows_of_cell(g, W.cell_start, me.cx, me.cy);
+        auto accx_q = [&](float4 b0) {
+            float dx, dy;
+            sep(me, b0, X.cs, dx, dy);
+            return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
+        };
+        const uint64_t c0 = scan_row_f(qW, LX.k0[0], LX.k1[0], -1, accx_q);
+        const uint64_t c1 = scan_row_f(qW, LX.k0[1], LX.k1[1], -1, accx_q);
+        const uint64_t c2 = scan_row_f(qW, LX.k0[2], LX.k1[2], -1, accx_q);
+        const CatMask cx3 = cat_masks(LX, c0, c1, c2);
+        const float cfs = (float)(vofa_d * (-K.numeric_gravity) / K.s_perm);
+        const float irhoa = a2.y;
+        const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)recq(qW, 0)[0].w + (double)me.h) : (double)me.h);
+        visit_f(cx3, LX, -1, [&](int k) { return accx_q(recq(qW, k)[0]); }, [&](int k) {
+            const float4* Q = recq(qW, k);
+            const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2];
+            float dx, dy;
+            sep(me, b0, X.cs, dx, dy);
+            dx = -dx;   // water - soil
+            dy = -dy;
+            const float dr = dx * dx + dy * dy;
+            const float ir = dr > 0.0f ? __builtin_amdgcn_rsqf(dr) : 0.0f;
+            const float r = dr * ir;
+            float ww, gx, gy;
+            if (F.uni_x) kern_f<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+            else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)b0.w + (double)me.h)), r, ir, dx, dy, ww, gx, gy);
+            ++cntx;
+            const float mw = b1.w, rhow = b1.z + X.rho0_w;
+            const float sc = mw * (b2.x * rhow * cfs) * (ww * b2.y * irhoa);
+            dg_x.add(sc * (b1.x - vxa));
+            dg_y.add(sc * (b1.y - vya));
+            const float mprr = mw * b2.w * (b2.y * irhoa);
+            po_x.add(mprr * gx);
+            po_y.add(mprr * gy);
+        });
+    }
+    if (!wc.valid) return;
+    const double ha_d = S.hsml[a], ca_d = S.c[a];
+    const double txx_d = txx.value() * SPH_C23_F, tyy_d = tyy.value() * SPH_C23_F, txy_d = txy.value();
+    const double vcc_d = vcc.value(), wxy_d = wxy.value() * 0.5;
+    S.drho[a] = con.value() + (K.delta * ha_d * ca_d) * drd.value();
+    S.txx[a] = txx_d; S.txy[a] = txy_d; S.tyy[a] = tyy_d; S.vcc[a] = vcc_d; S.wxy[a] = wxy_d;
+    double avx_d = avx.value(), avy_d = avy.value();
+    if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
+    S.avx[a] = avx_d; S.avy[a] = avy_d;
+    S.repx[a] = repx; S.repy[a] = repy;
+    if (cross) {
+        S.drgx[a] = dg_x.value(); S.drgy[a] = dg_y.value();
+        S.porx[a] = po_x.value() * vofa_d; S.pory[a] = po_y.value() * vofa_d;
+        S.countiac[a] = cntx;
+    } else {
+        S.countiac[a] = cnt;
+    }
+    S.c_next[a] = K.s_c;
+    qSw[(size_t)Q_STRIDE * t + 12] = (float)K.s_c;
+    {
+        const double p = S.p[a], sxx = S.sxx[a], sxy = S.sxy[a], syy = S.syy[a];
+        const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;
+        double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
+        if (K.plasticity == 3) {
+            const double I1 = 3.0 * p;
+            const double j2r = sxx * sxx + 2.0 * (sxy * sxy) + syy * syy + (sxx + syy) * (sxx + syy);
+            const double J2 = __builtin_sqrt(j2r / 2.0) + 1.0e-6;
+            const int fail = fabs(J2 - alpha * I1) < SPH_YIELD_TOL_F ? 1 : 0;
+            if (fail) {
+                const double exx = txx_d * 0.5 + vcc_d * (1.0 / 3.0);
+                const double exy = txy_d * 0.5;
+                const double eyy = tyy_d * 0.5 + vcc_d * (1.0 / 3.0);
+                const double sde = sxx * exx + 2.0 * sxy * exy + syy * eyy;
+                const double sj = __builtin_sqrt(j2r / 2.0 + 1.0e-10);
+                const double gs = G / sj;
+                const double dl = (3.0 * alpha * Kb * vcc_d + gs * sde) / (9.0 * (alpha * alpha) * Kb + G);
+                if (!(dl < 0.0)) {
+                    dsxx = -dl * (gs * sxx);
+                    dsxy = -dl * (gs * sxy);
+                    dsyy = -dl * (gs * syy);
+                    dpv = 3.0 * Kb * alpha * dl;
+                    S.eps_p[a] = S.eps_p[a] + dl * sxy / (2.0 * sj) * SPH_EPSP_F;
+                }
+            }
+            S.fail[a] = fail;
+        }
+        dsxx = dsxx + G * txx_d + 2.0 * sxy * wxy_d;
+        dsxy = dsxy + G * txy_d - (sxx - syy) * wxy_d;
+        dsyy = dsyy + G * tyy_d - 2.0 * sxy * wxy_d;
+        dpv = dpv - Kb * vcc_d;
+        S.dsxx[a] = dsxx; S.dsxy[a] = dsxy; S.dsyy[a] = dsyy; S.dp[a] = dpv;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Stage B, water (mixed): momentum, art_visc, delta_sph(vof).
+// ------------------------------------------------------------------------
+template <int SKF>
+__global__ __launch_bounds__(64) void k_water_b_mx(DPhase W, const float* qW, Masks MW, Grid g, KParams K, FusedK F,
+                                                   MixK X, int mask)
+{
+    const WaveCtx wc = wave_ctx(W.n);
+    const int t = wc.t;
+    const float4* QA = recq(qW, wc.tt);
+    const float4 a0 = QA[0], a1 = QA[1], a2 = QA[2], a3 = QA[3];
+    const SelfF me = self_of(a0);
+    const float vxa = a1.x, vya = a1.y, rhoa = a1.z + X.rho0_w;
+    const float vofa = a2.x, pa = a2.w, ca = a3.x, fxx_a = a3.y, fxy_a = a3.z, fyy_a = a3.w;
+    const int a = W.order[wc.tt];
+    const bool uni = F.uni_w;
+    const KCf kc = make_kcf(SKF, me.h);
+    const LaneRows L = rows_of_cell(g, W.cell_start, me.cx, me.cy);
+    auto acc_q = [&](float4 b0) {
+        float dx, dy;
+        sep(me, b0, X.cs, dx, dy);
+        return pred_f(uni, X.thr_w, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
+    };
+    CatMask cm{0, 0, 0};
+    if (wc.valid) {
+        cm.lo = MW.m[t];
+        cm.hi = MW.m[W.n + t];
+        cm.ovf = MW.ovf[t];
+    }
+    const float et2 = (float)(K.etq * K.etq);
+    const float beta = (float)K.beta, alpha = (float)K.alpha;
+    Kahan Px, Py, Sxx, Sxyy, Sxyx, Syy, artx, arty, dvd;
+    visit_f(cm, L, t, [&](int k) { return acc_q(recq(qW, k)[0]); }, [&](int k) {
+        const float4* Q = recq(qW, k);
+        const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2], b3 = Q[3];
+        float dx, dy;
+        sep(me, b0, X.cs, dx, dy);
+        const float dr = dx * dx + dy * dy;
+        const float ir = __builtin_amdgcn_rsqf(dr);
+        const float r = dr * ir, idr = ir * ir;
+        float ww, gx, gy;
+        const float amh = 0.5f * (me.h + b0.w);
+        if (uni) kern_f<SKF, false>(kc, r, ir, dx, dy, ww, gx, gy);
+        else kern_f<SKF, false>(make_kcf(SKF, amh), r, ir, dx, dy, ww, gx, gy);
+        const float Vb = b2.z, mb = b1.w;
+        const float ps = pa + b2.w;
+        Px.add(Vb * (ps * gx));
+        Py.add(Vb * (ps * gy));
+        const float sxy = fxy_a + b3.z;
+        Sxx.add(Vb * ((fxx_a + b3.y) * gx));
+        Sxyy.add(Vb * (sxy * gy));
+        Sxyx.add(Vb * (sxy * gx));
+        Syy.add(Vb * ((fyy_a + b3.w) * gy));
+        const float vr = (vxa - b1.x) * dx + (vya - b1.y) * dy;
+        const float muv = amh * vr * __builtin_amdgcn_rcpf(dr + amh * amh * et2);
+        const float mc = 0.5f * (ca + b3.x);
+        float piv = (beta * muv - alpha * mc) * muv * (2.0f * __builtin_amdgcn_rcpf(rhoa + b1.z + X.rho0_w));
+        piv = vr < 0.0f ? piv : 0.0f;
+        artx.add(-(mb * piv * gx));
+        arty.add(-(mb * piv * gy));
+        dvd.add(mb * (2.0f * (vofa - b2.x) * idr * (dx * gx + dy * gy)) * b2.y);
+    });
+    if (!wc.valid) return;
+    const double vofa_d = W.vof[a], rhoa_d = W.rho[a], ha_d = W.hsml[a], ca_d = W.c[a];   // new c
+    double ax = -(vofa_d * Px.value()) + Sxx.value() + Sxyy.value();
+    double ay = -(vofa_d * Py.value()) + Sxyx.value() + Syy.value();
+    if (rhoa_d > 0.0) {
+        ax = ax / rhoa_d;
+        ay = ay / rhoa_d;
+    }
+    if (mask & M_ART) { ax = ax + artx.value(); ay = ay + arty.value(); }
+    ax = ax + W.repx[a];
+    ay = ay + W.repy[a];
+    ay = ay + K.gravity;
+    double dvof = (mask & M_DELTA_FIELD) ? (K.delta * ha_d * ca_d) * dvd.value() : 0.0;
+    if (mask & M_CROSS) {
+        ax = ax - W.drgx[a];
+        ay = ay - W.drgy[a];
+        dvof = dvof - W.dvof_x[a];
+    }
+    W.ax[a] = ax;
+    W.ay[a] = ay;
+    W.dvof[a] = dvof;
+}
+
+// ------------------------------------------------------------------------
+// Stage B, soil (mixed): momentum (df3 form), art_visc, delta_sph on p, s.
+// ------------------------------------------------------------------------
+template <int SKF>
+__global__ __launch_bounds__(64) void k_soil_b_mx(DPhase S, const float* qS, Masks MS, Grid g, KParams K, FusedK F,
+                                                  MixK X, int mask)
+{
+    const WaveCtx wc = wave_ctx(S.n);
+    const int t = wc.t;
+    const float4* QA = recq(qS, wc.tt);
+    const float4 a0 = QA[0], a1 = QA[1], a2 = QA[2], a3 = QA[3];
+    const SelfF me = self_of(a0);
+    const float vxa = a1.x, vya = a1.y, rhoa = a1.z + X.rho0_s;
+    const float vofa = a2.x, irhoa = a2.y, pa = a2.w, ca = a3.x, sxxa = a3.y, sxya = a3.z, syya = a3.w;
+    const float ir2a = irhoa * irhoa;
+    const float fpa = vofa * pa * ir2a, fxxa = vofa * sxxa * ir2a, fxya = vofa * sxya * ir2a,
+                fyya = vofa * syya * ir2a;
+    const int a = S.order[wc.tt];
+    const bool uni = F.uni_s;
+    const KCf kc = make_kcf(SKF, me.h);
+    const LaneRows L = rows_of_cell(g, S.cell_start, me.cx, me.cy);
+    auto acc_q = [&](float4 b0) {
+        float dx, dy;
+        sep(me, b0, X.cs, dx, dy);
+        return pred_f(uni, X.thr_s, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
+    };
+    CatMask cm{0, 0, 0};
+    if (wc.valid) {
+        cm.lo = MS.m[t];
+        cm.hi = MS.m[S.n + t];
+        cm.ovf = MS.ovf[t];
+    }
+    const float et2 = (float)(K.etq * K.etq);
+    const float beta = (float)K.beta, alpha = (float)K.alpha;
+    Kahan Ax, Ay, Bxx, Cxyy, Cxyx, Dyy, artx, arty, dP, dXX, dXY, dYY;
+    visit_f(cm, L, t, [&](int k) { return acc_q(recq(qS, k)[0]); }, [&](int k) {
+        const float4* Q = recq(qS, k);
+        const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2], b3 = Q[3];
+        float dx, dy;
+        sep(me, b0, X.cs, dx, dy);
+        const float dr = dx * dx + dy * dy;
+        const float ir = __builtin_amdgcn_rsqf(dr);
+        const float r = dr * ir, idr = ir * ir;
+        float ww, gx, gy;
+        const float amh = 0.5f * (me.h + b0.w);
+        if (uni) kern_f<SKF, false>(kc, r, ir, dx, dy, ww, gx, gy);
+        else kern_f<SKF, false>(make_kcf(SKF, amh), r, ir, dx, dy, ww, gx, gy);
+        const float mb = b1.w, vofb = b2.x, irb = b2.y;
+        const float ir2b = irb * irb;
+        const float fp = fpa + vofb * b2.w * ir2b;
+        const float fxx = fxxa + vofb * b3.y * ir2b;
+        const float fxy = fxya + vofb * b3.z * ir2b;
+        const float fyy = fyya + vofb * b3.w * ir2b;
+        Ax.add(mb * (fp * gx));
+        Ay.add(mb * (fp * gy));
+        Bxx.add(mb * (fxx * gx));
+        Cxyy.add(mb * (fxy * gy));
+        Cxyx.add(mb * (fxy * gx));
+        Dyy.add(mb * (fyy * gy));
+        const float vr = (vxa - b1.x) * dx + (vya - b1.y) * dy;
+        const float muv = amh * vr * __builtin_amdgcn_rcpf(dr + amh * amh * et2);
+        const float mc = 0.5f * (ca + b3.x);
+        float piv = (beta * muv - alpha * mc) * muv * (2.0f * __builtin_amdgcn_rcpf(rhoa + b1.z + X.rho0_s));
+        piv = vr < 0.0f ? piv : 0.0f;
+        artx.add(-(mb * piv * gx));
+        arty.add(-(mb * piv * gy));
+        const float hh = 2.0f * idr * (dx * gx + dy * gy) * mb * irb;
+        dP.add(hh * (pa - b2.w));
+        dXX.add(hh * (sxxa - b3.y));
+        dXY.add(hh * (sxya - b3.z));
+        dYY.add(hh * (syya - b3.w));
+    });
+    if (!wc.valid) return;
+    const double rhoa_d = S.rho[a], ha_d = S.hsml[a];
+    double ax = -(Ax.value() * rhoa_d) + Bxx.value() * rhoa_d + Cxyy.value() * rhoa_d;
+    double ay = -(Ay.value() * rhoa_d) + Cxyx.value() * rhoa_d + Dyy.value() * rhoa_d;
+    if (rhoa_d > 0.0) { ax = ax / rhoa_d; ay = ay / rhoa_d; }
+    if (mask & M_ART) { ax = ax + artx.value(); ay = ay + arty.value(); }
+    ax = ax + S.repx[a];
+    ay = ay + S.repy[a];
+    ay = ay + K.gravity;
+    if (mask & M_CROSS) {
+        ax = ax + S.drgx[a]; ay = ay + S.drgy[a];
+        ax = ax + S.porx[a]; ay = ay + S.pory[a];
+    }
+    S.ax[a] = ax;
+    S.ay[a] = ay;
+    if (mask & M_DELTA_FIELD) {
+        const double cf = K.delta * ha_d * S.c[a];   // new c
+        S.dp[a] = S.dp[a] + cf * dP.value();
+        S.dsxx[a] = S.dsxx[a] + cf * dXX.value();
+        S.dsxy[a] = S.dsxy[a] + cf * dXY.value();
+        S.dsyy[a] = S.dsyy[a] + cf * dYY.value();
+    }
+}
+

@@ -13,6 +13,7 @@
 
 #include "../../include/sphg.h"
 #include "sph_fused.h"
+#include "sph_mixed.h"
 
 namespace {
 
@@ -76,6 +77,8 @@ struct sphg_ctx {
     std::vector<hipEvent_t> event_pool;
     int64_t pairs[3] = {0, 0, 0};
     FusedK F{};
+    MixK X{};                   // mixed-precision constants
+    double cs = 0.0;            // cell size
     bool legacy = false;        // SPHG_LEGACY=1: unfused reference-order kernels
     int reorder_every = 16;     // steps between permutations of the state into cell order
     int since_reorder = 0;
@@ -285,6 +288,7 @@ int check_params(const sphg_params& P)
     if (!(P.skf >= 1 && P.skf <= 4)) return SPHG_ERR_BAD_KERNEL;
     if (P.nnps != 1 && P.nnps != 2) return SPHG_ERR_UNSUPPORTED;
     if (!(P.x_maxgeom > P.x_mingeom && P.y_maxgeom > P.y_mingeom)) return SPHG_ERR_INVALID;
+    if (P.precision != SPHG_FP64 && P.precision != SPHG_MIXED) return SPHG_ERR_INVALID;
     return SPHG_OK;
 }
 
@@ -324,6 +328,7 @@ int setup_grid(sphg_ctx* c)
     g.ncx = (int)fx;
     g.ncy = (int)fy;
     c->g = g;
+    c->cs = cs;
     const int ncell = g.ncx * g.ncy;
     // per phase: reference link_list sorting grid (init_grid, m_particles.f90:1069-1113)
     for (int k = 0; k < 2; ++k) {
@@ -359,6 +364,14 @@ int setup_grid(sphg_ctx* c)
         F.thr_w = W.bound ? dr_threshold(sk * W.view.hsml[0]) : 0.0;
         F.thr_s = S.bound ? dr_threshold(sk * S.view.hsml[0]) : 0.0;
         F.thr_x = (W.bound && S.bound) ? dr_threshold(sk * ((W.view.hsml[0] + S.view.hsml[0]) / 2.0)) : 0.0;
+        MixK& X = c->X;
+        X.cs = (float)cs;
+        X.thr_w = (float)F.thr_w;
+        X.thr_s = (float)F.thr_s;
+        X.thr_x = (float)F.thr_x;
+        X.scale_k = (float)sk;
+        X.rho0_w = (float)c->K.w_rho0;
+        X.rho0_s = (float)c->K.s_rho0;
     }
     // (re)allocate cell arrays
     if (ncell != c->ncell) {
@@ -677,24 +690,37 @@ int stages_fused(sphg_ctx* c)
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
     const int cross = c->P.single_phase ? 0 : 1;
-    const FusedK& F = c->F;
-    const int link = c->P.nnps == 2 ? 1 : 0;
-    RC(launch(c, "masks", k_masks, nblk(W.n), TPB, (const double*)W.rec, (const int*)W.d.cell_start, W.n,
-              (const double*)S.rec, (const int*)S.d.cell_start, W.masks, W.xmasks, c->gl[0], c->K,
-              (link || F.uni_w) ? 1 : 0, F.thr_w, link, cross, F.uni_x, F.thr_x));
-    RC(launch(c, "masks", k_masks, nblk(S.n), TPB, (const double*)S.rec, (const int*)S.d.cell_start, S.n,
-              (const double*)W.rec, (const int*)W.d.cell_start, S.masks, S.xmasks, c->gl[1], c->K,
-              (link || F.uni_s) ? 1 : 0, F.thr_s, link, cross, F.uni_x, F.thr_x));
-    RC(launch(c, "water_a", k_water_a3<SKF>, nblk(W.n), TPB, W.d, S.d, (const double*)W.rec, W.rec,
-              (const double*)S.rec, W.masks, W.xmasks, c->gl[0], c->K, c->F, cross));
-    RC(launch(c, "soil_a", k_soil_a3<SKF>, nblk(S.n), TPB, S.d, W.d, (const double*)S.rec, S.rec,
-              (const double*)W.rec, S.masks, S.xmasks, c->gl[1], c->K, c->F, cross));
+    RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX>, nblk(W.n, 64), 64, W.d, S.d, (const double*)W.rec, W.rec,
+              (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
+    RC(launch(c, "soil_a", k_soil_a4<SKF, SPHG_LA, SPHG_LX>, nblk(S.n, 64), 64, S.d, W.d, (const double*)S.rec, S.rec,
+              (const double*)W.rec, S.masks, c->gl[1], c->K, c->F, cross));
     swap_c(W);
     swap_c(S);
-    RC(launch(c, "water_b", k_water_b3<SKF>, nblk(W.n), TPB, W.d, (const double*)W.rec, W.masks, c->gl[0], c->K,
+    RC(launch(c, "water_b", k_water_b4<SKF, SPHG_LB>, nblk(W.n, 64), 64, W.d, (const double*)W.rec, W.masks, c->gl[0], c->K,
               c->F, mask_b(c, 0)));
-    RC(launch(c, "soil_b", k_soil_b3<SKF>, nblk(S.n), TPB, S.d, (const double*)S.rec, S.masks, c->gl[1], c->K,
+    RC(launch(c, "soil_b", k_soil_b4<SKF, SPHG_LB>, nblk(S.n, 64), 64, S.d, (const double*)S.rec, S.masks, c->gl[1], c->K,
               c->F, mask_b(c, 1)));
+    return SPHG_OK;
+}
+
+// Mixed-precision stages: FP32 records and pair sums, FP64 per-particle tails.
+template <int SKF>
+int stages_mixed(sphg_ctx* c)
+{
+    PhaseBuf& W = c->ph[0];
+    PhaseBuf& S = c->ph[1];
+    float* qW = reinterpret_cast<float*>(W.rec);
+    float* qS = reinterpret_cast<float*>(S.rec);
+    RC(launch(c, "water_a", k_water_a_mx<SKF>, nblk(W.n, 64), 64, W.d, S.d, (const float*)qW, qW, (const float*)qS,
+              W.masks, c->gl[0], c->K, c->F, c->X, 1));
+    RC(launch(c, "soil_a", k_soil_a_mx<SKF>, nblk(S.n, 64), 64, S.d, W.d, (const float*)qS, qS, (const float*)qW,
+              S.masks, c->gl[1], c->K, c->F, c->X, 1));
+    swap_c(W);
+    swap_c(S);
+    RC(launch(c, "water_b", k_water_b_mx<SKF>, nblk(W.n, 64), 64, W.d, (const float*)qW, W.masks, c->gl[0], c->K,
+              c->F, c->X, mask_b(c, 0)));
+    RC(launch(c, "soil_b", k_soil_b_mx<SKF>, nblk(S.n, 64), 64, S.d, (const float*)qS, S.masks, c->gl[1], c->K,
+              c->F, c->X, mask_b(c, 1)));
     return SPHG_OK;
 }
 
@@ -704,8 +730,23 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
     PhaseBuf& S = c->ph[1];
     RC(build_cells(c, 0));
     RC(build_cells(c, 1));
-    RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half));
-    RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp));
+    if (c->P.precision == SPHG_MIXED) {
+        float* qW = reinterpret_cast<float*>(W.rec);
+        float* qS = reinterpret_cast<float*>(S.rec);
+        RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, c->gl[0], c->cs));
+        RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
+                  c->cs));
+        switch (c->P.skf) {
+        case 1: return stages_mixed<1>(c);
+        case 2: return stages_mixed<2>(c);
+        case 3: return stages_mixed<3>(c);
+        default: return stages_mixed<4>(c);
+        }
+    }
+    RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, (float*)nullptr, c->gl[0],
+              c->cs));
+    RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, (float*)nullptr,
+              c->gl[1], c->cs));
     switch (c->P.skf) {
     case 1: return stages_fused<1>(c);
     case 2: return stages_fused<2>(c);
@@ -734,6 +775,11 @@ int check_fused_switches(const sphg_ctx* c)
 {
     const sphg_params& P = c->P;
     if (P.summation_density || P.stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
+    // the mixed-precision form exists for the fused two-phase step with the
+    // direct_find predicate; its records hold 15-bit cell coordinates
+    if (P.precision == SPHG_MIXED &&
+        (P.single_phase || P.nnps != 1 || c->legacy || c->g.ncx > 32767 || c->g.ncy > 32767))
+        return SPHG_ERR_UNSUPPORTED;
     if (!P.single_phase) {
         if (!c->ph[1].bound) return SPHG_ERR_INVALID;
         if (P.yield_criterion != 2) return SPHG_ERR_UNSUPPORTED;

@@ -1,0 +1,85 @@
+"""Mixed-precision form (precision = SPHG_MIXED): FP32 pair sums with Kahan
+compensation, FP64 leapfrog / Drucker-Prager / EOS.  Its error is measured
+against the FP64 path, which the parity tests pin to the oracle, and after
+one step also against the oracle directly."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import field_errors
+from sources_amd import abi, scenarios
+from sources_amd.sph import Solver
+
+pytestmark = pytest.mark.gpu
+
+STATE = {1: ("x", "vx", "rho", "p", "vof"), 2: ("x", "vx", "rho", "p", "sxx", "sxy", "syy")}
+RATES = {1: ("drho", "dvx", "dvof"), 2: ("drho", "dvx", "dsxx", "dsxy", "dsyy", "dp")}
+
+CASES = {
+    "savage_skf4": lambda: scenarios.savage_column(),
+    "column_jitter": lambda: scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3,
+                                                        tank_h=0.12, jitter=0.05, seed=7),
+    "slope_small": lambda: scenarios.slope(dx=0.01, scale=0.25, jitter=0.1, seed=3),
+}
+
+
+def _run(make, precision, nsteps):
+    sc = make()
+    sc.params.precision = precision
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(nsteps, sync=True)
+    s.close()
+    return sc
+
+
+def _errs(a, b):
+    out = {}
+    for ph in ("water", "soil"):
+        pa, pb = getattr(a, ph), getattr(b, ph)
+        out[ph] = field_errors(pa, pb, STATE[pb.material] + RATES[pb.material])
+    return out
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_mixed_one_step(case):
+    print(f"case={case}", flush=True)
+    mx = _run(CASES[case], abi.SPHG_MIXED, 1)
+    fp = _run(CASES[case], abi.SPHG_FP64, 1)
+    ref = CASES[case]()
+    oracle.time_integration(ref.params, ref.water, ref.soil, 0, 1)
+    for label, other in (("fp64-gpu", fp), ("oracle", ref)):
+        e = _errs(mx, other)
+        print(label, {ph: {k: f"{v:.2e}" for k, v in d.items()} for ph, d in e.items()})
+        for ph, d in e.items():
+            mat = getattr(other, ph).material
+            for f in STATE[mat]:
+                assert d[f] <= 2e-6, (label, ph, f, d[f])
+            for f in RATES[mat]:
+                assert d[f] <= 2e-3, (label, ph, f, d[f])
+    assert np.isfinite(mx.water.rho[:mx.water.n]).all() and np.isfinite(mx.soil.rho[:mx.soil.n]).all()
+
+
+def test_mixed_many_steps_density_error():
+    """100 steps of the jittered slope: density within 1e-4 of FP64 (the
+    learned-step gate is 1%)."""
+    make = CASES["slope_small"]
+    mx = _run(make, abi.SPHG_MIXED, 100)
+    fp = _run(make, abi.SPHG_FP64, 100)
+    for ph in ("water", "soil"):
+        a, b = getattr(mx, ph), getattr(fp, ph)
+        n = b.n
+        rel = np.max(np.abs(a.rho[:n] - b.rho[:n]) / np.abs(b.rho[:n]))
+        ev = field_errors(a, b, ("x", "vx"))
+        print(ph, "max rel rho err", rel, ev)
+        assert rel <= 1e-4, (ph, rel)
+        assert ev["x"] <= 1e-5
+
+
+def test_mixed_rejects_single_phase():
+    sc = scenarios.dam_break(dx=0.02)
+    sc.params.precision = abi.SPHG_MIXED
+    s = Solver(sc.params, sc.water, None)
+    with pytest.raises(abi.SphgError) as ei:
+        s.time_integration(1, sync=True)
+    assert ei.value.code == abi.SPHG_ERR_UNSUPPORTED
+    s.close()
