@@ -26,17 +26,24 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at 1M two-phase SPH; learned-vs-plain speedup at ≤1% ρ err"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per particle of each fused pass, FP64: the
+# Algorithmic HBM bytes per particle of each fused pass: the
 # two-neighbour-stage model (state read once per stage, final state written
 # once, stage-A -> stage-B intermediates written and read once), split by stage.
+# FP64:
 #   water: A = 100 state + 40 intermediates = 140, B = 100 + 40 + 88 = 228 (368 total)
 #   soil:  A = 124 + 48 = 172,                    B = 124 + 48 + 112 = 284 (456 total)
+# mixed (integrated fields x, v, rho, vof, stress, p in FP64, the rest FP32):
+#   water: A = 76 + 20 = 96,   B = 76 + 20 + 68 = 164  (260 total)
+#   soil:  A = 100 + 24 = 124, B = 100 + 24 + 92 = 216 (340 total)
 #   cell list: 28 per particle and step
 ALGO_BYTES = {
-    "water_a": ("water", 140), "water_b": ("water", 228),
-    "soil_a": ("soil", 172), "soil_b": ("soil", 284),
+    "fp64": {"water_a": ("water", 140), "water_b": ("water", 228),
+             "soil_a": ("soil", 172), "soil_b": ("soil", 284)},
+    "mixed": {"water_a": ("water", 96), "water_b": ("water", 164),
+              "soil_a": ("soil", 124), "soil_b": ("soil", 216)},
 }
-STEP_BYTES = {"water": 368 + 28, "soil": 456 + 28}
+STEP_BYTES = {"fp64": {"water": 368 + 28, "soil": 456 + 28},
+              "mixed": {"water": 260 + 28, "soil": 340 + 28}}
 
 
 class Hip:
@@ -162,18 +169,23 @@ def main():
             kern[name] = {"launches": n, "avg_us": 1e3 * t / n, "total_ms_per_step": t / args.prof_steps}
     s.profile(False)
     pairs_w, pairs_s, pairs_x = s.pair_counts()
-    dom = max((k for k in kern if k in ALGO_BYTES), key=lambda k: kern[k]["total_ms_per_step"])
-    ph, bpp = ALGO_BYTES[dom]
+    algo_bytes = ALGO_BYTES[args.precision]
+    dom = max((k for k in kern if k in algo_bytes), key=lambda k: kern[k]["total_ms_per_step"])
+    ph, bpp = algo_bytes[dom]
     npart = nw if ph == "water" else ns
     algo = bpp * npart
     avg_s = kern[dom]["avg_us"] * 1e-6
     achieved = algo / avg_s / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_slope_1m_fp64.json")
-    if os.path.exists(tpath):
+    # HBM bytes per launch of the dominant kernel at this workload, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated by a copy)
+    tpath = os.path.join(ROOT, "profiles", f"traffic_slope_1m_{args.precision}.md")
+    if os.path.exists(tpath) and args.dx == 0.001 and args.scale == 1.0:
         with open(tpath) as f:
-            traffic = json.load(f).get(dom)
-    step_bytes = STEP_BYTES["water"] * nw + STEP_BYTES["soil"] * ns
+            txt = f.read()
+        blk = txt.split("```json", 1)[-1].split("```", 1)[0]
+        traffic = json.loads(blk).get(dom)
+    step_bytes = STEP_BYTES[args.precision]["water"] * nw + STEP_BYTES[args.precision]["soil"] * ns
     out = {
         "metric": METRIC,
         "value": value,
