@@ -92,6 +92,78 @@ def cpu_baseline(scale, steps):
     }
 
 
+def learned_section(args, hip):
+    """Mixed-precision plain step and the learned K-step surrogate with the
+    FP64 plain step as judge, against FP64 plain steps from the same state."""
+    import numpy as np
+    from sources_amd import abi, scenarios
+    from sources_amd.sph import Solver
+    from sources_amd.surrogate import Surrogate
+
+    K, S, M, J = args.learned_k, args.learned_samples, args.learned_advances, args.learned_judge
+    sc = scenarios.slope(dx=args.dx, scale=args.scale, max_interaction=0)
+    sc.params.precision = abi.SPHG_MIXED
+    ntot = sc.water.n + sc.soil.n
+    b = Solver(sc.params, sc.water, sc.soil)
+    b.time_integration(args.warmup, sync=False)
+    hip.sync()
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, b.stream)
+    b.time_integration(args.steps, sync=False)
+    hip.record(e1, b.stream)
+    ms_mixed = hip.elapsed_ms(e0, e1) / args.steps
+
+    g = Surrogate(b, ksteps=K, max_samples=S + 1, seed=1)
+    t0 = time.perf_counter()
+    g.record(S)
+    l0 = g.train(S, lr=0.0)
+    l1 = g.train(args.learned_iters, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    print(f"# surrogate: {S} samples, {args.learned_iters} Adam steps, loss {l0:.3f} -> {l1:.3f} "
+          f"in {t_train:.1f}s", file=sys.stderr, flush=True)
+
+    b.sync_all()
+    ref = sc.copy()
+    ref.params.precision = abi.SPHG_FP64
+    a = Solver(ref.params, ref.water, ref.soil)
+    a.itimestep = b.itimestep
+    hip.sync()
+    hip.record(e0, a.stream)
+    a.time_integration(M * K, sync=False)
+    hip.record(e1, a.stream)
+    ms_plain = hip.elapsed_ms(e0, e1)
+    print(f"# plain FP64: {M * K} steps in {ms_plain:.1f} ms", file=sys.stderr, flush=True)
+    hip.record(e0, b.stream)
+    nfb, jmax, jrms = g.rollout(M, judge_every=J, tol=0.01, use_net=True)
+    hip.record(e1, b.stream)
+    ms_learned = hip.elapsed_ms(e0, e1)
+    print(f"# learned: {M} advances of {K} steps in {ms_learned:.1f} ms, {nfb} fallbacks, "
+          f"judge max rho err {jmax:.2e}", file=sys.stderr, flush=True)
+    a.sync_all()
+    b.sync_all()
+    err = 0.0
+    for pa, pb in ((ref.water, sc.water), (ref.soil, sc.soil)):
+        real = pa.itype[:pa.n] > 0
+        err = max(err, float(np.max(np.abs(pb.rho[:pb.n][real] - pa.rho[:pa.n][real]) / np.abs(pa.rho[:pa.n][real]))))
+    g.close()
+    a.close()
+    b.close()
+    return {
+        "mixed": {"ms_per_step": ms_mixed, "value": ntot / (ms_mixed * 1e-3), "unit": "particle-steps/s",
+                  "speedup_vs_fp64": None},
+        "learned": {
+            "k_steps": K, "advances": M, "judge_every": J, "tol": 0.01,
+            "steps_covered": M * K, "ms_plain_fp64": ms_plain, "ms_learned": ms_learned,
+            "value": ntot * M * K / (ms_learned * 1e-3), "unit": "particle-steps/s",
+            "speedup_vs_plain_fp64": ms_plain / ms_learned,
+            "rho_rel_err_max_vs_fp64": err, "meets_1pct_gate": err <= 0.01, "fallbacks": nfb, "judge_max_rho_err": jmax, "judge_rms_rho_err": jrms,
+            "train": {"samples": S, "iters": args.learned_iters, "loss_before": l0, "loss_after": l1,
+                      "seconds": t_train},
+            "rates_precision": "mixed",
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +174,12 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the mixed and learned-surrogate section")
+    ap.add_argument("--learned-k", type=int, default=4)
+    ap.add_argument("--learned-samples", type=int, default=8)
+    ap.add_argument("--learned-iters", type=int, default=400)
+    ap.add_argument("--learned-advances", type=int, default=16)
+    ap.add_argument("--learned-judge", type=int, default=4)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="fp64: reference arithmetic; mixed: FP32 Kahan pair sums, FP64 integration")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -219,6 +297,10 @@ def main():
         "wall_ms_timed": wall * 1e3,
     }
     s.close()
+    if not args.no_learned:
+        extra = learned_section(args, hip)
+        extra["mixed"]["speedup_vs_fp64"] = ms_step / extra["mixed"]["ms_per_step"]
+        out.update(extra)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(1.0, args.cpu_steps)
     else:

@@ -226,6 +226,48 @@ int sphg_profile_get(sphg_ctx *ctx, const char *name, int64_t *launches, double 
 /* Number of same-phase / cross pairs found by the most recent step. */
 int sphg_pair_counts(sphg_ctx *ctx, int64_t *water_pairs, int64_t *soil_pairs, int64_t *cross_pairs);
 
+/* ---- learned multi-step surrogate ---------------------------------------
+   No counterpart in the reference.  A message-passing network on the
+   same-phase neighbour graph (BF16 MFMA, FP32 accumulation) corrects the
+   constant-rate extrapolation of velocity and density over K leapfrog steps.
+   It is trained on samples that the plain FP64 step generates, and the plain
+   step judges it: sphg_gnn_rollout runs the plain K steps beside every
+   judge_every-th learned advance and keeps the plain result whenever the
+   learned density deviates by more than tol (relative, max over real
+   particles); the judge always runs in FP64, while the rates the network
+   starts from use the context's precision.  Two-phase contexts only
+   (SPHG_ERR_UNSUPPORTED otherwise). */
+typedef struct sphg_gnn sphg_gnn;
+int sphg_gnn_nparams(void);
+/* Network for K = ksteps steps with room for max_samples training samples;
+   weights Xavier-uniform from SplitMix64(seed). */
+int sphg_gnn_create(sphg_ctx *ctx, int ksteps, int max_samples, uint64_t seed, sphg_gnn **out);
+void sphg_gnn_destroy(sphg_gnn *gnn);
+/* Copies the FP32 master weights (sphg_gnn_nparams floats) to (to_host=1) or
+   from (to_host=0) host memory. */
+int sphg_gnn_weights(sphg_gnn *gnn, float *host, int to_host);
+int sphg_gnn_num_samples(const sphg_gnn *gnn);
+/* One training sample: rates and graph of the current state, K plain steps
+   (the state advances), targets = plain result - extrapolation. */
+int sphg_gnn_record(sphg_gnn *gnn);
+/* iters Adam steps (one sample each, cycling); loss_out: mean normalised
+   loss over the last pass. */
+int sphg_gnn_train(sphg_gnn *gnn, int iters, float lr, float *loss_out);
+/* One learned advance of K steps (use_net=0: extrapolation only). */
+int sphg_gnn_step(sphg_gnn *gnn, int use_net);
+/* nlearned learned advances with the plain step as judge and fallback:
+   nfallback judged advances replaced by the plain steps; max_err / rms_err:
+   largest and root-mean-square relative density deviation of the judged
+   learned advances from the plain steps (real particles). */
+int sphg_gnn_rollout(sphg_gnn *gnn, int nlearned, int judge_every, double tol, int use_net, int *nfallback,
+                     double *max_err, double *rms_err);
+/* Nodes of a recorded sample (-1 if out of range). */
+int sphg_gnn_sample_size(const sphg_gnn *gnn, int sample);
+/* Inputs, outputs and weight gradient of one sample (for gradient checks):
+   any pointer may be NULL.  grad is d(sum_w (y - t/out)^2)/dW. */
+int sphg_gnn_debug(sphg_gnn *gnn, int sample, float *nf, float *raw, int *nb, int *nd, float *tgt, float *tw,
+                   float *y, float *grad, float *loss, float *out_scale);
+
 #ifdef __cplusplus
 }
 #endif

@@ -153,6 +153,21 @@ def load_library(path=None):
     lib.sphg_last_hip_error.argtypes = [vp]
     lib.sphg_stream.argtypes = [vp]
     lib.sphg_stream.restype = vp
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int)
+    lib.sphg_gnn_nparams.argtypes = []
+    lib.sphg_gnn_create.argtypes = [vp, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]
+    lib.sphg_gnn_destroy.argtypes = [vp]
+    lib.sphg_gnn_destroy.restype = None
+    lib.sphg_gnn_weights.argtypes = [vp, fp, C.c_int]
+    lib.sphg_gnn_num_samples.argtypes = [vp]
+    lib.sphg_gnn_record.argtypes = [vp]
+    lib.sphg_gnn_train.argtypes = [vp, C.c_int, C.c_float, fp]
+    lib.sphg_gnn_step.argtypes = [vp, C.c_int]
+    lib.sphg_gnn_rollout.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, ip, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+    lib.sphg_gnn_sample_size.argtypes = [vp, C.c_int]
+    lib.sphg_gnn_debug.argtypes = [vp, C.c_int, fp, fp, ip, ip, fp, fp, fp, fp, fp, fp]
     lib.sphg_profile_enable.argtypes = [vp, C.c_int]
     lib.sphg_profile_reset.argtypes = [vp]
     lib.sphg_profile_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
@@ -172,4 +187,7 @@ EXPORTED_SYMBOLS = (
     "sphg_single_step", "sphg_time_integration",
     "sphg_strerror", "sphg_last_hip_error", "sphg_stream",
     "sphg_profile_enable", "sphg_profile_reset", "sphg_profile_get", "sphg_pair_counts",
+    "sphg_gnn_nparams", "sphg_gnn_create", "sphg_gnn_destroy", "sphg_gnn_weights", "sphg_gnn_num_samples",
+    "sphg_gnn_record", "sphg_gnn_train", "sphg_gnn_step", "sphg_gnn_rollout", "sphg_gnn_sample_size",
+    "sphg_gnn_debug",
 )

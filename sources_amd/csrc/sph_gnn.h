@@ -1,0 +1,596 @@
+// sph_gnn.h -- learned multi-step surrogate on the cell-list graph.
+//
+// A message-passing network advances the particles K leapfrog steps at once.
+// Its graph is the same-phase neighbour graph of the SPH step (pairs with
+// r < 2h, at most DEG per node).  It predicts corrections to the constant-
+// rate extrapolation of velocity and density over K dt:
+//     v(t + K dt)   = v + K dt a + c_v
+//     rho(t + K dt) = rho + K dt drho/dt + c_rho
+// from node features (rates, velocity, density, stress) and edge features
+// (relative position / h, velocity, density and rate differences).
+//
+// Network (weights FP32 master copy, BF16 operands, FP32 accumulation):
+//   edge layer  z_ij = e_ij W1            (e_ij: FE features, e_ij[0] = 1 = bias)
+//   message     m_i  = (sum_j relu(z_ij) / 16) W2
+//   node MLP    y_i  = W5 relu(W4 relu(W3 [n_i, m_i] + b3) + b4) + b5
+// Since W2 is linear, the sum over edges is taken before it: the per-edge
+// work is one 32 x 32 x 16 MFMA per neighbour slot for 32 particles.
+//
+// One wavefront handles 32 particles.  Every matrix product is a
+// v_mfma_f32_32x32x16_bf16 tile product whose operands are read through
+// accessors (LDS or global), so forward and backward (training) share one
+// primitive.  MFMA operand layout (gfx950, wave64): lane l holds row l % 32
+// of A and column l % 32 of B, k = (l / 32) * 8 + [0, 8); the 16 results
+// of lane l are C[(v / 4) * 8 + (l / 32) * 4 + v % 4][l % 32], v = 0..15.
+#pragma once
+#include "sph_fused.h"
+
+namespace gnn {
+
+constexpr int FE = 16;    // edge input features
+constexpr int HE = 32;    // edge hidden width = message width
+constexpr int FN = 16;    // node input features
+constexpr int FI = FN + HE;  // node MLP input width
+constexpr int D = 64;     // node hidden width
+constexpr int NO = 4;     // outputs used: c_vx, c_vy, c_rho, (spare)
+constexpr int NOP = 32;   // output tile width
+constexpr int DEG = 48;   // neighbour slots per node
+constexpr int RAW = 12;   // per-node floats feeding the edge features
+constexpr float EDGE_NORM = 1.0f / 16.0f;
+
+// parameter offsets (floats), row-major [in][out]
+constexpr int O_W1 = 0;
+constexpr int O_W2 = O_W1 + FE * HE;
+constexpr int O_W3 = O_W2 + HE * HE;
+constexpr int O_B3 = O_W3 + FI * D;
+constexpr int O_W4 = O_B3 + D;
+constexpr int O_B4 = O_W4 + D * D;
+constexpr int O_W5 = O_B4 + D;
+constexpr int O_B5 = O_W5 + D * NOP;
+constexpr int NPARAM = O_B5 + NOP;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ f32x16 zero16()
+{
+    f32x16 z;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) z[v] = 0.0f;
+    return z;
+}
+
+// acc += A (32 x K) B (K x 32); a(row, k), b(k, col) return FP32 values that
+// are rounded to BF16 operands.  K is a multiple of 16.
+template <class FA, class FB>
+__device__ __forceinline__ f32x16 mma(FA&& a, FB&& b, int K, f32x16 acc)
+{
+    const int l = lane_id(), r = l & 31, h = l >> 5;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x8 av, bv;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            av[i] = (__bf16)a(r, k0 + h * 8 + i);
+            bv[i] = (__bf16)b(k0 + h * 8 + i, r);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// f(row, col, value&) over the 16 results a lane holds.
+template <class F>
+__device__ __forceinline__ void tile_each(f32x16& c, F&& f)
+{
+    const int l = lane_id(), col = l & 31, h = l >> 5;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        float x = c[v];
+        f((v >> 2) * 8 + h * 4 + (v & 3), col, x);
+        c[v] = x;
+    }
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Scales of the features and outputs (set from the first recorded sample).
+struct Scales {
+    float inv_v;     // 1 / velocity scale
+    float inv_a;     // K dt / velocity scale (acceleration -> velocity change)
+    float inv_r;     // 1 / density-deviation scale (relative)
+    float inv_s;     // 1 / stress scale
+    float kdt;       // K dt
+    float out[NO];   // output scales: c = y * out
+};
+
+// One graph: nodes [0, n) = water (cell order) then soil.
+struct Graph {
+    int n;
+    const float* nf;   // [n][FN]
+    const float* raw;  // [n][RAW]
+    const int* nb;     // [n][DEG]
+    const int* nd;     // [n]
+};
+
+// Edge features of node i (raw ri) and neighbour j (raw rj); zero when j < 0.
+__device__ __forceinline__ void edge_features(const float* ri, const float* rj, bool has, float* e)
+{
+#pragma unroll
+    for (int f = 0; f < FE; ++f) e[f] = 0.0f;
+    if (!has) return;
+    const float ih = 1.0f / ri[7];
+    const float dx = ri[0] - rj[0], dy = ri[1] - rj[1];
+    const float r = sqrtf(dx * dx + dy * dy);
+    const float q = fminf(r * ih * 0.5f, 1.0f);
+    e[0] = 1.0f;
+    e[1] = dx * ih;
+    e[2] = dy * ih;
+    e[3] = r * ih;
+    e[4] = (1.0f - q) * (1.0f - q);
+    e[5] = ri[2] - rj[2];
+    e[6] = ri[3] - rj[3];
+    e[7] = ri[4] - rj[4];
+    e[8] = ri[5] - rj[5];
+    e[9] = ri[6] - rj[6];
+    e[10] = rj[8];
+}
+
+struct GnnArgs {
+    Graph G;
+    const float* W;      // [NPARAM]
+    float* out;          // [n][NO] (forward) or nullptr
+    // training
+    const float* tgt;    // [n][NO] targets (normalised)
+    const float* tw;     // [n] loss weight (1 real, 0 virtual)
+    float* gW;           // [NPARAM] gradient accumulator
+    float* loss;         // [1] accumulator of sum w (y - t)^2
+    float inv_count;     // gradient scale: 1 / (number of weighted terms)
+    float inv_out[NO];   // targets are physical corrections; y predicts target * inv_out
+    float cw[NO];        // loss weight of each output
+};
+
+// Forward (TRAIN = false) or forward + backward (TRAIN = true) over 32
+// nodes per workgroup (one wavefront).
+template <bool TRAIN>
+__global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
+{
+    __shared__ float X0[32][FI + 1];
+    __shared__ float H3[32][D + 1];
+    __shared__ float H4[32][D + 1];
+    __shared__ float T[32][D + 1];
+    __shared__ float E[32][FE + 1];
+    __shared__ float HS[32][HE + 1];
+    const int l = lane_id(), p = l & 31, h = l >> 5;
+    const int base = blockIdx.x * 32;
+    const int i = base + p;
+    const bool valid = i < A.G.n;
+    const float* W = A.W;
+    if (h == 0) {
+#pragma unroll
+        for (int f = 0; f < FN; ++f) X0[p][f] = valid ? A.G.nf[(size_t)i * FN + f] : 0.0f;
+    }
+    const int deg = valid ? A.G.nd[i] : 0;
+    const int maxdeg = wave_max(deg);
+    float ri[RAW];
+#pragma unroll
+    for (int f = 0; f < RAW; ++f) ri[f] = valid ? A.G.raw[(size_t)i * RAW + f] : 1.0f;
+    // W1 as the B operand: lane holds column p, rows h*8 .. h*8+7
+    bf16x8 w1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) w1[q] = (__bf16)W[O_W1 + (h * 8 + q) * HE + p];
+
+    auto edge_operand = [&](int s) {
+        const int j = (s < deg) ? A.G.nb[(size_t)i * DEG + s] : -1;
+        float rj[RAW];
+#pragma unroll
+        for (int f = 0; f < RAW; ++f) rj[f] = (j >= 0) ? A.G.raw[(size_t)j * RAW + f] : 0.0f;
+        float e[FE];
+        edge_features(ri, rj, j >= 0, e);
+        bf16x8 av;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) av[q] = (__bf16)(h ? e[8 + q] : e[q]);
+        return av;
+    };
+
+    // ---- edge layer and sum over neighbours
+    f32x16 hsum = zero16();
+#pragma unroll 1
+    for (int s = 0; s < maxdeg; ++s) {
+        const bf16x8 av = edge_operand(s);
+        f32x16 z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, w1, zero16(), 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) hsum[v] += fmaxf(z[v], 0.0f);
+    }
+    tile_each(hsum, [&](int row, int col, float& v) { HS[row][col] = v * EDGE_NORM; });
+    __syncthreads();
+    // ---- message
+    {
+        f32x16 m = mma([&](int r, int k) { return HS[r][k]; }, [&](int k, int c) { return W[O_W2 + k * HE + c]; },
+                       HE, zero16());
+        tile_each(m, [&](int row, int col, float& v) { X0[row][FN + col] = v; });
+    }
+    __syncthreads();
+    // ---- node MLP
+#pragma unroll 1
+    for (int nt = 0; nt < 2; ++nt) {
+        f32x16 z = mma([&](int r, int k) { return X0[r][k]; },
+                       [&](int k, int c) { return W[O_W3 + k * D + nt * 32 + c]; }, FI, zero16());
+        tile_each(z, [&](int row, int col, float& v) {
+            H3[row][nt * 32 + col] = fmaxf(v + W[O_B3 + nt * 32 + col], 0.0f);
+        });
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int nt = 0; nt < 2; ++nt) {
+        f32x16 z = mma([&](int r, int k) { return H3[r][k]; },
+                       [&](int k, int c) { return W[O_W4 + k * D + nt * 32 + c]; }, D, zero16());
+        tile_each(z, [&](int row, int col, float& v) {
+            H4[row][nt * 32 + col] = fmaxf(v + W[O_B4 + nt * 32 + col], 0.0f);
+        });
+    }
+    __syncthreads();
+    f32x16 y = mma([&](int r, int k) { return H4[r][k]; }, [&](int k, int c) { return W[O_W5 + k * NOP + c]; }, D,
+                   zero16());
+    if constexpr (!TRAIN) {
+        tile_each(y, [&](int row, int col, float& v) {
+            if (col < NO && base + row < A.G.n) A.out[(size_t)(base + row) * NO + col] = v + W[O_B5 + col];
+        });
+        return;
+    } else {
+        // ---- loss and dL/dy
+        float lsum = 0.0f;
+        tile_each(y, [&](int row, int col, float& v) {
+            const int node = base + row;
+            float d = 0.0f;
+            if (col < NO && node < A.G.n) {
+                const float wgt = A.tw[node];
+                const float err = v + W[O_B5 + col] - A.tgt[(size_t)node * NO + col] * A.inv_out[col];
+                lsum += A.cw[col] * wgt * err * err;
+                d = 2.0f * A.cw[col] * wgt * err * A.inv_count;
+            }
+            T[row][col] = d;
+        });
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+        if (l == 0) unsafeAtomicAdd(A.loss, lsum);
+        __syncthreads();
+        float* gW = A.gW;
+        // dW5 = H4^T dy, db5 = sum dy
+#pragma unroll 1
+        for (int mt = 0; mt < 2; ++mt) {
+            f32x16 g = mma([&](int r, int k) { return H4[k][mt * 32 + r]; }, [&](int k, int c) { return T[k][c]; },
+                           32, zero16());
+            tile_each(g, [&](int row, int col, float& v) {
+                if (col < NO) unsafeAtomicAdd(&gW[O_W5 + (mt * 32 + row) * NOP + col], v);
+            });
+        }
+        if (l < NO) {
+            float sb = 0.0f;
+            for (int k = 0; k < 32; ++k) sb += T[k][l];
+            unsafeAtomicAdd(&gW[O_B5 + l], sb);
+        }
+        // dz4 = (dy W5^T) * [H4 > 0], into H4
+        f32x16 g4[2];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+            g4[nt] = mma([&](int r, int k) { return T[r][k]; },
+                         [&](int k, int c) { return W[O_W5 + (nt * 32 + c) * NOP + k]; }, 32, zero16());
+        __syncthreads();
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+            tile_each(g4[nt], [&](int row, int col, float& v) {
+                float& hv = H4[row][nt * 32 + col];
+                hv = hv > 0.0f ? v : 0.0f;
+            });
+        __syncthreads();
+        // dW4 = H3^T dz4, db4
+#pragma unroll 1
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll 1
+            for (int nt = 0; nt < 2; ++nt) {
+                f32x16 g = mma([&](int r, int k) { return H3[k][mt * 32 + r]; },
+                               [&](int k, int c) { return H4[k][nt * 32 + c]; }, 32, zero16());
+                tile_each(g, [&](int row, int col, float& v) {
+                    unsafeAtomicAdd(&gW[O_W4 + (mt * 32 + row) * D + nt * 32 + col], v);
+                });
+            }
+        {
+            float sb = 0.0f;
+            for (int k = 0; k < 32; ++k) sb += H4[k][l];
+            unsafeAtomicAdd(&gW[O_B4 + l], sb);
+        }
+        // dz3 = (dz4 W4^T) * [H3 > 0], into T
+#pragma unroll 1
+        for (int nt = 0; nt < 2; ++nt) {
+            f32x16 g = mma([&](int r, int k) { return H4[r][k]; },
+                           [&](int k, int c) { return W[O_W4 + (nt * 32 + c) * D + k]; }, D, zero16());
+            tile_each(g, [&](int row, int col, float& v) {
+                T[row][nt * 32 + col] = H3[row][nt * 32 + col] > 0.0f ? v : 0.0f;
+            });
+        }
+        __syncthreads();
+        // dW3 = X0^T dz3 (rows 0..FI-1), db3
+#pragma unroll 1
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll 1
+            for (int nt = 0; nt < 2; ++nt) {
+                f32x16 g = mma([&](int r, int k) { const int f = mt * 32 + r; return f < FI ? X0[k][f] : 0.0f; },
+                               [&](int k, int c) { return T[k][nt * 32 + c]; }, 32, zero16());
+                tile_each(g, [&](int row, int col, float& v) {
+                    const int f = mt * 32 + row;
+                    if (f < FI) unsafeAtomicAdd(&gW[O_W3 + f * D + nt * 32 + col], v);
+                });
+            }
+        {
+            float sb = 0.0f;
+            for (int k = 0; k < 32; ++k) sb += T[k][l];
+            unsafeAtomicAdd(&gW[O_B3 + l], sb);
+        }
+        // dm = dz3 W3[FN:, :]^T, into H3 (32 x HE)
+        {
+            f32x16 g = mma([&](int r, int k) { return T[r][k]; },
+                           [&](int k, int c) { return W[O_W3 + (FN + c) * D + k]; }, D, zero16());
+            __syncthreads();
+            tile_each(g, [&](int row, int col, float& v) { H3[row][col] = v; });
+        }
+        __syncthreads();
+        // dW2 = HS^T dm
+        {
+            f32x16 g = mma([&](int r, int k) { return HS[k][r]; }, [&](int k, int c) { return H3[k][c]; }, 32,
+                           zero16());
+            tile_each(g, [&](int row, int col, float& v) { unsafeAtomicAdd(&gW[O_W2 + row * HE + col], v); });
+        }
+        // d hsum = (dm W2^T) / 16, in the layout of hsum
+        f32x16 dh = mma([&](int r, int k) { return H3[r][k]; }, [&](int k, int c) { return W[O_W2 + c * HE + k]; },
+                        HE, zero16());
+#pragma unroll
+        for (int v = 0; v < 16; ++v) dh[v] *= EDGE_NORM;
+        // edge layer: dW1 = sum_s e_s^T (dh * [z_s > 0])
+        f32x16 g1 = zero16();
+#pragma unroll 1
+        for (int s = 0; s < maxdeg; ++s) {
+            const bf16x8 av = edge_operand(s);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) E[p][h * 8 + q] = (float)av[q];
+            f32x16 z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, w1, zero16(), 0, 0, 0);
+            int v = 0;
+            tile_each(z, [&](int row, int col, float& zv) {
+                T[row][col] = zv > 0.0f ? dh[v] : 0.0f;
+                ++v;
+            });
+            __syncthreads();
+            g1 = mma([&](int r, int k) { return r < FE ? E[k][r] : 0.0f; }, [&](int k, int c) { return T[k][c]; }, 32,
+                     g1);
+            __syncthreads();
+        }
+        tile_each(g1, [&](int row, int col, float& v) {
+            if (row < FE) unsafeAtomicAdd(&gW[O_W1 + row * HE + col], v);
+        });
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights.
+__global__ void k_adam(float* W, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, float clip)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = fminf(fmaxf(g[i], -clip), clip);
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    W[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// ------------------------------------------------------------------------
+// Graph and features of one phase, in cell order: node = off + t for the
+// particle a = order[t].  Neighbours: same-phase pairs with r < 2h (the
+// exact predicate of the step), at most DEG (nd counts the kept ones).
+// base[node] holds the constant-rate extrapolation the network corrects.
+// ------------------------------------------------------------------------
+struct GatherOut {
+    float* nf;
+    float* raw;
+    int* nb;
+    int* nd;
+    float* tw;         // [n]
+    int* node_of_id;   // [n of the phase] (phase-local ids)
+    int* overflow;     // [1]
+};
+
+// base[node] (FP64): vx + K dt ax, vy + K dt ay, rho + K dt drho, 0.
+__global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, int off, int phase, Scales S,
+                             GatherOut O, double* base)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const int a = P.order[t];
+    const int node = off + t;
+    const double x = P.x[a], y = P.y[a], h = P.hsml[a];
+    const double rho0 = phase == 0 ? K.w_rho0 : K.s_rho0;
+    const bool virt = P.itype[a] < 0;
+    const double kdt = S.kdt;
+    float* nf = O.nf + (size_t)node * FN;
+    nf[0] = 1.0f;
+    nf[1] = (float)phase;
+    nf[2] = virt ? 1.0f : 0.0f;
+    nf[3] = (float)(P.ax[a] * S.inv_a);
+    nf[4] = (float)(P.ay[a] * S.inv_a);
+    nf[5] = (float)(P.vx[a] * S.inv_v);
+    nf[6] = (float)(P.vy[a] * S.inv_v);
+    nf[7] = (float)((P.rho[a] - rho0) / rho0 * S.inv_r);
+    nf[8] = (float)(P.drho[a] * kdt / rho0 * S.inv_r);
+    nf[9] = (float)P.vof[a];
+    if (phase == 1) {
+        nf[10] = (float)(P.sxx[a] * S.inv_s);
+        nf[11] = (float)(P.sxy[a] * S.inv_s);
+        nf[12] = (float)(P.syy[a] * S.inv_s);
+        nf[13] = (float)(P.p[a] * S.inv_s);
+    } else {
+        nf[10] = nf[11] = nf[12] = nf[13] = 0.0f;
+    }
+    nf[14] = nf[15] = 0.0f;
+    float* rw = O.raw + (size_t)node * RAW;
+    rw[0] = (float)x;
+    rw[1] = (float)y;
+    rw[2] = nf[5];
+    rw[3] = nf[6];
+    rw[4] = nf[7];
+    rw[5] = nf[3];
+    rw[6] = nf[4];
+    rw[7] = (float)h;
+    rw[8] = nf[2];
+    rw[9] = rw[10] = rw[11] = 0.0f;
+    double* bs = base + (size_t)node * NO;
+    bs[0] = P.vx[a] + kdt * P.ax[a];
+    bs[1] = P.vy[a] + kdt * P.ay[a];
+    bs[2] = P.rho[a] + kdt * P.drho[a];
+    bs[3] = 0.0;
+    O.tw[node] = virt ? 0.0f : 1.0f;
+    O.node_of_id[P.id[a]] = node;
+    // neighbours (cell order of the same phase)
+    const Rows R = block_rows(g, P.cell_start, x, y);
+    int cnt = 0;
+    int* nb = O.nb + (size_t)node * DEG;
+#pragma unroll 1
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll 1
+        for (int k = R.k0[r]; k < R.k1[r]; ++k) {
+            if (k == t) continue;
+            const int b = P.order[k];
+            const double dx = x - P.x[b], dy = y - P.y[b];
+            double dr;
+            {
+#pragma clang fp contract(off)
+                dr = dx * dx + dy * dy;
+            }
+            if (!pred(uni != 0, thr, dr, h, P.hsml[b], K)) continue;
+            if (cnt < DEG) nb[cnt] = off + k;
+            ++cnt;
+        }
+    }
+    if (cnt > DEG) atomicOr(O.overflow, 1);
+    O.nd[node] = min(cnt, DEG);
+    for (int s = min(cnt, DEG); s < DEG; ++s) nb[s] = -1;
+}
+
+// Targets after K plain steps: the corrections state - base (physical
+// units), by particle id.
+__global__ void k_gnn_target(DPhase P, const int* node_of_id, const double* base, float* tgt)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    const int node = node_of_id[P.id[a]];
+    const double* bs = base + (size_t)node * NO;
+    float* tg = tgt + (size_t)node * NO;
+    tg[0] = (float)(P.vx[a] - bs[0]);
+    tg[1] = (float)(P.vy[a] - bs[1]);
+    tg[2] = (float)(P.rho[a] - bs[2]);
+    tg[3] = 0.0f;
+}
+
+// Learned K-step update of one phase (real particles): FP64 state, the
+// network's corrections added to the constant-rate extrapolation.
+__global__ void k_gnn_apply(DPhase P, KParams K, const int* node_of_id, const float* y, Scales S, int phase,
+                            int use_net)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    if (P.itype[a] <= 0) return;
+    const double kdt = S.kdt;
+    const int node = node_of_id[P.id[a]];
+    const float* yy = y + (size_t)node * NO;
+    const double cvx = use_net ? (double)yy[0] * S.out[0] : 0.0;
+    const double cvy = use_net ? (double)yy[1] * S.out[1] : 0.0;
+    const double crh = use_net ? (double)yy[2] * S.out[2] : 0.0;
+    const double vx0 = P.vx[a], vy0 = P.vy[a];
+    const double vx1 = vx0 + kdt * P.ax[a] + cvx;
+    const double vy1 = vy0 + kdt * P.ay[a] + cvy;
+    P.vx[a] = vx1;
+    P.vy[a] = vy1;
+    P.x[a] = P.x[a] + kdt * 0.5 * (vx0 + vx1);
+    P.y[a] = P.y[a] + kdt * 0.5 * (vy0 + vy1);
+    P.rho[a] = P.rho[a] + kdt * P.drho[a] + crh;
+    if (phase == 0) {
+        if (K.volume_fraction) P.vof[a] = P.vof[a] + kdt * P.dvof[a];
+    } else {
+        double p = P.p[a] + kdt * P.dp[a];
+        double sxx = P.sxx[a] + kdt * P.dsxx[a], sxy = P.sxy[a] + kdt * P.dsxy[a], syy = P.syy[a] + kdt * P.dsyy[a];
+        P.fail[a] = dp_project(K.s_alpha1, p, sxx, sxy, syy);
+        P.p[a] = p; P.sxx[a] = sxx; P.sxy[a] = sxy; P.syy[a] = syy;
+        P.vof[a] = P.rho[a] / K.s_rho0;
+    }
+}
+
+// Largest relative density deviation |rho_a - rho_b| / rho_b over real particles
+// (b: the reference state, addressed by id through node_of_id maps).
+// out[0]: max (as float bits: non-negative floats order as their bits);
+// sq[0] += sum of squares, sq[1] += count.  One atomic per block.
+__global__ __launch_bounds__(256) void k_rho_err(DPhase A, DPhase B, const int* slot_of_id_b, unsigned int* out,
+                                                 double* sq)
+{
+    __shared__ float smax[4];
+    __shared__ double ssum[4], scnt[4];
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    float e = 0.0f;
+    double e2 = 0.0, c = 0.0;
+    if (a < A.n && A.itype[a] > 0) {
+        const int b = slot_of_id_b[A.id[a]];
+        const double ed = fabs(A.rho[a] - B.rho[b]) / fabs(B.rho[b]);
+        e = (float)ed;
+        e2 = ed * ed;
+        c = 1.0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        e = fmaxf(e, __shfl_xor(e, o));
+        e2 += __shfl_xor(e2, o);
+        c += __shfl_xor(c, o);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        smax[w] = e;
+        ssum[w] = e2;
+        scnt[w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 1; i < nw; ++i) {
+            e = fmaxf(e, smax[i]);
+            e2 += ssum[i];
+            c += scnt[i];
+        }
+        atomicMax(out, __float_as_uint(e));
+        unsafeAtomicAdd(&sq[0], e2);
+        unsafeAtomicAdd(&sq[1], c);
+    }
+}
+
+// Flags a state a learned advance has broken: a non-finite or non-positive
+// density, a non-finite velocity, or a particle outside the cell grid.
+__global__ void k_sanity(DPhase P, Grid g, int* flag)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    const double x = P.x[a], y = P.y[a], r = P.rho[a];
+    const bool ok = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1 && r > 0.0 && r < 1.0e300 &&
+                    P.vx[a] == P.vx[a] && P.vy[a] == P.vy[a];
+    if (!ok) atomicOr(flag, 1);
+}
+
+__global__ void k_slot_of_id(DPhase P, int* slot_of_id)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < P.n) slot_of_id[P.id[a]] = a;
+}
+
+}  // namespace gnn
