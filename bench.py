@@ -164,6 +164,47 @@ def learned_section(args, hip):
     }
 
 
+def sweep_section(args, hip):
+    """64 independent submerged-column cases of uneven size (one seed):
+    eager per-context launches against the batched hipGraph replays."""
+    from sources_amd import scenarios
+    from sources_amd.batch import Batch
+    from sources_amd.sph import Solver
+
+    cases = scenarios.sweep(args.sweep_cases, seed=11)
+    ntot = sum(c.water.n + c.soil.n for c in cases)
+    ss = [Solver(c.params, c.water, c.soil) for c in cases]
+    steps = 32
+    for s in ss:
+        s.time_integration(2, sync=False)
+    hip.sync()
+    t0 = time.perf_counter()
+    for s in ss:
+        s.time_integration(steps, sync=False)
+    hip.sync()
+    ms_eager = (time.perf_counter() - t0) * 1e3 / steps
+    b = Batch(ss)
+    ns, _ = b.tune(trial_periods=1)
+    b.run(16)
+    t0 = time.perf_counter()
+    b.run(steps)
+    ms_graph = (time.perf_counter() - t0) * 1e3 / steps
+    print(f"# sweep: {len(cases)} cases, {ntot} particles: eager {ms_eager:.2f} ms/step, "
+          f"batched {ms_graph:.2f} ms/step (tuned: {ns or 'eager'} streams)", file=sys.stderr, flush=True)
+    b.close()
+    for s in ss:
+        s.close()
+    return {"sweep": {
+        "cases": len(cases), "particles": ntot, "steps": steps, "streams": ns,
+        "ms_per_step_eager": ms_eager, "ms_per_step_batched": ms_graph,
+        "batched_mode": "eager" if ns == 0 else f"hipGraph, {ns} streams",
+        "value_batched": ntot / (ms_graph * 1e-3), "unit": "particle-steps/s",
+        "speedup_batched_vs_eager": ms_eager / ms_graph,
+        "workload": "submerged column collapses, N ~ logU[5k,120k], aspect ~ U[0.5,2], "
+                    "phi ~ U[25,40] deg, k_s ~ logU[1e-4,1e-2], seed 11, FP64",
+    }}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +216,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the mixed and learned-surrogate section")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched scenario sweep section")
+    ap.add_argument("--sweep-cases", type=int, default=64)
     ap.add_argument("--learned-k", type=int, default=4)
     ap.add_argument("--learned-samples", type=int, default=8)
     ap.add_argument("--learned-iters", type=int, default=400)
@@ -301,6 +344,8 @@ def main():
         extra = learned_section(args, hip)
         extra["mixed"]["speedup_vs_fp64"] = ms_step / extra["mixed"]["ms_per_step"]
         out.update(extra)
+    if not args.no_sweep:
+        out.update(sweep_section(args, hip))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(1.0, args.cpu_steps)
     else:

@@ -268,6 +268,28 @@ int sphg_gnn_sample_size(const sphg_gnn *gnn, int sample);
 int sphg_gnn_debug(sphg_gnn *gnn, int sample, float *nf, float *raw, int *nb, int *nd, float *tgt, float *tw,
                    float *y, float *grad, float *loss, float *out_scale);
 
+/* ---- batched scenarios -------------------------------------------------
+   No counterpart in the reference (one case per process there).  Many
+   independent two-phase contexts advance together: each reorder period
+   (16 steps) of all of them is captured once into a hipGraph, with the
+   contexts dealt largest-first over a few streams, and replayed.  Results are
+   bit-identical to advancing each context alone.  While batched, a context
+   keeps its device buffers in place (graph mode); use it only through the
+   batch until sphg_batch_destroy.  Per-launch profiling must be off. */
+typedef struct sphg_batch sphg_batch;
+int sphg_batch_create(sphg_ctx **ctxs, int n, sphg_batch **out);
+void sphg_batch_destroy(sphg_batch *batch);
+/* nsteps leapfrog steps of every context (eager up to the next period
+   boundary, graph replays, eager remainder); checks every context's flags. */
+int sphg_batch_run(sphg_batch *batch, int nsteps);
+/* Number of streams the contexts are dealt over (drops the captured graph). */
+int sphg_batch_set_streams(sphg_batch *batch, int nstreams);
+/* Autotuner: times trial_periods replays with 1, 2, 4, ... streams (up to
+   one per context) and the same steps launched eagerly on the contexts' own
+   streams, and keeps the fastest (best_streams = 0: eager).  The contexts
+   advance during tuning; numerics do not depend on the choice. */
+int sphg_batch_tune(sphg_batch *batch, int trial_periods, int *best_streams, double *best_ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

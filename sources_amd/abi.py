@@ -168,6 +168,12 @@ def load_library(path=None):
                                      C.POINTER(C.c_double)]
     lib.sphg_gnn_sample_size.argtypes = [vp, C.c_int]
     lib.sphg_gnn_debug.argtypes = [vp, C.c_int, fp, fp, ip, ip, fp, fp, fp, fp, fp, fp]
+    lib.sphg_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    lib.sphg_batch_destroy.argtypes = [vp]
+    lib.sphg_batch_destroy.restype = None
+    lib.sphg_batch_run.argtypes = [vp, C.c_int]
+    lib.sphg_batch_set_streams.argtypes = [vp, C.c_int]
+    lib.sphg_batch_tune.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_double)]
     lib.sphg_profile_enable.argtypes = [vp, C.c_int]
     lib.sphg_profile_reset.argtypes = [vp]
     lib.sphg_profile_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
@@ -190,4 +196,5 @@ EXPORTED_SYMBOLS = (
     "sphg_gnn_nparams", "sphg_gnn_create", "sphg_gnn_destroy", "sphg_gnn_weights", "sphg_gnn_num_samples",
     "sphg_gnn_record", "sphg_gnn_train", "sphg_gnn_step", "sphg_gnn_rollout", "sphg_gnn_sample_size",
     "sphg_gnn_debug",
+    "sphg_batch_create", "sphg_batch_destroy", "sphg_batch_run", "sphg_batch_set_streams", "sphg_batch_tune",
 )

@@ -93,6 +93,14 @@ __global__ void k_iota(int* a, int n)
     if (i < n) a[i] = i;
 }
 
+// dst = src (element-wise copy).
+template <class T>
+__global__ void k_copy(const T* src, T* dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 // dst[t] = src[idx[t]]: permutation into cell order, and host -> device
 // transfer by reference particle number.
 template <class T>

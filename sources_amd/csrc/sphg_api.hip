@@ -3,6 +3,7 @@
 // host<->device sync and per-kernel event timing.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -83,6 +84,10 @@ struct sphg_ctx {
     bool legacy = false;        // SPHG_LEGACY=1: unfused reference-order kernels
     int reorder_every = 16;     // steps between permutations of the state into cell order
     int since_reorder = 0;
+    // Graph mode: the step keeps every device pointer of the state where it
+    // is (the reorder copies back instead of swapping buffers), so that a
+    // captured sequence of steps can be replayed.
+    bool stable_ptrs = false;
 };
 
 namespace {
@@ -224,12 +229,18 @@ int reorder(sphg_ctx* c, int k)
     const int* ord = d.order;
     auto pd = [&](double*& arr) -> int {
         RC(launch(c, "permute", k_gather<double>, nblk(b.n), TPB, (const double*)arr, b.spare_d, ord, b.n));
-        std::swap(arr, b.spare_d);
+        if (c->stable_ptrs)
+            RC(launch(c, "permute", k_copy<double>, nblk(b.n), TPB, (const double*)b.spare_d, arr, b.n));
+        else
+            std::swap(arr, b.spare_d);
         return SPHG_OK;
     };
     auto pi = [&](int*& arr) -> int {
         RC(launch(c, "permute", k_gather<int>, nblk(b.n), TPB, (const int*)arr, b.spare_i, ord, b.n));
-        std::swap(arr, b.spare_i);
+        if (c->stable_ptrs)
+            RC(launch(c, "permute", k_copy<int>, nblk(b.n), TPB, (const int*)b.spare_i, arr, b.n));
+        else
+            std::swap(arr, b.spare_i);
         return SPHG_OK;
     };
     RC(pi(d.id)); RC(pi(d.itype)); RC(pi(d.fail));
@@ -1738,6 +1749,264 @@ int sphg_gnn_debug(sphg_gnn* g, int sample, float* nf, float* raw, int* nb, int*
     RC(dl(loss, g->loss, sizeof(float)));
     if (out_scale) for (int o = 0; o < gnn::NO; ++o) out_scale[o] = g->S.out[o];
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SPHG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched scenarios: many independent contexts advanced by one hipGraph.
+// (No counterpart in the reference: its host runs one case per process.)
+// ---------------------------------------------------------------------------
+struct sphg_batch {
+    std::vector<sphg_ctx*> ctx;
+    std::vector<hipStream_t> streams;   // contexts are dealt round-robin, largest first
+    std::vector<int> order;             // contexts by decreasing particle count
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int nstreams = 0;
+    int period = 0;                     // steps per graph replay
+    bool eager = false;                 // autotuner's choice: per-context launches, no graph
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+};
+
+namespace {
+
+void batch_release_graph(sphg_batch* b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int batch_set_streams(sphg_batch* b, int nstreams)
+{
+    batch_release_graph(b);
+    for (hipStream_t s : b->streams) hipStreamDestroy(s);
+    for (hipEvent_t e : b->join) hipEventDestroy(e);
+    b->streams.clear();
+    b->join.clear();
+    b->nstreams = std::max(1, std::min(nstreams, (int)b->ctx.size()));
+    for (int i = 0; i < b->nstreams; ++i) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return SPHG_ERR_HIP;
+        b->streams.push_back(s);
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return SPHG_ERR_HIP;
+        b->join.push_back(e);
+    }
+    return SPHG_OK;
+}
+
+// Eager steps of one context until its next step starts a reorder period.
+int batch_align(sphg_ctx* c, int* left)
+{
+    while (*left > 0 && (c->itimestep < 1 || c->since_reorder != c->reorder_every - 1)) {
+        RC(one_step(c));
+        --*left;
+    }
+    return SPHG_OK;
+}
+
+// Capture `period` steps of every context, one stream per group of contexts.
+int batch_capture(sphg_batch* b)
+{
+    batch_release_graph(b);
+    hipStream_t origin = b->streams[0];
+    if (hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPHG_ERR_HIP;
+    int rc = SPHG_OK;
+    if (hipEventRecord(b->fork, origin) != hipSuccess) rc = SPHG_ERR_HIP;
+    for (int i = 1; i < b->nstreams && rc == SPHG_OK; ++i)
+        if (hipStreamWaitEvent(b->streams[i], b->fork, 0) != hipSuccess) rc = SPHG_ERR_HIP;
+    for (size_t r = 0; r < b->order.size() && rc == SPHG_OK; ++r) {
+        sphg_ctx* c = b->ctx[(size_t)b->order[r]];
+        hipStream_t own = c->stream;
+        const int it0 = c->itimestep;
+        c->stream = b->streams[r % (size_t)b->nstreams];
+        for (int s = 0; s < b->period && rc == SPHG_OK; ++s) rc = one_step(c);
+        c->stream = own;
+        c->itimestep = it0;   // the capture issued no work: the step count advances on replay
+    }
+    for (int i = 1; i < b->nstreams && rc == SPHG_OK; ++i) {
+        if (hipEventRecord(b->join[(size_t)i], b->streams[(size_t)i]) != hipSuccess ||
+            hipStreamWaitEvent(origin, b->join[(size_t)i], 0) != hipSuccess)
+            rc = SPHG_ERR_HIP;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(origin, &g);
+    if (rc != SPHG_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return SPHG_ERR_HIP;
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) return SPHG_ERR_HIP;
+    return SPHG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sphg_batch_create(sphg_ctx** ctxs, int n, sphg_batch** out)
+{
+    if (!ctxs || n < 1 || !out) return SPHG_ERR_INVALID;
+    sphg_batch* b = new sphg_batch();
+    for (int i = 0; i < n; ++i) {
+        sphg_ctx* c = ctxs[i];
+        if (!c || c->legacy || c->P.single_phase || c->prof) {
+            delete b;
+            return c ? SPHG_ERR_UNSUPPORTED : SPHG_ERR_INVALID;
+        }
+        const int rc = check_fused_switches(c);
+        if (rc != SPHG_OK) {
+            delete b;
+            return rc;
+        }
+        b->ctx.push_back(c);
+    }
+    b->order.resize((size_t)n);
+    for (int i = 0; i < n; ++i) b->order[(size_t)i] = i;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+        return b->ctx[(size_t)x]->ph[0].n + b->ctx[(size_t)x]->ph[1].n >
+               b->ctx[(size_t)y]->ph[0].n + b->ctx[(size_t)y]->ph[1].n;
+    });
+    b->period = b->ctx[0]->reorder_every;
+    for (sphg_ctx* c : b->ctx) {
+        if (c->reorder_every != b->period) {
+            delete b;
+            return SPHG_ERR_UNSUPPORTED;
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->stable_ptrs = true;
+    }
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return SPHG_ERR_HIP;
+    }
+    const int rc = batch_set_streams(b, 4);
+    if (rc != SPHG_OK) {
+        sphg_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return SPHG_OK;
+}
+
+void sphg_batch_destroy(sphg_batch* b)
+{
+    if (!b) return;
+    for (hipStream_t s : b->streams) hipStreamSynchronize(s);
+    batch_release_graph(b);
+    for (hipStream_t s : b->streams) hipStreamDestroy(s);
+    for (hipEvent_t e : b->join) hipEventDestroy(e);
+    if (b->fork) hipEventDestroy(b->fork);
+    for (sphg_ctx* c : b->ctx) c->stable_ptrs = false;
+    delete b;
+}
+
+int sphg_batch_set_streams(sphg_batch* b, int nstreams)
+{
+    if (!b || nstreams < 1) return SPHG_ERR_INVALID;
+    return batch_set_streams(b, nstreams);
+}
+
+int sphg_batch_run(sphg_batch* b, int nsteps)
+{
+    if (!b || nsteps < 0) return SPHG_ERR_INVALID;
+    if (b->eager) {
+        // step-major over the contexts' own streams
+        for (int s = 0; s < nsteps; ++s)
+            for (sphg_ctx* c : b->ctx) RC(one_step(c));
+        for (sphg_ctx* c : b->ctx) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            RC(check_flags(c));
+        }
+        return SPHG_OK;
+    }
+    std::vector<int> left(b->ctx.size(), nsteps);
+    // eager steps up to the start of a reorder period
+    for (size_t i = 0; i < b->ctx.size(); ++i) RC(batch_align(b->ctx[i], &left[i]));
+    int reps = nsteps;
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        sphg_ctx* c = b->ctx[i];
+        const bool aligned = c->itimestep >= 1 && c->since_reorder == c->reorder_every - 1;
+        reps = std::min(reps, aligned ? left[i] / b->period : 0);
+    }
+    if (reps > 0) {
+        for (sphg_ctx* c : b->ctx) HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (!b->exec) RC(batch_capture(b));
+        for (int r = 0; r < reps; ++r)
+            if (hipGraphLaunch(b->exec, b->streams[0]) != hipSuccess) return SPHG_ERR_HIP;
+        if (hipStreamSynchronize(b->streams[0]) != hipSuccess) return SPHG_ERR_HIP;
+        for (size_t i = 0; i < b->ctx.size(); ++i) {
+            b->ctx[i]->itimestep += reps * b->period;
+            left[i] -= reps * b->period;
+        }
+    }
+    // the remainder eagerly
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        sphg_ctx* c = b->ctx[i];
+        for (int s = 0; s < left[i]; ++s) RC(one_step(c));
+    }
+    for (sphg_ctx* c : b->ctx) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        RC(check_flags(c));
+    }
+    return SPHG_OK;
+}
+
+// Autotuner: times trial_periods graph periods for 1, 2, 4, ... streams (up
+// to one per context) and the same steps launched eagerly on the contexts'
+// own streams, and keeps the fastest.  Numerics do not depend on the choice.
+int sphg_batch_tune(sphg_batch* b, int trial_periods, int* best_streams, double* best_ms_per_step)
+{
+    if (!b || trial_periods < 1) return SPHG_ERR_INVALID;
+    int best = 0;   // 0: eager
+    double best_ms = 1e300;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return SPHG_ERR_HIP;
+    int rc = SPHG_OK;
+    const int nctx = (int)b->ctx.size();
+    // eager candidate
+    b->eager = true;
+    if ((rc = sphg_batch_run(b, 1)) == SPHG_OK) {
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = sphg_batch_run(b, trial_periods * b->period);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        best_ms = ms / (trial_periods * b->period);
+    }
+    b->eager = false;
+    for (int ns = 1; ns <= nctx && rc == SPHG_OK; ns *= 2) {
+        if ((rc = batch_set_streams(b, ns)) != SPHG_OK) break;
+        std::vector<int> left(b->ctx.size(), 1 << 30);
+        for (size_t i = 0; i < b->ctx.size() && rc == SPHG_OK; ++i) rc = batch_align(b->ctx[i], &left[i]);
+        if (rc != SPHG_OK) break;
+        for (sphg_ctx* c : b->ctx) hipStreamSynchronize(c->stream);
+        if ((rc = batch_capture(b)) != SPHG_OK) break;
+        hipGraphLaunch(b->exec, b->streams[0]);   // warm
+        hipEventRecord(e0, b->streams[0]);
+        for (int r = 0; r < trial_periods; ++r) hipGraphLaunch(b->exec, b->streams[0]);
+        hipEventRecord(e1, b->streams[0]);
+        hipEventSynchronize(e1);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        for (sphg_ctx* c : b->ctx) c->itimestep += (trial_periods + 1) * b->period;
+        const double per = ms / (trial_periods * b->period);
+        if (per < best_ms) {
+            best_ms = per;
+            best = ns;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    RC(rc);
+    b->eager = best == 0;
+    if (!b->eager) RC(batch_set_streams(b, best));
+    if (best_streams) *best_streams = best;
+    if (best_ms_per_step) *best_ms_per_step = best_ms;
     return SPHG_OK;
 }
 

@@ -318,3 +318,28 @@ def slope(dx=0.001, scale=1.0, layers=3, jitter=0.1, seed=2, skf=1, porosity=0.4
     _soil_fields(so, P, p0, 1.0 - porosity)
     return Scenario("slope", P, wa, so, nsteps=1000,
                     notes=f"water {nw}+{nv} virt, soil {ns}+{nv} virt, dx={dx}, scale={scale}")
+
+
+def sweep(ncases=64, seed=11, nmin=5000, nmax=120000, jitter=0.05):
+    """A scenario sweep of submerged granular column collapses with uneven
+    sizes and materials, all drawn from one seed (SplitMix64):
+    total particles ~ logU[nmin, nmax], column aspect (width / height)
+    ~ U[0.5, 2], friction angle ~ U[25, 40] degrees, permeability
+    ~ logU[1e-4, 1e-2] m/s.  The tank is three column widths wide and 1.5
+    column heights tall; dx follows from the particle target."""
+    u = uniform(seed, 4 * ncases, 7)
+    cases = []
+    for i in range(ncases):
+        a, b, c, d = u[4 * i:4 * i + 4]
+        n_target = math.exp(math.log(nmin) + a * (math.log(nmax) - math.log(nmin)))
+        aspect = 0.5 + 1.5 * b
+        phi = 25.0 + 15.0 * c
+        perm = math.exp(math.log(1e-4) + d * (math.log(1e-2) - math.log(1e-4)))
+        soil_h = 0.1
+        soil_w = aspect * soil_h
+        tank_w, tank_h = 3.0 * soil_w, 1.5 * soil_h
+        dx = math.sqrt((tank_w * tank_h + soil_w * soil_h) / n_target)
+        cases.append(submerged_column(dx=dx, soil_w=soil_w, soil_h=soil_h, tank_w=tank_w, tank_h=tank_h,
+                                      jitter=jitter, seed=seed * 1000 + i, phi_deg=phi, permeability=perm,
+                                      max_interaction=0))
+    return cases
