@@ -164,6 +164,66 @@ def learned_section(args, hip):
     }
 
 
+def mixed_error_section(args):
+    """Mixed precision against FP64 on the 1M slope from the same state:
+    norm-wise errors (max|mixed - fp64| / max|fp64|) of density, velocity and
+    stress, and the largest relative density error, after 1 and N steps."""
+    import numpy as np
+    from sources_amd import abi, scenarios
+    from sources_amd.sph import Solver
+
+    sf = scenarios.slope(dx=args.dx, scale=args.scale, max_interaction=0)
+    sm = sf.copy()
+    sm.params.precision = abi.SPHG_MIXED
+    a = Solver(sf.params, sf.water, sf.soil)
+    b = Solver(sm.params, sm.water, sm.soil)
+    out = {}
+    done = 0
+    for n in (1, args.mixed_error_steps):
+        a.time_integration(n - done, sync=True)
+        b.time_integration(n - done, sync=True)
+        done = n
+        e = {}
+        for f, phases in (("rho", ("water", "soil")), ("vx", ("water", "soil")), ("sxx", ("soil",)),
+                          ("sxy", ("soil",)), ("syy", ("soil",)), ("p", ("water", "soil"))):
+            for ph in phases:
+                x, y = getattr(getattr(sm, ph), f), getattr(getattr(sf, ph), f)
+                nn = getattr(sf, ph).n
+                e[f"{ph}_{f}"] = float(np.max(np.abs(x[:nn] - y[:nn])) / max(np.max(np.abs(y[:nn])), 1e-300))
+        rr = 0.0
+        for ph in ("water", "soil"):
+            pf, pm = getattr(sf, ph), getattr(sm, ph)
+            real = pf.itype[:pf.n] > 0
+            rr = max(rr, float(np.max(np.abs(pm.rho[:pm.n][real] - pf.rho[:pf.n][real]) / np.abs(pf.rho[:pf.n][real]))))
+        e["rho_max_rel"] = rr
+        out[f"after_{n}_steps"] = e
+        print(f"# mixed vs fp64 after {n} steps: rho max rel {rr:.2e}", file=sys.stderr, flush=True)
+    a.close()
+    b.close()
+    return {"mixed_error_vs_fp64": out}
+
+
+def pcie_section(args, hip):
+    """Drop-in (host-buffer) rate: one step per call with the whole state
+    uploaded before and downloaded after, as at the reference call site."""
+    from sources_amd import scenarios
+    from sources_amd.sph import Solver
+
+    sc = scenarios.slope(dx=args.dx, scale=args.scale, max_interaction=0)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(2, sync=True)
+    n = 3
+    t0 = time.perf_counter()
+    for _ in range(n):
+        s.upload_all()
+        s.time_integration(1, sync=True)
+    ms = (time.perf_counter() - t0) * 1e3 / n
+    ntot = sc.water.n + sc.soil.n
+    s.close()
+    return {"pcie_inclusive": {"ms_per_step": ms, "value": ntot / (ms * 1e-3), "unit": "particle-steps/s",
+                               "what": "per call: upload of the whole state, one step, download of the whole state"}}
+
+
 def sweep_section(args, hip):
     """64 independent submerged-column cases of uneven size (one seed):
     eager per-context launches against the batched hipGraph replays."""
@@ -218,6 +278,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the mixed and learned-surrogate section")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched scenario sweep section")
     ap.add_argument("--sweep-cases", type=int, default=64)
+    ap.add_argument("--mixed-error-steps", type=int, default=1000,
+                    help="mixed vs FP64 error is reported after 1 step and after this many (0: skip)")
+    ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--learned-k", type=int, default=4)
     ap.add_argument("--learned-samples", type=int, default=8)
     ap.add_argument("--learned-iters", type=int, default=400)
@@ -346,6 +409,10 @@ def main():
         out.update(extra)
     if not args.no_sweep:
         out.update(sweep_section(args, hip))
+    if args.mixed_error_steps > 0:
+        out.update(mixed_error_section(args))
+    if not args.no_pcie:
+        out.update(pcie_section(args, hip))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(1.0, args.cpu_steps)
     else:

@@ -68,6 +68,12 @@ class Solver:
         if self.soil is not None:
             self.sync("soil", mask)
 
+    def upload_all(self, mask=abi.F_ALL):
+        """Host arrays -> device for every bound phase."""
+        self.sync("water", mask, to_host=False)
+        if self.soil is not None:
+            self.sync("soil", mask, to_host=False)
+
     @property
     def itimestep(self):
         return self.lib.sphg_get_itimestep(self._ctx)
