@@ -101,17 +101,14 @@ def learned_section(args, hip):
     from sources_amd.surrogate import Surrogate
 
     K, S, M, J = args.learned_k, args.learned_samples, args.learned_advances, args.learned_judge
-    sc = scenarios.slope(dx=args.dx, scale=args.scale, max_interaction=0)
+    # the slope at half the linear size: ~250k particles
+    sc = scenarios.slope(dx=args.dx, scale=args.scale * args.learned_scale, max_interaction=0)
     sc.params.precision = abi.SPHG_MIXED
     ntot = sc.water.n + sc.soil.n
     b = Solver(sc.params, sc.water, sc.soil)
     b.time_integration(args.warmup, sync=False)
     hip.sync()
     e0, e1 = hip.event(), hip.event()
-    hip.record(e0, b.stream)
-    b.time_integration(args.steps, sync=False)
-    hip.record(e1, b.stream)
-    ms_mixed = hip.elapsed_ms(e0, e1) / args.steps
 
     g = Surrogate(b, ksteps=K, max_samples=S + 1, seed=1)
     t0 = time.perf_counter()
@@ -149,10 +146,8 @@ def learned_section(args, hip):
     a.close()
     b.close()
     return {
-        "mixed": {"ms_per_step": ms_mixed, "value": ntot / (ms_mixed * 1e-3), "unit": "particle-steps/s",
-                  "speedup_vs_fp64": None},
         "learned": {
-            "k_steps": K, "advances": M, "judge_every": J, "tol": 0.01,
+            "particles": ntot, "k_steps": K, "advances": M, "judge_every": J, "tol": 0.01,
             "steps_covered": M * K, "ms_plain_fp64": ms_plain, "ms_learned": ms_learned,
             "value": ntot * M * K / (ms_learned * 1e-3), "unit": "particle-steps/s",
             "speedup_vs_plain_fp64": ms_plain / ms_learned,
@@ -164,10 +159,11 @@ def learned_section(args, hip):
     }
 
 
-def mixed_error_section(args):
+def mixed_error_section(args, hip):
     """Mixed precision against FP64 on the 1M slope from the same state:
     norm-wise errors (max|mixed - fp64| / max|fp64|) of density, velocity and
-    stress, and the largest relative density error, after 1 and N steps."""
+    stress, and the largest relative density error, after 1 and N steps; the
+    N - 1 steps are timed for both precisions."""
     import numpy as np
     from sources_amd import abi, scenarios
     from sources_amd.sph import Solver
@@ -179,9 +175,15 @@ def mixed_error_section(args):
     b = Solver(sm.params, sm.water, sm.soil)
     out = {}
     done = 0
+    e0, e1 = hip.event(), hip.event()
+    timing = {}
     for n in (1, args.mixed_error_steps):
-        a.time_integration(n - done, sync=True)
-        b.time_integration(n - done, sync=True)
+        for tag, s in (("fp64", a), ("mixed", b)):
+            hip.record(e0, s.stream)
+            s.time_integration(n - done, sync=False)
+            hip.record(e1, s.stream)
+            timing[tag] = hip.elapsed_ms(e0, e1) / max(n - done, 1)
+            s.sync_all()
         done = n
         e = {}
         for f, phases in (("rho", ("water", "soil")), ("vx", ("water", "soil")), ("sxx", ("soil",)),
@@ -200,7 +202,12 @@ def mixed_error_section(args):
         print(f"# mixed vs fp64 after {n} steps: rho max rel {rr:.2e}", file=sys.stderr, flush=True)
     a.close()
     b.close()
-    return {"mixed_error_vs_fp64": out}
+    ntot = sf.water.n + sf.soil.n
+    return {"mixed_error_vs_fp64": out,
+            "mixed": {"ms_per_step": timing["mixed"], "value": ntot / (timing["mixed"] * 1e-3),
+                      "unit": "particle-steps/s", "particles": ntot, "steps_timed": args.mixed_error_steps - 1,
+                      "fp64_ms_per_step_same_run": timing["fp64"],
+                      "speedup_vs_fp64": timing["fp64"] / timing["mixed"]}}
 
 
 def pcie_section(args, hip):
@@ -231,7 +238,10 @@ def sweep_section(args, hip):
     from sources_amd.batch import Batch
     from sources_amd.sph import Solver
 
+    from sources_amd import abi
     cases = scenarios.sweep(args.sweep_cases, seed=11)
+    for c in cases:
+        c.params.precision = abi.SPHG_MIXED
     ntot = sum(c.water.n + c.soil.n for c in cases)
     ss = [Solver(c.params, c.water, c.soil) for c in cases]
     steps = 32
@@ -261,7 +271,7 @@ def sweep_section(args, hip):
         "value_batched": ntot / (ms_graph * 1e-3), "unit": "particle-steps/s",
         "speedup_batched_vs_eager": ms_eager / ms_graph,
         "workload": "submerged column collapses, N ~ logU[5k,120k], aspect ~ U[0.5,2], "
-                    "phi ~ U[25,40] deg, k_s ~ logU[1e-4,1e-2], seed 11, FP64",
+                    "phi ~ U[25,40] deg, k_s ~ logU[1e-4,1e-2], seed 11, mixed precision",
     }}
 
 
@@ -284,8 +294,9 @@ def main():
     ap.add_argument("--learned-k", type=int, default=4)
     ap.add_argument("--learned-samples", type=int, default=8)
     ap.add_argument("--learned-iters", type=int, default=400)
-    ap.add_argument("--learned-advances", type=int, default=16)
-    ap.add_argument("--learned-judge", type=int, default=4)
+    ap.add_argument("--learned-advances", type=int, default=20)
+    ap.add_argument("--learned-judge", type=int, default=5, help="plain FP64 check every this many advances")
+    ap.add_argument("--learned-scale", type=float, default=0.5, help="slope size for the learned section")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="fp64: reference arithmetic; mixed: FP32 Kahan pair sums, FP64 integration")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -404,13 +415,11 @@ def main():
     }
     s.close()
     if not args.no_learned:
-        extra = learned_section(args, hip)
-        extra["mixed"]["speedup_vs_fp64"] = ms_step / extra["mixed"]["ms_per_step"]
-        out.update(extra)
+        out.update(learned_section(args, hip))
     if not args.no_sweep:
         out.update(sweep_section(args, hip))
-    if args.mixed_error_steps > 0:
-        out.update(mixed_error_section(args))
+    if args.mixed_error_steps > 1:
+        out.update(mixed_error_section(args, hip))
     if not args.no_pcie:
         out.update(pcie_section(args, hip))
     if not args.no_cpu:

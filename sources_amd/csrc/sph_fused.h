@@ -34,15 +34,12 @@ enum : int {
 #define SPHG_LB 0
 #endif
 #ifndef SPHG_STAGE_WAVES
-#define SPHG_STAGE_WAVES 0   // 0: compiler's choice; n: ask for n waves per SIMD
-#endif
-#ifndef SPHG_PIPELINE
-#define SPHG_PIPELINE 1
+#define SPHG_STAGE_WAVES 0   // 0: compiler's choice; n: ask for at least n waves per SIMD
 #endif
 #if SPHG_STAGE_WAVES > 0
-#define STAGE_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPHG_STAGE_WAVES, 8)))
+#define STAGE_ATTR __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPHG_STAGE_WAVES, 8)))
 #else
-#define STAGE_ATTR __launch_bounds__(256)
+#define STAGE_ATTR __launch_bounds__(64)
 #endif
 
 struct FusedK {
@@ -926,7 +923,7 @@ constexpr int NEED_B = 0xFF;
 // chunks staged in the LDS window for the same-phase and cross loops.
 // ------------------------------------------------------------------------
 template <int SKF, int LA, int LX>
-__global__ __launch_bounds__(64) void k_water_a4(DPhase W, DPhase S, const double* recW, double* recWw,
+__global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, double* recWw,
                                                  const double* recS, Masks MW, Grid g, KParams K, FusedK F,
                                                  int cross)
 {
@@ -1067,7 +1064,7 @@ __global__ __launch_bounds__(64) void k_water_a4(DPhase W, DPhase S, const doubl
 // (new water p), then plastic_or_not + plastic_flow_rule3 + Jaumann_rate.
 // ------------------------------------------------------------------------
 template <int SKF, int LA, int LX>
-__global__ __launch_bounds__(64) void k_soil_a4(DPhase S, DPhase W, const double* recS, double* recSw,
+__global__ STAGE_ATTR void k_soil_a4(DPhase S, DPhase W, const double* recS, double* recSw,
                                                 const double* recW, Masks MS, Grid g, KParams K, FusedK F,
                                                 int cross)
 {
@@ -1237,7 +1234,7 @@ __global__ __launch_bounds__(64) void k_soil_a4(DPhase S, DPhase W, const double
 // The partner masks come from stage A.
 // ------------------------------------------------------------------------
 template <int SKF, int LB>
-__global__ __launch_bounds__(64) void k_water_b4(DPhase W, const double* recW, Masks MW, Grid g, KParams K,
+__global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Grid g, KParams K,
                                                  FusedK F, int mask)
 {
     static_assert(LB == 0 || (LB & 1), "window needs positions");
@@ -1329,7 +1326,7 @@ __global__ __launch_bounds__(64) void k_water_b4(DPhase W, const double* recW, M
 // + pore-water pressure force.
 // ------------------------------------------------------------------------
 template <int SKF, int LB>
-__global__ __launch_bounds__(64) void k_soil_b4(DPhase S, const double* recS, Masks MS, Grid g, KParams K,
+__global__ STAGE_ATTR void k_soil_b4(DPhase S, const double* recS, Masks MS, Grid g, KParams K,
                                                 FusedK F, int mask)
 {
     static_assert(LB == 0 || (LB & 1), "window needs positions");

@@ -18,6 +18,10 @@
 #pragma once
 #include "sph_fused.h"
 
+// The FP32 stage kernels fit five waves per SIMD without spilling (measured
+// faster than the compiler's default of four).
+#define STAGE_ATTR_MX __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
+
 __device__ __forceinline__ const float4* recq(const float* rec, int k)
 {
     return reinterpret_cast<const float4*>(rec + (size_t)Q_STRIDE * k);
@@ -183,7 +187,7 @@ __device__ __forceinline__ void visit_f(const CatMask& cm, const LaneRows& L, in
 // Stage A, water (mixed): the terms of k_water_a4 with FP32 pair sums.
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ __launch_bounds__(64) void k_water_a_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS,
+__global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS,
                                                    Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
 {
     const WaveCtx wc = wave_ctx(W.n);
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(64) void k_water_a_mx(DPhase W, DPhase S, const flo
 // Stage A, soil (mixed).
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ __launch_bounds__(64) void k_soil_a_mx(DPhase S, DPhase W, const float* qS, float* qSw, const float* qW,
+__global__ STAGE_ATTR_MX void k_soil_a_mx(DPhase S, DPhase W, const float* qS, float* qSw, const float* qW,
                                                   Masks MS, Grid g, KParams K, FusedK F, MixK X, int cross)
 {
     const WaveCtx wc = wave_ctx(S.n);
@@ -481,7 +485,7 @@ __global__ __launch_bounds__(64) void k_soil_a_mx(DPhase S, DPhase W, const floa
 // Stage B, water (mixed): momentum, art_visc, delta_sph(vof).
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ __launch_bounds__(64) void k_water_b_mx(DPhase W, const float* qW, Masks MW, Grid g, KParams K, FusedK F,
+__global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, Grid g, KParams K, FusedK F,
                                                    MixK X, int mask)
 {
     const WaveCtx wc = wave_ctx(W.n);
@@ -566,7 +570,7 @@ __global__ __launch_bounds__(64) void k_water_b_mx(DPhase W, const float* qW, Ma
 // Stage B, soil (mixed): momentum (df3 form), art_visc, delta_sph on p, s.
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ __launch_bounds__(64) void k_soil_b_mx(DPhase S, const float* qS, Masks MS, Grid g, KParams K, FusedK F,
+__global__ STAGE_ATTR_MX void k_soil_b_mx(DPhase S, const float* qS, Masks MS, Grid g, KParams K, FusedK F,
                                                   MixK X, int mask)
 {
     const WaveCtx wc = wave_ctx(S.n);
