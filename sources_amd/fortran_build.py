@@ -7,6 +7,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 FDIR = os.path.join(HERE, "fortran")
 EXE = os.path.join(FDIR, "sph_gpu")
+EXE_CAS = os.path.join(FDIR, "sph_cas")
 
 
 def flang():
@@ -22,6 +23,10 @@ def build(verbose=False):
          "-o", os.path.join(out, "sphg_mod.o")],
         [flang(), "-O2", "-I", out, os.path.join(FDIR, "sph_main.f90"), os.path.join(out, "sphg_mod.o"),
          "-L", lib_dir, "-lsphg", f"-Wl,-rpath,$ORIGIN/..", "-o", EXE],
+        [flang(), "-O2", "-c", "-I", out, os.path.join(FDIR, "sphg_io.f90"), "-module-dir", out,
+         "-o", os.path.join(out, "sphg_io.o")],
+        [flang(), "-O2", "-I", out, os.path.join(FDIR, "sph_cas.f90"), os.path.join(out, "sphg_io.o"),
+         os.path.join(out, "sphg_mod.o"), "-L", lib_dir, "-lsphg", f"-Wl,-rpath,$ORIGIN/..", "-o", EXE_CAS],
     ]
     for cmd in cmds:
         if verbose:

@@ -2,6 +2,7 @@
 Fortran host build (no GPU calls)."""
 import os
 import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -60,3 +61,55 @@ def test_fortran_host_builds_and_links():
     from sources_amd import fortran_build
     exe = fortran_build.build()
     assert os.path.exists(exe)
+
+
+CAS = """# a steering file in the reference's syntax (declarations_sph.f90:61-335)
+Print step = 10
+SAVE STEP = 20
+particle pairs find method = 2
+Results file = xv.dat, state.dat,
+   other.dat
+Single phase = .false.
+First phase = water
+Second phase = soil
+Water property = 1000.d0, 1.4d5, 7.0, 31.3, 1.0d-3
+Soil property = 2650., 1.5d6, 1.8d6, 0.3, 40.
+Darcy parameter = 0.4, 1.0d-3
+Soil failure criteria = 0.0, 0.5235987755982988
+Time step size = 2.0d-5
+&
+Print step = 99
+"""
+
+
+@pytest.mark.skipif(not (shutil.which("flang") or os.path.exists("/opt/rocm/lib/llvm/bin/flang")),
+                    reason="flang not available")
+def test_steering_file_is_parsed_like_the_reference(tmp_path):
+    from sources_amd import fortran_build
+    fortran_build.build()
+    f = tmp_path / "run.cas"
+    f.write_text(CAS)
+    r = subprocess.run([fortran_build.EXE_CAS, str(f), "--parse-only"], capture_output=True, text=True)
+    kv = dict(line.split(" = ", 1) for line in r.stdout.splitlines() if " = " in line)
+    assert kv["print_step"] == "10"          # the record after '&' is not read
+    assert kv["save_step"] == "20"
+    assert kv["nnps"] == "2"
+    assert (kv["results"], kv["res_soil"], kv["res_other"]) == ("xv.dat", "state.dat", "other.dat")
+    assert kv["single_phase"] == "F"
+    assert [float(v) for v in kv["water"].split()] == [1000.0, 1.4e5, 7.0, 31.3, 1.0e-3]
+    assert [float(v) for v in kv["soil"].split()] == [2650.0, 1.5e6, 1.8e6, 0.3, 40.0]
+    assert [float(v) for v in kv["darcy"].split()] == [0.4, 1.0e-3]
+    assert [float(v) for v in kv["failure"].split()] == [0.0, 0.5235987755982988]
+    assert float(kv["dt"]) == 2.0e-5
+
+
+@pytest.mark.skipif(not (shutil.which("flang") or os.path.exists("/opt/rocm/lib/llvm/bin/flang")),
+                    reason="flang not available")
+def test_unknown_steering_keyword_stops(tmp_path):
+    from sources_amd import fortran_build
+    fortran_build.build()
+    f = tmp_path / "bad.cas"
+    f.write_text("Print step = 10\nNo such thing = 3\n")
+    r = subprocess.run([fortran_build.EXE_CAS, str(f), "--parse-only"], capture_output=True, text=True)
+    assert "No such keyword" in r.stdout
+    assert "print_step" not in r.stdout
