@@ -227,15 +227,16 @@ int sphg_profile_get(sphg_ctx *ctx, const char *name, int64_t *launches, double 
 int sphg_pair_counts(sphg_ctx *ctx, int64_t *water_pairs, int64_t *soil_pairs, int64_t *cross_pairs);
 
 /* ---- learned multi-step surrogate ---------------------------------------
-   No counterpart in the reference.  A message-passing network on the
-   same-phase neighbour graph (BF16 MFMA, FP32 accumulation) corrects the
-   constant-rate extrapolation of velocity and density over K leapfrog steps.
-   It is trained on samples that the plain FP64 step generates, and the plain
-   step judges it: sphg_gnn_rollout runs the plain K steps beside every
+   No counterpart in the reference.  A learned advance of K steps is one
+   leapfrog step of size K dt (one right-hand side, the context's precision)
+   plus the correction of velocity and density towards K steps of size dt that
+   a message-passing network on the same-phase neighbour graph (BF16 MFMA,
+   FP32 accumulation) predicts from the state the advance starts from.  It is
+   trained on samples that the plain FP64 step generates, and the plain step
+   judges it: sphg_gnn_rollout runs the plain K steps beside every
    judge_every-th learned advance and keeps the plain result whenever the
    learned density deviates by more than tol (relative, max over real
-   particles); the judge always runs in FP64, while the rates the network
-   starts from use the context's precision.  Two-phase contexts only
+   particles); the judge always runs in FP64.  Two-phase contexts only
    (SPHG_ERR_UNSUPPORTED otherwise). */
 typedef struct sphg_gnn sphg_gnn;
 int sphg_gnn_nparams(void);
@@ -247,13 +248,14 @@ void sphg_gnn_destroy(sphg_gnn *gnn);
    from (to_host=0) host memory. */
 int sphg_gnn_weights(sphg_gnn *gnn, float *host, int to_host);
 int sphg_gnn_num_samples(const sphg_gnn *gnn);
-/* One training sample: rates and graph of the current state, K plain steps
-   (the state advances), targets = plain result - extrapolation. */
+/* One training sample: graph and features of the current state, the coarse
+   step from it, then K plain FP64 steps from it (the state advances along
+   these); targets = plain result - coarse result. */
 int sphg_gnn_record(sphg_gnn *gnn);
 /* iters Adam steps (one sample each, cycling); loss_out: mean normalised
    loss over the last pass. */
 int sphg_gnn_train(sphg_gnn *gnn, int iters, float lr, float *loss_out);
-/* One learned advance of K steps (use_net=0: extrapolation only). */
+/* One learned advance of K steps (use_net=0: the coarse step alone). */
 int sphg_gnn_step(sphg_gnn *gnn, int use_net);
 /* nlearned learned advances with the plain step as judge and fallback:
    nfallback judged advances replaced by the plain steps; max_err / rms_err:

@@ -1,13 +1,13 @@
 // sph_gnn.h -- learned multi-step surrogate on the cell-list graph.
 //
-// A message-passing network advances the particles K leapfrog steps at once.
+// A message-passing network advances the particles K leapfrog steps at once:
+// one leapfrog step of size K dt (the coarse step, one right-hand side), plus
+// the network's correction of velocity and density towards K steps of size
+// dt:  v += c_v,  rho += c_rho,  x += K dt c_v / 2.
 // Its graph is the same-phase neighbour graph of the SPH step (pairs with
-// r < 2h, at most DEG per node).  It predicts corrections to the constant-
-// rate extrapolation of velocity and density over K dt:
-//     v(t + K dt)   = v + K dt a + c_v
-//     rho(t + K dt) = rho + K dt drho/dt + c_rho
-// from node features (rates, velocity, density, stress) and edge features
-// (relative position / h, velocity, density and rate differences).
+// r < 2h, at most DEG per node); it sees node features (rates, velocity,
+// density, stress) and edge features (relative position / h, velocity,
+// density and rate differences) of the state the advance starts from.
 //
 // Network (weights FP32 master copy, BF16 operands, FP32 accumulation):
 //   edge layer  z_ij = e_ij W1            (e_ij: FE features, e_ij[0] = 1 = bias)
@@ -158,15 +158,30 @@ struct GnnArgs {
 
 // Forward (TRAIN = false) or forward + backward (TRAIN = true) over 32
 // nodes per workgroup (one wavefront).
+// LDS tiles of one wavefront.  Inference reuses two regions: X0 then H4 in
+// the first, HS then H3 in the second (each dead before the other is
+// written); training keeps all of them for the backward pass.
+constexpr int TILE_D = 32 * (D + 1);
+constexpr int lds_floats(bool train)
+{
+    return train ? 32 * (FI + 1) + 3 * TILE_D + 32 * (FE + 1) + 32 * (HE + 1) : 2 * TILE_D;
+}
+
 template <bool TRAIN>
 __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
 {
-    __shared__ float X0[32][FI + 1];
-    __shared__ float H3[32][D + 1];
-    __shared__ float H4[32][D + 1];
-    __shared__ float T[32][D + 1];
-    __shared__ float E[32][FE + 1];
-    __shared__ float HS[32][HE + 1];
+    __shared__ float lds[lds_floats(TRAIN)];
+    typedef float RowFI[FI + 1];
+    typedef float RowD[D + 1];
+    typedef float RowFE[FE + 1];
+    typedef float RowHE[HE + 1];
+    RowFI* X0 = reinterpret_cast<RowFI*>(lds);
+    RowD* H4 = TRAIN ? reinterpret_cast<RowD*>(lds + 32 * (FI + 1)) : reinterpret_cast<RowD*>(lds);
+    RowHE* HS = TRAIN ? reinterpret_cast<RowHE*>(lds + 32 * (FI + 1) + TILE_D) : reinterpret_cast<RowHE*>(lds + TILE_D);
+    RowD* H3 = TRAIN ? reinterpret_cast<RowD*>(lds + 32 * (FI + 1) + TILE_D + 32 * (HE + 1))
+                     : reinterpret_cast<RowD*>(lds + TILE_D);
+    RowD* T = reinterpret_cast<RowD*>(lds + 32 * (FI + 1) + 2 * TILE_D + 32 * (HE + 1));
+    RowFE* E = reinterpret_cast<RowFE*>(lds + 32 * (FI + 1) + 3 * TILE_D + 32 * (HE + 1));
     const int l = lane_id(), p = l & 31, h = l >> 5;
     const int base = blockIdx.x * 32;
     const int i = base + p;
@@ -394,7 +409,6 @@ __global__ void k_adam(float* W, const float* g, float* m, float* v, int n, floa
 // Graph and features of one phase, in cell order: node = off + t for the
 // particle a = order[t].  Neighbours: same-phase pairs with r < 2h (the
 // exact predicate of the step), at most DEG (nd counts the kept ones).
-// base[node] holds the constant-rate extrapolation the network corrects.
 // ------------------------------------------------------------------------
 struct GatherOut {
     float* nf;
@@ -406,9 +420,8 @@ struct GatherOut {
     int* overflow;     // [1]
 };
 
-// base[node] (FP64): vx + K dt ax, vy + K dt ay, rho + K dt drho, 0.
 __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, int off, int phase, Scales S,
-                             GatherOut O, double* base)
+                             GatherOut O)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
@@ -449,11 +462,6 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
     rw[7] = (float)h;
     rw[8] = nf[2];
     rw[9] = rw[10] = rw[11] = 0.0f;
-    double* bs = base + (size_t)node * NO;
-    bs[0] = P.vx[a] + kdt * P.ax[a];
-    bs[1] = P.vy[a] + kdt * P.ay[a];
-    bs[2] = P.rho[a] + kdt * P.drho[a];
-    bs[3] = 0.0;
     O.tw[node] = virt ? 0.0f : 1.0f;
     O.node_of_id[P.id[a]] = node;
     // neighbours (cell order of the same phase)
@@ -482,56 +490,41 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
     for (int s = min(cnt, DEG); s < DEG; ++s) nb[s] = -1;
 }
 
-// Targets after K plain steps: the corrections state - base (physical
-// units), by particle id.
-__global__ void k_gnn_target(DPhase P, const int* node_of_id, const double* base, float* tgt)
+// Targets: fine state (K plain steps, current) minus coarse state (one step
+// of K dt, snapshot C), by particle id, in physical units.
+__global__ void k_gnn_target(DPhase P, const int* node_of_id, DPhase C, const int* slot_of_id_c, float* tgt)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
-    const int node = node_of_id[P.id[a]];
-    const double* bs = base + (size_t)node * NO;
+    const int id = P.id[a];
+    const int node = node_of_id[id];
+    const int b = slot_of_id_c[id];
     float* tg = tgt + (size_t)node * NO;
-    tg[0] = (float)(P.vx[a] - bs[0]);
-    tg[1] = (float)(P.vy[a] - bs[1]);
-    tg[2] = (float)(P.rho[a] - bs[2]);
+    tg[0] = (float)(P.vx[a] - C.vx[b]);
+    tg[1] = (float)(P.vy[a] - C.vy[b]);
+    tg[2] = (float)(P.rho[a] - C.rho[b]);
     tg[3] = 0.0f;
 }
 
-// Learned K-step update of one phase (real particles): FP64 state, the
-// network's corrections added to the constant-rate extrapolation.
-__global__ void k_gnn_apply(DPhase P, KParams K, const int* node_of_id, const float* y, Scales S, int phase,
-                            int use_net)
+// The network's correction of the coarse step (real particles).
+__global__ void k_gnn_correct(DPhase P, KParams K, const int* node_of_id, const float* y, Scales S, int phase)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
     if (P.itype[a] <= 0) return;
-    const double kdt = S.kdt;
     const int node = node_of_id[P.id[a]];
     const float* yy = y + (size_t)node * NO;
-    const double cvx = use_net ? (double)yy[0] * S.out[0] : 0.0;
-    const double cvy = use_net ? (double)yy[1] * S.out[1] : 0.0;
-    const double crh = use_net ? (double)yy[2] * S.out[2] : 0.0;
-    const double vx0 = P.vx[a], vy0 = P.vy[a];
-    const double vx1 = vx0 + kdt * P.ax[a] + cvx;
-    const double vy1 = vy0 + kdt * P.ay[a] + cvy;
-    P.vx[a] = vx1;
-    P.vy[a] = vy1;
-    P.x[a] = P.x[a] + kdt * 0.5 * (vx0 + vx1);
-    P.y[a] = P.y[a] + kdt * 0.5 * (vy0 + vy1);
-    P.rho[a] = P.rho[a] + kdt * P.drho[a] + crh;
-    if (phase == 0) {
-        if (K.volume_fraction) P.vof[a] = P.vof[a] + kdt * P.dvof[a];
-    } else {
-        double p = P.p[a] + kdt * P.dp[a];
-        double sxx = P.sxx[a] + kdt * P.dsxx[a], sxy = P.sxy[a] + kdt * P.dsxy[a], syy = P.syy[a] + kdt * P.dsyy[a];
-        P.fail[a] = dp_project(K.s_alpha1, p, sxx, sxy, syy);
-        P.p[a] = p; P.sxx[a] = sxx; P.sxy[a] = sxy; P.syy[a] = syy;
-        P.vof[a] = P.rho[a] / K.s_rho0;
-    }
+    const double cvx = (double)yy[0] * S.out[0];
+    const double cvy = (double)yy[1] * S.out[1];
+    const double crh = (double)yy[2] * S.out[2];
+    P.vx[a] = P.vx[a] + cvx;
+    P.vy[a] = P.vy[a] + cvy;
+    P.x[a] = P.x[a] + 0.5 * (double)S.kdt * cvx;
+    P.y[a] = P.y[a] + 0.5 * (double)S.kdt * cvy;
+    P.rho[a] = P.rho[a] + crh;
+    if (phase == 1) P.vof[a] = P.rho[a] / K.s_rho0;
 }
 
-// Largest relative density deviation |rho_a - rho_b| / rho_b over real particles
-// (b: the reference state, addressed by id through node_of_id maps).
 // out[0]: max (as float bits: non-negative floats order as their bits);
 // sq[0] += sum of squares, sq[1] += count.  One atomic per block.
 __global__ __launch_bounds__(256) void k_rho_err(DPhase A, DPhase B, const int* slot_of_id_b, unsigned int* out,

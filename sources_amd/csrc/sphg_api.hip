@@ -1291,7 +1291,6 @@ struct sphg_gnn {
     };
     std::vector<Graph> samples;
     Graph cur;                      // graph of the state being advanced
-    double* base = nullptr;         // [n][NO] extrapolation of the sample being recorded
     float* y = nullptr;             // [n][NO] network output
     int* node_of_id[2] = {nullptr, nullptr};
     int* slot_of_id[2] = {nullptr, nullptr};
@@ -1369,19 +1368,41 @@ int snap_copy(sphg_gnn* g, int slot, bool to_snap)
     return SPHG_OK;
 }
 
-// The snapshot of a phase seen as a DPhase (fields used by k_rho_err).
+// The snapshot of a phase seen as a DPhase (id, itype, vx, vy, rho).
 DPhase snap_view(sphg_gnn* g, int slot, int k)
 {
     DPhase d = g->c->ph[k].d;
     StateCopy& s = g->snap[slot][k];
     d.id = s.id;
     d.itype = s.itype;
+    d.vx = s.f[2];
+    d.vy = s.f[3];
     d.rho = s.f[6];
     return d;
 }
 
-// Rates at the current state: the fused right-hand side without a half step.
-int rates_now(sphg_ctx* c) { return rhs_two_phase_fused(c, 0, 0, 0); }
+// One leapfrog step of size K dt (the coarse step of a learned advance).
+int coarse_step(sphg_ctx* c, int K)
+{
+    const double dt0 = c->K.dt;
+    c->K.dt = dt0 * K;
+    const int rc = one_step(c);
+    c->K.dt = dt0;
+    RC(rc);
+    c->itimestep += K - 1;
+    return SPHG_OK;
+}
+
+// K plain FP64 steps (targets and the judge), whatever precision the context uses.
+int plain_steps(sphg_ctx* c, int n)
+{
+    const int prec = c->P.precision;
+    c->P.precision = SPHG_FP64;
+    int rc = SPHG_OK;
+    for (int s = 0; s < n && rc == SPHG_OK; ++s) rc = one_step(c);
+    c->P.precision = prec;
+    return rc;
+}
 
 // Graph, features and extrapolation base of the current state into G.
 int build_graph(sphg_gnn* g, sphg_gnn::Graph& G)
@@ -1398,7 +1419,7 @@ int build_graph(sphg_gnn* g, sphg_gnn::Graph& G)
         gnn::GatherOut O{G.nf, G.raw, G.nb, G.nd, G.tw, g->node_of_id[k], g->overflow};
         RC(launch(c, "gnn_gather", gnn::k_gnn_gather, nblk(b.n), TPB, b.d, c->gl[k], c->K,
                   k == 0 ? c->F.thr_w : c->F.thr_s, k == 0 ? c->F.uni_w : c->F.uni_s, k == 0 ? 0 : Wb.n, k,
-                  g->S, O, g->base));
+                  g->S, O));
     }
     (void)Sb;
     return SPHG_OK;
@@ -1433,7 +1454,7 @@ int sphg_gnn_create(sphg_ctx* c, int ksteps, int max_samples, uint64_t seed, sph
     int rc;
     if ((rc = galloc(g, &g->W, gnn::NPARAM)) || (rc = galloc(g, &g->gW, gnn::NPARAM)) ||
         (rc = galloc(g, &g->m, gnn::NPARAM)) || (rc = galloc(g, &g->v, gnn::NPARAM)) ||
-        (rc = galloc(g, &g->loss, 1)) || (rc = galloc(g, &g->base, (size_t)n * gnn::NO)) ||
+        (rc = galloc(g, &g->loss, 1)) || 
         (rc = galloc(g, &g->y, (size_t)n * gnn::NO)) || (rc = galloc(g, &g->overflow, 1)) ||
         (rc = galloc(g, &g->err, 1)) || (rc = galloc(g, &g->errsq, 2)) || (rc = graph_alloc(g, g->cur, n)))
         return fail(rc);
@@ -1513,13 +1534,20 @@ int sphg_gnn_record(sphg_gnn* g)
     RC(check_fused_switches(c));
     sphg_gnn::Graph G;
     RC(graph_alloc(g, G, c->ph[0].n + c->ph[1].n));
-    RC(rates_now(c));
+    const int step0 = c->itimestep;
+    RC(snap_copy(g, 0, true));
     RC(build_graph(g, G));
-    for (int it = 0; it < g->K; ++it) RC(one_step(c));
+    RC(coarse_step(c, g->K));           // coarse state -> snapshot 1
+    RC(snap_copy(g, 1, true));
+    RC(snap_copy(g, 0, false));
+    c->itimestep = step0;
+    RC(plain_steps(c, g->K));           // fine state: the run continues from here
     for (int k = 0; k < 2; ++k) {
         PhaseBuf& b = c->ph[k];
-        RC(launch(c, "gnn_target", gnn::k_gnn_target, nblk(b.n), TPB, b.d, (const int*)g->node_of_id[k],
-                  (const double*)g->base, G.tgt));
+        const DPhase cv = snap_view(g, 1, k);
+        RC(launch(c, "slot_of_id", gnn::k_slot_of_id, nblk(b.n), TPB, cv, g->slot_of_id[k]));
+        RC(launch(c, "gnn_target", gnn::k_gnn_target, nblk(b.n), TPB, b.d, (const int*)g->node_of_id[k], cv,
+                  (const int*)g->slot_of_id[k], G.tgt));
     }
     // running statistics of the targets (host side, for the output scales)
     std::vector<float> t((size_t)G.n * gnn::NO), w((size_t)G.n);
@@ -1591,16 +1619,19 @@ int sphg_gnn_step(sphg_gnn* g, int use_net)
     if (!g) return SPHG_ERR_INVALID;
     sphg_ctx* c = g->c;
     RC(check_fused_switches(c));
-    RC(rates_now(c));
-    RC(build_graph(g, g->cur));
-    if (use_net) RC(forward(g, g->cur));
-    for (int k = 0; k < 2; ++k) {
-        PhaseBuf& b = c->ph[k];
-        RC(launch(c, "gnn_apply", gnn::k_gnn_apply, nblk(b.n), TPB, b.d, c->K, (const int*)g->node_of_id[k],
-                  (const float*)g->y, g->S, k, use_net ? 1 : 0));
-        b.cells_valid = false;
+    if (use_net) {
+        RC(build_graph(g, g->cur));
+        RC(forward(g, g->cur));
     }
-    c->itimestep += g->K;
+    RC(coarse_step(c, g->K));
+    if (use_net) {
+        for (int k = 0; k < 2; ++k) {
+            PhaseBuf& b = c->ph[k];
+            RC(launch(c, "gnn_correct", gnn::k_gnn_correct, nblk(b.n), TPB, b.d, c->K,
+                      (const int*)g->node_of_id[k], (const float*)g->y, g->S, k));
+            b.cells_valid = false;
+        }
+    }
     return SPHG_OK;
 }
 
@@ -1616,17 +1647,6 @@ static int state_broken(sphg_gnn* g, int* broken)
     HIPCHK(c, hipMemcpyAsync(broken, g->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return SPHG_OK;
-}
-
-// K plain FP64 steps (the judge), whatever precision the rates use.
-static int plain_steps(sphg_ctx* c, int n)
-{
-    const int prec = c->P.precision;
-    c->P.precision = SPHG_FP64;
-    int rc = SPHG_OK;
-    for (int s = 0; s < n && rc == SPHG_OK; ++s) rc = one_step(c);
-    c->P.precision = prec;
-    return rc;
 }
 
 int sphg_gnn_rollout(sphg_gnn* g, int nlearned, int judge_every, double tol, int use_net, int* nfallback,

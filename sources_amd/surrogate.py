@@ -1,11 +1,12 @@
 """Learned multi-step surrogate over a Solver (the sphg_gnn_* entry points).
 
-A message-passing network on the same-phase neighbour graph (BF16 MFMA,
-FP32 accumulation, hand-written forward and backward kernels) corrects the
-constant-rate extrapolation of velocity and density over K leapfrog steps.
-It trains on samples that the plain FP64 step generates; the plain step
-judges it and takes over whenever the learned density deviates by more than
-the tolerance.  No counterpart in the reference.
+A learned advance of K steps is one leapfrog step of size K dt plus the
+velocity and density correction that a message-passing network on the
+same-phase neighbour graph (BF16 MFMA, FP32 accumulation, hand-written
+forward and backward kernels) predicts towards K steps of size dt.  It
+trains on samples that the plain FP64 step generates; the plain step judges
+it and takes over whenever the learned density deviates by more than the
+tolerance.  No counterpart in the reference.
 """
 import ctypes as C
 
@@ -17,7 +18,7 @@ FN, RAW, DEG, NO = 16, 12, 48, 4
 
 
 class Surrogate:
-    def __init__(self, solver, ksteps=4, max_samples=64, seed=1):
+    def __init__(self, solver, ksteps=2, max_samples=64, seed=1):
         self.solver = solver
         self.lib = solver.lib
         self.ksteps = ksteps
