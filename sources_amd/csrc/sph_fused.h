@@ -4,12 +4,17 @@
 //   pack      thread t of the cell order: half-step update of particle order[t]
 //             and its 128-byte neighbour record rec[t] (cell-sorted, so the
 //             records of a cell row are contiguous and a neighbour costs one
-//             cache line instead of one line per SoA field);
+//             cache line instead of one line per SoA field).  Record chunks
+//             (16 bytes): 0 x,y | 1 vx,vy | 2 rho,m | 3 vof,s3 | 4 p,c |
+//             5 s1,s2 | 6 h,itype; m carries the sign of itype (negative for
+//             virtual particles).  The loops load only the chunks they use
+//             (partner loads are limited by the address unit, one address per
+//             lane and 16 bytes): 1/rho and m/rho of a partner are recomputed;
 //   stage A   candidate scan of the 3 cell rows with a sqrt-free exact
 //             predicate (dr < T, T precomputed so that dr < T <=> sqrt(dr) < R),
-//             accepted partners kept as one 64-bit mask per row; then a dense
-//             loop over the mask bits evaluates the pair terms.  The masks are
-//             stored for stage B, so each pair is tested once per step;
+//             accepted partners kept as one concatenated 128-bit mask; then a
+//             dense loop over the mask bits evaluates the pair terms.  The masks
+//             are stored for stage B, so each pair is tested once per step;
 //   stage B   dense loop over the stored masks.
 // Lanes of a wavefront hold consecutive cell-sorted particles; the dense loops
 // diverge only by the lanes' neighbour-count spread.
@@ -18,7 +23,7 @@
 
 // record slots (doubles)
 enum : int {
-    R_X = 0, R_Y, R_VX, R_VY, R_RHO, R_M, R_H, R_VOF, R_P, R_C, R_S1, R_S2, R_S3, R_IT, R_IRHO, R_V,
+    R_X = 0, R_Y, R_VX, R_VY, R_RHO, R_M, R_VOF, R_S3, R_P, R_C, R_S1, R_S2, R_H, R_IT, R_PAD0, R_PAD1,
     R_STRIDE = 16
 };
 
@@ -263,10 +268,9 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
     double2* R = reinterpret_cast<double2*>(rec + (size_t)R_STRIDE * t);
     R[0] = make_double2(P.x[i], P.y[i]);
     R[1] = make_double2(vx, vy);
-    R[2] = make_double2(rho, m);
-    R[3] = make_double2(P.hsml[i], vof);
-    R[6] = make_double2(0.0, (double)it);
-    R[7] = make_double2(irho, m * irho);
+    R[2] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
+    R[3] = make_double2(vof, 0.0);
+    R[6] = make_double2(P.hsml[i], (double)it);
 }
 
 __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs)
@@ -310,235 +314,24 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
     double2* R = reinterpret_cast<double2*>(rec + (size_t)R_STRIDE * t);
     R[0] = make_double2(P.x[i], P.y[i]);
     R[1] = make_double2(vx, vy);
-    R[2] = make_double2(rho, m);
-    R[3] = make_double2(P.hsml[i], P.vof[i]);
+    R[2] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
+    R[3] = make_double2(P.vof[i], syy);
     R[4] = make_double2(p, P.c[i]);
     R[5] = make_double2(sxx, sxy);
-    R[6] = make_double2(syy, (double)it);
-    R[7] = make_double2(irho, m * irho);
+    R[6] = make_double2(P.hsml[i], (double)it);
 }
 
 
 
 // ------------------------------------------------------------------------
-// Partner masks.  For particle t, bit j of row mask r marks partner
-// k0[r] + j of the 3 x 3 block's row r (cell order).  A row with more than 64
-// candidates sets bit r of `ovf`; such rows are walked directly.
+// Partner masks of stage A, reused by stage B: per particle t a 128-bit mask
+// over its candidate list (the three cell rows of its 3 x 3 block, in cell
+// order, concatenated): m[t] low and m[n + t] high word, and an overflow
+// flag (rows walked directly).
 // ------------------------------------------------------------------------
 struct Masks {
-    uint64_t* m;   // [3][n]
+    uint64_t* m;   // [3][n] (the first two words are used)
     int* ovf;      // [n]
-};
-
-struct MaskIter {
-    uint64_t m0, m1, m2;
-    int b0, b1, b2;
-    int d0, d1, d2;   // window index - record index, per row
-    __device__ __forceinline__ int next()
-    {
-        if (m0) { const int j = __builtin_ctzll(m0); m0 &= m0 - 1; return b0 + j; }
-        if (m1) { const int j = __builtin_ctzll(m1); m1 &= m1 - 1; return b1 + j; }
-        if (m2) { const int j = __builtin_ctzll(m2); m2 &= m2 - 1; return b2 + j; }
-        return -1;
-    }
-    // next record index k; i receives its window index
-    __device__ __forceinline__ int next(int& i)
-    {
-        if (m0) { const int k = b0 + __builtin_ctzll(m0); m0 &= m0 - 1; i = k + d0; return k; }
-        if (m1) { const int k = b1 + __builtin_ctzll(m1); m1 &= m1 - 1; i = k + d1; return k; }
-        if (m2) { const int k = b2 + __builtin_ctzll(m2); m2 &= m2 - 1; i = k + d2; return k; }
-        return -1;
-    }
-};
-
-__device__ __forceinline__ MaskIter mask_iter(const Masks& M, int n, int t, const Rows& R)
-{
-    MaskIter it;
-    it.m0 = M.m[t];
-    it.m1 = M.m[n + t];
-    it.m2 = M.m[2 * (size_t)n + t];
-    it.b0 = R.k0[0]; it.b1 = R.k0[1]; it.b2 = R.k0[2];
-    it.d0 = it.d1 = it.d2 = 0;
-    return it;
-}
-
-// Batched scan of one candidate row: eight independent record loads are in
-// flight before the compares.
-template <class P>
-__device__ __forceinline__ uint64_t scan_row(const double* rec, int k0, int k1, int self, P&& accept)
-{
-    uint64_t m = 0;
-#pragma unroll 1
-    for (int k = k0; k < k1; k += 8) {
-        double2 xy[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) xy[u] = recp(rec, min(k + u, k1 - 1))[0];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int kk = k + u;
-            const bool ok = kk < k1 && kk != self && accept(kk, xy[u]);
-            m |= (uint64_t)ok << (kk - k0);
-        }
-    }
-    return m;
-}
-
-// Same-phase masks of phase A and cross masks A -> B.  Low register count:
-// this is the latency-bound part of the neighbour search.
-__global__ __launch_bounds__(256) void k_masks(const double* recA, const int* startA, int nA, const double* recB,
-                                               const int* startB, Masks MS, Masks MX, Grid g, KParams K,
-                                               int uni_s, double thr_s, int link, int do_cross, int uni_x,
-                                               double thr_x)
-{
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= nA) return;
-    const double2 xy = recp(recA, t)[0];
-    const double xa = xy.x, ya = xy.y;
-    const double ha = recp(recA, t)[3].x;
-    {
-        const Rows R = block_rows(g, startA, xa, ya);
-        int ovf = 0;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            uint64_t m = 0;
-            if (R.k1[r] - R.k0[r] > 64) ovf |= 1 << r;
-            else
-                m = scan_row(recA, R.k0[r], R.k1[r], t, [&](int k, double2 b) {
-                    const double dx = xa - b.x, dy = ya - b.y;
-                    double dr;
-                    {
-#pragma clang fp contract(off)
-                        dr = dx * dx + dy * dy;
-                    }
-                    const double hb = uni_s ? ha : recp(recA, k)[3].x;
-                    if (!pred(uni_s, thr_s, dr, ha, hb, K)) return false;
-                    return !link || ll_reach(g, xa, ya, b.x, b.y);
-                });
-            MS.m[(size_t)r * nA + t] = m;
-        }
-        MS.ovf[t] = ovf;
-    }
-    if (do_cross) {
-        const Rows R = block_rows(g, startB, xa, ya);
-        int ovf = 0;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            uint64_t m = 0;
-            if (R.k1[r] - R.k0[r] > 64) ovf |= 1 << r;
-            else
-                m = scan_row(recB, R.k0[r], R.k1[r], -1, [&](int k, double2 b) {
-                    const double dx = xa - b.x, dy = ya - b.y;
-                    double dr;
-                    {
-#pragma clang fp contract(off)
-                        dr = dx * dx + dy * dy;
-                    }
-                    const double hb = uni_x ? ha : recp(recB, k)[3].x;
-                    return pred(uni_x, thr_x, dr, ha, hb, K);
-                });
-            MX.m[(size_t)r * nA + t] = m;
-        }
-        MX.ovf[t] = ovf;
-    }
-}
-
-// Visit every partner of t: the mask bits with the next partner's record
-// loads issued before the current partner's arithmetic, then the direct walk
-// of overflowed rows.  LOAD(k) returns the partner data, BODY(k, data).
-template <class L, class B, class P>
-__device__ __forceinline__ void visit_pf(MaskIter it, const Rows& R, int ovf, int self, L&& load, P&& accept,
-                                         B&& body)
-{
-    int k = it.next();
-#if SPHG_PIPELINE
-    // Software pipeline: the next partner's loads are issued before the
-    // current partner's arithmetic; the scheduling barrier keeps the
-    // compiler from sinking them back below it.
-    auto cur = load(k < 0 ? 0 : k);
-#pragma unroll 1
-    while (k >= 0) {
-        const int kn = it.next();
-        const auto nxt = load(kn < 0 ? k : kn);
-        __builtin_amdgcn_sched_barrier(0);
-        body(k, cur);
-        cur = nxt;
-        k = kn;
-    }
-#else
-#pragma unroll 1
-    while (k >= 0) {
-        const int k2 = it.next();
-        const auto d1 = load(k);
-        if (k2 >= 0) {
-            const auto d2 = load(k2);
-            body(k, d1);
-            body(k2, d2);
-            k = it.next();
-        } else {
-            body(k, d1);
-            break;
-        }
-    }
-#endif
-    if (ovf) {
-#pragma unroll 1
-        for (int r = 0; r < 3; ++r) {
-            if (!(ovf & (1 << r))) continue;
-#pragma unroll 1
-            for (int kk = R.k0[r]; kk < R.k1[r]; ++kk) {
-                if (kk == self || !accept(kk)) continue;
-                body(kk, load(kk));
-            }
-        }
-    }
-}
-
-struct NbS {   // same-phase partner, stage A
-    double2 xy, v, rm, hv, it, iv;
-};
-struct NbX {   // cross partner
-    double2 xy, v, rm, hv, pc, iv;
-};
-struct NbB {   // stage B partner: full record
-    double2 xy, v, rm, hv, pc, s12, s3, iv;
-};
-
-__device__ __forceinline__ NbS load_s(const double* rec, int k)
-{
-    const double2* R = recp(rec, k);
-    return NbS{R[0], R[1], R[2], R[3], R[6], R[7]};
-}
-__device__ __forceinline__ NbX load_x(const double* rec, int k)
-{
-    const double2* R = recp(rec, k);
-    return NbX{R[0], R[1], R[2], R[3], R[4], R[7]};
-}
-__device__ __forceinline__ NbB load_b(const double* rec, int k)
-{
-    const double2* R = recp(rec, k);
-    return NbB{R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7]};
-}
-
-// Exact predicate for the direct walk of overflowed rows (same as k_masks).
-struct Accept {
-    const double* rec;
-    double xa, ya, ha, thr;
-    int uni, link;
-    const Grid* g;
-    const KParams* K;
-    __device__ __forceinline__ bool operator()(int k) const
-    {
-        const double2 b = recp(rec, k)[0];
-        const double dx = xa - b.x, dy = ya - b.y;
-        double dr;
-        {
-#pragma clang fp contract(off)
-            dr = dx * dx + dy * dy;
-        }
-        const double hb = uni ? ha : recp(rec, k)[3].x;
-        if (!pred(uni, thr, dr, ha, hb, *K)) return false;
-        return !link || ll_reach(*g, xa, ya, b.x, b.y);
-    }
 };
 
 // ------------------------------------------------------------------------
@@ -688,18 +481,6 @@ __device__ __forceinline__ LaneRows lane_rows(const Rows& R, const Win& w, int c
     return L;
 }
 
-__device__ __forceinline__ MaskIter iter_of(uint64_t m0, uint64_t m1, uint64_t m2, const LaneRows& L)
-{
-    MaskIter it;
-    it.m0 = m0; it.m1 = m1; it.m2 = m2;
-    it.b0 = L.k0[0];
-    it.b1 = L.k0[1];
-    it.b2 = L.k0[2];
-    it.d0 = L.base[0];
-    it.d1 = L.base[1];
-    it.d2 = L.base[2];
-    return it;
-}
 
 // Scan of one lane row: exact predicate on the candidates' positions
 // (LDS chunk 0 of the window, or the record).  accept(i, k, xy) gets the
@@ -814,8 +595,9 @@ __device__ __forceinline__ CatIter cat_iter(const CatMask& c, const LaneRows& L)
     return it;
 }
 
-// Partner data of the stage kernels: the eight record chunks
-// (x,y | vx,vy | rho,m | h,vof | p,c | s1,s2 | s3,itype | 1/rho,m/rho).
+// Partner data of the stage kernels: xy = (x, y), v = (vx, vy),
+// rm = (rho, m), hv = (h, vof), pc = (p, c), s12 = (s1, s2), s3 = (s3, itype),
+// iv = (1/rho, m/rho), filled from the record chunks.
 struct Nb {
     double2 xy, v, rm, hv, pc, s12, s3, iv;
     __device__ __forceinline__ void set(int c, double2 q)
@@ -824,27 +606,41 @@ struct Nb {
         case 0: xy = q; break;
         case 1: v = q; break;
         case 2: rm = q; break;
-        case 3: hv = q; break;
+        case 3: hv.y = q.x; s3.x = q.y; break;   // vof, s3
         case 4: pc = q; break;
         case 5: s12 = q; break;
-        case 6: s3 = q; break;
-        default: iv = q; break;
+        case 6: hv.x = q.x; s3.y = q.y; break;   // h, itype
+        default: break;
         }
     }
 };
 
-// Loads the chunks NEED of partner record k (window index i): chunks in LM
-// come from the LDS window when it is in use, the others from the record.
+constexpr int NEED_IV = 1 << 8;   // recompute 1/rho and m/rho of the partner
+constexpr int CH_H = 1 << 6;      // chunk holding h
+
+// Loads the chunks NEED of partner record k (window index i) -- plus the h
+// chunk when with_h (non-uniform smoothing length): chunks in LM come from
+// the LDS window when it is in use, the others from the record.
 template <int LM, int NEED>
-__device__ __forceinline__ Nb load_nb(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k)
+__device__ __forceinline__ Nb load_nb(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k,
+                                      bool with_h)
 {
     Nb b{};
     const double2* R = recp(rec, k);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        if (!(NEED & (1 << c))) continue;
+    for (int c = 0; c < 7; ++c) {
+        if (!(NEED & (1 << c)) && !(c == 6 && with_h)) continue;
         if ((LM & (1 << c)) && win) b.set(c, lds[Lds<LM>::slot(c)][i]);
         else b.set(c, R[c]);
+    }
+    if (NEED & 4) {
+        // chunk 2 carries m with the sign of itype (virtual < 0)
+        if (!((NEED & CH_H) || with_h)) b.s3.y = b.rm.y < 0.0 ? -1.0 : 1.0;
+        b.rm.y = fabs(b.rm.y);
+    }
+    if (NEED & NEED_IV) {
+        b.iv.x = fast_rcp(b.rm.x);
+        b.iv.y = b.rm.y * b.iv.x;
     }
     return b;
 }
@@ -902,17 +698,17 @@ __device__ __forceinline__ double2 xy_of(const double2 (*lds)[WIN_N], const doub
 template <int LM>
 __device__ __forceinline__ double h_of(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k)
 {
-    if ((LM & 8) && win) return lds[Lds<LM>::slot(3)][i].x;
-    return recp(rec, k)[3].x;
+    if ((LM & CH_H) && win) return lds[Lds<LM>::slot(6)][i].x;
+    return recp(rec, k)[6].x;
 }
 
 constexpr int lds_rows(int a, int b) { return (a > b ? a : b) > 0 ? (a > b ? a : b) : 1; }
 
-// Chunks read per partner by each loop.
-constexpr int NEED_SAME_A = 0xCF;    // x,y v rho,m h,vof s3,itype 1/rho,V
-constexpr int NEED_WX_A = 0x8F;      // soil partners of water: x,y v rho,m h,vof 1/rho,V
-constexpr int NEED_SX_A = 0x9F;      // water partners of soil: + p,c
-constexpr int NEED_B = 0xFF;
+// Chunks read per partner by each loop (h only when it is not uniform).
+constexpr int NEED_SAME_A = 0x07 | NEED_IV;          // x,y v rho,m(itype in its sign); 1/rho, m/rho
+constexpr int NEED_WX_A = 0x0F | NEED_IV;            // soil partners of water: x,y v rho,m vof
+constexpr int NEED_SX_A = 0x1F | NEED_IV;            // water partners of soil: + p,c
+constexpr int NEED_B = 0x3F | NEED_IV;               // x,y v rho,m vof,s3 p,c s1,s2
 
 // ------------------------------------------------------------------------
 // Stage A, water: same-phase terms (con_density, delta_sph(rho),
@@ -932,10 +728,10 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     const WaveCtx wc = wave_ctx(W.n);
     const int t = wc.t;
     const double2* RA = recp(recW, wc.tt);
-    const double2 xy = RA[0], va = RA[1], rm = RA[2], hv = RA[3];
-    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hv.x, vofa = hv.y;
-    const double irhoa = RA[7].x;
-    const int ita = (int)RA[6].y;
+    const double2 xy = RA[0], va = RA[1], rm = RA[2], hi = RA[6];
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = RA[3].x;
+    const double irhoa = 1.0 / rhoa;
+    const int ita = (int)hi.y;
     const int a = W.order[wc.tt];
     const double ca = W.c[a];
     const bool link = K.nnps == 2;
@@ -962,7 +758,7 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
         MW.m[W.n + t] = cm.hi;
         MW.ovf[t] = cm.ovf;
     }
-    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, recW, w.ok, i, k); };
+    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, recW, w.ok, i, k, !uni); };
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
     int cnt = 0;
     visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
@@ -1008,9 +804,9 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
         const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recS, accx_b);
         const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recS, accx_b);
         const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recS, accx_b);
-        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_WX_A>(lds, recS, wx.ok, i, k); };
+        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_WX_A>(lds, recS, wx.ok, i, k, !F.uni_x); };
         const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
-        const KC kx = make_kc(SKF, sph_mhsml(ha, F.uni_x ? recp(recS, 0)[3].x : ha));
+        const KC kx = make_kc(SKF, sph_mhsml(ha, F.uni_x ? recp(recS, 0)[6].x : ha));
         const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
         visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
             const double dx = xa - b.xy.x, dy = ya - b.xy.y;
@@ -1073,10 +869,10 @@ __global__ STAGE_ATTR void k_soil_a4(DPhase S, DPhase W, const double* recS, dou
     const WaveCtx wc = wave_ctx(S.n);
     const int t = wc.t;
     const double2* RA = recp(recS, wc.tt);
-    const double2 xy = RA[0], va = RA[1], rm = RA[2], hv = RA[3];
-    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hv.x, vofa = hv.y;
-    const double irhoa = RA[7].x;
-    const int ita = (int)RA[6].y;
+    const double2 xy = RA[0], va = RA[1], rm = RA[2], hi = RA[6];
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = RA[3].x;
+    const double irhoa = 1.0 / rhoa;
+    const int ita = (int)hi.y;
     const double ca = RA[4].y;
     const int a = S.order[wc.tt];
     const bool link = K.nnps == 2;
@@ -1102,7 +898,7 @@ __global__ STAGE_ATTR void k_soil_a4(DPhase S, DPhase W, const double* recS, dou
         MS.m[S.n + t] = cm.hi;
         MS.ovf[t] = cm.ovf;
     }
-    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, recS, w.ok, i, k); };
+    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, recS, w.ok, i, k, !uni); };
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;
     double avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
     int cnt = 0;
@@ -1151,9 +947,9 @@ __global__ STAGE_ATTR void k_soil_a4(DPhase S, DPhase W, const double* recS, dou
         const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recW, accx_b);
         const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recW, accx_b);
         const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recW, accx_b);
-        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_SX_A>(lds, recW, wx.ok, i, k); };
+        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_SX_A>(lds, recW, wx.ok, i, k, !F.uni_x); };
         const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
-        const KC kx = make_kc(SKF, sph_mhsml(F.uni_x ? recp(recW, 0)[3].x : ha, ha));
+        const KC kx = make_kc(SKF, sph_mhsml(F.uni_x ? recp(recW, 0)[6].x : ha, ha));
         const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
         visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
             const double dx = b.xy.x - xa, dy = b.xy.y - ya;   // water - soil
@@ -1193,8 +989,8 @@ __global__ STAGE_ATTR void k_soil_a4(DPhase S, DPhase W, const double* recS, dou
     S.c_next[a] = K.s_c;
     recSw[(size_t)R_STRIDE * t + R_C] = K.s_c;
     {
-        const double2 ps = RA[4], s12 = RA[5], s3 = RA[6];
-        const double p = ps.x, sxx = s12.x, sxy = s12.y, syy = s3.x;
+        const double2 ps = RA[4], s12 = RA[5];
+        const double p = ps.x, sxx = s12.x, sxy = s12.y, syy = RA[3].y;
         const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;
         double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
         if (K.plasticity == 3) {
@@ -1242,10 +1038,10 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
     const WaveCtx wc = wave_ctx(W.n);
     const int t = wc.t;
     const double2* RA = recp(recW, wc.tt);
-    const double2 axy = RA[0], av = RA[1], arm = RA[2], ahv = RA[3], apc = RA[4], a12 = RA[5], a3 = RA[6],
-                  aiv = RA[7];
-    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = ahv.x, vofa = ahv.y;
-    const double pa = apc.x, ca = apc.y, fxx_a = a12.x, fxy_a = a12.y, fyy_a = a3.x;
+    const double2 axy = RA[0], av = RA[1], arm = RA[2], a3 = RA[3], apc = RA[4], a12 = RA[5];
+    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = RA[6].x, vofa = a3.x;
+    const double pa = apc.x, ca = apc.y, fxx_a = a12.x, fxy_a = a12.y, fyy_a = a3.y;
+    const double irhoa = 1.0 / rhoa;
     const int a = W.order[wc.tt];
     const bool link = K.nnps == 2;
     const bool uni = link || F.uni_w;
@@ -1259,7 +1055,7 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
     auto acc_i = [&](int i, int k) {
         return P(xy_of<LB>(lds, recW, w.ok, i, k), uni ? ha : h_of<LB>(lds, recW, w.ok, i, k), g, K);
     };
-    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, recW, w.ok, i, k); };
+    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, recW, w.ok, i, k, !uni); };
     CatMask cm{0, 0, 0};
     if (wc.valid) {
         cm.lo = MW.m[t];
@@ -1275,7 +1071,7 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
         const Dist D = dist_of(dr);
         const double r = D.r, idr = D.idr;
         double ww, gx, gy;
-        const double amh = sph_mhsml(ha, b.hv.x);
+        const double amh = uni ? ha : sph_mhsml(ha, b.hv.x);
         if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, ww, gx, gy);
         else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, ww, gx, gy);
         const double Vb = b.iv.y, mb = b.rm.y;
@@ -1302,8 +1098,8 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
     double ax = -(vofa * Px) + Sxx + Sxyy;
     double ay = -(vofa * Py) + Sxyx + Syy;
     if (rhoa > 0.0) {
-        ax = ax * aiv.x;
-        ay = ay * aiv.x;
+        ax = ax * irhoa;
+        ay = ay * irhoa;
     }
     if (mask & M_ART) { ax = ax + artx; ay = ay + arty; }
     ax = ax + W.repx[a];
@@ -1334,10 +1130,9 @@ __global__ STAGE_ATTR void k_soil_b4(DPhase S, const double* recS, Masks MS, Gri
     const WaveCtx wc = wave_ctx(S.n);
     const int t = wc.t;
     const double2* RA = recp(recS, wc.tt);
-    const double2 axy = RA[0], av = RA[1], arm = RA[2], ahv = RA[3], apc = RA[4], a12 = RA[5], a3 = RA[6],
-                  aiv = RA[7];
-    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = ahv.x, vofa = ahv.y;
-    const double pa = apc.x, ca = apc.y, sxxa = a12.x, sxya = a12.y, syya = a3.x, irhoa = aiv.x;
+    const double2 axy = RA[0], av = RA[1], arm = RA[2], a3 = RA[3], apc = RA[4], a12 = RA[5];
+    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = RA[6].x, vofa = a3.x;
+    const double pa = apc.x, ca = apc.y, sxxa = a12.x, sxya = a12.y, syya = a3.y, irhoa = 1.0 / rhoa;
     const double ir2a = irhoa * irhoa;
     const double fpa = vofa * pa * ir2a, fxxa = vofa * sxxa * ir2a, fxya = vofa * sxya * ir2a,
                  fyya = vofa * syya * ir2a;
@@ -1354,7 +1149,7 @@ __global__ STAGE_ATTR void k_soil_b4(DPhase S, const double* recS, Masks MS, Gri
     auto acc_i = [&](int i, int k) {
         return P(xy_of<LB>(lds, recS, w.ok, i, k), uni ? ha : h_of<LB>(lds, recS, w.ok, i, k), g, K);
     };
-    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, recS, w.ok, i, k); };
+    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, recS, w.ok, i, k, !uni); };
     CatMask cm{0, 0, 0};
     if (wc.valid) {
         cm.lo = MS.m[t];
@@ -1370,7 +1165,7 @@ __global__ STAGE_ATTR void k_soil_b4(DPhase S, const double* recS, Masks MS, Gri
         const Dist D = dist_of(dr);
         const double r = D.r, idr = D.idr;
         double ww, gx, gy;
-        const double amh = sph_mhsml(ha, b.hv.x);
+        const double amh = uni ? ha : sph_mhsml(ha, b.hv.x);
         if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, ww, gx, gy);
         else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, ww, gx, gy);
         const double mb = b.rm.y, vofb = b.hv.y, irb = b.iv.x;
