@@ -5,7 +5,7 @@
 //             and its 128-byte neighbour record rec[t] (cell-sorted, so the
 //             records of a cell row are contiguous and a neighbour costs one
 //             cache line instead of one line per SoA field).  Record chunks
-//             (16 bytes): 0 x,y | 1 vx,vy | 2 rho,m | 3 vof,s3 | 4 p,c |
+//             (16 bytes): 0 x,y | 1 vx,vy | 2 rho,m | 3 vof,p | 4 s3,c |
 //             5 s1,s2 | 6 h,itype; m carries the sign of itype (negative for
 //             virtual particles).  The loops load only the chunks they use
 //             (partner loads are limited by the address unit, one address per
@@ -23,7 +23,7 @@
 
 // record slots (doubles)
 enum : int {
-    R_X = 0, R_Y, R_VX, R_VY, R_RHO, R_M, R_VOF, R_S3, R_P, R_C, R_S1, R_S2, R_H, R_IT, R_PAD0, R_PAD1,
+    R_X = 0, R_Y, R_VX, R_VY, R_RHO, R_M, R_VOF, R_P, R_S3, R_C, R_S1, R_S2, R_H, R_IT, R_PAD0, R_PAD1,
     R_STRIDE = 16
 };
 
@@ -315,8 +315,8 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
     R[0] = make_double2(P.x[i], P.y[i]);
     R[1] = make_double2(vx, vy);
     R[2] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
-    R[3] = make_double2(P.vof[i], syy);
-    R[4] = make_double2(p, P.c[i]);
+    R[3] = make_double2(P.vof[i], p);
+    R[4] = make_double2(syy, P.c[i]);
     R[5] = make_double2(sxx, sxy);
     R[6] = make_double2(P.hsml[i], (double)it);
 }
@@ -606,8 +606,8 @@ struct Nb {
         case 0: xy = q; break;
         case 1: v = q; break;
         case 2: rm = q; break;
-        case 3: hv.y = q.x; s3.x = q.y; break;   // vof, s3
-        case 4: pc = q; break;
+        case 3: hv.y = q.x; pc.x = q.y; break;   // vof, p
+        case 4: s3.x = q.x; pc.y = q.y; break;   // s3, c
         case 5: s12 = q; break;
         case 6: hv.x = q.x; s3.y = q.y; break;   // h, itype
         default: break;
@@ -707,8 +707,8 @@ constexpr int lds_rows(int a, int b) { return (a > b ? a : b) > 0 ? (a > b ? a :
 // Chunks read per partner by each loop (h only when it is not uniform).
 constexpr int NEED_SAME_A = 0x07 | NEED_IV;          // x,y v rho,m(itype in its sign); 1/rho, m/rho
 constexpr int NEED_WX_A = 0x0F | NEED_IV;            // soil partners of water: x,y v rho,m vof
-constexpr int NEED_SX_A = 0x1F | NEED_IV;            // water partners of soil: + p,c
-constexpr int NEED_B = 0x3F | NEED_IV;               // x,y v rho,m vof,s3 p,c s1,s2
+constexpr int NEED_SX_A = 0x0F | NEED_IV;            // water partners of soil: x,y v rho,m vof,p
+constexpr int NEED_B = 0x3F | NEED_IV;               // x,y v rho,m vof,p s3,c s1,s2
 
 // ------------------------------------------------------------------------
 // Stage A, water: same-phase terms (con_density, delta_sph(rho),
@@ -848,9 +848,9 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
         W.countiac[a] = cnt;
     }
     double* RW = recWw + (size_t)R_STRIDE * t;
-    reinterpret_cast<double2*>(RW)[4] = make_double2(p, cn);
+    RW[R_P] = p;
+    reinterpret_cast<double2*>(RW)[4] = make_double2(vofa * (mu * tyy), cn);
     reinterpret_cast<double2*>(RW)[5] = make_double2(vofa * (mu * txx), vofa * (mu * txy));
-    RW[R_S3] = vofa * (mu * tyy);
 }
 
 // ------------------------------------------------------------------------
@@ -989,8 +989,8 @@ __global__ STAGE_ATTR void k_soil_a4(DPhase S, DPhase W, const double* recS, dou
     S.c_next[a] = K.s_c;
     recSw[(size_t)R_STRIDE * t + R_C] = K.s_c;
     {
-        const double2 ps = RA[4], s12 = RA[5];
-        const double p = ps.x, sxx = s12.x, sxy = s12.y, syy = RA[3].y;
+        const double2 s12 = RA[5];
+        const double p = RA[3].y, sxx = s12.x, sxy = s12.y, syy = RA[4].x;
         const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;
         double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
         if (K.plasticity == 3) {
@@ -1040,7 +1040,7 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
     const double2* RA = recp(recW, wc.tt);
     const double2 axy = RA[0], av = RA[1], arm = RA[2], a3 = RA[3], apc = RA[4], a12 = RA[5];
     const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = RA[6].x, vofa = a3.x;
-    const double pa = apc.x, ca = apc.y, fxx_a = a12.x, fxy_a = a12.y, fyy_a = a3.y;
+    const double pa = a3.y, ca = apc.y, fxx_a = a12.x, fxy_a = a12.y, fyy_a = apc.x;
     const double irhoa = 1.0 / rhoa;
     const int a = W.order[wc.tt];
     const bool link = K.nnps == 2;
@@ -1132,7 +1132,7 @@ __global__ STAGE_ATTR void k_soil_b4(DPhase S, const double* recS, Masks MS, Gri
     const double2* RA = recp(recS, wc.tt);
     const double2 axy = RA[0], av = RA[1], arm = RA[2], a3 = RA[3], apc = RA[4], a12 = RA[5];
     const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = RA[6].x, vofa = a3.x;
-    const double pa = apc.x, ca = apc.y, sxxa = a12.x, sxya = a12.y, syya = a3.y, irhoa = 1.0 / rhoa;
+    const double pa = a3.y, ca = apc.y, sxxa = a12.x, sxya = a12.y, syya = apc.x, irhoa = 1.0 / rhoa;
     const double ir2a = irhoa * irhoa;
     const double fpa = vofa * pa * ir2a, fxxa = vofa * sxxa * ir2a, fxya = vofa * sxya * ir2a,
                  fyya = vofa * syya * ir2a;
