@@ -222,7 +222,7 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
     int cnt = 0;
     visit_f(cm, L, t, [&](int k) { return acc_q(recq(qW, k)[0]); }, [&](int k) {
         const float4* Q = recq(qW, k);
-        const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2];
+        const float4 b0 = Q[0], b1 = Q[1];   // 1/rho and m/rho recomputed: two loads per partner
         float dx, dy;
         sep(me, b0, X.cs, dx, dy);
         const float dr = dx * dx + dy * dy;
@@ -232,7 +232,7 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
         if (uni) kern_f<SKF, true>(kc, r, ir, dx, dy, ww, gx, gy);
         else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
         ++cnt;
-        const float mb = b1.w, Vb = b2.z, irb = b2.y;
+        const float mb = b1.w, irb = __builtin_amdgcn_rcpf(b1.z + X.rho0_w), Vb = mb * irb;
         const float dvx = vxa - b1.x, dvy = vya - b1.y;
         con.add(mb * (dvx * gx + dvy * gy));
         drd.add(mb * (2.0f * (dra - b1.z) * idr * (dx * gx + dy * gy)) * irb);
@@ -356,7 +356,7 @@ __global__ STAGE_ATTR_MX void k_soil_a_mx(DPhase S, DPhase W, const float* qS, f
     int cnt = 0;
     visit_f(cm, L, t, [&](int k) { return acc_q(recq(qS, k)[0]); }, [&](int k) {
         const float4* Q = recq(qS, k);
-        const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2];
+        const float4 b0 = Q[0], b1 = Q[1];   // 1/rho and m/rho recomputed: two loads per partner
         float dx, dy;
         sep(me, b0, X.cs, dx, dy);
         const float dr = dx * dx + dy * dy;
@@ -366,7 +366,7 @@ __global__ STAGE_ATTR_MX void k_soil_a_mx(DPhase S, DPhase W, const float* qS, f
         if (uni) kern_f<SKF, true>(kc, r, ir, dx, dy, ww, gx, gy);
         else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
         ++cnt;
-        const float mb = b1.w, Vb = b2.z, irb = b2.y;
+        const float mb = b1.w, irb = __builtin_amdgcn_rcpf(b1.z + X.rho0_s), Vb = mb * irb;
         const float dvx = vxa - b1.x, dvy = vya - b1.y;
         const float dvg = dvx * gx + dvy * gy;
         con.add(mb * dvg);
