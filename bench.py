@@ -31,18 +31,19 @@ HBM_PEAK_GBS = 8000.0
 # once, stage-A -> stage-B intermediates written and read once), split by stage.
 # FP64:
 #   water: A = 100 state + 40 intermediates = 140, B = 100 + 40 + 88 = 228 (368 total)
-#   soil:  A = 124 + 48 = 172,                    B = 124 + 48 + 112 = 284 (456 total)
+#   soil:  one pass for both stages (its stage B needs nothing from the other
+#          soil particles' stage A): 124 state + 48 + 112 outputs = 284
 # mixed (integrated fields x, v, rho, vof, stress, p in FP64, the rest FP32):
 #   water: A = 76 + 20 = 96,   B = 76 + 20 + 68 = 164  (260 total)
 #   soil:  A = 100 + 24 = 124, B = 100 + 24 + 92 = 216 (340 total)
 #   cell list: 28 per particle and step
 ALGO_BYTES = {
     "fp64": {"water_a": ("water", 140), "water_b": ("water", 228),
-             "soil_a": ("soil", 172), "soil_b": ("soil", 284)},
+             "soil_ab": ("soil", 284)},
     "mixed": {"water_a": ("water", 96), "water_b": ("water", 164),
               "soil_a": ("soil", 124), "soil_b": ("soil", 216)},
 }
-STEP_BYTES = {"fp64": {"water": 368 + 28, "soil": 456 + 28},
+STEP_BYTES = {"fp64": {"water": 368 + 28, "soil": 284 + 28},
               "mixed": {"water": 260 + 28, "soil": 340 + 28}}
 
 
@@ -315,7 +316,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is a design, see DESIGN.md)", file=sys.stderr)
+              "(the slab decomposition over several GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     from sources_amd import scenarios
@@ -358,7 +359,7 @@ def main():
     hip.sync()
     kern = {}
     for name in ("cell_key", "cell_scatter", "cell_sort", "scan", "fill", "first_half",
-                 "water_a", "soil_a", "water_b", "soil_b", "final_water", "final_soil"):
+                 "water_a", "soil_a", "soil_ab", "water_b", "soil_b", "final_water", "final_soil"):
         n, t = s.profile_get(name)
         if n:
             kern[name] = {"launches": n, "avg_us": 1e3 * t / n, "total_ms_per_step": t / args.prof_steps}

@@ -15,7 +15,9 @@
 //             accepted partners kept as one concatenated 128-bit mask; then a
 //             dense loop over the mask bits evaluates the pair terms.  The masks
 //             are stored for stage B, so each pair is tested once per step;
-//   stage B   dense loop over the stored masks.
+//   stage B   dense loop over the stored masks (water).  Soil stage B needs
+//             nothing from the stage A of other soil particles, so soil runs
+//             both stages in one kernel over one partner loop.
 // Lanes of a wavefront hold consecutive cell-sorted particles; the dense loops
 // diverge only by the lanes' neighbour-count spread.
 #pragma once
@@ -46,12 +48,32 @@ enum : int {
 #else
 #define STAGE_ATTR __launch_bounds__(64)
 #endif
+#ifndef SPHG_AB_WAVES
+#define SPHG_AB_WAVES 0      // the same for the one-pass soil kernel
+#endif
+#if SPHG_AB_WAVES > 0
+#define STAGE_ATTR_AB __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPHG_AB_WAVES, 8)))
+#else
+#define STAGE_ATTR_AB __launch_bounds__(64)
+#endif
 
 struct FusedK {
     // dr thresholds and uniform-h flags for same-phase (water, soil) and cross pairs
     double thr_w, thr_s, thr_x;
     int uni_w, uni_s, uni_x;
 };
+
+// q^n for an integer 0 <= n <= 64 by binary powering (within a few ulp of pow).
+__device__ __forceinline__ double ipow(double q, int n)
+{
+    double r = 1.0, b = q;
+#pragma unroll 1
+    for (; n > 0; n >>= 1) {
+        if (n & 1) r *= b;
+        b *= b;
+    }
+    return r;
+}
 
 // 1/d and 1/sqrt(d) from the hardware estimates plus two Newton steps
 // (each step doubles the correct bits), i.e. to about one ulp; far shorter
@@ -718,7 +740,7 @@ constexpr int NEED_B = 0x3F | NEED_IV;               // x,y v rho,m vof,p s3,c s
 // and the record slots p, c_new, vof*sigma.  LA / LX select the record
 // chunks staged in the LDS window for the same-phase and cross loops.
 // ------------------------------------------------------------------------
-template <int SKF, int LA, int LX>
+template <int SKF, int LA, int LX, bool U>
 __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, double* recWw,
                                                  const double* recS, Masks MW, Grid g, KParams K, FusedK F,
                                                  int cross)
@@ -735,7 +757,9 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     const int a = W.order[wc.tt];
     const double ca = W.c[a];
     const bool link = K.nnps == 2;
-    const bool uni = link || F.uni_w;
+    // U (checked by the host): h uniform in both phases, integer repulsion exponents
+    const bool uni = U || link || F.uni_w;
+    const bool ux = U || F.uni_x;
     const KC kc = make_kc(SKF, link ? K.h1_water : ha);
     int cx, cy;
     lane_cell(g, xa, ya, cx, cy);
@@ -782,7 +806,7 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
         avy -= dvy * f;
         if (ita > 0 && !(r > W.dspp) && (int)b.s3.y < 0) {
             const double q = W.dspp / r;
-            const double fr = (pow(q, K.p1) - pow(q, K.p2)) * idr;
+            const double fr = (U ? ipow(q, K.ip1) - ipow(q, K.ip2) : pow(q, K.p1) - pow(q, K.p2)) * idr;
             repx += K.dd * dx * fr;
             repy += K.dd * dy * fr;
         }
@@ -796,17 +820,17 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
         const Win wx = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
         if (wx.ok) win_load<LX>(wx, recS, lds, wc.lane);
         const LaneRows LXr = lane_rows(block_rows(g, S.cell_start, xa, ya), wx, cy);
-        const Pred PX{xa, ya, ha, F.thr_x, F.uni_x, 0};
+        const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
         auto accx_b = [&](int i, int k, double2 b) {
-            return PX(b, F.uni_x ? ha : h_of<LX>(lds, recS, wx.ok, i, k), g, K);
+            return PX(b, ux ? ha : h_of<LX>(lds, recS, wx.ok, i, k), g, K);
         };
         auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, recS, wx.ok, i, k)); };
         const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recS, accx_b);
         const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recS, accx_b);
         const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recS, accx_b);
-        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_WX_A>(lds, recS, wx.ok, i, k, !F.uni_x); };
+        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_WX_A>(lds, recS, wx.ok, i, k, !ux); };
         const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
-        const KC kx = make_kc(SKF, sph_mhsml(ha, F.uni_x ? recp(recS, 0)[6].x : ha));
+        const KC kx = make_kc(SKF, sph_mhsml(ha, ux ? recp(recS, 0)[6].x : ha));
         const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
         visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
             const double dx = xa - b.xy.x, dy = ya - b.xy.y;
@@ -814,7 +838,7 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
             const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
             const double r = dr * ir;
             double ww, gx, gy;
-            if (F.uni_x) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+            if (ux) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
             else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, ir, dx, dy, ww, gx, gy);
             ++cntx;
             const double mb = b.rm.y;
@@ -851,177 +875,6 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     RW[R_P] = p;
     reinterpret_cast<double2*>(RW)[4] = make_double2(vofa * (mu * tyy), cn);
     reinterpret_cast<double2*>(RW)[5] = make_double2(vofa * (mu * txx), vofa * (mu * txy));
-}
-
-// ------------------------------------------------------------------------
-// Stage A, soil: same-phase terms (con_density, delta_sph(rho),
-// shear_strain_rate, velocity_divergence, Jaumann spin, av_vel,
-// repulsive_force), Darcy drag and pore-water pressure from water partners
-// (new water p), then plastic_or_not + plastic_flow_rule3 + Jaumann_rate.
-// ------------------------------------------------------------------------
-template <int SKF, int LA, int LX>
-__global__ STAGE_ATTR void k_soil_a4(DPhase S, DPhase W, const double* recS, double* recSw,
-                                                const double* recW, Masks MS, Grid g, KParams K, FusedK F,
-                                                int cross)
-{
-    static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
-    __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
-    const WaveCtx wc = wave_ctx(S.n);
-    const int t = wc.t;
-    const double2* RA = recp(recS, wc.tt);
-    const double2 xy = RA[0], va = RA[1], rm = RA[2], hi = RA[6];
-    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = RA[3].x;
-    const double irhoa = 1.0 / rhoa;
-    const int ita = (int)hi.y;
-    const double ca = RA[4].y;
-    const int a = S.order[wc.tt];
-    const bool link = K.nnps == 2;
-    const bool uni = link || F.uni_s;
-    const KC kc = make_kc(SKF, link ? K.h1_soil : ha);
-    int cx, cy;
-    lane_cell(g, xa, ya, cx, cy);
-
-    const Win w = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
-    if (w.ok) win_load<LA>(w, recS, lds, wc.lane);
-    const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya), w, cy);
-    const Pred P{xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0};
-    auto acc_b = [&](int i, int k, double2 b) {
-        return P(b, uni ? ha : h_of<LA>(lds, recS, w.ok, i, k), g, K);
-    };
-    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, recS, w.ok, i, k)); };
-    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], recS, acc_b);
-    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], recS, acc_b);
-    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], recS, acc_b);
-    const CatMask cm = cat_masks(L, m0, m1, m2);
-    if (wc.valid) {
-        MS.m[t] = cm.lo;
-        MS.m[S.n + t] = cm.hi;
-        MS.ovf[t] = cm.ovf;
-    }
-    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, recS, w.ok, i, k, !uni); };
-    double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;
-    double avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
-    int cnt = 0;
-    visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
-        const double dx = xa - b.xy.x, dy = ya - b.xy.y;
-        const double dr = dx * dx + dy * dy;
-        const Dist D = dist_of(dr);
-        const double r = D.r, idr = D.idr;
-        double ww, gx, gy;
-        if (uni) kern_fast<SKF, true>(kc, r, D.ir, dx, dy, ww, gx, gy);
-        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, D.ir, dx, dy, ww, gx, gy);
-        ++cnt;
-        const double mb = b.rm.y, Vb = b.iv.y;
-        const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
-        const double dvg = dvx * gx + dvy * gy;
-        con += mb * dvg;
-        drd += mb * (2.0 * (rhoa - b.rm.x) * idr * (dx * gx + dy * gy)) * b.iv.x;
-        txx += Vb * (-2.0 * dvx * gx + dvy * gy);
-        txy += Vb * (-(dvx * gy) - dvy * gx);
-        tyy += Vb * (-2.0 * dvy * gy + dvx * gx);
-        vcc -= Vb * dvg;
-        wxy += Vb * (-(dvx * gy) + dvy * gx);
-        const double f = mb * ww * (2.0 * fast_rcp(rhoa + b.rm.x));
-        avx -= dvx * f;
-        avy -= dvy * f;
-        if (ita > 0 && !(r > S.dspp) && (int)b.s3.y < 0) {
-            const double q = S.dspp / r;
-            const double fr = (pow(q, K.p1) - pow(q, K.p2)) * idr;
-            repx += K.dd * dx * fr;
-            repy += K.dd * dy * fr;
-        }
-    });
-
-    double dg_x = 0.0, dg_y = 0.0, po_x = 0.0, po_y = 0.0;
-    int cntx = 0;
-    if (cross) {
-        if (LA != 0) __syncthreads();
-        const Win wx = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
-        if (wx.ok) win_load<LX>(wx, recW, lds, wc.lane);
-        const LaneRows LXr = lane_rows(block_rows(g, W.cell_start, xa, ya), wx, cy);
-        const Pred PX{xa, ya, ha, F.thr_x, F.uni_x, 0};
-        auto accx_b = [&](int i, int k, double2 b) {
-            return PX(b, F.uni_x ? ha : h_of<LX>(lds, recW, wx.ok, i, k), g, K);
-        };
-        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, recW, wx.ok, i, k)); };
-        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recW, accx_b);
-        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recW, accx_b);
-        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recW, accx_b);
-        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_SX_A>(lds, recW, wx.ok, i, k, !F.uni_x); };
-        const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
-        const KC kx = make_kc(SKF, sph_mhsml(F.uni_x ? recp(recW, 0)[6].x : ha, ha));
-        const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
-        visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
-            const double dx = b.xy.x - xa, dy = b.xy.y - ya;   // water - soil
-            const double dr = dx * dx + dy * dy;
-            const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
-            const double r = dr * ir;
-            double ww, gx, gy;
-            if (F.uni_x) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
-            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(b.hv.x, ha)), r, ir, dx, dy, ww, gx, gy);
-            ++cntx;
-            const double mw = b.rm.y, rhow = b.rm.x;
-            const double sc = mw * (b.hv.y * rhow * cfs) * (ww * b.iv.x * irhoa);
-            dg_x += sc * (b.v.x - vxa);
-            dg_y += sc * (b.v.y - vya);
-            const double mprr = mw * b.pc.x * (b.iv.x * irhoa);
-            po_x += mprr * gx;
-            po_y += mprr * gy;
-        });
-        po_x *= vofa;
-        po_y *= vofa;
-    }
-    if (!wc.valid) return;
-    txx *= SPH_C23_F;
-    tyy *= SPH_C23_F;
-    wxy *= 0.5;
-    S.drho[a] = con + (K.delta * ha * ca) * drd;
-    S.txx[a] = txx; S.txy[a] = txy; S.tyy[a] = tyy; S.vcc[a] = vcc; S.wxy[a] = wxy;
-    if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
-    S.avx[a] = avx; S.avy[a] = avy;
-    S.repx[a] = repx; S.repy[a] = repy;
-    if (cross) {
-        S.drgx[a] = dg_x; S.drgy[a] = dg_y; S.porx[a] = po_x; S.pory[a] = po_y;
-        S.countiac[a] = cntx;
-    } else {
-        S.countiac[a] = cnt;
-    }
-    S.c_next[a] = K.s_c;
-    recSw[(size_t)R_STRIDE * t + R_C] = K.s_c;
-    {
-        const double2 s12 = RA[5];
-        const double p = RA[3].y, sxx = s12.x, sxy = s12.y, syy = RA[4].x;
-        const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;
-        double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
-        if (K.plasticity == 3) {
-            const double I1 = 3.0 * p;
-            const double j2r = sxx * sxx + 2.0 * (sxy * sxy) + syy * syy + (sxx + syy) * (sxx + syy);
-            const double J2 = __builtin_sqrt(j2r / 2.0) + 1.0e-6;
-            const int fail = fabs(J2 - alpha * I1) < SPH_YIELD_TOL_F ? 1 : 0;
-            if (fail) {
-                const double exx = txx * 0.5 + vcc * (1.0 / 3.0);
-                const double exy = txy * 0.5;
-                const double eyy = tyy * 0.5 + vcc * (1.0 / 3.0);
-                const double sde = sxx * exx + 2.0 * sxy * exy + syy * eyy;
-                const double sj = __builtin_sqrt(j2r / 2.0 + 1.0e-10);
-                const double gs = G / sj;
-                const double dl = (3.0 * alpha * Kb * vcc + gs * sde) / (9.0 * (alpha * alpha) * Kb + G);
-                if (!(dl < 0.0)) {
-                    dsxx = -dl * (gs * sxx);
-                    dsxy = -dl * (gs * sxy);
-                    dsyy = -dl * (gs * syy);
-                    dpv = 3.0 * Kb * alpha * dl;
-                    S.eps_p[a] = S.eps_p[a] + dl * sxy / (2.0 * sj) * SPH_EPSP_F;
-                }
-            }
-            S.fail[a] = fail;
-        }
-        dsxx = dsxx + G * txx + 2.0 * sxy * wxy;
-        dsxy = dsxy + G * txy - (sxx - syy) * wxy;
-        dsyy = dsyy + G * tyy - 2.0 * sxy * wxy;
-        dpv = dpv - Kb * vcc;
-        S.dsxx[a] = dsxx; S.dsxy[a] = dsxy; S.dsyy[a] = dsyy; S.dp[a] = dpv;
-    }
 }
 
 // ------------------------------------------------------------------------
@@ -1117,48 +970,104 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
 }
 
 // ------------------------------------------------------------------------
-// Stage B, soil: -df3(vof*p) + df3(vof*sigma) (:1270-1277), art_visc,
-// delta_sph on p, sxx, sxy, syy; /rho, + repulsion, + gravity, + Darcy drag,
-// + pore-water pressure force.
+// Soil, stages A and B in one kernel.  Soil stage B reads only what the pack
+// wrote (p, sigma, vof of the partners) and the new sound speed, which is the
+// constant s_c for every soil particle, so both stages share one scan and one
+// dense loop over the same partner set (no stored masks, one partner load):
+//   same phase  con_density, delta_sph(rho), shear_strain_rate,
+//               velocity_divergence, Jaumann spin, av_vel, repulsive_force
+//               (stage A); -df3(vof*p) + df3(vof*sigma) (:1270-1277),
+//               art_visc, delta_sph on p, sxx, sxy, syy (stage B);
+//   cross       Darcy drag and pore-water pressure from the water partners
+//               (new water p);
+//   tail        plastic_or_not + plastic_flow_rule3 + Jaumann_rate, then
+//               /rho, + repulsion, + gravity, + Darcy drag, + pore pressure.
 // ------------------------------------------------------------------------
-template <int SKF, int LB>
-__global__ STAGE_ATTR void k_soil_b4(DPhase S, const double* recS, Masks MS, Grid g, KParams K,
-                                                FusedK F, int mask)
+template <int SKF, int LA, int LX, bool U>
+__global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS, const double* recW, Grid g,
+                                      KParams K, FusedK F, int cross, int mask)
 {
-    static_assert(LB == 0 || (LB & 1), "window needs positions");
-    __shared__ double2 lds[lds_rows(Lds<LB>::n, 0)][WIN_N];
+    static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
+    __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
     const WaveCtx wc = wave_ctx(S.n);
     const int t = wc.t;
     const double2* RA = recp(recS, wc.tt);
-    const double2 axy = RA[0], av = RA[1], arm = RA[2], a3 = RA[3], apc = RA[4], a12 = RA[5];
-    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = RA[6].x, vofa = a3.x;
-    const double pa = a3.y, ca = apc.y, sxxa = a12.x, sxya = a12.y, syya = apc.x, irhoa = 1.0 / rhoa;
-    const double ir2a = irhoa * irhoa;
+    const double2 xy = RA[0], va = RA[1], rm = RA[2], a3 = RA[3], a4 = RA[4], a12 = RA[5], hi = RA[6];
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = a3.x;
+    const double pa = a3.y, sxxa = a12.x, sxya = a12.y, syya = a4.x;
+    const double ca = a4.y;          // sound speed before this step (delta_sph of rho)
+    const double cn = K.s_c;         // sound speed after it (stage B)
+    const double irhoa = 1.0 / rhoa, ir2a = irhoa * irhoa;
     const double fpa = vofa * pa * ir2a, fxxa = vofa * sxxa * ir2a, fxya = vofa * sxya * ir2a,
                  fyya = vofa * syya * ir2a;
+    const int ita = (int)hi.y;
     const int a = S.order[wc.tt];
     const bool link = K.nnps == 2;
-    const bool uni = link || F.uni_s;
+    // U (checked by the host): h uniform in both phases, integer repulsion exponents
+    const bool uni = U || link || F.uni_s;
+    const bool ux = U || F.uni_x;
     const KC kc = make_kc(SKF, link ? K.h1_soil : ha);
     int cx, cy;
     lane_cell(g, xa, ya, cx, cy);
-    const Win w = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LB != 0);
-    if (w.ok) win_load<LB>(w, recS, lds, wc.lane);
+
+    double dg_x = 0.0, dg_y = 0.0, po_x = 0.0, po_y = 0.0;
+    int cntx = 0;
+    if (cross) {
+        const Win wx = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
+        if (wx.ok) win_load<LX>(wx, recW, lds, wc.lane);
+        const LaneRows LXr = lane_rows(block_rows(g, W.cell_start, xa, ya), wx, cy);
+        const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
+        auto accx_b = [&](int i, int k, double2 b) {
+            return PX(b, ux ? ha : h_of<LX>(lds, recW, wx.ok, i, k), g, K);
+        };
+        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, recW, wx.ok, i, k)); };
+        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recW, accx_b);
+        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recW, accx_b);
+        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recW, accx_b);
+        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_SX_A>(lds, recW, wx.ok, i, k, !ux); };
+        const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
+        const KC kx = make_kc(SKF, sph_mhsml(ux ? recp(recW, 0)[6].x : ha, ha));
+        const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
+        visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
+            const double dx = b.xy.x - xa, dy = b.xy.y - ya;   // water - soil
+            const double dr = dx * dx + dy * dy;
+            const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
+            const double r = dr * ir;
+            double ww, gx, gy;
+            if (ux) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(b.hv.x, ha)), r, ir, dx, dy, ww, gx, gy);
+            ++cntx;
+            const double mw = b.rm.y, rhow = b.rm.x;
+            const double sc = mw * (b.hv.y * rhow * cfs) * (ww * b.iv.x * irhoa);
+            dg_x += sc * (b.v.x - vxa);
+            dg_y += sc * (b.v.y - vya);
+            const double mprr = mw * b.pc.x * (b.iv.x * irhoa);
+            po_x += mprr * gx;
+            po_y += mprr * gy;
+        });
+        po_x *= vofa;
+        po_y *= vofa;
+    }
+    if (cross && LX != 0) __syncthreads();
+    const Win w = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
+    if (w.ok) win_load<LA>(w, recS, lds, wc.lane);
     const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya), w, cy);
     const Pred P{xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0};
-    auto acc_i = [&](int i, int k) {
-        return P(xy_of<LB>(lds, recS, w.ok, i, k), uni ? ha : h_of<LB>(lds, recS, w.ok, i, k), g, K);
+    auto acc_b = [&](int i, int k, double2 b) {
+        return P(b, uni ? ha : h_of<LA>(lds, recS, w.ok, i, k), g, K);
     };
-    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, recS, w.ok, i, k, !uni); };
-    CatMask cm{0, 0, 0};
-    if (wc.valid) {
-        cm.lo = MS.m[t];
-        cm.hi = MS.m[S.n + t];
-        cm.ovf = MS.ovf[t];
-    }
+    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, recS, w.ok, i, k)); };
+    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], recS, acc_b);
+    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], recS, acc_b);
+    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], recS, acc_b);
+    const CatMask cm = cat_masks(L, m0, m1, m2);
+    auto load = [&](int i, int k) { return load_nb<LA, NEED_B>(lds, recS, w.ok, i, k, !uni); };
     const double et2 = K.etq * K.etq;
-    double Ax = 0.0, Ay = 0.0, Bxx = 0.0, Cxyy = 0.0, Cxyx = 0.0, Dyy = 0.0;
+    double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;
+    double avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
+    double Fx = 0.0, Fy = 0.0;
     double artx = 0.0, arty = 0.0, dP = 0.0, dXX = 0.0, dXY = 0.0, dYY = 0.0;
+    int cnt = 0;
     visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
         const double dx = xa - b.xy.x, dy = ya - b.xy.y;
         const double dr = dx * dx + dy * dy;
@@ -1166,25 +1075,41 @@ __global__ STAGE_ATTR void k_soil_b4(DPhase S, const double* recS, Masks MS, Gri
         const double r = D.r, idr = D.idr;
         double ww, gx, gy;
         const double amh = uni ? ha : sph_mhsml(ha, b.hv.x);
-        if (uni) kern_fast<SKF, false>(kc, r, D.ir, dx, dy, ww, gx, gy);
-        else kern_fast<SKF, false>(make_kc(SKF, amh), r, D.ir, dx, dy, ww, gx, gy);
-        const double mb = b.rm.y, vofb = b.hv.y, irb = b.iv.x;
-        const double ir2b = irb * irb;
+        if (uni) kern_fast<SKF, true>(kc, r, D.ir, dx, dy, ww, gx, gy);
+        else kern_fast<SKF, true>(make_kc(SKF, amh), r, D.ir, dx, dy, ww, gx, gy);
+        ++cnt;
+        const double mb = b.rm.y, Vb = b.iv.y, irb = b.iv.x;
+        // stage A
+        const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
+        const double dvg = dvx * gx + dvy * gy;
+        con += mb * dvg;
+        drd += mb * (2.0 * (rhoa - b.rm.x) * idr * (dx * gx + dy * gy)) * irb;
+        txx += Vb * (-2.0 * dvx * gx + dvy * gy);
+        txy += Vb * (-(dvx * gy) - dvy * gx);
+        tyy += Vb * (-2.0 * dvy * gy + dvx * gx);
+        vcc -= Vb * dvg;
+        wxy += Vb * (-(dvx * gy) + dvy * gx);
+        const double f = mb * ww * (2.0 * fast_rcp(rhoa + b.rm.x));
+        avx -= dvx * f;
+        avy -= dvy * f;
+        if (ita > 0 && !(r > S.dspp) && (int)b.s3.y < 0) {
+            const double q = S.dspp / r;
+            const double fr = (U ? ipow(q, K.ip1) - ipow(q, K.ip2) : pow(q, K.p1) - pow(q, K.p2)) * idr;
+            repx += K.dd * dx * fr;
+            repy += K.dd * dy * fr;
+        }
+        // stage B
+        const double vofb = b.hv.y, ir2b = irb * irb;
         const double fp = fpa + vofb * b.pc.x * ir2b;
         const double fxx = fxxa + vofb * b.s12.x * ir2b;
         const double fxy = fxya + vofb * b.s12.y * ir2b;
         const double fyy = fyya + vofb * b.s3.x * ir2b;
-        Ax += mb * (fp * gx);
-        Ay += mb * (fp * gy);
-        Bxx += mb * (fxx * gx);
-        Cxyy += mb * (fxy * gy);
-        Cxyx += mb * (fxy * gx);
-        Dyy += mb * (fyy * gy);
-        const double vr = (vxa - b.v.x) * dx + (vya - b.v.y) * dy;
+        Fx += mb * ((fxx - fp) * gx + fxy * gy);
+        Fy += mb * ((fyy - fp) * gy + fxy * gx);
+        const double vr = dvx * dx + dvy * dy;
         {   // art_visc, branch-free: approaching pairs only (vr < 0)
             const double muv = amh * vr * fast_rcp(dr + amh * amh * et2);
-            const double mc = 0.5 * (ca + b.pc.y);
-            double piv = (K.beta * muv - K.alpha * mc) * muv * (2.0 * fast_rcp(rhoa + b.rm.x));
+            double piv = (K.beta * muv - K.alpha * cn) * muv * (2.0 * fast_rcp(rhoa + b.rm.x));
             piv = vr < 0.0 ? piv : 0.0;
             artx -= mb * piv * gx;
             arty -= mb * piv * gy;
@@ -1195,25 +1120,73 @@ __global__ STAGE_ATTR void k_soil_b4(DPhase S, const double* recS, Masks MS, Gri
         dXY += hh * (sxya - b.s12.y);
         dYY += hh * (syya - b.s3.x);
     });
+
     if (!wc.valid) return;
-    double ax = -(Ax * rhoa) + Bxx * rhoa + Cxyy * rhoa;
-    double ay = -(Ay * rhoa) + Cxyx * rhoa + Dyy * rhoa;
+    txx *= SPH_C23_F;
+    tyy *= SPH_C23_F;
+    wxy *= 0.5;
+    S.drho[a] = con + (K.delta * ha * ca) * drd;
+    S.txx[a] = txx; S.txy[a] = txy; S.tyy[a] = tyy; S.vcc[a] = vcc; S.wxy[a] = wxy;
+    if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
+    S.avx[a] = avx; S.avy[a] = avy;
+    S.repx[a] = repx; S.repy[a] = repy;
+    if (cross) {
+        S.drgx[a] = dg_x; S.drgy[a] = dg_y; S.porx[a] = po_x; S.pory[a] = po_y;
+        S.countiac[a] = cntx;
+    } else {
+        S.countiac[a] = cnt;
+    }
+    S.c_next[a] = cn;
+
+    const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;
+    double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
+    if (K.plasticity == 3) {
+        const double I1 = 3.0 * pa;
+        const double j2r = sxxa * sxxa + 2.0 * (sxya * sxya) + syya * syya + (sxxa + syya) * (sxxa + syya);
+        const double J2 = __builtin_sqrt(j2r / 2.0) + 1.0e-6;
+        const int fail = fabs(J2 - alpha * I1) < SPH_YIELD_TOL_F ? 1 : 0;
+        if (fail) {
+            const double exx = txx * 0.5 + vcc * (1.0 / 3.0);
+            const double exy = txy * 0.5;
+            const double eyy = tyy * 0.5 + vcc * (1.0 / 3.0);
+            const double sde = sxxa * exx + 2.0 * sxya * exy + syya * eyy;
+            const double sj = __builtin_sqrt(j2r / 2.0 + 1.0e-10);
+            const double gs = G / sj;
+            const double dl = (3.0 * alpha * Kb * vcc + gs * sde) / (9.0 * (alpha * alpha) * Kb + G);
+            if (!(dl < 0.0)) {
+                dsxx = -dl * (gs * sxxa);
+                dsxy = -dl * (gs * sxya);
+                dsyy = -dl * (gs * syya);
+                dpv = 3.0 * Kb * alpha * dl;
+                S.eps_p[a] = S.eps_p[a] + dl * sxya / (2.0 * sj) * SPH_EPSP_F;
+            }
+        }
+        S.fail[a] = fail;
+    }
+    dsxx = dsxx + G * txx + 2.0 * sxya * wxy;
+    dsxy = dsxy + G * txy - (sxxa - syya) * wxy;
+    dsyy = dsyy + G * tyy - 2.0 * sxya * wxy;
+    dpv = dpv - Kb * vcc;
+
+    double ax = Fx * rhoa;
+    double ay = Fy * rhoa;
     if (rhoa > 0.0) { ax = ax * irhoa; ay = ay * irhoa; }
     if (mask & M_ART) { ax = ax + artx; ay = ay + arty; }
-    ax = ax + S.repx[a];
-    ay = ay + S.repy[a];
+    ax = ax + repx;
+    ay = ay + repy;
     ay = ay + K.gravity;
     if (mask & M_CROSS) {
-        ax = ax + S.drgx[a]; ay = ay + S.drgy[a];
-        ax = ax + S.porx[a]; ay = ay + S.pory[a];
+        ax = ax + dg_x; ay = ay + dg_y;
+        ax = ax + po_x; ay = ay + po_y;
     }
+    if (mask & M_DELTA_FIELD) {
+        const double cf = K.delta * ha * cn;
+        dpv = dpv + cf * dP;
+        dsxx = dsxx + cf * dXX;
+        dsxy = dsxy + cf * dXY;
+        dsyy = dsyy + cf * dYY;
+    }
+    S.dsxx[a] = dsxx; S.dsxy[a] = dsxy; S.dsyy[a] = dsyy; S.dp[a] = dpv;
     S.ax[a] = ax;
     S.ay[a] = ay;
-    if (mask & M_DELTA_FIELD) {
-        const double cf = K.delta * ha * ca;
-        S.dp[a] = S.dp[a] + cf * dP;
-        S.dsxx[a] = S.dsxx[a] + cf * dXX;
-        S.dsxy[a] = S.dsxy[a] + cf * dXY;
-        S.dsyy[a] = S.dsyy[a] + cf * dYY;
-    }
 }

@@ -42,6 +42,7 @@ struct KParams {
     int artificial_density, visc_artificial, average_velocity;
     int soil_artificial_stress, water_artificial_volume, volume_fraction;
     int plasticity, yield_criterion, two_phase;
+    int ip1, ip2;               // p1, p2 when they are integers in [0, 64], else -1
 };
 
 // Stage masks (granular entry points run a subset of a fused pass).

@@ -274,6 +274,9 @@ void set_kparams(sphg_ctx* c)
     K.scale_k = (double)sk;
     K.alpha = P.alpha; K.beta = P.beta; K.etq = P.etq;
     K.dd = P.dd; K.p1 = P.p1; K.p2 = P.p2;
+    auto small_int = [](double v) { return (v >= 0.0 && v <= 64.0 && v == (double)(int)v) ? (int)v : -1; };
+    K.ip1 = small_int(P.p1);
+    K.ip2 = small_int(P.p2);
     K.delta = P.delta; K.epsilon = P.epsilon;
     K.gravity = P.gravity; K.numeric_gravity = P.numeric_gravity;
     K.w_rho0 = P.water_rho0; K.w_b = P.water_b; K.w_gamma = P.water_gamma; K.w_c = P.water_c;
@@ -702,16 +705,23 @@ int stages_fused(sphg_ctx* c)
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
     const int cross = c->P.single_phase ? 0 : 1;
-    RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX>, nblk(W.n, 64), 64, W.d, S.d, (const double*)W.rec, W.rec,
-              (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
-    RC(launch(c, "soil_a", k_soil_a4<SKF, SPHG_LA, SPHG_LX>, nblk(S.n, 64), 64, S.d, W.d, (const double*)S.rec, S.rec,
-              (const double*)W.rec, S.masks, c->gl[1], c->K, c->F, cross));
+    const bool rep_int = c->K.ip1 >= 0 && c->K.ip2 >= 0;
+    if ((c->K.nnps == 2 || c->F.uni_w) && c->F.uni_x && rep_int)
+        RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX, true>, nblk(W.n, 64), 64, W.d, S.d,
+                  (const double*)W.rec, W.rec, (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
+    else
+        RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX, false>, nblk(W.n, 64), 64, W.d, S.d,
+                  (const double*)W.rec, W.rec, (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
+    if ((c->K.nnps == 2 || c->F.uni_s) && c->F.uni_x && rep_int)
+        RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true>, nblk(S.n, 64), 64, S.d, W.d,
+                  (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
+    else
+        RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, false>, nblk(S.n, 64), 64, S.d, W.d,
+                  (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
     swap_c(W);
     swap_c(S);
     RC(launch(c, "water_b", k_water_b4<SKF, SPHG_LB>, nblk(W.n, 64), 64, W.d, (const double*)W.rec, W.masks, c->gl[0], c->K,
               c->F, mask_b(c, 0)));
-    RC(launch(c, "soil_b", k_soil_b4<SKF, SPHG_LB>, nblk(S.n, 64), 64, S.d, (const double*)S.rec, S.masks, c->gl[1], c->K,
-              c->F, mask_b(c, 1)));
     return SPHG_OK;
 }
 

@@ -14,12 +14,37 @@ pytestmark = pytest.mark.gpu
 FP64_TOL_1STEP = 1e-12
 
 
+def _column():
+    return scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3, tank_h=0.12, jitter=0.05,
+                                      seed=7)
+
+
+def _column_varied_h(seed=5):
+    """Smoothing length varied by up to 4% per particle: the kernels' general
+    form (pair h = mean of the two, h read from the partner record)."""
+    sc = _column()
+    rng = np.random.default_rng(seed)
+    for ph in (sc.water, sc.soil):
+        n = ph.ntotal + ph.nvirt
+        ph.hsml[:n] *= 1.0 + 0.04 * rng.uniform(-1.0, 1.0, n)
+    return sc
+
+
+def _column_fractional_repulsion():
+    """Non-integer repulsion exponents: the kernels' general pow() form
+    (integer exponents take binary powering)."""
+    sc = _column()
+    sc.params.p1, sc.params.p2 = 12.5, 3.5
+    return sc
+
+
 def small_cases():
     return {
         "savage_skf4": lambda: scenarios.savage_column(),
         "savage_skf1": lambda: scenarios.savage_column(skf=1),
-        "column_jitter": lambda: scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3,
-                                                            tank_h=0.12, jitter=0.05, seed=7),
+        "column_jitter": _column,
+        "column_varied_h": _column_varied_h,
+        "column_frac_rep": _column_fractional_repulsion,
         "slope_small": lambda: scenarios.slope(dx=0.01, scale=0.25, jitter=0.1, seed=3),
     }
 
@@ -118,7 +143,7 @@ def test_one_step_two_phase(case):
 
 @pytest.mark.parametrize("nsteps,tol", [(2, 1e-12), (10, 1e-10)])
 def test_steps_two_phase(nsteps, tol):
-    sc, ref = _both(small_cases()["column_jitter"])
+    sc, ref = _both(_column)
     s = Solver(sc.params, sc.water, sc.soil)
     s.time_integration(nsteps, sync=True)
     oracle.time_integration(ref.params, ref.water, ref.soil, 0, nsteps)
