@@ -35,16 +35,16 @@ HBM_PEAK_GBS = 8000.0
 #          soil particles' stage A): 124 state + 48 + 112 outputs = 284
 # mixed (integrated fields x, v, rho, vof, stress, p in FP64, the rest FP32):
 #   water: A = 76 + 20 = 96,   B = 76 + 20 + 68 = 164  (260 total)
-#   soil:  A = 100 + 24 = 124, B = 100 + 24 + 92 = 216 (340 total)
+#   soil:  one pass, 100 + 24 + 92 = 216
 #   cell list: 28 per particle and step
 ALGO_BYTES = {
     "fp64": {"water_a": ("water", 140), "water_b": ("water", 228),
              "soil_ab": ("soil", 284)},
     "mixed": {"water_a": ("water", 96), "water_b": ("water", 164),
-              "soil_a": ("soil", 124), "soil_b": ("soil", 216)},
+              "soil_ab": ("soil", 216)},
 }
 STEP_BYTES = {"fp64": {"water": 368 + 28, "soil": 284 + 28},
-              "mixed": {"water": 260 + 28, "soil": 340 + 28}}
+              "mixed": {"water": 260 + 28, "soil": 216 + 28}}
 
 
 class Hip:
@@ -358,7 +358,7 @@ def main():
     s.time_integration(args.prof_steps, sync=False)
     hip.sync()
     kern = {}
-    for name in ("cell_key", "cell_scatter", "cell_sort", "scan", "fill", "first_half",
+    for name in ("cell_key", "cell_scatter", "cell_sort", "scan", "fill", "first_half", "pack", "permute", "iota",
                  "water_a", "soil_a", "soil_ab", "water_b", "soil_b", "final_water", "final_soil"):
         n, t = s.profile_get(name)
         if n:

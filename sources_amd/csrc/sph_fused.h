@@ -2,14 +2,16 @@
 //
 // Per phase and step:
 //   pack      thread t of the cell order: half-step update of particle order[t]
-//             and its 128-byte neighbour record rec[t] (cell-sorted, so the
-//             records of a cell row are contiguous and a neighbour costs one
-//             cache line instead of one line per SoA field).  Record chunks
-//             (16 bytes): 0 x,y | 1 vx,vy | 2 rho,m | 3 vof,p | 4 s3,c |
+//             and its neighbour record t, cell-sorted and stored as chunk
+//             planes (chunk c of record t is element t of plane c).  Record
+//             chunks (16 bytes): 0 x,y | 1 vx,vy | 2 rho,m | 3 vof,p | 4 s3,c |
 //             5 s1,s2 | 6 h,itype; m carries the sign of itype (negative for
-//             virtual particles).  The loops load only the chunks they use
-//             (partner loads are limited by the address unit, one address per
-//             lane and 16 bytes): 1/rho and m/rho of a partner are recomputed;
+//             virtual particles).  The lanes of a wave hold nearby particles
+//             whose k-th partners are nearby in the cell order too, so one
+//             chunk load of all lanes touches a few cache lines of a plane (a
+//             128-byte record per particle would touch one line per lane: the
+//             address unit serialises on those).  The loops load only the
+//             chunks they use: 1/rho and m/rho of a partner are recomputed;
 //   stage A   candidate scan of the 3 cell rows with a sqrt-free exact
 //             predicate (dr < T, T precomputed so that dr < T <=> sqrt(dr) < R),
 //             accepted partners kept as one concatenated 128-bit mask; then a
@@ -183,9 +185,19 @@ __device__ __forceinline__ void kern_fast(const KC& k, double r, double inv_r, d
     }
 }
 
-__device__ __forceinline__ const double2* recp(const double* rec, int k)
+// The records are stored as chunk planes: chunk c of record k is element k
+// of plane c (n 16-byte elements per plane).  The lanes of a wave read one
+// chunk of nearby partners at a time, so a plane layout touches a few cache
+// lines per load where a 128-byte-per-record layout touches one per lane.
+struct RecV {
+    const double2* p;
+    int n;
+    __device__ __forceinline__ double2 operator()(int k, int c) const { return p[(size_t)c * n + k]; }
+};
+
+__device__ __forceinline__ RecV recv(const double* rec, int n)
 {
-    return reinterpret_cast<const double2*>(rec + (size_t)R_STRIDE * k);
+    return RecV{reinterpret_cast<const double2*>(rec), n};
 }
 
 // Exact pair predicate: sqrt(dx^2+dy^2) < scale_k * mh  (direct_find) or
@@ -223,7 +235,19 @@ __device__ __forceinline__ Rows block_rows(const Grid& g, const int* cell_start,
     return R;
 }
 
-constexpr int Q_STRIDE = 16;   // floats per record
+constexpr int Q_STRIDE = 16;   // floats per record: four float4 chunk planes
+
+// FP32 records of the mixed-precision form, as chunk planes like RecV.
+struct RecQ {
+    const float4* p;
+    int n;
+    __device__ __forceinline__ float4 operator()(int k, int c) const { return p[(size_t)c * n + k]; }
+};
+
+__device__ __forceinline__ RecQ recqv(const float* q, int n)
+{
+    return RecQ{reinterpret_cast<const float4*>(q), n};
+}
 
 __device__ __forceinline__ int cell_tag(int cx, int cy, bool is_virtual)
 {
@@ -232,19 +256,19 @@ __device__ __forceinline__ int cell_tag(int cx, int cy, bool is_virtual)
 __device__ __forceinline__ int tag_cx(int tag) { return (tag >> 15) & 0x7fff; }
 __device__ __forceinline__ int tag_cy(int tag) { return tag & 0x7fff; }
 
-// FP32 neighbour record of the mixed-precision form (16 floats; the layout
-// is described in sph_mixed.h): chunks q0 and q1.
-__device__ __forceinline__ void write_q01(float* Q, const Grid& g, double cs, double x, double y, double h, bool virt,
-                                          double vx, double vy, double drho, double m)
+// FP32 neighbour record of the mixed-precision form (the layout is
+// described in sph_mixed.h): chunks q0 and q1 of the record at R (plane 0),
+// planes pn elements apart.
+__device__ __forceinline__ void write_q01(float4* R, size_t pn, const Grid& g, double cs, double x, double y, double h,
+                                          bool virt, double vx, double vy, double drho, double m)
 {
     int cx, cy;
     grid_cell(g, x, y, cx, cy);
     cx = min(max(cx, 0), g.ncx - 1);
     cy = min(max(cy, 0), g.ncy - 1);
     const double xc = g.x0 + ((double)cx + 0.5) * cs, yc = g.y0 + ((double)cy + 0.5) * cs;
-    float4* R = reinterpret_cast<float4*>(Q);
     R[0] = make_float4((float)(x - xc), (float)(y - yc), __int_as_float(cell_tag(cx, cy, virt)), (float)h);
-    R[1] = make_float4((float)vx, (float)vy, (float)drho, (float)m);
+    R[pn] = make_float4((float)vx, (float)vy, (float)drho, (float)m);
 }
 
 // ------------------------------------------------------------------------
@@ -282,17 +306,19 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
     const double m = P.mass[i];
     const double irho = 1.0 / rho;
     if (q) {
-        float* Q = q + (size_t)Q_STRIDE * t;
-        write_q01(Q, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.w_rho0, m);
-        reinterpret_cast<float4*>(Q)[2] = make_float4((float)vof, (float)irho, (float)(m * irho), 0.0f);
+        float4* Q = reinterpret_cast<float4*>(q) + t;
+        const size_t pn = (size_t)P.n;
+        write_q01(Q, pn, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.w_rho0, m);
+        Q[2 * pn] = make_float4((float)vof, (float)irho, (float)(m * irho), 0.0f);
         return;
     }
-    double2* R = reinterpret_cast<double2*>(rec + (size_t)R_STRIDE * t);
-    R[0] = make_double2(P.x[i], P.y[i]);
-    R[1] = make_double2(vx, vy);
-    R[2] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
-    R[3] = make_double2(vof, 0.0);
-    R[6] = make_double2(P.hsml[i], (double)it);
+    double2* R = reinterpret_cast<double2*>(rec) + t;
+    const size_t pn = (size_t)P.n;
+    R[0 * pn] = make_double2(P.x[i], P.y[i]);
+    R[1 * pn] = make_double2(vx, vy);
+    R[2 * pn] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
+    R[3 * pn] = make_double2(vof, 0.0);
+    R[6 * pn] = make_double2(P.hsml[i], (double)it);
 }
 
 __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs)
@@ -327,20 +353,22 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
     const double m = P.mass[i];
     const double irho = 1.0 / rho;
     if (q) {
-        float* Q = q + (size_t)Q_STRIDE * t;
-        write_q01(Q, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.s_rho0, m);
-        reinterpret_cast<float4*>(Q)[2] = make_float4((float)P.vof[i], (float)irho, (float)(m * irho), (float)p);
-        reinterpret_cast<float4*>(Q)[3] = make_float4((float)P.c[i], (float)sxx, (float)sxy, (float)syy);
+        float4* Q = reinterpret_cast<float4*>(q) + t;
+        const size_t pn = (size_t)P.n;
+        write_q01(Q, pn, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.s_rho0, m);
+        Q[2 * pn] = make_float4((float)P.vof[i], (float)irho, (float)(m * irho), (float)p);
+        Q[3 * pn] = make_float4((float)P.c[i], (float)sxx, (float)sxy, (float)syy);
         return;
     }
-    double2* R = reinterpret_cast<double2*>(rec + (size_t)R_STRIDE * t);
-    R[0] = make_double2(P.x[i], P.y[i]);
-    R[1] = make_double2(vx, vy);
-    R[2] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
-    R[3] = make_double2(P.vof[i], p);
-    R[4] = make_double2(syy, P.c[i]);
-    R[5] = make_double2(sxx, sxy);
-    R[6] = make_double2(P.hsml[i], (double)it);
+    double2* R = reinterpret_cast<double2*>(rec) + t;
+    const size_t pn = (size_t)P.n;
+    R[0 * pn] = make_double2(P.x[i], P.y[i]);
+    R[1 * pn] = make_double2(vx, vy);
+    R[2 * pn] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
+    R[3 * pn] = make_double2(P.vof[i], p);
+    R[4 * pn] = make_double2(syy, P.c[i]);
+    R[5 * pn] = make_double2(sxx, sxy);
+    R[6 * pn] = make_double2(P.hsml[i], (double)it);
 }
 
 
@@ -456,7 +484,7 @@ struct Lds {
 // Copy the chunks (16-byte pairs of record slots) selected by LM of every
 // window record into lds[slot][i].
 template <int LM>
-__device__ __forceinline__ void win_load(const Win& w, const double* rec, double2 (*lds)[WIN_N], int lane)
+__device__ __forceinline__ void win_load(const Win& w, const RecV& rec, double2 (*lds)[WIN_N], int lane)
 {
     constexpr int NC = Lds<LM>::n;
     if constexpr (NC == 0) return;
@@ -481,7 +509,7 @@ __device__ __forceinline__ void win_load(const Win& w, const double* rec, double
 #pragma unroll
         for (int q = 1; q < NC; ++q)
             if (c == q) ch = CH[q];
-        lds[c][i] = reinterpret_cast<const double2*>(rec + (size_t)R_STRIDE * k)[ch];
+        lds[c][i] = rec(k, ch);
     }
     __syncthreads();
 }
@@ -509,13 +537,13 @@ __device__ __forceinline__ LaneRows lane_rows(const Rows& R, const Win& w, int c
 // window index i and the record index k.
 template <class P>
 __device__ __forceinline__ uint64_t scan_lane_row(int k0, int k1, int base, int self, bool win, const double2* ldsxy,
-                                                  const double* rec, P&& accept)
+                                                  const RecV& rec, P&& accept)
 {
     uint64_t m = 0;
     if (k1 - k0 > 64) return 0;
 #pragma unroll 4
     for (int k = k0; k < k1; ++k) {
-        const double2 b = win ? ldsxy[k + base] : recp(rec, k)[0];
+        const double2 b = win ? ldsxy[k + base] : rec(k, 0);
         const bool ok = k != self && accept(k + base, k, b);
         m |= (uint64_t)ok << (k - k0);
     }
@@ -644,16 +672,15 @@ constexpr int CH_H = 1 << 6;      // chunk holding h
 // chunk when with_h (non-uniform smoothing length): chunks in LM come from
 // the LDS window when it is in use, the others from the record.
 template <int LM, int NEED>
-__device__ __forceinline__ Nb load_nb(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k,
+__device__ __forceinline__ Nb load_nb(const double2 (*lds)[WIN_N], const RecV& rec, bool win, int i, int k,
                                       bool with_h)
 {
     Nb b{};
-    const double2* R = recp(rec, k);
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
         if (!(NEED & (1 << c)) && !(c == 6 && with_h)) continue;
         if ((LM & (1 << c)) && win) b.set(c, lds[Lds<LM>::slot(c)][i]);
-        else b.set(c, R[c]);
+        else b.set(c, rec(k, c));
     }
     if (NEED & 4) {
         // chunk 2 carries m with the sign of itype (virtual < 0)
@@ -711,17 +738,17 @@ __device__ __forceinline__ void lane_cell(const Grid& g, double x, double y, int
 
 
 template <int LM>
-__device__ __forceinline__ double2 xy_of(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k)
+__device__ __forceinline__ double2 xy_of(const double2 (*lds)[WIN_N], const RecV& rec, bool win, int i, int k)
 {
     if ((LM & 1) && win) return lds[0][i];
-    return recp(rec, k)[0];
+    return rec(k, 0);
 }
 
 template <int LM>
-__device__ __forceinline__ double h_of(const double2 (*lds)[WIN_N], const double* rec, bool win, int i, int k)
+__device__ __forceinline__ double h_of(const double2 (*lds)[WIN_N], const RecV& rec, bool win, int i, int k)
 {
     if ((LM & CH_H) && win) return lds[Lds<LM>::slot(6)][i].x;
-    return recp(rec, k)[6].x;
+    return rec(k, 6).x;
 }
 
 constexpr int lds_rows(int a, int b) { return (a > b ? a : b) > 0 ? (a > b ? a : b) : 1; }
@@ -747,11 +774,12 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
 {
     static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
     __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
+    const RecV RW = recv(recW, W.n), RS = recv(recS, S.n);
     const WaveCtx wc = wave_ctx(W.n);
     const int t = wc.t;
-    const double2* RA = recp(recW, wc.tt);
-    const double2 xy = RA[0], va = RA[1], rm = RA[2], hi = RA[6];
-    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = RA[3].x;
+    auto RA = [&](int c) { return RW(wc.tt, c); };
+    const double2 xy = RA(0), va = RA(1), rm = RA(2), hi = RA(6);
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = RA(3).x;
     const double irhoa = 1.0 / rhoa;
     const int ita = (int)hi.y;
     const int a = W.order[wc.tt];
@@ -766,23 +794,23 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
 
     // ---- same-phase window, scan, masks
     const Win w = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
-    if (w.ok) win_load<LA>(w, recW, lds, wc.lane);
+    if (w.ok) win_load<LA>(w, RW, lds, wc.lane);
     const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya), w, cy);
     const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
     auto acc_b = [&](int i, int k, double2 b) {
-        return P(b, uni ? ha : h_of<LA>(lds, recW, w.ok, i, k), g, K);
+        return P(b, uni ? ha : h_of<LA>(lds, RW, w.ok, i, k), g, K);
     };
-    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, recW, w.ok, i, k)); };
-    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], recW, acc_b);
-    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], recW, acc_b);
-    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], recW, acc_b);
+    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, RW, w.ok, i, k)); };
+    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], RW, acc_b);
+    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], RW, acc_b);
+    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], RW, acc_b);
     const CatMask cm = cat_masks(L, m0, m1, m2);
     if (wc.valid) {
         MW.m[t] = cm.lo;
         MW.m[W.n + t] = cm.hi;
         MW.ovf[t] = cm.ovf;
     }
-    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, recW, w.ok, i, k, !uni); };
+    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, RW, w.ok, i, k, !uni); };
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
     int cnt = 0;
     visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
@@ -818,19 +846,19 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     if (cross) {
         if (LA != 0) __syncthreads();
         const Win wx = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
-        if (wx.ok) win_load<LX>(wx, recS, lds, wc.lane);
+        if (wx.ok) win_load<LX>(wx, RS, lds, wc.lane);
         const LaneRows LXr = lane_rows(block_rows(g, S.cell_start, xa, ya), wx, cy);
         const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
         auto accx_b = [&](int i, int k, double2 b) {
-            return PX(b, ux ? ha : h_of<LX>(lds, recS, wx.ok, i, k), g, K);
+            return PX(b, ux ? ha : h_of<LX>(lds, RS, wx.ok, i, k), g, K);
         };
-        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, recS, wx.ok, i, k)); };
-        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recS, accx_b);
-        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recS, accx_b);
-        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recS, accx_b);
-        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_WX_A>(lds, recS, wx.ok, i, k, !ux); };
+        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, RS, wx.ok, i, k)); };
+        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], RS, accx_b);
+        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], RS, accx_b);
+        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], RS, accx_b);
+        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_WX_A>(lds, RS, wx.ok, i, k, !ux); };
         const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
-        const KC kx = make_kc(SKF, sph_mhsml(ha, ux ? recp(recS, 0)[6].x : ha));
+        const KC kx = make_kc(SKF, sph_mhsml(ha, ux ? RS(0, 6).x : ha));
         const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
         visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
             const double dx = xa - b.xy.x, dy = ya - b.xy.y;
@@ -871,10 +899,11 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     } else {
         W.countiac[a] = cnt;
     }
-    double* RW = recWw + (size_t)R_STRIDE * t;
-    RW[R_P] = p;
-    reinterpret_cast<double2*>(RW)[4] = make_double2(vofa * (mu * tyy), cn);
-    reinterpret_cast<double2*>(RW)[5] = make_double2(vofa * (mu * txx), vofa * (mu * txy));
+    double2* WP = reinterpret_cast<double2*>(recWw) + t;
+    const size_t nw = (size_t)W.n;
+    reinterpret_cast<double*>(WP + 3 * nw)[1] = p;   // chunk 3 = (vof, p)
+    WP[4 * nw] = make_double2(vofa * (mu * tyy), cn);
+    WP[5 * nw] = make_double2(vofa * (mu * txx), vofa * (mu * txy));
 }
 
 // ------------------------------------------------------------------------
@@ -888,11 +917,12 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
 {
     static_assert(LB == 0 || (LB & 1), "window needs positions");
     __shared__ double2 lds[lds_rows(Lds<LB>::n, 0)][WIN_N];
+    const RecV RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(W.n);
     const int t = wc.t;
-    const double2* RA = recp(recW, wc.tt);
-    const double2 axy = RA[0], av = RA[1], arm = RA[2], a3 = RA[3], apc = RA[4], a12 = RA[5];
-    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = RA[6].x, vofa = a3.x;
+    auto RA = [&](int c) { return RW(wc.tt, c); };
+    const double2 axy = RA(0), av = RA(1), arm = RA(2), a3 = RA(3), apc = RA(4), a12 = RA(5);
+    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = RA(6).x, vofa = a3.x;
     const double pa = a3.y, ca = apc.y, fxx_a = a12.x, fxy_a = a12.y, fyy_a = apc.x;
     const double irhoa = 1.0 / rhoa;
     const int a = W.order[wc.tt];
@@ -902,13 +932,13 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
     int cx, cy;
     lane_cell(g, xa, ya, cx, cy);
     const Win w = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LB != 0);
-    if (w.ok) win_load<LB>(w, recW, lds, wc.lane);
+    if (w.ok) win_load<LB>(w, RW, lds, wc.lane);
     const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya), w, cy);
     const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
     auto acc_i = [&](int i, int k) {
-        return P(xy_of<LB>(lds, recW, w.ok, i, k), uni ? ha : h_of<LB>(lds, recW, w.ok, i, k), g, K);
+        return P(xy_of<LB>(lds, RW, w.ok, i, k), uni ? ha : h_of<LB>(lds, RW, w.ok, i, k), g, K);
     };
-    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, recW, w.ok, i, k, !uni); };
+    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, RW, w.ok, i, k, !uni); };
     CatMask cm{0, 0, 0};
     if (wc.valid) {
         cm.lo = MW.m[t];
@@ -989,10 +1019,11 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
 {
     static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
     __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
+    const RecV RS = recv(recS, S.n), RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(S.n);
     const int t = wc.t;
-    const double2* RA = recp(recS, wc.tt);
-    const double2 xy = RA[0], va = RA[1], rm = RA[2], a3 = RA[3], a4 = RA[4], a12 = RA[5], hi = RA[6];
+    auto RA = [&](int c) { return RS(wc.tt, c); };
+    const double2 xy = RA(0), va = RA(1), rm = RA(2), a3 = RA(3), a4 = RA(4), a12 = RA(5), hi = RA(6);
     const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = a3.x;
     const double pa = a3.y, sxxa = a12.x, sxya = a12.y, syya = a4.x;
     const double ca = a4.y;          // sound speed before this step (delta_sph of rho)
@@ -1014,19 +1045,19 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     int cntx = 0;
     if (cross) {
         const Win wx = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
-        if (wx.ok) win_load<LX>(wx, recW, lds, wc.lane);
+        if (wx.ok) win_load<LX>(wx, RW, lds, wc.lane);
         const LaneRows LXr = lane_rows(block_rows(g, W.cell_start, xa, ya), wx, cy);
         const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
         auto accx_b = [&](int i, int k, double2 b) {
-            return PX(b, ux ? ha : h_of<LX>(lds, recW, wx.ok, i, k), g, K);
+            return PX(b, ux ? ha : h_of<LX>(lds, RW, wx.ok, i, k), g, K);
         };
-        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, recW, wx.ok, i, k)); };
-        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], recW, accx_b);
-        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], recW, accx_b);
-        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], recW, accx_b);
-        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_SX_A>(lds, recW, wx.ok, i, k, !ux); };
+        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, RW, wx.ok, i, k)); };
+        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], RW, accx_b);
+        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], RW, accx_b);
+        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], RW, accx_b);
+        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_SX_A>(lds, RW, wx.ok, i, k, !ux); };
         const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
-        const KC kx = make_kc(SKF, sph_mhsml(ux ? recp(recW, 0)[6].x : ha, ha));
+        const KC kx = make_kc(SKF, sph_mhsml(ux ? RW(0, 6).x : ha, ha));
         const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
         visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
             const double dx = b.xy.x - xa, dy = b.xy.y - ya;   // water - soil
@@ -1050,18 +1081,18 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     }
     if (cross && LX != 0) __syncthreads();
     const Win w = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
-    if (w.ok) win_load<LA>(w, recS, lds, wc.lane);
+    if (w.ok) win_load<LA>(w, RS, lds, wc.lane);
     const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya), w, cy);
     const Pred P{xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0};
     auto acc_b = [&](int i, int k, double2 b) {
-        return P(b, uni ? ha : h_of<LA>(lds, recS, w.ok, i, k), g, K);
+        return P(b, uni ? ha : h_of<LA>(lds, RS, w.ok, i, k), g, K);
     };
-    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, recS, w.ok, i, k)); };
-    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], recS, acc_b);
-    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], recS, acc_b);
-    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], recS, acc_b);
+    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, RS, w.ok, i, k)); };
+    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], RS, acc_b);
+    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], RS, acc_b);
+    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], RS, acc_b);
     const CatMask cm = cat_masks(L, m0, m1, m2);
-    auto load = [&](int i, int k) { return load_nb<LA, NEED_B>(lds, recS, w.ok, i, k, !uni); };
+    auto load = [&](int i, int k) { return load_nb<LA, NEED_B>(lds, RS, w.ok, i, k, !uni); };
     const double et2 = K.etq * K.etq;
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;
     double avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;

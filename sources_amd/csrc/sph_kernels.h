@@ -139,7 +139,9 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total)
     return incl - v;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile(const int* in, int* out, int n, int* tile_sums)
+// clear (optional, the same array as in): zeroed after it is read, so that a
+// counting pass can start from zeros without a fill.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile(const int* in, int* out, int n, int* tile_sums, int* clear)
 {
     __shared__ int lds[SCAN_BLOCK];
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
@@ -150,6 +152,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile(const int* in, int* ou
         const int t = v[i];
         v[i] = s;
         s += t;
+    }
+    if (clear) {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i)
+            if (base + i < n) clear[base + i] = 0;
     }
     int total;
     const int pre = block_exclusive_scan(s, lds, total);
@@ -190,26 +197,39 @@ __global__ void k_scan_add(int* out, int n, const int* tile_sums, int ntiles)
 // ------------------------------------------------------------------------
 // Cell list: key + rank (counting sort), scatter, per-cell ordering.
 // ------------------------------------------------------------------------
-__global__ void k_cell_key(DPhase P, Grid g, int* err)
+__global__ __launch_bounds__(256) void k_cell_key(DPhase P, Grid g, int* err)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    const double x = P.x[i], y = P.y[i];
-    int cx, cy;
-    grid_cell(g, x, y, cx, cy);
-    // grid_geom's range check (m_particles.f90:1131-1136): the grid spans the
-    // geometry box [x_mingeom, x_maxgeom] x [y_mingeom, y_maxgeom].
-    const bool bad = !(x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1) || cx < 0 || cy < 0 ||
-                     cx >= g.ncx || cy >= g.ncy;
-    if (bad) {
-        atomicOr(err, 1);
-        P.key[i] = 0;
-        P.slot[i] = atomicAdd(&P.cell_count[0], 1);
-        return;
+    const bool valid = i < P.n;
+    int key = -1;
+    if (valid) {
+        const double x = P.x[i], y = P.y[i];
+        int cx, cy;
+        grid_cell(g, x, y, cx, cy);
+        // grid_geom's range check (m_particles.f90:1131-1136): the grid spans the
+        // geometry box [x_mingeom, x_maxgeom] x [y_mingeom, y_maxgeom].
+        const bool bad = !(x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1) || cx < 0 || cy < 0 ||
+                         cx >= g.ncx || cy >= g.ncy;
+        if (bad) atomicOr(err, 1);
+        key = bad ? 0 : cy * g.ncx + cx;
+        P.key[i] = key;
     }
-    const int key = cy * g.ncx + cx;
-    P.key[i] = key;
-    P.slot[i] = atomicAdd(&P.cell_count[key], 1);
+    // Rank within the cell.  The state is kept in cell order, so lanes of a
+    // wave mostly hold runs of equal keys: the head lane of each run takes
+    // the run's slots with one atomic and the run's lanes share them out.  The
+    // rank order inside a cell is arbitrary; k_cell_sort fixes it.
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || prev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);   // lanes 0..lane
+    const int start = 63 - __clzll((long long)(heads & upto));
+    const uint64_t after = heads & ~upto;
+    const int end = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (head && key >= 0) base = atomicAdd(&P.cell_count[key], end - start);
+    base = __shfl(base, start, 64);
+    if (valid) P.slot[i] = base + (lane - start);
 }
 
 __global__ void k_cell_scatter(DPhase P)

@@ -3,7 +3,8 @@
 // The state, the leapfrog update, the Drucker-Prager projections, the EOS,
 // the flow rule and the Jaumann rate stay FP64 (pack and final kernels and
 // the per-particle tails of the stage kernels).  The pair sums run in FP32
-// with compensated (Kahan) accumulation over 64-byte FP32 neighbour records:
+// with compensated (Kahan) accumulation over FP32 neighbour records of four
+// float4 chunks, stored as chunk planes like the FP64 records:
 //   q0: xl, yl, tag, h         xl, yl: position relative to the centre of the
 //                              particle's cell; tag: cx << 15 | cy, bit 31 set
 //                              for a virtual particle (int bits)
@@ -21,11 +22,16 @@
 // The FP32 stage kernels fit five waves per SIMD without spilling (measured
 // faster than the compiler's default of four).
 #define STAGE_ATTR_MX __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
+#ifndef SPHG_MX_AB_WAVES
+#define SPHG_MX_AB_WAVES 4   // the one-pass soil kernel: 4 waves (a few spills) beat 3
+#endif
+#if SPHG_MX_AB_WAVES > 0
+#define STAGE_ATTR_MX_AB __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPHG_MX_AB_WAVES, 8)))
+#else
+#define STAGE_ATTR_MX_AB __launch_bounds__(64)
+#endif
 
-__device__ __forceinline__ const float4* recq(const float* rec, int k)
-{
-    return reinterpret_cast<const float4*>(rec + (size_t)Q_STRIDE * k);
-}
+
 
 // Compensated FP32 sum (Kahan): value() = s - c.
 struct Kahan {
@@ -147,13 +153,13 @@ __device__ __forceinline__ void sep(const SelfF& a, float4 b0, float cs, float& 
 
 // Scan of one candidate row (FP32 predicate).
 template <class P>
-__device__ __forceinline__ uint64_t scan_row_f(const float* rec, int k0, int k1, int self, P&& accept)
+__device__ __forceinline__ uint64_t scan_row_f(const RecQ& rec, int k0, int k1, int self, P&& accept)
 {
     uint64_t m = 0;
     if (k1 - k0 > 64) return 0;
 #pragma unroll 4
     for (int k = k0; k < k1; ++k) {
-        const bool ok = k != self && accept(recq(rec, k)[0]);
+        const bool ok = k != self && accept(rec(k, 0));
         m |= (uint64_t)ok << (k - k0);
     }
     return m;
@@ -190,10 +196,11 @@ template <int SKF>
 __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS,
                                                    Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
 {
+    const RecQ QWv = recqv(qW, W.n), QSv = recqv(qS, S.n);
     const WaveCtx wc = wave_ctx(W.n);
     const int t = wc.t;
-    const float4* QA = recq(qW, wc.tt);
-    const float4 a0 = QA[0], a1 = QA[1], a2 = QA[2];
+    auto QA = [&](int c) { return QWv(wc.tt, c); };
+    const float4 a0 = QA(0), a1 = QA(1), a2 = QA(2);
     const SelfF me = self_of(a0);
     const float vxa = a1.x, vya = a1.y, dra = a1.z;
     const float rhoa = dra + X.rho0_w;
@@ -206,9 +213,9 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
         sep(me, b0, X.cs, dx, dy);
         return pred_f(uni, X.thr_w, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
     };
-    const uint64_t m0 = scan_row_f(qW, L.k0[0], L.k1[0], t, acc_q);
-    const uint64_t m1 = scan_row_f(qW, L.k0[1], L.k1[1], t, acc_q);
-    const uint64_t m2 = scan_row_f(qW, L.k0[2], L.k1[2], t, acc_q);
+    const uint64_t m0 = scan_row_f(QWv, L.k0[0], L.k1[0], t, acc_q);
+    const uint64_t m1 = scan_row_f(QWv, L.k0[1], L.k1[1], t, acc_q);
+    const uint64_t m2 = scan_row_f(QWv, L.k0[2], L.k1[2], t, acc_q);
     const CatMask cm = cat_masks(L, m0, m1, m2);
     if (wc.valid) {
         MW.m[t] = cm.lo;
@@ -220,9 +227,9 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
     Kahan con, drd, txx, txy, tyy, avx, avy;
     float repx = 0.0f, repy = 0.0f;
     int cnt = 0;
-    visit_f(cm, L, t, [&](int k) { return acc_q(recq(qW, k)[0]); }, [&](int k) {
-        const float4* Q = recq(qW, k);
-        const float4 b0 = Q[0], b1 = Q[1];   // 1/rho and m/rho recomputed: two loads per partner
+    visit_f(cm, L, t, [&](int k) { return acc_q(QWv(k, 0)); }, [&](int k) {
+        auto Q = [&](int c) { return QWv(k, c); };
+        const float4 b0 = Q(0), b1 = Q(1);   // 1/rho and m/rho recomputed: two loads per partner
         float dx, dy;
         sep(me, b0, X.cs, dx, dy);
         const float dr = dx * dx + dy * dy;
@@ -259,17 +266,17 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
             sep(me, b0, X.cs, dx, dy);
             return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
         };
-        const uint64_t c0 = scan_row_f(qS, LX.k0[0], LX.k1[0], -1, accx_q);
-        const uint64_t c1 = scan_row_f(qS, LX.k0[1], LX.k1[1], -1, accx_q);
-        const uint64_t c2 = scan_row_f(qS, LX.k0[2], LX.k1[2], -1, accx_q);
+        const uint64_t c0 = scan_row_f(QSv, LX.k0[0], LX.k1[0], -1, accx_q);
+        const uint64_t c1 = scan_row_f(QSv, LX.k0[1], LX.k1[1], -1, accx_q);
+        const uint64_t c2 = scan_row_f(QSv, LX.k0[2], LX.k1[2], -1, accx_q);
         const CatMask cx3 = cat_masks(LX, c0, c1, c2);
         const double vofa_d = W.vof[a], rhoa_d = W.rho[a];
         const float cfa = (float)(vofa_d * rhoa_d * (-K.numeric_gravity) / K.s_perm);
         const float irhoa = a2.y;
-        const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)me.h + (double)recq(qS, 0)[0].w) : (double)me.h);
-        visit_f(cx3, LX, -1, [&](int k) { return accx_q(recq(qS, k)[0]); }, [&](int k) {
-            const float4* Q = recq(qS, k);
-            const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2];
+        const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)me.h + (double)QSv(0, 0).w) : (double)me.h);
+        visit_f(cx3, LX, -1, [&](int k) { return accx_q(QSv(k, 0)); }, [&](int k) {
+            auto Q = [&](int c) { return QSv(k, c); };
+            const float4 b0 = Q(0), b1 = Q(1), b2 = Q(2);
             float dx, dy;
             sep(me, b0, X.cs, dx, dy);
             const float dr = dx * dx + dy * dy;
@@ -310,175 +317,11 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
     } else {
         W.countiac[a] = cnt;
     }
-    float* QW = qWw + (size_t)Q_STRIDE * t;
-    QW[11] = (float)p;
-    reinterpret_cast<float4*>(QW)[3] =
-        make_float4((float)cn, (float)(vofa_d * (mu * txx_d)), (float)(vofa_d * (mu * txy_d)),
-                    (float)(vofa_d * (mu * tyy_d)));
-}
-
-// ------------------------------------------------------------------------
-// Stage A, soil (mixed).
-// ------------------------------------------------------------------------
-template <int SKF>
-__global__ STAGE_ATTR_MX void k_soil_a_mx(DPhase S, DPhase W, const float* qS, float* qSw, const float* qW,
-                                                  Masks MS, Grid g, KParams K, FusedK F, MixK X, int cross)
-{
-    const WaveCtx wc = wave_ctx(S.n);
-    const int t = wc.t;
-    const float4* QA = recq(qS, wc.tt);
-    const float4 a0 = QA[0], a1 = QA[1], a2 = QA[2];
-    const SelfF me = self_of(a0);
-    const float vxa = a1.x, vya = a1.y, dra = a1.z;
-    const float rhoa = dra + X.rho0_s;
-    const int a = S.order[wc.tt];
-    const bool uni = F.uni_s;
-    const KCf kc = make_kcf(SKF, me.h);
-    const LaneRows L = rows_of_cell(g, S.cell_start, me.cx, me.cy);
-    auto acc_q = [&](float4 b0) {
-        float dx, dy;
-        sep(me, b0, X.cs, dx, dy);
-        return pred_f(uni, X.thr_s, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
-    };
-    const uint64_t m0 = scan_row_f(qS, L.k0[0], L.k1[0], t, acc_q);
-    const uint64_t m1 = scan_row_f(qS, L.k0[1], L.k1[1], t, acc_q);
-    const uint64_t m2 = scan_row_f(qS, L.k0[2], L.k1[2], t, acc_q);
-    const CatMask cm = cat_masks(L, m0, m1, m2);
-    if (wc.valid) {
-        MS.m[t] = cm.lo;
-        MS.m[S.n + t] = cm.hi;
-        MS.ovf[t] = cm.ovf;
-    }
-    const bool real_a = S.itype[a] > 0;
-    const float dspp = (float)S.dspp;
-    Kahan con, drd, txx, txy, tyy, vcc, wxy, avx, avy;
-    float repx = 0.0f, repy = 0.0f;
-    int cnt = 0;
-    visit_f(cm, L, t, [&](int k) { return acc_q(recq(qS, k)[0]); }, [&](int k) {
-        const float4* Q = recq(qS, k);
-        const float4 b0 = Q[0], b1 = Q[1];   // 1/rho and m/rho recomputed: two loads per partner
-        float dx, dy;
-        sep(me, b0, X.cs, dx, dy);
-        const float dr = dx * dx + dy * dy;
-        const float ir = __builtin_amdgcn_rsqf(dr);
-        const float r = dr * ir, idr = ir * ir;
-        float ww, gx, gy;
-        if (uni) kern_f<SKF, true>(kc, r, ir, dx, dy, ww, gx, gy);
-        else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
-        ++cnt;
-        const float mb = b1.w, irb = __builtin_amdgcn_rcpf(b1.z + X.rho0_s), Vb = mb * irb;
-        const float dvx = vxa - b1.x, dvy = vya - b1.y;
-        const float dvg = dvx * gx + dvy * gy;
-        con.add(mb * dvg);
-        drd.add(mb * (2.0f * (dra - b1.z) * idr * (dx * gx + dy * gy)) * irb);
-        txx.add(Vb * (-2.0f * dvx * gx + dvy * gy));
-        txy.add(Vb * (-(dvx * gy) - dvy * gx));
-        tyy.add(Vb * (-2.0f * dvy * gy + dvx * gx));
-        vcc.add(-(Vb * dvg));
-        wxy.add(Vb * (-(dvx * gy) + dvy * gx));
-        const float f = mb * ww * (2.0f * __builtin_amdgcn_rcpf(rhoa + b1.z + X.rho0_s));
-        avx.add(-(dvx * f));
-        avy.add(-(dvy * f));
-        if (real_a && !(r > dspp) && __float_as_int(b0.z) < 0) {
-            const float q = dspp / r;
-            const float fr = (__powf(q, (float)K.p1) - __powf(q, (float)K.p2)) * idr;
-            repx += (float)K.dd * dx * fr;
-            repy += (float)K.dd * dy * fr;
-        }
-    });
-
-    Kahan dg_x, dg_y, po_x, po_y;
-    int cntx = 0;
-    const double vofa_d = S.vof[a], rhoa_d = S.rho[a];
-    if (cross) {
-        const LaneRows LX = rows_of_cell(g, W.cell_start, me.cx, me.cy);
-        auto accx_q = [&](float4 b0) {
-            float dx, dy;
-            sep(me, b0, X.cs, dx, dy);
-            return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
-        };
-        const uint64_t c0 = scan_row_f(qW, LX.k0[0], LX.k1[0], -1, accx_q);
-        const uint64_t c1 = scan_row_f(qW, LX.k0[1], LX.k1[1], -1, accx_q);
-        const uint64_t c2 = scan_row_f(qW, LX.k0[2], LX.k1[2], -1, accx_q);
-        const CatMask cx3 = cat_masks(LX, c0, c1, c2);
-        const float cfs = (float)(vofa_d * (-K.numeric_gravity) / K.s_perm);
-        const float irhoa = a2.y;
-        const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)recq(qW, 0)[0].w + (double)me.h) : (double)me.h);
-        visit_f(cx3, LX, -1, [&](int k) { return accx_q(recq(qW, k)[0]); }, [&](int k) {
-            const float4* Q = recq(qW, k);
-            const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2];
-            float dx, dy;
-            sep(me, b0, X.cs, dx, dy);
-            dx = -dx;   // water - soil
-            dy = -dy;
-            const float dr = dx * dx + dy * dy;
-            const float ir = dr > 0.0f ? __builtin_amdgcn_rsqf(dr) : 0.0f;
-            const float r = dr * ir;
-            float ww, gx, gy;
-            if (F.uni_x) kern_f<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
-            else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)b0.w + (double)me.h)), r, ir, dx, dy, ww, gx, gy);
-            ++cntx;
-            const float mw = b1.w, rhow = b1.z + X.rho0_w;
-            const float sc = mw * (b2.x * rhow * cfs) * (ww * b2.y * irhoa);
-            dg_x.add(sc * (b1.x - vxa));
-            dg_y.add(sc * (b1.y - vya));
-            const float mprr = mw * b2.w * (b2.y * irhoa);
-            po_x.add(mprr * gx);
-            po_y.add(mprr * gy);
-        });
-    }
-    if (!wc.valid) return;
-    const double ha_d = S.hsml[a], ca_d = S.c[a];
-    const double txx_d = txx.value() * SPH_C23_F, tyy_d = tyy.value() * SPH_C23_F, txy_d = txy.value();
-    const double vcc_d = vcc.value(), wxy_d = wxy.value() * 0.5;
-    S.drho[a] = con.value() + (K.delta * ha_d * ca_d) * drd.value();
-    S.txx[a] = txx_d; S.txy[a] = txy_d; S.tyy[a] = tyy_d; S.vcc[a] = vcc_d; S.wxy[a] = wxy_d;
-    double avx_d = avx.value(), avy_d = avy.value();
-    if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
-    S.avx[a] = avx_d; S.avy[a] = avy_d;
-    S.repx[a] = repx; S.repy[a] = repy;
-    if (cross) {
-        S.drgx[a] = dg_x.value(); S.drgy[a] = dg_y.value();
-        S.porx[a] = po_x.value() * vofa_d; S.pory[a] = po_y.value() * vofa_d;
-        S.countiac[a] = cntx;
-    } else {
-        S.countiac[a] = cnt;
-    }
-    S.c_next[a] = K.s_c;
-    qSw[(size_t)Q_STRIDE * t + 12] = (float)K.s_c;
-    {
-        const double p = S.p[a], sxx = S.sxx[a], sxy = S.sxy[a], syy = S.syy[a];
-        const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;
-        double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
-        if (K.plasticity == 3) {
-            const double I1 = 3.0 * p;
-            const double j2r = sxx * sxx + 2.0 * (sxy * sxy) + syy * syy + (sxx + syy) * (sxx + syy);
-            const double J2 = __builtin_sqrt(j2r / 2.0) + 1.0e-6;
-            const int fail = fabs(J2 - alpha * I1) < SPH_YIELD_TOL_F ? 1 : 0;
-            if (fail) {
-                const double exx = txx_d * 0.5 + vcc_d * (1.0 / 3.0);
-                const double exy = txy_d * 0.5;
-                const double eyy = tyy_d * 0.5 + vcc_d * (1.0 / 3.0);
-                const double sde = sxx * exx + 2.0 * sxy * exy + syy * eyy;
-                const double sj = __builtin_sqrt(j2r / 2.0 + 1.0e-10);
-                const double gs = G / sj;
-                const double dl = (3.0 * alpha * Kb * vcc_d + gs * sde) / (9.0 * (alpha * alpha) * Kb + G);
-                if (!(dl < 0.0)) {
-                    dsxx = -dl * (gs * sxx);
-                    dsxy = -dl * (gs * sxy);
-                    dsyy = -dl * (gs * syy);
-                    dpv = 3.0 * Kb * alpha * dl;
-                    S.eps_p[a] = S.eps_p[a] + dl * sxy / (2.0 * sj) * SPH_EPSP_F;
-                }
-            }
-            S.fail[a] = fail;
-        }
-        dsxx = dsxx + G * txx_d + 2.0 * sxy * wxy_d;
-        dsxy = dsxy + G * txy_d - (sxx - syy) * wxy_d;
-        dsyy = dsyy + G * tyy_d - 2.0 * sxy * wxy_d;
-        dpv = dpv - Kb * vcc_d;
-        S.dsxx[a] = dsxx; S.dsxy[a] = dsxy; S.dsyy[a] = dsyy; S.dp[a] = dpv;
-    }
+    float4* QP = reinterpret_cast<float4*>(qWw) + t;
+    const size_t nw = (size_t)W.n;
+    reinterpret_cast<float*>(QP + 2 * nw)[3] = (float)p;   // q2.w
+    QP[3 * nw] = make_float4((float)cn, (float)(vofa_d * (mu * txx_d)), (float)(vofa_d * (mu * txy_d)),
+                             (float)(vofa_d * (mu * tyy_d)));
 }
 
 // ------------------------------------------------------------------------
@@ -488,10 +331,11 @@ template <int SKF>
 __global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, Grid g, KParams K, FusedK F,
                                                    MixK X, int mask)
 {
+    const RecQ QWv = recqv(qW, W.n);
     const WaveCtx wc = wave_ctx(W.n);
     const int t = wc.t;
-    const float4* QA = recq(qW, wc.tt);
-    const float4 a0 = QA[0], a1 = QA[1], a2 = QA[2], a3 = QA[3];
+    auto QA = [&](int c) { return QWv(wc.tt, c); };
+    const float4 a0 = QA(0), a1 = QA(1), a2 = QA(2), a3 = QA(3);
     const SelfF me = self_of(a0);
     const float vxa = a1.x, vya = a1.y, rhoa = a1.z + X.rho0_w;
     const float vofa = a2.x, pa = a2.w, ca = a3.x, fxx_a = a3.y, fxy_a = a3.z, fyy_a = a3.w;
@@ -513,9 +357,9 @@ __global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, 
     const float et2 = (float)(K.etq * K.etq);
     const float beta = (float)K.beta, alpha = (float)K.alpha;
     Kahan Px, Py, Sxx, Sxyy, Sxyx, Syy, artx, arty, dvd;
-    visit_f(cm, L, t, [&](int k) { return acc_q(recq(qW, k)[0]); }, [&](int k) {
-        const float4* Q = recq(qW, k);
-        const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2], b3 = Q[3];
+    visit_f(cm, L, t, [&](int k) { return acc_q(QWv(k, 0)); }, [&](int k) {
+        auto Q = [&](int c) { return QWv(k, c); };
+        const float4 b0 = Q(0), b1 = Q(1), b2 = Q(2), b3 = Q(3);
         float dx, dy;
         sep(me, b0, X.cs, dx, dy);
         const float dr = dx * dx + dy * dy;
@@ -567,43 +411,93 @@ __global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, 
 }
 
 // ------------------------------------------------------------------------
-// Stage B, soil (mixed): momentum (df3 form), art_visc, delta_sph on p, s.
+// Soil, stages A and B in one kernel (mixed): soil stage B needs nothing
+// from the stage A of other soil particles (its partners' p, sigma, vof come
+// from the pack and the new sound speed is the constant s_c), so both stages
+// share one scan and one partner loop; the terms are those of k_soil_ab4.
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ STAGE_ATTR_MX void k_soil_b_mx(DPhase S, const float* qS, Masks MS, Grid g, KParams K, FusedK F,
-                                                  MixK X, int mask)
+__global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* qS, const float* qW, Grid g,
+                                           KParams K, FusedK F, MixK X, int cross, int mask)
 {
+    const RecQ QSv = recqv(qS, S.n), QWv = recqv(qW, W.n);
     const WaveCtx wc = wave_ctx(S.n);
     const int t = wc.t;
-    const float4* QA = recq(qS, wc.tt);
-    const float4 a0 = QA[0], a1 = QA[1], a2 = QA[2], a3 = QA[3];
+    auto QA = [&](int c) { return QSv(wc.tt, c); };
+    const float4 a0 = QA(0), a1 = QA(1), a2 = QA(2), a3 = QA(3);
     const SelfF me = self_of(a0);
-    const float vxa = a1.x, vya = a1.y, rhoa = a1.z + X.rho0_s;
-    const float vofa = a2.x, irhoa = a2.y, pa = a2.w, ca = a3.x, sxxa = a3.y, sxya = a3.z, syya = a3.w;
+    const float vxa = a1.x, vya = a1.y, dra = a1.z;
+    const float rhoa = dra + X.rho0_s;
+    const float vofa = a2.x, irhoa = a2.y, pa = a2.w, sxxa = a3.y, sxya = a3.z, syya = a3.w;
     const float ir2a = irhoa * irhoa;
     const float fpa = vofa * pa * ir2a, fxxa = vofa * sxxa * ir2a, fxya = vofa * sxya * ir2a,
                 fyya = vofa * syya * ir2a;
+    const float cn = (float)K.s_c;   // sound speed after this step (art_visc)
     const int a = S.order[wc.tt];
     const bool uni = F.uni_s;
     const KCf kc = make_kcf(SKF, me.h);
+    const bool real_a = S.itype[a] > 0;
+    const float dspp = (float)S.dspp;
+
+    Kahan dg_x, dg_y, po_x, po_y;
+    int cntx = 0;
+    const double vofa_d = S.vof[a];
+    if (cross) {
+        const LaneRows LX = rows_of_cell(g, W.cell_start, me.cx, me.cy);
+        auto accx_q = [&](float4 b0) {
+            float dx, dy;
+            sep(me, b0, X.cs, dx, dy);
+            return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
+        };
+        const uint64_t c0 = scan_row_f(QWv, LX.k0[0], LX.k1[0], -1, accx_q);
+        const uint64_t c1 = scan_row_f(QWv, LX.k0[1], LX.k1[1], -1, accx_q);
+        const uint64_t c2 = scan_row_f(QWv, LX.k0[2], LX.k1[2], -1, accx_q);
+        const CatMask cx3 = cat_masks(LX, c0, c1, c2);
+        const float cfs = (float)(vofa_d * (-K.numeric_gravity) / K.s_perm);
+        const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)QWv(0, 0).w + (double)me.h) : (double)me.h);
+        visit_f(cx3, LX, -1, [&](int k) { return accx_q(QWv(k, 0)); }, [&](int k) {
+            auto Q = [&](int c) { return QWv(k, c); };
+            const float4 b0 = Q(0), b1 = Q(1), b2 = Q(2);
+            float dx, dy;
+            sep(me, b0, X.cs, dx, dy);
+            dx = -dx;   // water - soil
+            dy = -dy;
+            const float dr = dx * dx + dy * dy;
+            const float ir = dr > 0.0f ? __builtin_amdgcn_rsqf(dr) : 0.0f;
+            const float r = dr * ir;
+            float ww, gx, gy;
+            if (F.uni_x) kern_f<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+            else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)b0.w + (double)me.h)), r, ir, dx, dy, ww, gx, gy);
+            ++cntx;
+            const float mw = b1.w, rhow = b1.z + X.rho0_w;
+            const float sc = mw * (b2.x * rhow * cfs) * (ww * b2.y * irhoa);
+            dg_x.add(sc * (b1.x - vxa));
+            dg_y.add(sc * (b1.y - vya));
+            const float mprr = mw * b2.w * (b2.y * irhoa);
+            po_x.add(mprr * gx);
+            po_y.add(mprr * gy);
+        });
+    }
+
     const LaneRows L = rows_of_cell(g, S.cell_start, me.cx, me.cy);
     auto acc_q = [&](float4 b0) {
         float dx, dy;
         sep(me, b0, X.cs, dx, dy);
         return pred_f(uni, X.thr_s, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
     };
-    CatMask cm{0, 0, 0};
-    if (wc.valid) {
-        cm.lo = MS.m[t];
-        cm.hi = MS.m[S.n + t];
-        cm.ovf = MS.ovf[t];
-    }
+    const uint64_t m0 = scan_row_f(QSv, L.k0[0], L.k1[0], t, acc_q);
+    const uint64_t m1 = scan_row_f(QSv, L.k0[1], L.k1[1], t, acc_q);
+    const uint64_t m2 = scan_row_f(QSv, L.k0[2], L.k1[2], t, acc_q);
+    const CatMask cm = cat_masks(L, m0, m1, m2);
     const float et2 = (float)(K.etq * K.etq);
     const float beta = (float)K.beta, alpha = (float)K.alpha;
-    Kahan Ax, Ay, Bxx, Cxyy, Cxyx, Dyy, artx, arty, dP, dXX, dXY, dYY;
-    visit_f(cm, L, t, [&](int k) { return acc_q(recq(qS, k)[0]); }, [&](int k) {
-        const float4* Q = recq(qS, k);
-        const float4 b0 = Q[0], b1 = Q[1], b2 = Q[2], b3 = Q[3];
+    Kahan con, drd, txx, txy, tyy, vcc, wxy, avx, avy;
+    Kahan Fx, Fy, artx, arty, dP, dXX, dXY, dYY;
+    float repx = 0.0f, repy = 0.0f;
+    int cnt = 0;
+    visit_f(cm, L, t, [&](int k) { return acc_q(QSv(k, 0)); }, [&](int k) {
+        auto Q = [&](int c) { return QSv(k, c); };
+        const float4 b0 = Q(0), b1 = Q(1), b2 = Q(2), b3 = Q(3);
         float dx, dy;
         sep(me, b0, X.cs, dx, dy);
         const float dr = dx * dx + dy * dy;
@@ -611,24 +505,41 @@ __global__ STAGE_ATTR_MX void k_soil_b_mx(DPhase S, const float* qS, Masks MS, G
         const float r = dr * ir, idr = ir * ir;
         float ww, gx, gy;
         const float amh = 0.5f * (me.h + b0.w);
-        if (uni) kern_f<SKF, false>(kc, r, ir, dx, dy, ww, gx, gy);
-        else kern_f<SKF, false>(make_kcf(SKF, amh), r, ir, dx, dy, ww, gx, gy);
-        const float mb = b1.w, vofb = b2.x, irb = b2.y;
+        if (uni) kern_f<SKF, true>(kc, r, ir, dx, dy, ww, gx, gy);
+        else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
+        ++cnt;
+        const float mb = b1.w, vofb = b2.x, irb = b2.y, Vb = b2.z;
+        // stage A
+        const float dvx = vxa - b1.x, dvy = vya - b1.y;
+        const float dvg = dvx * gx + dvy * gy;
+        con.add(mb * dvg);
+        drd.add(mb * (2.0f * (dra - b1.z) * idr * (dx * gx + dy * gy)) * irb);
+        txx.add(Vb * (-2.0f * dvx * gx + dvy * gy));
+        txy.add(Vb * (-(dvx * gy) - dvy * gx));
+        tyy.add(Vb * (-2.0f * dvy * gy + dvx * gx));
+        vcc.add(-(Vb * dvg));
+        wxy.add(Vb * (-(dvx * gy) + dvy * gx));
+        const float rs = 2.0f * __builtin_amdgcn_rcpf(rhoa + b1.z + X.rho0_s);
+        const float f = mb * ww * rs;
+        avx.add(-(dvx * f));
+        avy.add(-(dvy * f));
+        if (real_a && !(r > dspp) && __float_as_int(b0.z) < 0) {
+            const float q = dspp / r;
+            const float fr = (__powf(q, (float)K.p1) - __powf(q, (float)K.p2)) * idr;
+            repx += (float)K.dd * dx * fr;
+            repy += (float)K.dd * dy * fr;
+        }
+        // stage B
         const float ir2b = irb * irb;
         const float fp = fpa + vofb * b2.w * ir2b;
         const float fxx = fxxa + vofb * b3.y * ir2b;
         const float fxy = fxya + vofb * b3.z * ir2b;
         const float fyy = fyya + vofb * b3.w * ir2b;
-        Ax.add(mb * (fp * gx));
-        Ay.add(mb * (fp * gy));
-        Bxx.add(mb * (fxx * gx));
-        Cxyy.add(mb * (fxy * gy));
-        Cxyx.add(mb * (fxy * gx));
-        Dyy.add(mb * (fyy * gy));
-        const float vr = (vxa - b1.x) * dx + (vya - b1.y) * dy;
+        Fx.add(mb * ((fxx - fp) * gx + fxy * gy));
+        Fy.add(mb * ((fyy - fp) * gy + fxy * gx));
+        const float vr = dvx * dx + dvy * dy;
         const float muv = amh * vr * __builtin_amdgcn_rcpf(dr + amh * amh * et2);
-        const float mc = 0.5f * (ca + b3.x);
-        float piv = (beta * muv - alpha * mc) * muv * (2.0f * __builtin_amdgcn_rcpf(rhoa + b1.z + X.rho0_s));
+        float piv = (beta * muv - alpha * cn) * muv * rs;
         piv = vr < 0.0f ? piv : 0.0f;
         artx.add(-(mb * piv * gx));
         arty.add(-(mb * piv * gy));
@@ -639,26 +550,74 @@ __global__ STAGE_ATTR_MX void k_soil_b_mx(DPhase S, const float* qS, Masks MS, G
         dYY.add(hh * (syya - b3.w));
     });
     if (!wc.valid) return;
-    const double rhoa_d = S.rho[a], ha_d = S.hsml[a];
-    double ax = -(Ax.value() * rhoa_d) + Bxx.value() * rhoa_d + Cxyy.value() * rhoa_d;
-    double ay = -(Ay.value() * rhoa_d) + Cxyx.value() * rhoa_d + Dyy.value() * rhoa_d;
+    const double ha_d = S.hsml[a], ca_d = S.c[a], rhoa_d = S.rho[a];
+    const double txx_d = txx.value() * SPH_C23_F, tyy_d = tyy.value() * SPH_C23_F, txy_d = txy.value();
+    const double vcc_d = vcc.value(), wxy_d = wxy.value() * 0.5;
+    S.drho[a] = con.value() + (K.delta * ha_d * ca_d) * drd.value();
+    S.txx[a] = txx_d; S.txy[a] = txy_d; S.tyy[a] = tyy_d; S.vcc[a] = vcc_d; S.wxy[a] = wxy_d;
+    double avx_d = avx.value(), avy_d = avy.value();
+    if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
+    S.avx[a] = avx_d; S.avy[a] = avy_d;
+    S.repx[a] = repx; S.repy[a] = repy;
+    const double dgx_d = dg_x.value(), dgy_d = dg_y.value();
+    const double pox_d = po_x.value() * vofa_d, poy_d = po_y.value() * vofa_d;
+    if (cross) {
+        S.drgx[a] = dgx_d; S.drgy[a] = dgy_d; S.porx[a] = pox_d; S.pory[a] = poy_d;
+        S.countiac[a] = cntx;
+    } else {
+        S.countiac[a] = cnt;
+    }
+    S.c_next[a] = K.s_c;
+    const double p = S.p[a], sxx = S.sxx[a], sxy = S.sxy[a], syy = S.syy[a];
+    const double alpha_d = K.s_alpha1, G = K.s_G, Kb = K.s_k;
+    double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
+    if (K.plasticity == 3) {
+        const double I1 = 3.0 * p;
+        const double j2r = sxx * sxx + 2.0 * (sxy * sxy) + syy * syy + (sxx + syy) * (sxx + syy);
+        const double J2 = __builtin_sqrt(j2r / 2.0) + 1.0e-6;
+        const int fail = fabs(J2 - alpha_d * I1) < SPH_YIELD_TOL_F ? 1 : 0;
+        if (fail) {
+            const double exx = txx_d * 0.5 + vcc_d * (1.0 / 3.0);
+            const double exy = txy_d * 0.5;
+            const double eyy = tyy_d * 0.5 + vcc_d * (1.0 / 3.0);
+            const double sde = sxx * exx + 2.0 * sxy * exy + syy * eyy;
+            const double sj = __builtin_sqrt(j2r / 2.0 + 1.0e-10);
+            const double gs = G / sj;
+            const double dl = (3.0 * alpha_d * Kb * vcc_d + gs * sde) / (9.0 * (alpha_d * alpha_d) * Kb + G);
+            if (!(dl < 0.0)) {
+                dsxx = -dl * (gs * sxx);
+                dsxy = -dl * (gs * sxy);
+                dsyy = -dl * (gs * syy);
+                dpv = 3.0 * Kb * alpha_d * dl;
+                S.eps_p[a] = S.eps_p[a] + dl * sxy / (2.0 * sj) * SPH_EPSP_F;
+            }
+        }
+        S.fail[a] = fail;
+    }
+    dsxx = dsxx + G * txx_d + 2.0 * sxy * wxy_d;
+    dsxy = dsxy + G * txy_d - (sxx - syy) * wxy_d;
+    dsyy = dsyy + G * tyy_d - 2.0 * sxy * wxy_d;
+    dpv = dpv - Kb * vcc_d;
+
+    double ax = Fx.value() * rhoa_d;
+    double ay = Fy.value() * rhoa_d;
     if (rhoa_d > 0.0) { ax = ax / rhoa_d; ay = ay / rhoa_d; }
     if (mask & M_ART) { ax = ax + artx.value(); ay = ay + arty.value(); }
-    ax = ax + S.repx[a];
-    ay = ay + S.repy[a];
+    ax = ax + (double)repx;
+    ay = ay + (double)repy;
     ay = ay + K.gravity;
     if (mask & M_CROSS) {
-        ax = ax + S.drgx[a]; ay = ay + S.drgy[a];
-        ax = ax + S.porx[a]; ay = ay + S.pory[a];
+        ax = ax + dgx_d; ay = ay + dgy_d;
+        ax = ax + pox_d; ay = ay + poy_d;
     }
+    if (mask & M_DELTA_FIELD) {
+        const double cf = K.delta * ha_d * K.s_c;   // new c
+        dpv = dpv + cf * dP.value();
+        dsxx = dsxx + cf * dXX.value();
+        dsxy = dsxy + cf * dXY.value();
+        dsyy = dsyy + cf * dYY.value();
+    }
+    S.dsxx[a] = dsxx; S.dsxy[a] = dsxy; S.dsyy[a] = dsyy; S.dp[a] = dpv;
     S.ax[a] = ax;
     S.ay[a] = ay;
-    if (mask & M_DELTA_FIELD) {
-        const double cf = K.delta * ha_d * S.c[a];   // new c
-        S.dp[a] = S.dp[a] + cf * dP.value();
-        S.dsxx[a] = S.dsxx[a] + cf * dXX.value();
-        S.dsxy[a] = S.dsxy[a] + cf * dXY.value();
-        S.dsyy[a] = S.dsyy[a] + cf * dYY.value();
-    }
 }
-

@@ -41,6 +41,7 @@ struct PhaseBuf {
     DPhase d{};
     std::vector<void*> allocs;
     bool cells_valid = false;
+    bool counts_zero = false;   // d.cell_count known to be all zero
     // materialised pairs
     int *fwd = nullptr, *off = nullptr;
     int *pi = nullptr, *pj = nullptr;
@@ -192,12 +193,13 @@ int fill_int(sphg_ctx* c, int* a, int n, int v)
     return launch(c, "fill", k_fill_int, nblk(n), TPB, a, n, v);
 }
 
-// Exclusive scan of n ints into out[0..n], total at out[n].
-int exclusive_scan(sphg_ctx* c, const int* in, int* out, int n)
+// Exclusive scan of n ints into out[0..n], total at out[n]; clear (the
+// input array or null) is zeroed as it is read.
+int exclusive_scan(sphg_ctx* c, const int* in, int* out, int n, int* clear = nullptr)
 {
     const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (ntiles > SCAN_TILE) return SPHG_ERR_UNSUPPORTED;
-    RC(launch(c, "scan", k_scan_tile, ntiles > 0 ? ntiles : 1, SCAN_BLOCK, in, out, n, c->d_scan));
+    RC(launch(c, "scan", k_scan_tile, ntiles > 0 ? ntiles : 1, SCAN_BLOCK, in, out, n, c->d_scan, clear));
     RC(launch(c, "scan", k_scan_sums, 1, SCAN_BLOCK, c->d_scan, ntiles > 0 ? ntiles : 1));
     RC(launch(c, "scan", k_scan_add, nblk(n > 0 ? n : 1), TPB, out, n, (const int*)c->d_scan, ntiles > 0 ? ntiles : 1));
     return SPHG_OK;
@@ -210,9 +212,12 @@ int build_cells(sphg_ctx* c, int k)
 {
     PhaseBuf& b = c->ph[k];
     if (b.cells_valid) return SPHG_OK;
-    RC(fill_int(c, b.d.cell_count, c->ncell, 0));
+    // cell_count is all zero between builds: the scan clears what it reads
+    if (!b.counts_zero) RC(fill_int(c, b.d.cell_count, c->ncell, 0));
+    b.counts_zero = false;
     RC(launch(c, "cell_key", k_cell_key, nblk(b.n), TPB, b.d, c->gl[k], c->d_err));
-    RC(exclusive_scan(c, b.d.cell_count, b.d.cell_start, c->ncell));
+    RC(exclusive_scan(c, b.d.cell_count, b.d.cell_start, c->ncell, b.d.cell_count));
+    b.counts_zero = true;
     RC(launch(c, "cell_scatter", k_cell_scatter, nblk(b.n), TPB, b.d));
     RC(launch(c, "cell_sort", k_cell_sort, nblk(c->ncell), TPB, b.d, c->ncell));
     b.cells_valid = true;
@@ -395,6 +400,7 @@ int setup_grid(sphg_ctx* c)
             PhaseBuf& b = c->ph[k];
             if (!b.bound) continue;
             RC(dalloc(c, b, &b.d.cell_count, (size_t)ncell));
+            b.counts_zero = false;
             RC(dalloc(c, b, &b.d.cell_start, (size_t)ncell + 1));
             b.cells_valid = false;
         }
@@ -735,14 +741,12 @@ int stages_mixed(sphg_ctx* c)
     float* qS = reinterpret_cast<float*>(S.rec);
     RC(launch(c, "water_a", k_water_a_mx<SKF>, nblk(W.n, 64), 64, W.d, S.d, (const float*)qW, qW, (const float*)qS,
               W.masks, c->gl[0], c->K, c->F, c->X, 1));
-    RC(launch(c, "soil_a", k_soil_a_mx<SKF>, nblk(S.n, 64), 64, S.d, W.d, (const float*)qS, qS, (const float*)qW,
-              S.masks, c->gl[1], c->K, c->F, c->X, 1));
+    RC(launch(c, "soil_ab", k_soil_ab_mx<SKF>, nblk(S.n, 64), 64, S.d, W.d, (const float*)qS, (const float*)qW,
+              c->gl[1], c->K, c->F, c->X, 1, mask_b(c, 1)));
     swap_c(W);
     swap_c(S);
     RC(launch(c, "water_b", k_water_b_mx<SKF>, nblk(W.n, 64), 64, W.d, (const float*)qW, W.masks, c->gl[0], c->K,
               c->F, c->X, mask_b(c, 0)));
-    RC(launch(c, "soil_b", k_soil_b_mx<SKF>, nblk(S.n, 64), 64, S.d, (const float*)qS, S.masks, c->gl[1], c->K,
-              c->F, c->X, mask_b(c, 1)));
     return SPHG_OK;
 }
 
