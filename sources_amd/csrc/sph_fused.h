@@ -42,21 +42,28 @@ enum : int {
 #ifndef SPHG_LB
 #define SPHG_LB 0
 #endif
+#ifndef SPHG_WPB
+#define SPHG_WPB 4           // waves per workgroup of the stage kernels (consecutive particles share a CU)
+#endif
+constexpr int STAGE_BLOCK = 64 * SPHG_WPB;
+#ifndef SPHG_PIPE
+#define SPHG_PIPE 0          // 1: software-pipelined partner loads in the dense loops
+#endif
 #ifndef SPHG_STAGE_WAVES
 #define SPHG_STAGE_WAVES 0   // 0: compiler's choice; n: ask for at least n waves per SIMD
 #endif
 #if SPHG_STAGE_WAVES > 0
-#define STAGE_ATTR __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPHG_STAGE_WAVES, 8)))
+#define STAGE_ATTR __launch_bounds__(STAGE_BLOCK) __attribute__((amdgpu_waves_per_eu(SPHG_STAGE_WAVES, 8)))
 #else
-#define STAGE_ATTR __launch_bounds__(64)
+#define STAGE_ATTR __launch_bounds__(STAGE_BLOCK)
 #endif
 #ifndef SPHG_AB_WAVES
 #define SPHG_AB_WAVES 0      // the same for the one-pass soil kernel
 #endif
 #if SPHG_AB_WAVES > 0
-#define STAGE_ATTR_AB __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPHG_AB_WAVES, 8)))
+#define STAGE_ATTR_AB __launch_bounds__(STAGE_BLOCK) __attribute__((amdgpu_waves_per_eu(SPHG_AB_WAVES, 8)))
 #else
-#define STAGE_ATTR_AB __launch_bounds__(64)
+#define STAGE_ATTR_AB __launch_bounds__(STAGE_BLOCK)
 #endif
 
 struct FusedK {
@@ -305,11 +312,20 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
     }
     const double m = P.mass[i];
     const double irho = 1.0 / rho;
+    // equation of state (pressure(), new sound speed) at the half-step
+    // density: pointwise, so the soil stage can read the new water pressure
+    // without waiting for the water stage
+    const double rr = rho / (K.w_rho0 * vof);
+    const double p = K.w_b * (pow(rr, K.w_gamma) - 1.0);
+    const double cn = K.w_c * pow(rr, 3.0);
+    P.p[i] = p;
+    P.c_next[i] = cn;
     if (q) {
         float4* Q = reinterpret_cast<float4*>(q) + t;
         const size_t pn = (size_t)P.n;
         write_q01(Q, pn, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.w_rho0, m);
-        Q[2 * pn] = make_float4((float)vof, (float)irho, (float)(m * irho), 0.0f);
+        Q[2 * pn] = make_float4((float)vof, (float)irho, (float)(m * irho), (float)p);
+        Q[3 * pn] = make_float4((float)cn, 0.0f, 0.0f, 0.0f);
         return;
     }
     double2* R = reinterpret_cast<double2*>(rec) + t;
@@ -317,7 +333,8 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
     R[0 * pn] = make_double2(P.x[i], P.y[i]);
     R[1 * pn] = make_double2(vx, vy);
     R[2 * pn] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
-    R[3 * pn] = make_double2(vof, 0.0);
+    R[3 * pn] = make_double2(vof, p);
+    R[4 * pn] = make_double2(0.0, cn);
     R[6 * pn] = make_double2(P.hsml[i], (double)it);
 }
 
@@ -565,8 +582,28 @@ __device__ __forceinline__ void visit_win(IT it, const LaneRows& L, int ovf, int
                                           B&& body)
 {
     int i = 0;
+#if SPHG_PIPE
+    // next partner's loads issued before the current partner's terms
+    int k = it.next(i);
+    if (k >= 0) {
+        auto b = load(i, k);
+#pragma unroll 1
+        for (;;) {
+            int i2 = 0;
+            const int k2 = it.next(i2);
+            if (k2 < 0) {
+                body(b);
+                break;
+            }
+            auto b2 = load(i2, k2);
+            body(b);
+            b = b2;
+        }
+    }
+#else
 #pragma unroll 1
     for (int k = it.next(i); k >= 0; k = it.next(i)) body(load(i, k));
+#endif
     if (ovf) {
 #pragma unroll 1
         for (int r = 0; r < 3; ++r) {
@@ -720,8 +757,8 @@ struct WaveCtx {
 __device__ __forceinline__ WaveCtx wave_ctx(int n)
 {
     WaveCtx c;
-    c.lane = threadIdx.x;
-    const int t0 = xcd_block() * 64;
+    c.lane = threadIdx.x & 63;
+    const int t0 = (xcd_block() * SPHG_WPB + (int)(threadIdx.x >> 6)) * 64;
     c.t = t0 + c.lane;
     c.valid = c.t < n;
     c.last = min(63, n - 1 - t0);
@@ -773,6 +810,7 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
                                                  int cross)
 {
     static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
+    static_assert(SPHG_WPB == 1 || (LA == 0 && LX == 0), "windows need one wave per workgroup");
     __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
     const RecV RW = recv(recW, W.n), RS = recv(recS, S.n);
     const WaveCtx wc = wave_ctx(W.n);
@@ -879,9 +917,6 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
         });
     }
     if (!wc.valid) return;
-    const double rr = rhoa / (K.w_rho0 * vofa);
-    const double p = K.w_b * (pow(rr, K.w_gamma) - 1.0);
-    const double cn = K.w_c * pow(rr, 3.0);
     const double mu = K.w_mu;
     txx *= SPH_C23_F;
     tyy *= SPH_C23_F;
@@ -890,8 +925,6 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
     W.avx[a] = avx; W.avy[a] = avy;
     W.repx[a] = repx; W.repy[a] = repy;
-    W.p[a] = p;
-    W.c_next[a] = cn;
     if (cross) {
         W.drgx[a] = dg_x; W.drgy[a] = dg_y; W.dvof_x[a] = dvof_x * (1.0 / K.s_rho0);
         W.vof2[a] = 1.0 - vof2 / K.s_rho0;
@@ -899,10 +932,10 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     } else {
         W.countiac[a] = cnt;
     }
+    // p and the new c (chunks 3 and 4) come from the pack
     double2* WP = reinterpret_cast<double2*>(recWw) + t;
     const size_t nw = (size_t)W.n;
-    reinterpret_cast<double*>(WP + 3 * nw)[1] = p;   // chunk 3 = (vof, p)
-    WP[4 * nw] = make_double2(vofa * (mu * tyy), cn);
+    reinterpret_cast<double*>(WP + 4 * nw)[0] = vofa * (mu * tyy);
     WP[5 * nw] = make_double2(vofa * (mu * txx), vofa * (mu * txy));
 }
 
@@ -916,6 +949,7 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
                                                  FusedK F, int mask)
 {
     static_assert(LB == 0 || (LB & 1), "window needs positions");
+    static_assert(SPHG_WPB == 1 || LB == 0, "windows need one wave per workgroup");
     __shared__ double2 lds[lds_rows(Lds<LB>::n, 0)][WIN_N];
     const RecV RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(W.n);
@@ -1018,6 +1052,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
                                       KParams K, FusedK F, int cross, int mask)
 {
     static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
+    static_assert(SPHG_WPB == 1 || (LA == 0 && LX == 0), "windows need one wave per workgroup");
     __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
     const RecV RS = recv(recS, S.n), RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(S.n);

@@ -232,20 +232,59 @@ __global__ __launch_bounds__(256) void k_cell_key(DPhase P, Grid g, int* err)
     if (valid) P.slot[i] = base + (lane - start);
 }
 
-__global__ void k_cell_scatter(DPhase P)
+// Exclusive prefix of the tile sums of k_scan_tile (ntiles <= SCAN_TILE)
+// into LDS tp[0..ntiles], by the whole workgroup (SCAN_BLOCK threads).
+__device__ __forceinline__ void tile_prefix(const int* tile_sums, int ntiles, int* tp, int* lds)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    P.order[P.cell_start[P.key[i]] + P.slot[i]] = i;
+    const int base = threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS], s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = (base + i < ntiles) ? tile_sums[base + i] : 0;
+        const int t = v[i];
+        v[i] = s;
+        s += t;
+    }
+    int total;
+    const int pre = block_exclusive_scan(s, lds, total);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i)
+        if (base + i <= ntiles) tp[base + i] = pre + v[i];
+    if (threadIdx.x == 0 && ntiles == SCAN_TILE) tp[SCAN_TILE] = total;
+    __syncthreads();
 }
 
-// Ascending particle number inside each cell: makes the neighbour visiting
-// order (and so every floating-point sum) independent of atomic arrival order.
-__global__ void k_cell_sort(DPhase P, int ncell)
+// Cell-list scatter on the tile-local prefix of the counts (part, from
+// k_scan_tile) plus the prefix of the tile sums, computed per workgroup:
+// the three-pass scan becomes one pass.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_cell_scatter_t(DPhase P, const int* part, const int* tile_sums,
+                                                               int ntiles)
 {
+    __shared__ int tp[SCAN_TILE + 1];
+    __shared__ int lds[SCAN_BLOCK];
+    tile_prefix(tile_sums, ntiles, tp, lds);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int key = P.key[i];
+    P.order[part[key] + tp[key / SCAN_TILE] + P.slot[i]] = i;
+}
+
+// Ascending particle number inside each cell (insertion sort; the ranks of
+// k_cell_key depend on atomic arrival order), which makes the neighbour
+// visiting order and so every floating-point sum deterministic; and the
+// final cell_start from the tile-local prefix and the tile sums.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_cell_sort_t(DPhase P, const int* part, const int* tile_sums,
+                                                            int ntiles, int ncell)
+{
+    __shared__ int tp[SCAN_TILE + 1];
+    __shared__ int lds[SCAN_BLOCK];
+    tile_prefix(tile_sums, ntiles, tp, lds);
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    const int s = P.cell_start[c], e = P.cell_start[c + 1];
+    const int s = part[c] + tp[c / SCAN_TILE];
+    const int e = c + 1 < ncell ? part[c + 1] + tp[(c + 1) / SCAN_TILE] : tp[ntiles];
+    P.cell_start[c] = s;
+    if (c + 1 == ncell) P.cell_start[ncell] = e;
     for (int k = s + 1; k < e; ++k) {
         const int v = P.order[k];
         int m = k - 1;
@@ -256,6 +295,7 @@ __global__ void k_cell_sort(DPhase P, int ncell)
         P.order[m + 1] = v;
     }
 }
+
 
 // ------------------------------------------------------------------------
 // Per-particle helpers

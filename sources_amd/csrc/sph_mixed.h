@@ -21,14 +21,14 @@
 
 // The FP32 stage kernels fit five waves per SIMD without spilling (measured
 // faster than the compiler's default of four).
-#define STAGE_ATTR_MX __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
+#define STAGE_ATTR_MX __launch_bounds__(STAGE_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)))
 #ifndef SPHG_MX_AB_WAVES
 #define SPHG_MX_AB_WAVES 4   // the one-pass soil kernel: 4 waves (a few spills) beat 3
 #endif
 #if SPHG_MX_AB_WAVES > 0
-#define STAGE_ATTR_MX_AB __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPHG_MX_AB_WAVES, 8)))
+#define STAGE_ATTR_MX_AB __launch_bounds__(STAGE_BLOCK) __attribute__((amdgpu_waves_per_eu(SPHG_MX_AB_WAVES, 8)))
 #else
-#define STAGE_ATTR_MX_AB __launch_bounds__(64)
+#define STAGE_ATTR_MX_AB __launch_bounds__(STAGE_BLOCK)
 #endif
 
 
@@ -296,10 +296,7 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
         });
     }
     if (!wc.valid) return;
-    const double rhoa_d = W.rho[a], vofa_d = W.vof[a], ha_d = W.hsml[a], ca_d = W.c[a];
-    const double rr = rhoa_d / (K.w_rho0 * vofa_d);
-    const double p = K.w_b * (pow(rr, K.w_gamma) - 1.0);
-    const double cn = K.w_c * pow(rr, 3.0);
+    const double vofa_d = W.vof[a], ha_d = W.hsml[a], ca_d = W.c[a];
     const double mu = K.w_mu;
     const double txx_d = txx.value() * SPH_C23_F, tyy_d = tyy.value() * SPH_C23_F, txy_d = txy.value();
     W.drho[a] = con.value() + (K.delta * ha_d * ca_d) * drd.value();
@@ -308,8 +305,6 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
     if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
     W.avx[a] = avx_d; W.avy[a] = avy_d;
     W.repx[a] = repx; W.repy[a] = repy;
-    W.p[a] = p;
-    W.c_next[a] = cn;
     if (cross) {
         W.drgx[a] = dg_x.value(); W.drgy[a] = dg_y.value(); W.dvof_x[a] = dvof_x.value() * (1.0 / K.s_rho0);
         W.vof2[a] = 1.0 - vof2.value() / K.s_rho0;
@@ -317,11 +312,11 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
     } else {
         W.countiac[a] = cnt;
     }
-    float4* QP = reinterpret_cast<float4*>(qWw) + t;
-    const size_t nw = (size_t)W.n;
-    reinterpret_cast<float*>(QP + 2 * nw)[3] = (float)p;   // q2.w
-    QP[3 * nw] = make_float4((float)cn, (float)(vofa_d * (mu * txx_d)), (float)(vofa_d * (mu * txy_d)),
-                             (float)(vofa_d * (mu * tyy_d)));
+    // p (q2.w) and the new c (q3.x) come from the pack
+    float* Q3 = reinterpret_cast<float*>(reinterpret_cast<float4*>(qWw) + 3 * (size_t)W.n + t);
+    Q3[1] = (float)(vofa_d * (mu * txx_d));
+    Q3[2] = (float)(vofa_d * (mu * txy_d));
+    Q3[3] = (float)(vofa_d * (mu * tyy_d));
 }
 
 // ------------------------------------------------------------------------

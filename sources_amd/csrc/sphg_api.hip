@@ -42,6 +42,7 @@ struct PhaseBuf {
     std::vector<void*> allocs;
     bool cells_valid = false;
     bool counts_zero = false;   // d.cell_count known to be all zero
+    int* cell_part = nullptr;   // tile-local exclusive prefix of the cell counts
     // materialised pairs
     int *fwd = nullptr, *off = nullptr;
     int *pi = nullptr, *pj = nullptr;
@@ -69,6 +70,12 @@ struct sphg_ctx {
     int itimestep = 0;
     int device = 0;
     hipStream_t stream = nullptr;
+    // side stream: the soil stage runs on it concurrently with the water
+    // stages (fork and join events on the main stream)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool use_side = true;       // SPHG_SIDE_STREAM=0 keeps every kernel on the main stream
+    hipStream_t lstream = nullptr;   // stream of the next launches when set
     PhaseBuf ph[2];
     int* d_err = nullptr;       // [0] out of range
     int* d_scan = nullptr;      // tile sums
@@ -142,19 +149,20 @@ int launch(sphg_ctx* c, const char* name, void (*kern)(KArgs...), int grid, int 
 {
     if (grid <= 0) return SPHG_OK;
     hipEvent_t ea = nullptr, eb = nullptr;
+    hipStream_t st = c->lstream ? c->lstream : c->stream;
     if (c->prof) {
         ea = get_event(c);
         eb = get_event(c);
-        hipEventRecord(ea, c->stream);
+        hipEventRecord(ea, st);
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, args...);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, args...);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         c->last_hip = (int)e;
         return SPHG_ERR_HIP;
     }
     if (c->prof) {
-        hipEventRecord(eb, c->stream);
+        hipEventRecord(eb, st);
         c->pending.push_back({name, ea, eb});
         if (c->pending.size() > 4096) drain_events(c);
     }
@@ -216,10 +224,17 @@ int build_cells(sphg_ctx* c, int k)
     if (!b.counts_zero) RC(fill_int(c, b.d.cell_count, c->ncell, 0));
     b.counts_zero = false;
     RC(launch(c, "cell_key", k_cell_key, nblk(b.n), TPB, b.d, c->gl[k], c->d_err));
-    RC(exclusive_scan(c, b.d.cell_count, b.d.cell_start, c->ncell, b.d.cell_count));
+    // one scan pass: tile-local prefixes and tile sums; the scatter and the
+    // per-cell sort add the prefix of the tile sums themselves
+    const int ntiles = std::max((c->ncell + SCAN_TILE - 1) / SCAN_TILE, 1);
+    if (ntiles > SCAN_TILE) return SPHG_ERR_UNSUPPORTED;
+    RC(launch(c, "scan", k_scan_tile, ntiles, SCAN_BLOCK, (const int*)b.d.cell_count, b.cell_part, c->ncell, c->d_scan,
+              b.d.cell_count));
     b.counts_zero = true;
-    RC(launch(c, "cell_scatter", k_cell_scatter, nblk(b.n), TPB, b.d));
-    RC(launch(c, "cell_sort", k_cell_sort, nblk(c->ncell), TPB, b.d, c->ncell));
+    RC(launch(c, "cell_scatter", k_cell_scatter_t, nblk(b.n, SCAN_BLOCK), SCAN_BLOCK, b.d, (const int*)b.cell_part,
+              (const int*)c->d_scan, ntiles));
+    RC(launch(c, "cell_sort", k_cell_sort_t, nblk(c->ncell, SCAN_BLOCK), SCAN_BLOCK, b.d, (const int*)b.cell_part,
+              (const int*)c->d_scan, ntiles, c->ncell));
     b.cells_valid = true;
     return SPHG_OK;
 }
@@ -400,6 +415,7 @@ int setup_grid(sphg_ctx* c)
             PhaseBuf& b = c->ph[k];
             if (!b.bound) continue;
             RC(dalloc(c, b, &b.d.cell_count, (size_t)ncell));
+            RC(dalloc(c, b, &b.cell_part, (size_t)ncell));
             b.counts_zero = false;
             RC(dalloc(c, b, &b.d.cell_start, (size_t)ncell + 1));
             b.cells_valid = false;
@@ -674,6 +690,29 @@ int mask_a(const sphg_ctx* c, int base)
     return m;
 }
 
+// Fork the side stream off the main stream (it waits for the work issued so
+// far) and join it back (the main stream waits for the side work).
+int fork_side(sphg_ctx* c)
+{
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    return SPHG_OK;
+}
+
+int join_side(sphg_ctx* c)
+{
+    HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return SPHG_OK;
+}
+
+// Launches issued inside its scope go to the side stream (when enabled).
+struct SideScope {
+    sphg_ctx* c;
+    explicit SideScope(sphg_ctx* ctx) : c(ctx) { if (c->use_side) c->lstream = c->side; }
+    ~SideScope() { c->lstream = nullptr; }
+};
+
 int mask_b(const sphg_ctx* c, int phase)
 {
     int m = MASK_B;
@@ -712,22 +751,31 @@ int stages_fused(sphg_ctx* c)
     PhaseBuf& S = c->ph[1];
     const int cross = c->P.single_phase ? 0 : 1;
     const bool rep_int = c->K.ip1 >= 0 && c->K.ip2 >= 0;
+    // The soil stage reads only the packed records of both phases (the water
+    // pressure comes from the pack): it runs on the side stream while the
+    // water stages run on the main stream.
+    if (c->use_side) RC(fork_side(c));
+    {
+        SideScope side(c);
+        if ((c->K.nnps == 2 || c->F.uni_s) && c->F.uni_x && rep_int)
+            RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d,
+                      W.d, (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
+        else
+            RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
+                      S.d, W.d, (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross,
+                      mask_b(c, 1)));
+    }
     if ((c->K.nnps == 2 || c->F.uni_w) && c->F.uni_x && rep_int)
-        RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX, true>, nblk(W.n, 64), 64, W.d, S.d,
+        RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d,
                   (const double*)W.rec, W.rec, (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
     else
-        RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX, false>, nblk(W.n, 64), 64, W.d, S.d,
-                  (const double*)W.rec, W.rec, (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
-    if ((c->K.nnps == 2 || c->F.uni_s) && c->F.uni_x && rep_int)
-        RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true>, nblk(S.n, 64), 64, S.d, W.d,
-                  (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
-    else
-        RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, false>, nblk(S.n, 64), 64, S.d, W.d,
-                  (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
+        RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d,
+                  S.d, (const double*)W.rec, W.rec, (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
     swap_c(W);
     swap_c(S);
-    RC(launch(c, "water_b", k_water_b4<SKF, SPHG_LB>, nblk(W.n, 64), 64, W.d, (const double*)W.rec, W.masks, c->gl[0], c->K,
-              c->F, mask_b(c, 0)));
+    RC(launch(c, "water_b", k_water_b4<SKF, SPHG_LB>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, (const double*)W.rec,
+              W.masks, c->gl[0], c->K, c->F, mask_b(c, 0)));
+    if (c->use_side) RC(join_side(c));
     return SPHG_OK;
 }
 
@@ -739,14 +787,19 @@ int stages_mixed(sphg_ctx* c)
     PhaseBuf& S = c->ph[1];
     float* qW = reinterpret_cast<float*>(W.rec);
     float* qS = reinterpret_cast<float*>(S.rec);
-    RC(launch(c, "water_a", k_water_a_mx<SKF>, nblk(W.n, 64), 64, W.d, S.d, (const float*)qW, qW, (const float*)qS,
-              W.masks, c->gl[0], c->K, c->F, c->X, 1));
-    RC(launch(c, "soil_ab", k_soil_ab_mx<SKF>, nblk(S.n, 64), 64, S.d, W.d, (const float*)qS, (const float*)qW,
-              c->gl[1], c->K, c->F, c->X, 1, mask_b(c, 1)));
+    if (c->use_side) RC(fork_side(c));   // soil stage beside the water stages
+    {
+        SideScope side(c);
+        RC(launch(c, "soil_ab", k_soil_ab_mx<SKF>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, (const float*)qS,
+                  (const float*)qW, c->gl[1], c->K, c->F, c->X, 1, mask_b(c, 1)));
+    }
+    RC(launch(c, "water_a", k_water_a_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, (const float*)qW, qW,
+              (const float*)qS, W.masks, c->gl[0], c->K, c->F, c->X, 1));
     swap_c(W);
     swap_c(S);
-    RC(launch(c, "water_b", k_water_b_mx<SKF>, nblk(W.n, 64), 64, W.d, (const float*)qW, W.masks, c->gl[0], c->K,
-              c->F, c->X, mask_b(c, 0)));
+    RC(launch(c, "water_b", k_water_b_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, (const float*)qW, W.masks,
+              c->gl[0], c->K, c->F, c->X, mask_b(c, 0)));
+    if (c->use_side) RC(join_side(c));
     return SPHG_OK;
 }
 
@@ -914,7 +967,14 @@ int sphg_create(const sphg_params* params, sphg_ctx** out)
         c->since_reorder = c->reorder_every;   // first fused step sorts the state
     }
     set_kparams(c);
+    {
+        const char* e = std::getenv("SPHG_SIDE_STREAM");
+        if (e && e[0] == '0') c->use_side = false;
+    }
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&c->d_err, sizeof(int) * 4) != hipSuccess) {
         delete c;
         return SPHG_ERR_HIP;
@@ -937,6 +997,12 @@ void sphg_destroy(sphg_ctx* c)
     for (hipEvent_t e : c->event_pool) hipEventDestroy(e);
     if (c->d_err) hipFree(c->d_err);
     if (c->d_scan) hipFree(c->d_scan);
+    if (c->side) {
+        hipStreamSynchronize(c->side);
+        hipStreamDestroy(c->side);
+    }
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
