@@ -74,7 +74,9 @@ struct sphg_ctx {
     // stages (fork and join events on the main stream)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool use_side = true;       // SPHG_SIDE_STREAM=0 keeps every kernel on the main stream
+    // SPHG_SIDE_STREAM=0 keeps every kernel on the main stream; so does the
+    // per-kernel profile mode, whose durations are then not shared
+    bool use_side = true;
     hipStream_t lstream = nullptr;   // stream of the next launches when set
     PhaseBuf ph[2];
     int* d_err = nullptr;       // [0] out of range
@@ -692,6 +694,8 @@ int mask_a(const sphg_ctx* c, int base)
 
 // Fork the side stream off the main stream (it waits for the work issued so
 // far) and join it back (the main stream waits for the side work).
+bool side_on(const sphg_ctx* c) { return c->use_side && !c->prof; }
+
 int fork_side(sphg_ctx* c)
 {
     HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
@@ -709,7 +713,7 @@ int join_side(sphg_ctx* c)
 // Launches issued inside its scope go to the side stream (when enabled).
 struct SideScope {
     sphg_ctx* c;
-    explicit SideScope(sphg_ctx* ctx) : c(ctx) { if (c->use_side) c->lstream = c->side; }
+    explicit SideScope(sphg_ctx* ctx) : c(ctx) { if (side_on(c)) c->lstream = c->side; }
     ~SideScope() { c->lstream = nullptr; }
 };
 
@@ -754,7 +758,7 @@ int stages_fused(sphg_ctx* c)
     // The soil stage reads only the packed records of both phases (the water
     // pressure comes from the pack): it runs on the side stream while the
     // water stages run on the main stream.
-    if (c->use_side) RC(fork_side(c));
+    if (side_on(c)) RC(fork_side(c));
     {
         SideScope side(c);
         if ((c->K.nnps == 2 || c->F.uni_s) && c->F.uni_x && rep_int)
@@ -775,7 +779,7 @@ int stages_fused(sphg_ctx* c)
     swap_c(S);
     RC(launch(c, "water_b", k_water_b4<SKF, SPHG_LB>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, (const double*)W.rec,
               W.masks, c->gl[0], c->K, c->F, mask_b(c, 0)));
-    if (c->use_side) RC(join_side(c));
+    if (side_on(c)) RC(join_side(c));
     return SPHG_OK;
 }
 
@@ -787,7 +791,7 @@ int stages_mixed(sphg_ctx* c)
     PhaseBuf& S = c->ph[1];
     float* qW = reinterpret_cast<float*>(W.rec);
     float* qS = reinterpret_cast<float*>(S.rec);
-    if (c->use_side) RC(fork_side(c));   // soil stage beside the water stages
+    if (side_on(c)) RC(fork_side(c));   // soil stage beside the water stages
     {
         SideScope side(c);
         RC(launch(c, "soil_ab", k_soil_ab_mx<SKF>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, (const float*)qS,
@@ -799,7 +803,7 @@ int stages_mixed(sphg_ctx* c)
     swap_c(S);
     RC(launch(c, "water_b", k_water_b_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, (const float*)qW, W.masks,
               c->gl[0], c->K, c->F, c->X, mask_b(c, 0)));
-    if (c->use_side) RC(join_side(c));
+    if (side_on(c)) RC(join_side(c));
     return SPHG_OK;
 }
 
@@ -1869,6 +1873,7 @@ struct sphg_batch {
     bool eager = false;                 // autotuner's choice: per-context launches, no graph
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
+    std::vector<char> side;             // the contexts' side-stream setting, restored on destroy
 };
 
 namespace {
@@ -1979,8 +1984,17 @@ int sphg_batch_create(sphg_ctx** ctxs, int n, sphg_batch** out)
             delete b;
             return SPHG_ERR_UNSUPPORTED;
         }
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (hipStreamSynchronize(c->stream) != hipSuccess || hipStreamSynchronize(c->side) != hipSuccess) {
+            delete b;
+            return SPHG_ERR_HIP;
+        }
+    }
+    for (sphg_ctx* c : b->ctx) {
         c->stable_ptrs = true;
+        // a context in a batch keeps every kernel on its own (captured)
+        // stream: the batch overlaps whole contexts instead
+        b->side.push_back(c->use_side ? 1 : 0);
+        c->use_side = false;
     }
     if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
         delete b;
@@ -2003,7 +2017,10 @@ void sphg_batch_destroy(sphg_batch* b)
     for (hipStream_t s : b->streams) hipStreamDestroy(s);
     for (hipEvent_t e : b->join) hipEventDestroy(e);
     if (b->fork) hipEventDestroy(b->fork);
-    for (sphg_ctx* c : b->ctx) c->stable_ptrs = false;
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        b->ctx[i]->stable_ptrs = false;
+        if (i < b->side.size()) b->ctx[i]->use_side = b->side[i];
+    }
     delete b;
 }
 
