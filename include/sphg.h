@@ -118,7 +118,11 @@ typedef struct sphg_params {
     int32_t maxngx, maxngy;
     double dt;
     int32_t precision;                /* SPHG_FP64 or SPHG_MIXED */
-    int32_t reserved[7];
+    /* param.f90:78 integrate_scheme: 1 (or 0) leapfrog; 2 Verlet
+       (time_integration_for_water_by_verlet, declarations_sph.f90:889-974,
+       single-phase water only, as sph.f90:51-57 dispatches it) */
+    int32_t integrate_scheme;
+    int32_t reserved[6];
 } sphg_params;
 
 /*

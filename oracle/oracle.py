@@ -57,6 +57,9 @@ def lib():
         L.oracle_single_step_for_water.argtypes = [P, V, C.c_int]
         L.oracle_time_integration.argtypes = [P, V, V, C.POINTER(C.c_int), C.c_int]
         L.oracle_time_integration_for_water.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
+        L.oracle_time_integration_for_water_by_verlet.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
+        L.oracle_verlet_update.argtypes = [P, V, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+        L.oracle_verlet_update.restype = None
         L.oracle_init_grid.argtypes = [P, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int), D, D]
         L.oracle_init_grid.restype = None
         _lib = L
@@ -129,7 +132,9 @@ def time_integration(P, wa, so, itimestep, nsteps):
     it = C.c_int(itimestep)
     if P.single_phase:
         v = wa.view()
-        rc = lib().oracle_time_integration_for_water(C.byref(P), C.byref(v), C.byref(it), nsteps)
+        f = (lib().oracle_time_integration_for_water_by_verlet if P.integrate_scheme == 2
+             else lib().oracle_time_integration_for_water)
+        rc = f(C.byref(P), C.byref(v), C.byref(it), nsteps)
         wa.pull_scalars(v)
     else:
         vw, vs = wa.view(), so.view()
@@ -138,6 +143,13 @@ def time_integration(P, wa, so, itimestep, nsteps):
         so.pull_scalars(vs)
     _check(rc, "time_integration")
     return it.value
+
+
+def verlet_update(P, wa, lastvx, lastrho, step):
+    """The update of the Verlet scheme after the right-hand side of `step`
+    (lastvx / lastrho: the other levels, updated in place)."""
+    v = wa.view()
+    lib().oracle_verlet_update(C.byref(P), C.byref(v), _dp(lastvx), _dp(lastrho), step)
 
 
 def init_grid(P, ntotal, hsml):

@@ -1062,3 +1062,59 @@ int oracle_time_integration_for_water(const sphg_params *P, sphg_phase_view *wa,
     }
     return SPHG_OK;
 }
+
+/* The update of time_integration_for_water_by_verlet after single_step_for_water
+ * of step `step` (declarations_sph.f90:902-948): real particles (the first
+ * ntotal) move by dt v + dt**2./2. dvx and advance their other velocity level
+ * by 2 dt dvx (every 50th step: restart from v + dt dvx); every particle's
+ * other density level advances by 2 dt drho (step 1 and every 50th step:
+ * restart); then both levels swap (whole arrays, as the reference does).
+ * Pinned against the scheme's closed form under constant dvx and drho: with
+ * u_k = (v_k - v_0) / (dt a), u_k = 2 ceil(k/2) for k < 50 and k + 1 from the
+ * first restart on; the same for the density; x by the sum of the velocities
+ * (tests/test_oracle_verlet.py). */
+void oracle_verlet_update(const sphg_params *P, sphg_phase_view *wa, double *lastvx, double *lastrho, int step)
+{
+    int n = wa->ntotal + wa->nvirt, nr = wa->ntotal, i, d;
+    double dt = P->dt, h2 = pow(dt, 2.0) / 2.0;   /* dt**2./2. */
+    for (i = 0; i < nr; i++)
+        for (d = 0; d < 2; d++) {
+            int k = 2 * i + d;
+            wa->x[k] = wa->x[k] + dt * wa->vx[k] + h2 * wa->dvx[k];
+            if (step % 50 != 0) lastvx[k] = lastvx[k] + 2.0 * dt * wa->dvx[k];
+            else lastvx[k] = wa->vx[k] + dt * wa->dvx[k];
+        }
+    for (i = 0; i < 2 * n; i++) { double t = wa->vx[i]; wa->vx[i] = lastvx[i]; lastvx[i] = t; }
+    for (i = 0; i < n; i++) {
+        if (step == 1) lastrho[i] = wa->rho[i] + dt * 2.0 * wa->drho[i];
+        else if (step % 50 != 0) lastrho[i] = lastrho[i] + dt * 2.0 * wa->drho[i];
+        else lastrho[i] = wa->rho[i] + dt * wa->drho[i];
+    }
+    for (i = 0; i < n; i++) { double t = wa->rho[i]; wa->rho[i] = lastrho[i]; lastrho[i] = t; }
+}
+
+/* time_integration_for_water_by_verlet, declarations_sph.f90:889-974
+ * (integrate_scheme = 2).  lastvx / lastrho are the routine's local arrays:
+ * they start as copies of vx and rho at every call (the reference allocates
+ * and fills them per call).  Composition of oracle_single_step_for_water and
+ * oracle_verlet_update, each pinned on its own. */
+int oracle_time_integration_for_water_by_verlet(const sphg_params *P, sphg_phase_view *wa, int *itimestep,
+                                                int nsteps)
+{
+    int n = wa->ntotal + wa->nvirt, it, rc = SPHG_OK;
+    double *lastvx, *lastrho;
+    if (P->summation_density) return SPHG_ERR_UNSUPPORTED;
+    lastvx = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
+    lastrho = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    memcpy(lastvx, wa->vx, sizeof(double) * 2 * (size_t)n);
+    memcpy(lastrho, wa->rho, sizeof(double) * (size_t)n);
+    for (it = 0; it < nsteps; it++) {
+        int step = ++(*itimestep);
+        rc = oracle_single_step_for_water(P, wa, step);
+        if (rc) break;
+        oracle_verlet_update(P, wa, lastvx, lastrho, step);
+    }
+    free(lastvx);
+    free(lastrho);
+    return rc;
+}

@@ -90,7 +90,8 @@ class Params(C.Structure):
         ("maxngx", C.c_int32), ("maxngy", C.c_int32),
         ("dt", C.c_double),
         ("precision", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("integrate_scheme", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 

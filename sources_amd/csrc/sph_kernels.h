@@ -413,6 +413,46 @@ __global__ void k_final_water(DPhase P, KParams K, int first, int with_vof)
     P.y[i] = P.y[i] + dt * vy;
 }
 
+// time_integration_for_water_by_verlet (declarations_sph.f90:889-974), the
+// update after single_step_for_water of step `step`: real particles (the
+// first ntotal of the reference numbering) move by dt v + h2 a with
+// h2 = dt**2./2. and advance their other velocity level by 2 dt a (every
+// 50th step: restart from v + dt a); every particle's other density level
+// advances by 2 dt drho (step 1 and every 50th step: restart); then both
+// levels swap, as the reference's whole-array swaps do (the virtual
+// particles' velocity levels swap unchanged).
+__global__ void k_verlet_water(DPhase P, double* lvx, double* lvy, double* lrho, double dt, double h2, int step)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double vx = P.vx[i], vy = P.vy[i];
+    double nvx = lvx[i], nvy = lvy[i];
+    if (P.id[i] < P.ntotal) {
+        const double ax = P.ax[i], ay = P.ay[i];
+        P.x[i] = P.x[i] + dt * vx + h2 * ax;
+        P.y[i] = P.y[i] + dt * vy + h2 * ay;
+        if (step % 50 != 0) {
+            nvx = nvx + 2.0 * dt * ax;
+            nvy = nvy + 2.0 * dt * ay;
+        } else {
+            nvx = vx + dt * ax;
+            nvy = vy + dt * ay;
+        }
+    }
+    P.vx[i] = nvx;
+    P.vy[i] = nvy;
+    lvx[i] = vx;
+    lvy[i] = vy;
+    const double r = P.rho[i], dr = P.drho[i];
+    double nr;
+    if (step == 1) nr = r + dt * 2.0 * dr;
+    else if (step % 50 != 0) nr = lrho[i] + dt * 2.0 * dr;
+    else nr = r + dt * dr;
+    P.rho[i] = nr;
+    lrho[i] = r;
+}
+
 // volume_fraction_soil (m_particles.f90:3970-3988, applied by single_step
 // :1411 to the pre-update density), then first_step / second_half
 // (stress_integration = 1) and the closing Drucker-Prager projection (:583/:601).

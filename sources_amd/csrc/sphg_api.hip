@@ -43,6 +43,8 @@ struct PhaseBuf {
     bool cells_valid = false;
     bool counts_zero = false;   // d.cell_count known to be all zero
     int* cell_part = nullptr;   // tile-local exclusive prefix of the cell counts
+    // the other velocity and density levels of the Verlet scheme (water)
+    double *lvx = nullptr, *lvy = nullptr, *lrho = nullptr;
     // materialised pairs
     int *fwd = nullptr, *off = nullptr;
     int *pi = nullptr, *pj = nullptr;
@@ -326,6 +328,7 @@ int check_params(const sphg_params& P)
     if (P.nnps != 1 && P.nnps != 2) return SPHG_ERR_UNSUPPORTED;
     if (!(P.x_maxgeom > P.x_mingeom && P.y_maxgeom > P.y_mingeom)) return SPHG_ERR_INVALID;
     if (P.precision != SPHG_FP64 && P.precision != SPHG_MIXED) return SPHG_ERR_INVALID;
+    if (P.integrate_scheme < 0 || P.integrate_scheme > 2) return SPHG_ERR_INVALID;
     return SPHG_OK;
 }
 
@@ -858,6 +861,8 @@ int check_fused_switches(const sphg_ctx* c)
 {
     const sphg_params& P = c->P;
     if (P.summation_density || P.stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
+    // the Verlet scheme is the reference's single-phase water integrator
+    if (P.integrate_scheme == 2 && !P.single_phase) return SPHG_ERR_UNSUPPORTED;
     // the mixed-precision form exists for the fused two-phase step with the
     // direct_find predicate; its records hold 15-bit cell coordinates
     if (P.precision == SPHG_MIXED &&
@@ -908,6 +913,29 @@ int one_step(sphg_ctx* c)
     RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, step == 1 ? 1 : 0));
     W.cells_valid = false;
     S.cells_valid = false;
+    return SPHG_OK;
+}
+
+// time_integration_for_water_by_verlet (declarations_sph.f90:889-974): the
+// routine's lastvx / lastrho start as copies of vx and rho at every call.
+int verlet_steps(sphg_ctx* c, int nsteps)
+{
+    PhaseBuf& W = c->ph[0];
+    if (!W.lvx) {
+        RC(dalloc(c, W, &W.lvx, (size_t)W.n));
+        RC(dalloc(c, W, &W.lvy, (size_t)W.n));
+        RC(dalloc(c, W, &W.lrho, (size_t)W.n));
+    }
+    RC(launch(c, "verlet", k_copy<double>, nblk(W.n), TPB, (const double*)W.d.vx, W.lvx, W.n));
+    RC(launch(c, "verlet", k_copy<double>, nblk(W.n), TPB, (const double*)W.d.vy, W.lvy, W.n));
+    RC(launch(c, "verlet", k_copy<double>, nblk(W.n), TPB, (const double*)W.d.rho, W.lrho, W.n));
+    const double dt = c->P.dt, h2 = std::pow(dt, 2.0) / 2.0;   // dt**2./2.
+    for (int it = 0; it < nsteps; ++it) {
+        const int step = ++c->itimestep;
+        RC(rhs_water(c));
+        RC(launch(c, "verlet", k_verlet_water, nblk(W.n), TPB, W.d, W.lvx, W.lvy, W.lrho, dt, h2, step));
+        W.cells_valid = false;
+    }
     return SPHG_OK;
 }
 
@@ -1246,7 +1274,9 @@ int sphg_time_integration(sphg_ctx* c, int nsteps, int sync)
 {
     if (!c || nsteps < 0) return SPHG_ERR_INVALID;
     RC(check_fused_switches(c));
-    for (int it = 0; it < nsteps; ++it) RC(one_step(c));
+    if (c->P.single_phase && c->P.integrate_scheme == 2) RC(verlet_steps(c, nsteps));
+    else
+        for (int it = 0; it < nsteps; ++it) RC(one_step(c));
     if (sync) {
         RC(download(c, 0, SPHG_F_ALL));
         if (!c->P.single_phase) RC(download(c, 1, SPHG_F_ALL));

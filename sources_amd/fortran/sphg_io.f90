@@ -189,6 +189,7 @@ subroutine apply_cas(p)
    type(sphg_params_t), intent(inout) :: p
    if (set_nnps) p%nnps = cas_nnps
    if (set_single) p%single_phase = merge(1, 0, single_phase_cas)
+   p%integrate_scheme = integrate_scheme
    if (set_water) then
       p%water_rho0 = cas_water(1); p%water_b = cas_water(2); p%water_gamma = cas_water(3)
       p%water_c = cas_water(4); p%water_viscosity = cas_water(5)

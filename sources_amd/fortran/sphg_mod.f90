@@ -31,7 +31,8 @@ type, bind(C) :: sphg_params_t
    integer(c_int32_t) :: maxngx, maxngy
    real(c_double) :: dt
    integer(c_int32_t) :: precision
-   integer(c_int32_t) :: reserved(7)
+   integer(c_int32_t) :: integrate_scheme
+   integer(c_int32_t) :: reserved(6)
 end type
 
 ! include/sphg.h: struct sphg_phase_view

@@ -206,6 +206,37 @@ def test_single_phase_dam_break(skf):
     s.close()
 
 
+def test_verlet_dam_break():
+    """integrate_scheme = 2 (time_integration_for_water_by_verlet): one call of
+    one step, then one call of 54 steps, across the 50th step's restart."""
+    def make():
+        sc = scenarios.dam_break(dx=0.01, skf=1, jitter=0.05, seed=5)
+        sc.params.integrate_scheme = 2
+        return sc
+    sched = ((1, 0), (55, 1))
+    sc, ref = _both(make)
+    fields = ("x", "vx", "rho", "p", "drho", "dvx", "av")
+    s = Solver(sc.params, sc.water)
+    for nsteps, done in sched:
+        print(f"verlet steps {done}->{nsteps}", flush=True)
+        s.time_integration(nsteps - done, sync=True)
+        oracle.time_integration(ref.params, ref.water, None, done, nsteps - done)
+    assert s.itimestep == 55
+    # conditioning: the same schedule from a one-ulp density perturbation
+    a, b = make(), make()
+    u = np.random.default_rng(123).choice([-1.0, 1.0], size=b.water.rho.shape)
+    b.water.rho[:] = b.water.rho * (1.0 + u * 2.0 ** -52)
+    for nsteps, done in sched:
+        oracle.time_integration(a.params, a.water, None, done, nsteps - done)
+        oracle.time_integration(b.params, b.water, None, done, nsteps - done)
+    sens = field_errors(b.water, a.water, fields)
+    errs = field_errors(sc.water, ref.water, fields)
+    print("errs", errs, "sens", sens)
+    bad = {f: (errs[f], sens[f]) for f in fields if not (errs[f] <= max(1e-12, 16.0 * sens[f]))}
+    assert not bad, bad
+    s.close()
+
+
 def test_nnps2_fused_step_matches_link_list_oracle():
     sc, ref = _both(lambda: _tight_box_column(nnps=2))
     s = Solver(sc.params, sc.water, sc.soil)
