@@ -60,6 +60,8 @@ def lib():
         L.oracle_time_integration_for_water_by_verlet.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
         L.oracle_verlet_update.argtypes = [P, V, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
         L.oracle_verlet_update.restype = None
+        L.oracle_single_step_for_soil.argtypes = [P, V, C.c_int]
+        L.oracle_time_integration_for_soil.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
         L.oracle_init_grid.argtypes = [P, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int), D, D]
         L.oracle_init_grid.restype = None
         _lib = L
@@ -115,7 +117,11 @@ def df(ph, f, d, which="df"):
 
 
 def single_step(P, wa, so, itimestep):
-    if P.single_phase:
+    if P.single_phase and wa is None:
+        v = so.view()
+        rc = lib().oracle_single_step_for_soil(C.byref(P), C.byref(v), itimestep)
+        so.pull_scalars(v)
+    elif P.single_phase:
         v = wa.view()
         rc = lib().oracle_single_step_for_water(C.byref(P), C.byref(v), itimestep)
         wa.pull_scalars(v)
@@ -130,7 +136,11 @@ def single_step(P, wa, so, itimestep):
 def time_integration(P, wa, so, itimestep, nsteps):
     """Advance nsteps from step counter itimestep; returns the new counter."""
     it = C.c_int(itimestep)
-    if P.single_phase:
+    if P.single_phase and wa is None:
+        v = so.view()
+        rc = lib().oracle_time_integration_for_soil(C.byref(P), C.byref(v), C.byref(it), nsteps)
+        so.pull_scalars(v)
+    elif P.single_phase:
         v = wa.view()
         f = (lib().oracle_time_integration_for_water_by_verlet if P.integrate_scheme == 2
              else lib().oracle_time_integration_for_water)

@@ -924,6 +924,62 @@ int oracle_single_step_for_water(const sphg_params *P, sphg_phase_view *pl, int 
     return SPHG_OK;
 }
 
+/* single_step_for_soil, declarations_sph.f90:1530-1655 (serial path,
+ * summation_density = .false., plasticity 0 or 3).  The same routines as the
+ * soil phase of single_step, except the momentum: -df(p) + df(sxx) + df(sxy)
+ * (the df form, no vof factors, :1596-1600) and no repulsive force.
+ * Pinned: the df-form internal force with art_visc is pairwise
+ * antisymmetric, so sum_i m_i (dvx_i - g) = 0 over all particles (to
+ * rounding), and a uniform-stress lattice interior has no internal force
+ * (tests/test_oracle_soil_only.py); the routines it composes are pinned in
+ * tests/test_oracle_physics.py. */
+int oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int itimestep)
+{
+    int n = pl->ntotal + pl->nvirt, i, rc;
+    double *a, *b, *c3;
+    (void)itimestep;
+    if (P->summation_density || (P->plasticity != 0 && P->plasticity != 3)) return SPHG_ERR_UNSUPPORTED;
+    for (i = 0; i < 2 * n; i++) pl->dvx[i] = 0.0;
+    for (i = 0; i < n; i++) {
+        pl->drho[i] = 0.0; pl->dsxx[i] = 0.0; pl->dsxy[i] = 0.0; pl->dsyy[i] = 0.0; pl->dp[i] = 0.0;
+    }
+    rc = oracle_find_pairs(P, pl);
+    if (rc) return rc;
+    oracle_con_density(P, pl);
+    if (P->artificial_density) oracle_delta_sph(P, pl, pl->rho, pl->drho);
+    oracle_shear_strain_rate(P, pl);
+    oracle_velocity_divergence(P, pl);
+    oracle_pressure(P, pl);
+    if (P->yield_criterion == 1) oracle_mohr_coulomb(P, pl);
+    else if (P->yield_criterion == 2) oracle_drucker_prager(P, pl);
+    a = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    b = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    c3 = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    /* dvx = -df(p,'x') + df(sxx,'x') + df(sxy,'y'), :1596-1597 */
+    oracle_df(pl, pl->p, 0, a); oracle_df(pl, pl->sxx, 0, b); oracle_df(pl, pl->sxy, 1, c3);
+    for (i = 0; i < n; i++) pl->dvx[2 * i] = -a[i] + b[i] + c3[i];
+    oracle_df(pl, pl->p, 1, a); oracle_df(pl, pl->sxy, 0, b); oracle_df(pl, pl->syy, 1, c3);
+    for (i = 0; i < n; i++) pl->dvx[2 * i + 1] = -a[i] + b[i] + c3[i];
+    for (i = 0; i < n; i++) if (pl->rho[i] > 0.0) pl->dvx[2 * i] = pl->dvx[2 * i] / pl->rho[i];
+    for (i = 0; i < n; i++) if (pl->rho[i] > 0.0) pl->dvx[2 * i + 1] = pl->dvx[2 * i + 1] / pl->rho[i];
+    free(a); free(b); free(c3);
+    if (P->plasticity == 3) {
+        oracle_plastic_or_not(P, pl);
+        oracle_plastic_flow_rule3(P, pl);
+    }
+    oracle_jaumann_rate(P, pl);
+    if (P->visc_artificial) oracle_art_visc(P, pl);
+    if (P->soil_artificial_stress) {
+        oracle_delta_sph(P, pl, pl->p, pl->dp);
+        oracle_delta_sph(P, pl, pl->sxx, pl->dsxx);
+        oracle_delta_sph(P, pl, pl->sxy, pl->dsxy);
+        oracle_delta_sph(P, pl, pl->syy, pl->dsyy);
+    }
+    for (i = 0; i < n; i++) pl->dvx[2 * i + 1] = pl->dvx[2 * i + 1] + P->gravity;
+    if (P->average_velocity) oracle_av_vel(P, pl);
+    return SPHG_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* time_integration, declarations_sph.f90:525-791 (stress_integration = 1,   */
 /* plasticity = 3; summation_density = .false.)                              */
@@ -1021,6 +1077,31 @@ int oracle_time_integration(const sphg_params *P, sphg_phase_view *wa, sphg_phas
             second_half(P, wa);
             second_half(P, so);
         }
+        oracle_drucker_prager(P, so);
+    }
+    return SPHG_OK;
+}
+
+/* time_integration_for_soil, declarations_sph.f90:978-1127
+ * (stress_integration = 1): first_half and a Drucker-Prager projection
+ * (steps > 1), single_step_for_soil, first_step / second_half and the closing
+ * projection -- the soil updates of time_integration without the water
+ * phase.  Pinned by the free-fall recurrence of isolated particles
+ * (tests/test_oracle_soil_only.py). */
+int oracle_time_integration_for_soil(const sphg_params *P, sphg_phase_view *so, int *itimestep, int nsteps)
+{
+    int it, rc;
+    if (P->summation_density || P->stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
+    for (it = 0; it < nsteps; it++) {
+        int step = ++(*itimestep);
+        if (step != 1) {
+            first_half(P, so);
+            oracle_drucker_prager(P, so);
+        }
+        rc = oracle_single_step_for_soil(P, so, step);
+        if (rc) return rc;
+        if (step == 1) first_step(P, so);
+        else second_half(P, so);
         oracle_drucker_prager(P, so);
     }
     return SPHG_OK;

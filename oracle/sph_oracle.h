@@ -49,6 +49,8 @@ int  oracle_single_step_for_water(const sphg_params *P, sphg_phase_view *pl, int
 int  oracle_time_integration(const sphg_params *P, sphg_phase_view *wa, sphg_phase_view *so,
                              int *itimestep, int nsteps);
 int  oracle_time_integration_for_water(const sphg_params *P, sphg_phase_view *wa, int *itimestep, int nsteps);
+int  oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int itimestep);
+int  oracle_time_integration_for_soil(const sphg_params *P, sphg_phase_view *so, int *itimestep, int nsteps);
 void oracle_verlet_update(const sphg_params *P, sphg_phase_view *wa, double *lastvx, double *lastrho, int step);
 int  oracle_time_integration_for_water_by_verlet(const sphg_params *P, sphg_phase_view *wa, int *itimestep,
                                                  int nsteps);

@@ -920,10 +920,12 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     const double mu = K.w_mu;
     txx *= SPH_C23_F;
     tyy *= SPH_C23_F;
-    W.drho[a] = con + (K.delta * ha * ca) * drd;
+    // delta_sph(rho) and av_vel only under their switches (artificial_density,
+    // average_velocity); without av_vel the reference leaves av as it was
+    W.drho[a] = K.artificial_density ? con + (K.delta * ha * ca) * drd : con;
     W.txx[a] = txx; W.txy[a] = txy; W.tyy[a] = tyy;
     if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
-    W.avx[a] = avx; W.avy[a] = avy;
+    if (K.average_velocity) { W.avx[a] = avx; W.avy[a] = avy; }
     W.repx[a] = repx; W.repy[a] = repy;
     if (cross) {
         W.drgx[a] = dg_x; W.drgy[a] = dg_y; W.dvof_x[a] = dvof_x * (1.0 / K.s_rho0);
@@ -1034,7 +1036,9 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
 }
 
 // ------------------------------------------------------------------------
-// Soil, stages A and B in one kernel.  Soil stage B reads only what the pack
+// Soil, stages A and B in one kernel (SOLO: single-phase soil,
+// single_step_for_soil, declarations_sph.f90:1530-1655: the df form of the
+// momentum without vof factors, no repulsion, no water partners).  Soil stage B reads only what the pack
 // wrote (p, sigma, vof of the partners) and the new sound speed, which is the
 // constant s_c for every soil particle, so both stages share one scan and one
 // dense loop over the same partner set (no stored masks, one partner load):
@@ -1047,7 +1051,7 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
 //   tail        plastic_or_not + plastic_flow_rule3 + Jaumann_rate, then
 //               /rho, + repulsion, + gravity, + Darcy drag, + pore pressure.
 // ------------------------------------------------------------------------
-template <int SKF, int LA, int LX, bool U>
+template <int SKF, int LA, int LX, bool U, bool SOLO>
 __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS, const double* recW, Grid g,
                                       KParams K, FusedK F, int cross, int mask)
 {
@@ -1078,7 +1082,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
 
     double dg_x = 0.0, dg_y = 0.0, po_x = 0.0, po_y = 0.0;
     int cntx = 0;
-    if (cross) {
+    if (!SOLO && cross) {
         const Win wx = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
         if (wx.ok) win_load<LX>(wx, RW, lds, wc.lane);
         const LaneRows LXr = lane_rows(block_rows(g, W.cell_start, xa, ya), wx, cy);
@@ -1114,7 +1118,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
         po_x *= vofa;
         po_y *= vofa;
     }
-    if (cross && LX != 0) __syncthreads();
+    if (!SOLO && cross && LX != 0) __syncthreads();
     const Win w = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
     if (w.ok) win_load<LA>(w, RS, lds, wc.lane);
     const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya), w, cy);
@@ -1158,7 +1162,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
         const double f = mb * ww * (2.0 * fast_rcp(rhoa + b.rm.x));
         avx -= dvx * f;
         avy -= dvy * f;
-        if (ita > 0 && !(r > S.dspp) && (int)b.s3.y < 0) {
+        if (!SOLO && ita > 0 && !(r > S.dspp) && (int)b.s3.y < 0) {
             const double q = S.dspp / r;
             const double fr = (U ? ipow(q, K.ip1) - ipow(q, K.ip2) : pow(q, K.p1) - pow(q, K.p2)) * idr;
             repx += K.dd * dx * fr;
@@ -1170,8 +1174,16 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
         const double fxx = fxxa + vofb * b.s12.x * ir2b;
         const double fxy = fxya + vofb * b.s12.y * ir2b;
         const double fyy = fyya + vofb * b.s3.x * ir2b;
-        Fx += mb * ((fxx - fp) * gx + fxy * gy);
-        Fy += mb * ((fyy - fp) * gy + fxy * gx);
+        if constexpr (SOLO) {
+            // -df(p) + df(sigma) (single_step_for_soil :1596-1597):
+            // m_b/rho_b (f_a + f_b) grad W, no vof factors
+            const double ps = pa + b.pc.x, sxy = sxya + b.s12.y;
+            Fx += Vb * ((sxxa + b.s12.x - ps) * gx + sxy * gy);
+            Fy += Vb * ((syya + b.s3.x - ps) * gy + sxy * gx);
+        } else {
+            Fx += mb * ((fxx - fp) * gx + fxy * gy);
+            Fy += mb * ((fyy - fp) * gy + fxy * gx);
+        }
         const double vr = dvx * dx + dvy * dy;
         {   // art_visc, branch-free: approaching pairs only (vr < 0)
             const double muv = amh * vr * fast_rcp(dr + amh * amh * et2);
@@ -1191,10 +1203,12 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     txx *= SPH_C23_F;
     tyy *= SPH_C23_F;
     wxy *= 0.5;
-    S.drho[a] = con + (K.delta * ha * ca) * drd;
+    // delta_sph(rho) and av_vel only under their switches (artificial_density,
+    // average_velocity); without av_vel the reference leaves av as it was
+    S.drho[a] = K.artificial_density ? con + (K.delta * ha * ca) * drd : con;
     S.txx[a] = txx; S.txy[a] = txy; S.tyy[a] = tyy; S.vcc[a] = vcc; S.wxy[a] = wxy;
     if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
-    S.avx[a] = avx; S.avy[a] = avy;
+    if (K.average_velocity) { S.avx[a] = avx; S.avy[a] = avy; }
     S.repx[a] = repx; S.repy[a] = repy;
     if (cross) {
         S.drgx[a] = dg_x; S.drgy[a] = dg_y; S.porx[a] = po_x; S.pory[a] = po_y;
@@ -1234,8 +1248,8 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     dsyy = dsyy + G * tyy - 2.0 * sxya * wxy;
     dpv = dpv - Kb * vcc;
 
-    double ax = Fx * rhoa;
-    double ay = Fy * rhoa;
+    double ax = SOLO ? Fx : Fx * rhoa;
+    double ay = SOLO ? Fy : Fy * rhoa;
     if (rhoa > 0.0) { ax = ax * irhoa; ay = ay * irhoa; }
     if (mask & M_ART) { ax = ax + artx; ay = ay + arty; }
     ax = ax + repx;

@@ -299,11 +299,11 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
     const double vofa_d = W.vof[a], ha_d = W.hsml[a], ca_d = W.c[a];
     const double mu = K.w_mu;
     const double txx_d = txx.value() * SPH_C23_F, tyy_d = tyy.value() * SPH_C23_F, txy_d = txy.value();
-    W.drho[a] = con.value() + (K.delta * ha_d * ca_d) * drd.value();
+    W.drho[a] = K.artificial_density ? con.value() + (K.delta * ha_d * ca_d) * drd.value() : con.value();
     W.txx[a] = txx_d; W.txy[a] = txy_d; W.tyy[a] = tyy_d;
     double avx_d = avx.value(), avy_d = avy.value();
     if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
-    W.avx[a] = avx_d; W.avy[a] = avy_d;
+    if (K.average_velocity) { W.avx[a] = avx_d; W.avy[a] = avy_d; }
     W.repx[a] = repx; W.repy[a] = repy;
     if (cross) {
         W.drgx[a] = dg_x.value(); W.drgy[a] = dg_y.value(); W.dvof_x[a] = dvof_x.value() * (1.0 / K.s_rho0);
@@ -548,11 +548,11 @@ __global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* q
     const double ha_d = S.hsml[a], ca_d = S.c[a], rhoa_d = S.rho[a];
     const double txx_d = txx.value() * SPH_C23_F, tyy_d = tyy.value() * SPH_C23_F, txy_d = txy.value();
     const double vcc_d = vcc.value(), wxy_d = wxy.value() * 0.5;
-    S.drho[a] = con.value() + (K.delta * ha_d * ca_d) * drd.value();
+    S.drho[a] = K.artificial_density ? con.value() + (K.delta * ha_d * ca_d) * drd.value() : con.value();
     S.txx[a] = txx_d; S.txy[a] = txy_d; S.tyy[a] = tyy_d; S.vcc[a] = vcc_d; S.wxy[a] = wxy_d;
     double avx_d = avx.value(), avy_d = avy.value();
     if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
-    S.avx[a] = avx_d; S.avy[a] = avy_d;
+    if (K.average_velocity) { S.avx[a] = avx_d; S.avy[a] = avy_d; }
     S.repx[a] = repx; S.repy[a] = repy;
     const double dgx_d = dg_x.value(), dgy_d = dg_y.value();
     const double pox_d = po_x.value() * vofa_d, poy_d = po_y.value() * vofa_d;

@@ -268,8 +268,9 @@ int reorder(sphg_ctx* c, int k)
         return SPHG_OK;
     };
     RC(pi(d.id)); RC(pi(d.itype)); RC(pi(d.fail));
+    // av persists across steps when average_velocity is off
     double** f[] = {&d.x, &d.y, &d.vx, &d.vy, &d.mass, &d.hsml, &d.rho, &d.vof, &d.p, &d.c,
-                    &d.drho, &d.ax, &d.ay, &d.dvof, &d.sxx, &d.sxy, &d.syy, &d.eps_p};
+                    &d.drho, &d.ax, &d.ay, &d.dvof, &d.sxx, &d.sxy, &d.syy, &d.eps_p, &d.avx, &d.avy};
     for (double** a : f) RC(pd(*a));
     RC(launch(c, "iota", k_iota, nblk(b.n), TPB, d.order, b.n));
     return SPHG_OK;
@@ -765,10 +766,10 @@ int stages_fused(sphg_ctx* c)
     {
         SideScope side(c);
         if ((c->K.nnps == 2 || c->F.uni_s) && c->F.uni_x && rep_int)
-            RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d,
+            RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d,
                       W.d, (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
         else
-            RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
+            RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, false, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
                       S.d, W.d, (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross,
                       mask_b(c, 1)));
     }
@@ -785,6 +786,39 @@ int stages_fused(sphg_ctx* c)
     if (side_on(c)) RC(join_side(c));
     return SPHG_OK;
 }
+
+// Single-phase soil (single_step_for_soil, declarations_sph.f90:1530-1655):
+// the one-pass soil kernel in its df form, without water partners.
+template <int SKF>
+int stages_soil_only(sphg_ctx* c)
+{
+    PhaseBuf& S = c->ph[1];
+    const double* rs = S.rec;
+    if (c->K.nnps == 2 || c->F.uni_s)
+        RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
+                  S.d, S.d, rs, rs, c->gl[1], c->K, c->F, 0, mask_b(c, 1)));
+    else
+        RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, false, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
+                  S.d, S.d, rs, rs, c->gl[1], c->K, c->F, 0, mask_b(c, 1)));
+    swap_c(S);
+    return SPHG_OK;
+}
+
+int rhs_soil_only(sphg_ctx* c, int soil_half, int soil_ndp)
+{
+    PhaseBuf& S = c->ph[1];
+    RC(build_cells(c, 1));
+    RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, (float*)nullptr,
+              c->gl[1], c->cs));
+    switch (c->P.skf) {
+    case 1: return stages_soil_only<1>(c);
+    case 2: return stages_soil_only<2>(c);
+    case 3: return stages_soil_only<3>(c);
+    default: return stages_soil_only<4>(c);
+    }
+}
+
+bool soil_only(const sphg_ctx* c) { return c->P.single_phase && !c->ph[0].bound && c->ph[1].bound; }
 
 // Mixed-precision stages: FP32 records and pair sums, FP64 per-particle tails.
 template <int SKF>
@@ -863,6 +897,12 @@ int check_fused_switches(const sphg_ctx* c)
     if (P.summation_density || P.stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
     // the Verlet scheme is the reference's single-phase water integrator
     if (P.integrate_scheme == 2 && !P.single_phase) return SPHG_ERR_UNSUPPORTED;
+    if (soil_only(c)) {
+        // time_integration_for_soil: leapfrog only; Drucker-Prager with flow rule 3
+        if (P.integrate_scheme == 2 || P.precision != SPHG_FP64) return SPHG_ERR_UNSUPPORTED;
+        if (P.yield_criterion != 2 || (P.plasticity != 3 && P.plasticity != 0)) return SPHG_ERR_UNSUPPORTED;
+        return SPHG_OK;
+    }
     // the mixed-precision form exists for the fused two-phase step with the
     // direct_find predicate; its records hold 15-bit cell coordinates
     if (P.precision == SPHG_MIXED &&
@@ -882,6 +922,22 @@ int one_step(sphg_ctx* c)
     const int step = ++c->itimestep;
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
+    if (soil_only(c)) {
+        // time_integration_for_soil (declarations_sph.f90:978-1127): first_half
+        // and a Drucker-Prager projection, single_step_for_soil (its failure
+        // criterion is the second projection of the pack), first_step /
+        // second_half and the closing projection
+        if (++c->since_reorder >= c->reorder_every) {
+            RC(build_cells(c, 1));
+            RC(reorder(c, 1));
+            c->since_reorder = 0;
+        }
+        if (step != 1) RC(rhs_soil_only(c, 1, 2));
+        else RC(rhs_soil_only(c, 0, 1));
+        RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, step == 1 ? 1 : 0));
+        S.cells_valid = false;
+        return SPHG_OK;
+    }
     if (c->P.single_phase) {
         if (step != 1) RC(launch(c, "first_half", k_first_half_water, nblk(W.n), TPB, W.d, c->K));
         RC(rhs_water(c));
@@ -1251,7 +1307,9 @@ int sphg_single_step(sphg_ctx* c, int sync)
 {
     if (!c) return SPHG_ERR_INVALID;
     RC(check_fused_switches(c));
-    if (c->P.single_phase) {
+    if (soil_only(c)) {
+        RC(rhs_soil_only(c, 0, 1));
+    } else if (c->P.single_phase) {
         RC(rhs_water(c));
     } else {
         PhaseBuf& S = c->ph[1];
@@ -1264,8 +1322,8 @@ int sphg_single_step(sphg_ctx* c, int sync)
     }
     if (sync) {
         const uint64_t m = SPHG_F_ALL & ~SPHG_F_X & ~SPHG_F_VX & ~SPHG_F_MIN;
-        RC(download(c, 0, m));
-        if (!c->P.single_phase) RC(download(c, 1, m));
+        for (int k = 0; k < 2; ++k)
+            if (c->ph[k].bound) RC(download(c, k, m));
     }
     return check_flags(c);
 }
@@ -1278,8 +1336,8 @@ int sphg_time_integration(sphg_ctx* c, int nsteps, int sync)
     else
         for (int it = 0; it < nsteps; ++it) RC(one_step(c));
     if (sync) {
-        RC(download(c, 0, SPHG_F_ALL));
-        if (!c->P.single_phase) RC(download(c, 1, SPHG_F_ALL));
+        for (int k = 0; k < 2; ++k)
+            if (c->ph[k].bound) RC(download(c, k, SPHG_F_ALL));
         return check_flags(c);
     }
     HIPCHK(c, hipGetLastError());

@@ -85,7 +85,8 @@ def _soil_fields(ph, P, p0, vof):
 
 
 class Scenario:
-    """A ready-to-run state: params, water phase and (two-phase) soil phase."""
+    """A ready-to-run state: params, water phase and (two-phase) soil phase;
+    a single-phase soil run has no water phase."""
 
     def __init__(self, name, params, water, soil=None, nsteps=1, notes=""):
         self.name = name
@@ -97,11 +98,11 @@ class Scenario:
 
     @property
     def n_particles(self):
-        return self.water.n + (self.soil.n if self.soil is not None else 0)
+        return (self.water.n if self.water is not None else 0) + (self.soil.n if self.soil is not None else 0)
 
     def copy(self):
         from .state import params_copy
-        return Scenario(self.name, params_copy(self.params), self.water.copy(),
+        return Scenario(self.name, params_copy(self.params), self.water.copy() if self.water is not None else None,
                         self.soil.copy() if self.soil is not None else None,
                         self.nsteps, self.notes)
 
@@ -190,6 +191,39 @@ def dam_break(dx=0.005, width=0.25, height=0.5, tank_w=1.0, tank_h=0.6, layers=3
     _water_fields(wa, P, p, 1.0)
     return Scenario("dam_break", P, wa, None, nsteps=2000,
                     notes=f"{len(xr)} water + {len(xv)} virtual, dx={dx}")
+
+
+def soil_collapse(dx=0.005, width=0.1, height=0.05, tank_w=0.3, tank_h=0.12, layers=3, skf=1,
+                  max_interaction=None, jitter=0.0, seed=0, **pover):
+    """Single-phase soil (the reference's time_integration_for_soil /
+    single_step_for_soil, declarations_sph.f90:978-1127,1530-1655): a dry
+    granular column width x height at the left wall of a tank_w x tank_h
+    tank, `layers` rows of fixed virtual particles on the bottom and both
+    sides, vof = 1, K0 stresses of the dry column with the
+    input_mix_Savage.f90 factors."""
+    P = default_params(single_phase=1, skf=skf, x_mingeom=-0.05, x_maxgeom=tank_w + 0.05,
+                       y_mingeom=-0.05, y_maxgeom=2.0 * tank_h)
+    for k, v in pover.items():
+        setattr(P, k, v)
+    m = int(round((tank_w + 2 * layers * dx) / dx))
+    n = int(round((tank_h + layers * dx) / dx))
+    x, y, ddx, ddy = block_centers(m * dx, n * dx, m, n)
+    x = x - layers * dx
+    y = y - layers * dx
+    real = (x > 0.0) & (x < width) & (y > 0.0) & (y < height)
+    virt = (x < 0.0) | (x > tank_w) | (y < 0.0)
+    xr, yr = x[real], y[real]
+    if jitter > 0.0:
+        u = uniform(seed, 2 * len(xr))
+        xr = xr + (2.0 * u[0::2] - 1.0) * jitter * dx
+        yr = yr + (2.0 * u[1::2] - 1.0) * jitter * dx
+    xv, yv = x[virt], y[virt]
+    mi = max_interaction if max_interaction is not None else 16 * (len(xr) + len(xv))
+    so = _make_phase(abi.SPHG_SOIL, xr, yr, xv, yv, ddx, ddy, f32(1.2), mi)
+    p0 = np.maximum(P.soil_rho0 * P.gravity * (so.x[:, 1] - height), 0.0)
+    _soil_fields(so, P, p0, 1.0)
+    return Scenario("soil_collapse", P, None, so, nsteps=1000,
+                    notes=f"{len(xr)} soil + {len(xv)} virtual, dx={dx}")
 
 
 def submerged_column(dx=0.0005, soil_w=0.2, soil_h=0.1, tank_w=0.8, tank_h=0.3, layers=3,

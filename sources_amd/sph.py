@@ -19,7 +19,8 @@ def _phase_id(phase):
 
 class Solver:
     """One device context holding the water phase (`parts`) and, for
-    two-phase runs, the soil phase (`soil`)."""
+    two-phase runs, the soil phase (`soil`); a single-phase soil run binds
+    the soil phase alone (water=None)."""
 
     def __init__(self, params, water, soil=None, lib=None):
         self.lib = lib or abi.load_library()
@@ -29,7 +30,8 @@ class Solver:
         self._ctx = C.c_void_p()
         self._check(self.lib.sphg_create(C.byref(params), C.byref(self._ctx)), "sphg_create")
         self._views = {}
-        self.bind(abi.SPHG_WATER, water)
+        if water is not None:
+            self.bind(abi.SPHG_WATER, water)
         if soil is not None:
             self.bind(abi.SPHG_SOIL, soil)
 
@@ -64,13 +66,15 @@ class Solver:
         self._pull(pid)
 
     def sync_all(self, mask=abi.F_ALL):
-        self.sync("water", mask)
+        if self.water is not None:
+            self.sync("water", mask)
         if self.soil is not None:
             self.sync("soil", mask)
 
     def upload_all(self, mask=abi.F_ALL):
         """Host arrays -> device for every bound phase."""
-        self.sync("water", mask, to_host=False)
+        if self.water is not None:
+            self.sync("water", mask, to_host=False)
         if self.soil is not None:
             self.sync("soil", mask, to_host=False)
 
@@ -151,18 +155,21 @@ class Solver:
         """single_step / single_step_for_water (declarations_sph.f90:1131, :1439)."""
         self._check(self.lib.sphg_single_step(self._ctx, 1 if sync else 0), "single_step")
         if sync:
-            self._pull("water")
-            if self.soil is not None:
-                self._pull("soil")
+            self._pull_bound()
 
     def time_integration(self, nsteps, sync=True):
-        """time_integration / time_integration_for_water (declarations_sph.f90:525, :794)."""
+        """time_integration / time_integration_for_water / _for_soil /
+        _for_water_by_verlet (declarations_sph.f90:525, :794, :978, :889)."""
         self._check(self.lib.sphg_time_integration(self._ctx, int(nsteps), 1 if sync else 0),
                     "time_integration")
         if sync:
+            self._pull_bound()
+
+    def _pull_bound(self):
+        if self.water is not None:
             self._pull("water")
-            if self.soil is not None:
-                self._pull("soil")
+        if self.soil is not None:
+            self._pull("soil")
 
     # -- measurement ------------------------------------------------------
     def profile(self, enable=True):

@@ -237,6 +237,58 @@ def test_verlet_dam_break():
     s.close()
 
 
+@pytest.mark.parametrize("skf,nnps", [(1, 1), (4, 1), (1, 2)])
+def test_soil_only_collapse(skf, nnps):
+    """Single-phase soil: time_integration_for_soil / single_step_for_soil."""
+    make = lambda: scenarios.soil_collapse(dx=0.005, skf=skf, nnps=nnps, jitter=0.05, seed=9)  # noqa: E731
+    sc, ref = _both(make)
+    s = Solver(sc.params, None, sc.soil)
+    for nsteps, done, tol in ((1, 0, FP64_TOL_1STEP), (10, 1, 1e-10)):
+        print(f"soil only skf={skf} nnps={nnps} steps {done}->{nsteps}", flush=True)
+        s.time_integration(nsteps - done, sync=True)
+        oracle.time_integration(ref.params, None, ref.soil, done, nsteps - done)
+        es = assert_fields(sc.soil, ref.soil, tol, label=f"soil only {nsteps} steps")
+        print(nsteps, "soil", max(es.values()))
+        _assert_fail_flags(sc.soil, ref.soil, ref.params)
+    s.close()
+
+
+def _assert_fail_flags(g, r, P):
+    """The Drucker-Prager flags agree, except where the oracle's state sits on
+    the yield surface to rounding (sqrt(J2) = alpha1 I1, as a projection
+    leaves it), where 1e-15 differences decide the comparison either way."""
+    n = r.n
+    bad = np.nonzero(g.fail[:n] != r.fail[:n])[0]
+    if len(bad) == 0:
+        return
+    t = np.tan(P.soil_phi)
+    alpha1 = t / np.sqrt(9.0 + 12.0 * t * t)
+    sxx, sxy, syy, p = r.sxx[bad], r.sxy[bad], r.syy[bad], r.p[bad]
+    j2 = np.sqrt((sxx * sxx + 2.0 * sxy * sxy + syy * syy + (sxx + syy) ** 2) / 2.0) + 1.0e-6
+    lim = alpha1 * 3.0 * p
+    print("fail flags differ at", bad.tolist(), "sqrt(J2)", j2.tolist(), "alpha1 I1", lim.tolist())
+    assert np.all(np.abs(j2 - lim) <= 1e-9 * np.maximum(np.abs(j2), np.abs(lim))), (j2, lim)
+
+
+def test_two_phase_without_delta_and_xsph():
+    """artificial_density and average_velocity off: no delta-SPH on rho, and
+    av keeps its value (the reference does not touch it)."""
+    def make():
+        sc = _column()
+        sc.params.artificial_density = 0
+        sc.params.average_velocity = 0
+        for ph in (sc.water, sc.soil):
+            ph.av[:] = np.random.default_rng(1).normal(0.0, 1e-4, ph.av.shape)
+        return sc
+    sc, ref = _both(make)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(3, sync=True)
+    oracle.time_integration(ref.params, ref.water, ref.soil, 0, 3)
+    assert_fields(sc.water, ref.water, 1e-11, label="water")
+    assert_fields(sc.soil, ref.soil, 1e-11, label="soil")
+    s.close()
+
+
 def test_nnps2_fused_step_matches_link_list_oracle():
     sc, ref = _both(lambda: _tight_box_column(nnps=2))
     s = Solver(sc.params, sc.water, sc.soil)
