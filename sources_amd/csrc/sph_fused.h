@@ -361,11 +361,20 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
         }
     }
     double p = P.p[i], sxx = P.sxx[i], sxy = P.sxy[i], syy = P.syy[i];
+    // the last projection is single_step's failure criterion: Mohr-Coulomb
+    // when yield_criterion = 1, which may also reset rho (the record then
+    // keeps the density before it in chunk 7 for the terms evaluated before
+    // the criterion: con_density's delta_sph, strain rate, divergence)
+    const double rho_pre = rho;
     if (ndp > 0) {
         int f = 0;
-        for (int k = 0; k < ndp; ++k) f = dp_project(K.s_alpha1, p, sxx, sxy, syy);
+        for (int k = 0; k < ndp; ++k) {
+            if (k == ndp - 1 && K.yield_criterion == 1) f = mc_project(K, p, sxx, sxy, syy, rho);
+            else f = dp_project(K.s_alpha1, p, sxx, sxy, syy);
+        }
         P.p[i] = p; P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy;
         P.fail[i] = f;
+        if (rho != rho_pre) P.rho[i] = rho;
     }
     const double m = P.mass[i];
     const double irho = 1.0 / rho;
@@ -386,6 +395,7 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
     R[4 * pn] = make_double2(syy, P.c[i]);
     R[5 * pn] = make_double2(sxx, sxy);
     R[6 * pn] = make_double2(P.hsml[i], (double)it);
+    if (K.yield_criterion == 1) R[7 * pn] = make_double2(rho_pre, 0.0);
 }
 
 
@@ -687,6 +697,7 @@ __device__ __forceinline__ CatIter cat_iter(const CatMask& c, const LaneRows& L)
 // iv = (1/rho, m/rho), filled from the record chunks.
 struct Nb {
     double2 xy, v, rm, hv, pc, s12, s3, iv;
+    double rpre;   // density before the Mohr-Coulomb criterion (record chunk 7)
     __device__ __forceinline__ void set(int c, double2 q)
     {
         switch (c) {
@@ -1068,6 +1079,11 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     const double ca = a4.y;          // sound speed before this step (delta_sph of rho)
     const double cn = K.s_c;         // sound speed after it (stage B)
     const double irhoa = 1.0 / rhoa, ir2a = irhoa * irhoa;
+    // Mohr-Coulomb (general instantiation only): the criterion may have reset
+    // rho; con_density's delta_sph, the strain rate and the divergence come
+    // before it in single_step and use the density before it (chunk 7)
+    const bool mc = !U && K.yield_criterion == 1;
+    const double rhoa_pre = mc ? RA(7).x : rhoa;
     const double fpa = vofa * pa * ir2a, fxxa = vofa * sxxa * ir2a, fxya = vofa * sxya * ir2a,
                  fyya = vofa * syya * ir2a;
     const int ita = (int)hi.y;
@@ -1131,7 +1147,11 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], RS, acc_b);
     const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], RS, acc_b);
     const CatMask cm = cat_masks(L, m0, m1, m2);
-    auto load = [&](int i, int k) { return load_nb<LA, NEED_B>(lds, RS, w.ok, i, k, !uni); };
+    auto load = [&](int i, int k) {
+        Nb b = load_nb<LA, NEED_B>(lds, RS, w.ok, i, k, !uni);
+        if (mc) b.rpre = RS(k, 7).x;
+        return b;
+    };
     const double et2 = K.etq * K.etq;
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;
     double avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
@@ -1152,12 +1172,15 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
         // stage A
         const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
         const double dvg = dvx * gx + dvy * gy;
+        const double rb_pre = mc ? b.rpre : b.rm.x;
+        const double irb_pre = mc ? fast_rcp(rb_pre) : irb;
+        const double Vb_pre = mc ? mb * irb_pre : Vb;
         con += mb * dvg;
-        drd += mb * (2.0 * (rhoa - b.rm.x) * idr * (dx * gx + dy * gy)) * irb;
-        txx += Vb * (-2.0 * dvx * gx + dvy * gy);
-        txy += Vb * (-(dvx * gy) - dvy * gx);
-        tyy += Vb * (-2.0 * dvy * gy + dvx * gx);
-        vcc -= Vb * dvg;
+        drd += mb * (2.0 * (rhoa_pre - rb_pre) * idr * (dx * gx + dy * gy)) * irb_pre;
+        txx += Vb_pre * (-2.0 * dvx * gx + dvy * gy);
+        txy += Vb_pre * (-(dvx * gy) - dvy * gx);
+        tyy += Vb_pre * (-2.0 * dvy * gy + dvx * gx);
+        vcc -= Vb_pre * dvg;
         wxy += Vb * (-(dvx * gy) + dvy * gx);
         const double f = mb * ww * (2.0 * fast_rcp(rhoa + b.rm.x));
         avx -= dvx * f;

@@ -43,6 +43,9 @@ struct KParams {
     int soil_artificial_stress, water_artificial_volume, volume_fraction;
     int plasticity, yield_criterion, two_phase;
     int ip1, ip2;               // p1, p2 when they are integers in [0, 64], else -1
+    // Mohr-Coulomb criterion (yield_criterion = 1): cohesion cos(phi), sin(phi)
+    // and the tension cut-off pressure -cohesion tan(phi)**(-1.0), host-evaluated
+    double mc_ccos, mc_sin, mc_pcut;
 };
 
 // Stage masks (granular entry points run a subset of a fused pass).
@@ -311,6 +314,33 @@ __device__ __forceinline__ int dp_project(double alpha1, double& p, double& sxx,
     J2 = __builtin_sqrt(J2 / 2.0) + 1.0e-6;
     if (J2 > alpha1 * I1) {
         const double skale = alpha1 * I1 / J2;
+        sxx = skale * sxx;
+        sxy = skale * sxy;
+        syy = skale * syy;
+        return 1;
+    }
+    return 0;
+}
+
+// mohr_coulomb_failure_criterion for one particle (m_particles.f90:3220-3266):
+// where the yield stress c cos(phi) + p sin(phi) is not positive, p drops to
+// the tension cut-off and rho to rho0; a maximum shear stress above the yield
+// stress scales the deviator down to it.
+__device__ __forceinline__ int mc_project(const KParams& K, double& p, double& sxx, double& sxy, double& syy,
+                                          double& rho)
+{
+#pragma clang fp contract(off)
+    const double h = (sxx - syy) / 2.0;
+    const double tmax = __builtin_sqrt(h * h + sxy * sxy);
+    if (tmax < (double)1.e-6f) return 0;
+    double yield = K.mc_ccos + p * K.mc_sin;
+    if (yield <= 0.0) {
+        yield = 0.0;
+        p = K.mc_pcut;
+        rho = K.s_rho0;
+    }
+    if (tmax > yield) {
+        const double skale = yield / tmax;
         sxx = skale * sxx;
         sxy = skale * sxy;
         syy = skale * syy;

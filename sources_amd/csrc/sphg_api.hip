@@ -300,6 +300,9 @@ void set_kparams(sphg_ctx* c)
     K.alpha = P.alpha; K.beta = P.beta; K.etq = P.etq;
     K.dd = P.dd; K.p1 = P.p1; K.p2 = P.p2;
     auto small_int = [](double v) { return (v >= 0.0 && v <= 64.0 && v == (double)(int)v) ? (int)v : -1; };
+    K.mc_ccos = P.soil_cohesion * std::cos(P.soil_phi);
+    K.mc_sin = std::sin(P.soil_phi);
+    K.mc_pcut = -P.soil_cohesion * std::pow(std::tan(P.soil_phi), -1.0);
     K.ip1 = small_int(P.p1);
     K.ip2 = small_int(P.p2);
     K.delta = P.delta; K.epsilon = P.epsilon;
@@ -765,7 +768,8 @@ int stages_fused(sphg_ctx* c)
     if (side_on(c)) RC(fork_side(c));
     {
         SideScope side(c);
-        if ((c->K.nnps == 2 || c->F.uni_s) && c->F.uni_x && rep_int)
+        // U: uniform h in both phases, integer repulsion exponents, Drucker-Prager
+        if ((c->K.nnps == 2 || c->F.uni_s) && c->F.uni_x && rep_int && c->K.yield_criterion != 1)
             RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d,
                       W.d, (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
         else
@@ -794,7 +798,7 @@ int stages_soil_only(sphg_ctx* c)
 {
     PhaseBuf& S = c->ph[1];
     const double* rs = S.rec;
-    if (c->K.nnps == 2 || c->F.uni_s)
+    if ((c->K.nnps == 2 || c->F.uni_s) && c->K.yield_criterion != 1)
         RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
                   S.d, S.d, rs, rs, c->gl[1], c->K, c->F, 0, mask_b(c, 1)));
     else
@@ -900,7 +904,8 @@ int check_fused_switches(const sphg_ctx* c)
     if (soil_only(c)) {
         // time_integration_for_soil: leapfrog only; Drucker-Prager with flow rule 3
         if (P.integrate_scheme == 2 || P.precision != SPHG_FP64) return SPHG_ERR_UNSUPPORTED;
-        if (P.yield_criterion != 2 || (P.plasticity != 3 && P.plasticity != 0)) return SPHG_ERR_UNSUPPORTED;
+        if ((P.yield_criterion != 1 && P.yield_criterion != 2) || (P.plasticity != 3 && P.plasticity != 0))
+            return SPHG_ERR_UNSUPPORTED;
         return SPHG_OK;
     }
     // the mixed-precision form exists for the fused two-phase step with the
@@ -910,7 +915,9 @@ int check_fused_switches(const sphg_ctx* c)
         return SPHG_ERR_UNSUPPORTED;
     if (!P.single_phase) {
         if (!c->ph[1].bound) return SPHG_ERR_INVALID;
-        if (P.yield_criterion != 2) return SPHG_ERR_UNSUPPORTED;
+        // Mohr-Coulomb in the FP64 fused step only
+        if (P.yield_criterion != 2 && !(P.yield_criterion == 1 && P.precision == SPHG_FP64 && !c->legacy))
+            return SPHG_ERR_UNSUPPORTED;
         if (P.plasticity != 3 && P.plasticity != 0) return SPHG_ERR_UNSUPPORTED;
     }
     if (!c->ph[0].bound) return SPHG_ERR_INVALID;

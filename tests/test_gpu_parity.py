@@ -270,6 +270,52 @@ def _assert_fail_flags(g, r, P):
     assert np.all(np.abs(j2 - lim) <= 1e-9 * np.maximum(np.abs(j2), np.abs(lim))), (j2, lim)
 
 
+@pytest.mark.parametrize("cohesion,ptop", [(0.0, -40.0), (100.0, -600.0)])
+def test_two_phase_mohr_coulomb(cohesion, ptop):
+    """yield_criterion = 1: the soil criterion of single_step is Mohr-Coulomb.
+    phi = 10 degrees and a shear-stress perturbation make it scale the
+    deviator of most soil particles; a top layer in tension (p below
+    -c cot(phi)) takes the cut-off, which resets rho between the terms
+    evaluated before the criterion and after it."""
+    def make():
+        sc = _column()
+        P = sc.params
+        P.yield_criterion = 1
+        P.soil_cohesion = cohesion
+        P.soil_phi = np.deg2rad(10.0)
+        so = sc.soil
+        n = so.ntotal
+        so.sxy[:n] += np.random.default_rng(3).normal(0.0, 60.0, n)
+        top = so.x[:n, 1] > so.x[:n, 1].max() - 0.006
+        so.p[:n][top] = ptop
+        return sc
+    sc, ref = _both(make)
+    probe = make()
+    oracle.single_step(probe.params, probe.water, probe.soil, 1)
+    assert (probe.soil.rho == probe.params.soil_rho0).sum() > 0   # the cut-off is exercised
+    s = Solver(sc.params, sc.water, sc.soil)
+    for nsteps, done, tol in ((1, 0, FP64_TOL_1STEP), (6, 1, 1e-10)):
+        print(f"mohr-coulomb c={cohesion} steps {done}->{nsteps}", flush=True)
+        s.time_integration(nsteps - done, sync=True)
+        oracle.time_integration(ref.params, ref.water, ref.soil, done, nsteps - done)
+        assert_fields(sc.water, ref.water, tol, label="water")
+        es = assert_fields(sc.soil, ref.soil, tol, label="soil")
+        print(nsteps, "soil", max(es.values()))
+    s.close()
+
+
+def test_soil_only_mohr_coulomb():
+    make = lambda: scenarios.soil_collapse(dx=0.005, jitter=0.05, seed=9, yield_criterion=1,  # noqa: E731
+                                           soil_cohesion=50.0, soil_phi=np.deg2rad(10.0))
+    sc, ref = _both(make)
+    s = Solver(sc.params, None, sc.soil)
+    s.time_integration(5, sync=True)
+    oracle.time_integration(ref.params, None, ref.soil, 0, 5)
+    es = assert_fields(sc.soil, ref.soil, 1e-10, label="soil only, Mohr-Coulomb")
+    print("soil", max(es.values()))
+    s.close()
+
+
 def test_two_phase_without_delta_and_xsph():
     """artificial_density and average_velocity off: no delta-SPH on rho, and
     av keeps its value (the reference does not touch it)."""

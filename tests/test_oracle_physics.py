@@ -246,3 +246,42 @@ def test_vof_renormalisation_has_no_effect_on_the_step():
     for f in ("x", "vx", "rho", "vof", "p"):
         assert np.array_equal(getattr(a.water, f), getattr(b.water, f)), f
         assert np.array_equal(getattr(a.soil, f), getattr(b.soil, f)), f
+
+
+def test_mohr_coulomb_projection():
+    # mohr_coulomb_failure_criterion (m_particles.f90:3220-3266), closed forms:
+    # the maximum shear stress tmax = sqrt(((sxx-syy)/2)^2 + sxy^2) is capped
+    # at the yield stress c cos(phi) + p sin(phi); below the tension cut-off
+    # (yield <= 0) p becomes -c cot(phi), rho becomes rho0 and the deviator
+    # vanishes; tmax < 1e-6 is left alone.
+    from sources_amd.state import Phase, default_params
+    c, phi = 2000.0, np.deg2rad(30.0)
+    P = default_params(soil_cohesion=c, soil_phi=phi, yield_criterion=1)
+    ph = Phase(abi.SPHG_SOIL, 5, 0, max_interaction=10, dspp=0.005)
+    #            inside   over    tension  tiny     over, p < 0 but yield > 0
+    ph.p[:] = [1.0e4, 1.0e4, -5.0e3, 1.0e4, -1.0e3]
+    ph.sxx[:] = [100.0, 9.0e3, 300.0, 1e-7, 3.0e3]
+    ph.syy[:] = [-100.0, -9.0e3, -300.0, -1e-7, -3.0e3]
+    ph.sxy[:] = [50.0, 4.0e3, 100.0, 0.0, 1.0e3]
+    ph.rho[:] = 1700.0
+    before = {f: getattr(ph, f).copy() for f in ("p", "sxx", "syy", "sxy", "rho")}
+    oracle.call_phase("mohr_coulomb", P, ph)
+    tmax0 = np.sqrt(((before["sxx"] - before["syy"]) / 2) ** 2 + before["sxy"] ** 2)
+    yield0 = c * np.cos(phi) + before["p"] * np.sin(phi)
+    assert list(ph.fail) == [0, 1, 1, 0, 1]
+    assert ph.nfail == 3
+    for i in (0, 3):   # untouched
+        for f in before:
+            assert getattr(ph, f)[i] == before[f][i]
+    for i in (1, 4):   # scaled onto the yield surface, direction kept, p and rho kept
+        k = yield0[i] / tmax0[i]
+        assert 0.0 < k < 1.0
+        for f in ("sxx", "syy", "sxy"):
+            np.testing.assert_allclose(getattr(ph, f)[i], k * before[f][i], rtol=1e-15)
+        tmax = np.sqrt(((ph.sxx[i] - ph.syy[i]) / 2) ** 2 + ph.sxy[i] ** 2)
+        np.testing.assert_allclose(tmax, yield0[i], rtol=1e-14)
+        assert ph.p[i] == before["p"][i] and ph.rho[i] == before["rho"][i]
+    # tension cut-off
+    np.testing.assert_allclose(ph.p[2], -c / np.tan(phi), rtol=1e-15)
+    assert ph.rho[2] == P.soil_rho0
+    assert ph.sxx[2] == 0.0 and ph.syy[2] == 0.0 and ph.sxy[2] == 0.0
