@@ -515,8 +515,9 @@ void oracle_plastic_or_not(const sphg_params *P, sphg_phase_view *ph)
     ph->nfail = k;
 }
 
-/* plastic_flow_rule3, m_particles.f90:3700-3762 */
-void oracle_plastic_flow_rule3(const sphg_params *P, sphg_phase_view *ph)
+/* plastic_flow_rule3, m_particles.f90:3700-3762 (failed points only) and
+   plastic_flow_rule, :3571-3633 (every point; the same correction). */
+static void flow_rule(const sphg_params *P, sphg_phase_view *ph, int failed_only)
 {
     int n = ph->ntotal + ph->nvirt, i;
     double phi = P->soil_phi, K = P->soil_k, e = P->soil_E, niu = P->soil_niu;
@@ -524,7 +525,7 @@ void oracle_plastic_flow_rule3(const sphg_params *P, sphg_phase_view *ph)
     double G = e / (2.0 * (1.0 + niu));
     for (i = 0; i < n; i++) {
         double exx, exy, eyy, sde, J2, dlambda, sxx, sxy, syy;
-        if (ph->fail[i] != 1) continue;
+        if (failed_only && ph->fail[i] != 1) continue;
         sxx = ph->sxx[i]; sxy = ph->sxy[i]; syy = ph->syy[i];
         exx = ph->txx[i] / 2.0 + ph->vcc[i] / 3.0;
         exy = ph->txy[i] / 2.0;
@@ -541,6 +542,12 @@ void oracle_plastic_flow_rule3(const sphg_params *P, sphg_phase_view *ph)
         ph->epsilon_p[i] = ph->epsilon_p[i] + dlambda * sxy / (2.0 * sqrt(J2)) * EPSP_SCALE_F;
     }
 }
+
+void oracle_plastic_flow_rule3(const sphg_params *P, sphg_phase_view *ph) { flow_rule(P, ph, 1); }
+
+/* Pinned: equals plastic_flow_rule3 bit for bit where every point is flagged,
+   and acts where none is (tests/test_oracle_physics.py). */
+void oracle_plastic_flow_rule(const sphg_params *P, sphg_phase_view *ph) { flow_rule(P, ph, 0); }
 
 /* Jaumann_rate, m_particles.f90:3047-3119.  `wxy` is the spin; in the
    reference it aliases the shared txy buffer and is zeroed after Hooke's law
@@ -826,6 +833,8 @@ static int phase_rhs(const sphg_params *P, sphg_phase_view *pl, int itimestep)
         if (P->plasticity == 3) {
             oracle_plastic_or_not(P, pl);
             oracle_plastic_flow_rule3(P, pl);
+        } else if (P->plasticity == 1) {
+            oracle_plastic_flow_rule(P, pl);
         } else if (P->plasticity != 0) {
             return SPHG_ERR_UNSUPPORTED;
         }
@@ -925,7 +934,7 @@ int oracle_single_step_for_water(const sphg_params *P, sphg_phase_view *pl, int 
 }
 
 /* single_step_for_soil, declarations_sph.f90:1530-1655 (serial path,
- * summation_density = .false., plasticity 0 or 3).  The same routines as the
+ * summation_density = .false., plasticity 0, 1 or 3).  The same routines as the
  * soil phase of single_step, except the momentum: -df(p) + df(sxx) + df(sxy)
  * (the df form, no vof factors, :1596-1600) and no repulsive force.
  * Pinned: the df-form internal force with art_visc is pairwise
@@ -938,7 +947,7 @@ int oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int i
     int n = pl->ntotal + pl->nvirt, i, rc;
     double *a, *b, *c3;
     (void)itimestep;
-    if (P->summation_density || (P->plasticity != 0 && P->plasticity != 3)) return SPHG_ERR_UNSUPPORTED;
+    if (P->summation_density || P->plasticity == 2) return SPHG_ERR_UNSUPPORTED;
     for (i = 0; i < 2 * n; i++) pl->dvx[i] = 0.0;
     for (i = 0; i < n; i++) {
         pl->drho[i] = 0.0; pl->dsxx[i] = 0.0; pl->dsxy[i] = 0.0; pl->dsyy[i] = 0.0; pl->dp[i] = 0.0;
@@ -966,6 +975,8 @@ int oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int i
     if (P->plasticity == 3) {
         oracle_plastic_or_not(P, pl);
         oracle_plastic_flow_rule3(P, pl);
+    } else if (P->plasticity == 1) {
+        oracle_plastic_flow_rule(P, pl);
     }
     oracle_jaumann_rate(P, pl);
     if (P->visc_artificial) oracle_art_visc(P, pl);
@@ -1052,7 +1063,7 @@ static void second_half(const sphg_params *P, sphg_phase_view *pl)
 static int check_switches(const sphg_params *P)
 {
     if (P->summation_density || P->stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
-    if (!P->single_phase && P->plasticity != 3 && P->plasticity != 0) return SPHG_ERR_UNSUPPORTED;
+    if (!P->single_phase && P->plasticity == 2) return SPHG_ERR_UNSUPPORTED;
     return SPHG_OK;
 }
 

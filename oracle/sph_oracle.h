@@ -32,6 +32,7 @@ void oracle_drucker_prager(const sphg_params *P, sphg_phase_view *ph);
 void oracle_mohr_coulomb(const sphg_params *P, sphg_phase_view *ph);
 void oracle_plastic_or_not(const sphg_params *P, sphg_phase_view *ph);
 void oracle_plastic_flow_rule3(const sphg_params *P, sphg_phase_view *ph);
+void oracle_plastic_flow_rule(const sphg_params *P, sphg_phase_view *ph);
 void oracle_jaumann_rate(const sphg_params *P, sphg_phase_view *ph);
 void oracle_art_visc(const sphg_params *P, sphg_phase_view *ph);
 void oracle_repulsive_force(const sphg_params *P, sphg_phase_view *ph);

@@ -1243,12 +1243,15 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
 
     const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;
     double dsxx = 0.0, dsxy = 0.0, dsyy = 0.0, dpv = 0.0;
-    if (K.plasticity == 3) {
+    // plasticity 3: plastic_or_not + plastic_flow_rule3 (failed points);
+    // plasticity 1: plastic_flow_rule (m_particles.f90:3571-3633), the same
+    // correction at every point, with the criterion's fail flags left as they are
+    if (K.plasticity == 3 || K.plasticity == 1) {
         const double I1 = 3.0 * pa;
         const double j2r = sxxa * sxxa + 2.0 * (sxya * sxya) + syya * syya + (sxxa + syya) * (sxxa + syya);
         const double J2 = __builtin_sqrt(j2r / 2.0) + 1.0e-6;
         const int fail = fabs(J2 - alpha * I1) < SPH_YIELD_TOL_F ? 1 : 0;
-        if (fail) {
+        if (fail || K.plasticity == 1) {
             const double exx = txx * 0.5 + vcc * (1.0 / 3.0);
             const double exy = txy * 0.5;
             const double eyy = tyy * 0.5 + vcc * (1.0 / 3.0);
@@ -1264,7 +1267,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
                 S.eps_p[a] = S.eps_p[a] + dl * sxya / (2.0 * sj) * SPH_EPSP_F;
             }
         }
-        S.fail[a] = fail;
+        if (K.plasticity == 3) S.fail[a] = fail;
     }
     dsxx = dsxx + G * txx + 2.0 * sxya * wxy;
     dsxy = dsxy + G * txy - (sxxa - syya) * wxy;

@@ -904,8 +904,7 @@ int check_fused_switches(const sphg_ctx* c)
     if (soil_only(c)) {
         // time_integration_for_soil: leapfrog only; Drucker-Prager with flow rule 3
         if (P.integrate_scheme == 2 || P.precision != SPHG_FP64) return SPHG_ERR_UNSUPPORTED;
-        if ((P.yield_criterion != 1 && P.yield_criterion != 2) || (P.plasticity != 3 && P.plasticity != 0))
-            return SPHG_ERR_UNSUPPORTED;
+        if ((P.yield_criterion != 1 && P.yield_criterion != 2) || P.plasticity == 2) return SPHG_ERR_UNSUPPORTED;
         return SPHG_OK;
     }
     // the mixed-precision form exists for the fused two-phase step with the
@@ -918,7 +917,10 @@ int check_fused_switches(const sphg_ctx* c)
         // Mohr-Coulomb in the FP64 fused step only
         if (P.yield_criterion != 2 && !(P.yield_criterion == 1 && P.precision == SPHG_FP64 && !c->legacy))
             return SPHG_ERR_UNSUPPORTED;
-        if (P.plasticity != 3 && P.plasticity != 0) return SPHG_ERR_UNSUPPORTED;
+        // plastic_flow_rule (plasticity 1) in the FP64 fused step only
+        if (P.plasticity != 3 && P.plasticity != 0 &&
+            !(P.plasticity == 1 && P.precision == SPHG_FP64 && !c->legacy))
+            return SPHG_ERR_UNSUPPORTED;
     }
     if (!c->ph[0].bound) return SPHG_ERR_INVALID;
     return SPHG_OK;

@@ -316,6 +316,29 @@ def test_soil_only_mohr_coulomb():
     s.close()
 
 
+def test_plastic_flow_rule_1():
+    """plasticity = 1: plastic_flow_rule at every point, two-phase and soil only."""
+    def make2():
+        sc = _column()
+        sc.params.plasticity = 1
+        return sc
+    sc, ref = _both(make2)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(4, sync=True)
+    oracle.time_integration(ref.params, ref.water, ref.soil, 0, 4)
+    assert_fields(sc.water, ref.water, 1e-11, label="water")
+    es = assert_fields(sc.soil, ref.soil, 1e-11, label="soil")
+    print("two-phase, rule 1:", max(es.values()))
+    s.close()
+    sc, ref = _both(lambda: scenarios.soil_collapse(dx=0.005, jitter=0.05, seed=9, plasticity=1))
+    s = Solver(sc.params, None, sc.soil)
+    s.time_integration(4, sync=True)
+    oracle.time_integration(ref.params, None, ref.soil, 0, 4)
+    es = assert_fields(sc.soil, ref.soil, 1e-11, label="soil only, rule 1")
+    print("soil only, rule 1:", max(es.values()))
+    s.close()
+
+
 def test_two_phase_without_delta_and_xsph():
     """artificial_density and average_velocity off: no delta-SPH on rho, and
     av keeps its value (the reference does not touch it)."""

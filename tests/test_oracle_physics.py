@@ -285,3 +285,43 @@ def test_mohr_coulomb_projection():
     np.testing.assert_allclose(ph.p[2], -c / np.tan(phi), rtol=1e-15)
     assert ph.rho[2] == P.soil_rho0
     assert ph.sxx[2] == 0.0 and ph.syy[2] == 0.0 and ph.sxy[2] == 0.0
+
+
+def test_plastic_flow_rule_is_rule3_at_every_point():
+    # plastic_flow_rule (plasticity = 1, m_particles.f90:3571-3633) is
+    # plastic_flow_rule3's correction without the failure flag test: bit-equal
+    # to rule 3 where every point is flagged; with no point flagged, rule 3
+    # does nothing and rule 1 still corrects along the deviator.
+    from sources_amd.state import default_params
+    P = default_params(soil_phi=math.radians(30.0))
+
+    def state(fail):
+        _, ph = lattice_phase(n=10, material=abi.SPHG_SOIL)
+        rng = np.random.default_rng(9)
+        n = ph.maxn
+        ph.sxx[:] = rng.normal(scale=200, size=n)
+        ph.syy[:] = rng.normal(scale=200, size=n)
+        ph.sxy[:] = rng.normal(scale=100, size=n)
+        ph.txx[:] = rng.normal(size=n)
+        ph.tyy[:] = rng.normal(size=n)
+        ph.txy[:] = rng.normal(size=n)
+        ph.vcc[:] = rng.normal(size=n)
+        ph.fail[:] = fail
+        for f in ("dsxx", "dsxy", "dsyy", "dp", "epsilon_p"):
+            getattr(ph, f)[:] = 0.0
+        return ph
+
+    a, b = state(1), state(1)
+    oracle.call_phase("plastic_flow_rule", P, a)
+    oracle.call_phase("plastic_flow_rule3", P, b)
+    for f in ("dsxx", "dsxy", "dsyy", "dp", "epsilon_p"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    a, b = state(0), state(0)
+    oracle.call_phase("plastic_flow_rule", P, a)
+    oracle.call_phase("plastic_flow_rule3", P, b)
+    assert np.all(b.dp == 0.0) and np.all(b.dsxx == 0.0)
+    moved = a.dp != 0
+    assert moved.sum() > 10 and np.all(a.dp[moved] > 0)
+    k = a.dsxx[moved] / a.sxx[moved]
+    np.testing.assert_allclose(a.dsxy[moved] / a.sxy[moved], k, rtol=1e-12)
+    assert np.all(a.fail == 0)   # rule 1 leaves the flags alone
