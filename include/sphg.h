@@ -143,6 +143,12 @@ typedef struct sphg_phase_view {
     double *sxx, *syy, *sxy, *txx, *tyy, *txy, *wxy, *vcc, *epsilon_p;
     double *dsxx, *dsyy, *dsxy, *dp;
     double *w, *dwdx;
+    /* soil: the stress levels of stress_integration = 2 and the rates of
+       plastic_flow_rule2 (plasticity = 2; m_particles.f90:275-293).  May be
+       NULL: the device then keeps its own copies, starting at zero as
+       allocate_sph leaves them (declarations_sph.f90:493-507). */
+    double *sxx_min, *sxy_min, *syy_min, *p_min;
+    double *dsxx2, *dsxy2, *dsyy2, *dp2;
 } sphg_phase_view;
 
 typedef struct sphg_ctx sphg_ctx;

@@ -38,7 +38,9 @@ def lib():
         for name in ("oracle_con_density", "oracle_sum_density", "oracle_shear_strain_rate",
                      "oracle_velocity_divergence", "oracle_pressure", "oracle_newtonian_fluid",
                      "oracle_drucker_prager", "oracle_mohr_coulomb", "oracle_plastic_or_not",
-                     "oracle_plastic_flow_rule3", "oracle_plastic_flow_rule", "oracle_jaumann_rate", "oracle_art_visc",
+                     "oracle_plastic_flow_rule3", "oracle_plastic_flow_rule",
+                     "oracle_plastic_flow_rule2", "oracle_first_half", "oracle_first_step",
+                     "oracle_second_half", "oracle_return_mapping", "oracle_jaumann_rate", "oracle_art_visc",
                      "oracle_repulsive_force", "oracle_av_vel"):
             getattr(L, name).argtypes = [P, V]
             getattr(L, name).restype = None

@@ -545,6 +545,32 @@ static void flow_rule(const sphg_params *P, sphg_phase_view *ph, int failed_only
 
 void oracle_plastic_flow_rule3(const sphg_params *P, sphg_phase_view *ph) { flow_rule(P, ph, 1); }
 
+/* plastic_flow_rule2, m_particles.f90:3636-3697: the same dlambda at every
+   point, without the sign test, into the return-mapping rates dsxx2 .. dp2
+   (not accumulated; epsilon_p untouched).  Pinned: equal to plastic_flow_rule's
+   increments where dlambda >= 0 (tests/test_oracle_physics.py). */
+void oracle_plastic_flow_rule2(const sphg_params *P, sphg_phase_view *ph)
+{
+    int n = ph->ntotal + ph->nvirt, i;
+    double phi = P->soil_phi, K = P->soil_k, e = P->soil_E, niu = P->soil_niu;
+    double alpha = dp_alpha1(phi);
+    double G = e / (2.0 * (1.0 + niu));
+    for (i = 0; i < n; i++) {
+        double exx, exy, eyy, sde, J2, dlambda, sxx = ph->sxx[i], sxy = ph->sxy[i], syy = ph->syy[i];
+        exx = ph->txx[i] / 2.0 + ph->vcc[i] / 3.0;
+        exy = ph->txy[i] / 2.0;
+        eyy = ph->tyy[i] / 2.0 + ph->vcc[i] / 3.0;
+        sde = sxx * exx + 2.0 * sxy * exy + syy * eyy;
+        J2 = (sxx * sxx + 2.0 * (sxy * sxy) + syy * syy + (sxx + syy) * (sxx + syy)) / 2.0;
+        J2 = J2 + 1.0e-10;
+        dlambda = (3.0 * alpha * K * ph->vcc[i] + G / sqrt(J2) * sde) / (9.0 * (alpha * alpha) * K + G);
+        ph->dsxx2[i] = -dlambda * (G / sqrt(J2) * sxx);
+        ph->dsxy2[i] = -dlambda * (G / sqrt(J2) * sxy);
+        ph->dsyy2[i] = -dlambda * (G / sqrt(J2) * syy);
+        ph->dp2[i] = 3.0 * K * alpha * dlambda;
+    }
+}
+
 /* Pinned: equals plastic_flow_rule3 bit for bit where every point is flagged,
    and acts where none is (tests/test_oracle_physics.py). */
 void oracle_plastic_flow_rule(const sphg_params *P, sphg_phase_view *ph) { flow_rule(P, ph, 0); }
@@ -835,6 +861,8 @@ static int phase_rhs(const sphg_params *P, sphg_phase_view *pl, int itimestep)
             oracle_plastic_flow_rule3(P, pl);
         } else if (P->plasticity == 1) {
             oracle_plastic_flow_rule(P, pl);
+        } else if (P->plasticity == 2) {
+            oracle_plastic_flow_rule2(P, pl);
         } else if (P->plasticity != 0) {
             return SPHG_ERR_UNSUPPORTED;
         }
@@ -934,7 +962,7 @@ int oracle_single_step_for_water(const sphg_params *P, sphg_phase_view *pl, int 
 }
 
 /* single_step_for_soil, declarations_sph.f90:1530-1655 (serial path,
- * summation_density = .false., plasticity 0, 1 or 3).  The same routines as the
+ * summation_density = .false.).  The same routines as the
  * soil phase of single_step, except the momentum: -df(p) + df(sxx) + df(sxy)
  * (the df form, no vof factors, :1596-1600) and no repulsive force.
  * Pinned: the df-form internal force with art_visc is pairwise
@@ -947,7 +975,7 @@ int oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int i
     int n = pl->ntotal + pl->nvirt, i, rc;
     double *a, *b, *c3;
     (void)itimestep;
-    if (P->summation_density || P->plasticity == 2) return SPHG_ERR_UNSUPPORTED;
+    if (P->summation_density || P->plasticity < 0 || P->plasticity > 3) return SPHG_ERR_UNSUPPORTED;
     for (i = 0; i < 2 * n; i++) pl->dvx[i] = 0.0;
     for (i = 0; i < n; i++) {
         pl->drho[i] = 0.0; pl->dsxx[i] = 0.0; pl->dsxy[i] = 0.0; pl->dsyy[i] = 0.0; pl->dp[i] = 0.0;
@@ -977,6 +1005,8 @@ int oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int i
         oracle_plastic_flow_rule3(P, pl);
     } else if (P->plasticity == 1) {
         oracle_plastic_flow_rule(P, pl);
+    } else if (P->plasticity == 2) {
+        oracle_plastic_flow_rule2(P, pl);
     }
     oracle_jaumann_rate(P, pl);
     if (P->visc_artificial) oracle_art_visc(P, pl);
@@ -992,12 +1022,14 @@ int oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int i
 }
 
 /* ------------------------------------------------------------------------ */
-/* time_integration, declarations_sph.f90:525-791 (stress_integration = 1,   */
-/* plasticity = 3; summation_density = .false.)                              */
-/* Pinned by: free-fall recurrence of an isolated particle                   */
-/* (tests/test_oracle_integration.py).                                       */
+/* time_integration, declarations_sph.f90:525-791 (summation_density =       */
+/* .false.).  first_half / first_step / second_half / return_mapping are the */
+/* routines contained there (:643-789).  Pinned by the free-fall recurrence  */
+/* of an isolated particle and by closed forms of the stress levels under    */
+/* constant rates, both stress_integration values and the return mapping     */
+/* (tests/test_oracle_physics.py).                                           */
 /* ------------------------------------------------------------------------ */
-static void first_half(const sphg_params *P, sphg_phase_view *pl)
+void oracle_first_half(const sphg_params *P, sphg_phase_view *pl)
 {
     int n = pl->ntotal + pl->nvirt, i, d;
     double dt = P->dt;
@@ -1008,6 +1040,16 @@ static void first_half(const sphg_params *P, sphg_phase_view *pl)
             pl->vof_min[i] = pl->vof[i];
             pl->vof[i] = pl->vof[i] + (dt / 2.0) * pl->dvof[i];
         }
+        if (pl->material == SPHG_SOIL && P->stress_integration == 2) {   /* :664-672 */
+            pl->sxx_min[i] = pl->sxx[i];
+            pl->sxy_min[i] = pl->sxy[i];
+            pl->syy_min[i] = pl->syy[i];
+            pl->p_min[i] = pl->p[i];
+            pl->sxx[i] = pl->sxx[i] + (dt / 2.0) * pl->dsxx[i];
+            pl->sxy[i] = pl->sxy[i] + (dt / 2.0) * pl->dsxy[i];
+            pl->syy[i] = pl->syy[i] + (dt / 2.0) * pl->dsyy[i];
+            pl->p[i] = pl->p[i] + (dt / 2.0) * pl->dp[i];
+        }
         if (pl->itype[i] < 0) continue;
         for (d = 0; d < 2; d++) {
             pl->v_min[2 * i + d] = pl->vx[2 * i + d];
@@ -1016,7 +1058,7 @@ static void first_half(const sphg_params *P, sphg_phase_view *pl)
     }
 }
 
-static void first_step(const sphg_params *P, sphg_phase_view *pl)
+void oracle_first_step(const sphg_params *P, sphg_phase_view *pl)
 {
     int n = pl->ntotal + pl->nvirt, i, d;
     double dt = P->dt;
@@ -1024,7 +1066,12 @@ static void first_step(const sphg_params *P, sphg_phase_view *pl)
         pl->rho[i] = pl->rho[i] + (dt / 2.0) * pl->drho[i];
         if (pl->material == SPHG_WATER && P->volume_fraction)
             pl->vof[i] = pl->vof[i] + (dt / 2.0) * pl->dvof[i];
-        if (pl->material == SPHG_SOIL) {
+        if (pl->material == SPHG_SOIL && P->stress_integration == 2) {   /* :700-705 */
+            pl->sxx[i] = pl->sxx[i] + (dt / 2.0) * pl->dsxx[i];
+            pl->sxy[i] = pl->sxy[i] + (dt / 2.0) * pl->dsxy[i];
+            pl->syy[i] = pl->syy[i] + (dt / 2.0) * pl->dsyy[i];
+            pl->p[i] = pl->p[i] + (dt / 2.0) * pl->dp[i];
+        } else if (pl->material == SPHG_SOIL) {
             pl->sxx[i] = pl->sxx[i] + dt * pl->dsxx[i];
             pl->sxy[i] = pl->sxy[i] + dt * pl->dsxy[i];
             pl->syy[i] = pl->syy[i] + dt * pl->dsyy[i];
@@ -1038,7 +1085,7 @@ static void first_step(const sphg_params *P, sphg_phase_view *pl)
     }
 }
 
-static void second_half(const sphg_params *P, sphg_phase_view *pl)
+void oracle_second_half(const sphg_params *P, sphg_phase_view *pl)
 {
     int n = pl->ntotal + pl->nvirt, i, d;
     double dt = P->dt;
@@ -1046,7 +1093,12 @@ static void second_half(const sphg_params *P, sphg_phase_view *pl)
         pl->rho[i] = pl->rho_min[i] + dt * pl->drho[i];
         if (pl->material == SPHG_WATER && P->volume_fraction)
             pl->vof[i] = pl->vof_min[i] + dt * pl->dvof[i];
-        if (pl->material == SPHG_SOIL) {
+        if (pl->material == SPHG_SOIL && P->stress_integration == 2) {   /* :744-749 */
+            pl->sxx[i] = pl->sxx_min[i] + dt * pl->dsxx[i];
+            pl->sxy[i] = pl->sxy_min[i] + dt * pl->dsxy[i];
+            pl->syy[i] = pl->syy_min[i] + dt * pl->dsyy[i];
+            pl->p[i] = pl->p_min[i] + dt * pl->dp[i];
+        } else if (pl->material == SPHG_SOIL) {
             pl->sxx[i] = pl->sxx[i] + dt * pl->dsxx[i];
             pl->sxy[i] = pl->sxy[i] + dt * pl->dsxy[i];
             pl->syy[i] = pl->syy[i] + dt * pl->dsyy[i];
@@ -1062,9 +1114,32 @@ static void second_half(const sphg_params *P, sphg_phase_view *pl)
 
 static int check_switches(const sphg_params *P)
 {
-    if (P->summation_density || P->stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
-    if (!P->single_phase && P->plasticity == 2) return SPHG_ERR_UNSUPPORTED;
+    if (P->summation_density || (P->stress_integration != 1 && P->stress_integration != 2))
+        return SPHG_ERR_UNSUPPORTED;
     return SPHG_OK;
+}
+
+/* return_mapping, declarations_sph.f90:764-789: points flagged by
+   plastic_or_not take the plastic_flow_rule2 rates (stress_integration 1)
+   or are recomputed from the _min levels (stress_integration 2). */
+void oracle_return_mapping(const sphg_params *P, sphg_phase_view *pl)
+{
+    int n = pl->ntotal + pl->nvirt, i;
+    double dt = P->dt;
+    for (i = 0; i < n; i++) {
+        if (pl->fail[i] != 1) continue;
+        if (P->stress_integration == 1) {
+            pl->sxx[i] = pl->sxx[i] + dt * pl->dsxx2[i];
+            pl->sxy[i] = pl->sxy[i] + dt * pl->dsxy2[i];
+            pl->syy[i] = pl->syy[i] + dt * pl->dsyy2[i];
+            pl->p[i] = pl->p[i] + dt * pl->dp2[i];
+        } else {
+            pl->sxx[i] = pl->sxx_min[i] + dt * pl->dsxx[i];
+            pl->sxy[i] = pl->sxy_min[i] + dt * pl->dsxy[i];
+            pl->syy[i] = pl->syy_min[i] + dt * pl->dsyy[i];
+            pl->p[i] = pl->p_min[i] + dt * pl->dp[i];
+        }
+    }
 }
 
 int oracle_time_integration(const sphg_params *P, sphg_phase_view *wa, sphg_phase_view *so,
@@ -1075,18 +1150,22 @@ int oracle_time_integration(const sphg_params *P, sphg_phase_view *wa, sphg_phas
     for (it = 0; it < nsteps; it++) {
         int step = ++(*itimestep);
         if (step != 1) {
-            first_half(P, wa);
-            first_half(P, so);
+            oracle_first_half(P, wa);
+            oracle_first_half(P, so);
             oracle_drucker_prager(P, so);
         }
         rc = oracle_single_step(P, wa, so, step);
         if (rc) return rc;
         if (step == 1) {
-            first_step(P, wa);
-            first_step(P, so);
+            oracle_first_step(P, wa);
+            oracle_first_step(P, so);
         } else {
-            second_half(P, wa);
-            second_half(P, so);
+            oracle_second_half(P, wa);
+            oracle_second_half(P, so);
+        }
+        if (P->plasticity == 2) {   /* :576-579, :594-597 */
+            oracle_plastic_or_not(P, so);
+            oracle_return_mapping(P, so);
         }
         oracle_drucker_prager(P, so);
     }
@@ -1102,17 +1181,18 @@ int oracle_time_integration(const sphg_params *P, sphg_phase_view *wa, sphg_phas
 int oracle_time_integration_for_soil(const sphg_params *P, sphg_phase_view *so, int *itimestep, int nsteps)
 {
     int it, rc;
-    if (P->summation_density || P->stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
+    if (P->summation_density || (P->stress_integration != 1 && P->stress_integration != 2))
+        return SPHG_ERR_UNSUPPORTED;
     for (it = 0; it < nsteps; it++) {
         int step = ++(*itimestep);
         if (step != 1) {
-            first_half(P, so);
+            oracle_first_half(P, so);
             oracle_drucker_prager(P, so);
         }
         rc = oracle_single_step_for_soil(P, so, step);
         if (rc) return rc;
-        if (step == 1) first_step(P, so);
-        else second_half(P, so);
+        if (step == 1) oracle_first_step(P, so);
+        else oracle_second_half(P, so);
         oracle_drucker_prager(P, so);
     }
     return SPHG_OK;

@@ -33,6 +33,11 @@ void oracle_mohr_coulomb(const sphg_params *P, sphg_phase_view *ph);
 void oracle_plastic_or_not(const sphg_params *P, sphg_phase_view *ph);
 void oracle_plastic_flow_rule3(const sphg_params *P, sphg_phase_view *ph);
 void oracle_plastic_flow_rule(const sphg_params *P, sphg_phase_view *ph);
+void oracle_plastic_flow_rule2(const sphg_params *P, sphg_phase_view *ph);
+void oracle_first_half(const sphg_params *P, sphg_phase_view *pl);
+void oracle_first_step(const sphg_params *P, sphg_phase_view *pl);
+void oracle_second_half(const sphg_params *P, sphg_phase_view *pl);
+void oracle_return_mapping(const sphg_params *P, sphg_phase_view *pl);
 void oracle_jaumann_rate(const sphg_params *P, sphg_phase_view *ph);
 void oracle_art_visc(const sphg_params *P, sphg_phase_view *ph);
 void oracle_repulsive_force(const sphg_params *P, sphg_phase_view *ph);

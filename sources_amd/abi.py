@@ -106,6 +106,8 @@ VIEW_DBL_ARRAYS = (
     "sxx", "syy", "sxy", "txx", "tyy", "txy", "wxy", "vcc", "epsilon_p",
     "dsxx", "dsyy", "dsxy", "dp",
     "w", "dwdx",
+    "sxx_min", "sxy_min", "syy_min", "p_min",
+    "dsxx2", "dsxy2", "dsyy2", "dp2",
 )
 
 

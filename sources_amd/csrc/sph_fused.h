@@ -361,6 +361,15 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
         }
     }
     double p = P.p[i], sxx = P.sxx[i], sxy = P.sxy[i], syy = P.syy[i];
+    if (do_half && K.stress_integration == 2) {   // first_half, declarations_sph.f90:664-672
+        const double h = K.dt / 2.0;
+        P.sxx_min[i] = sxx; P.sxy_min[i] = sxy; P.syy_min[i] = syy; P.p_min[i] = p;
+        sxx = sxx + h * P.dsxx[i];
+        sxy = sxy + h * P.dsxy[i];
+        syy = syy + h * P.dsyy[i];
+        p = p + h * P.dp[i];
+        P.p[i] = p; P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy;
+    }
     // the last projection is single_step's failure criterion: Mohr-Coulomb
     // when yield_criterion = 1, which may also reset rho (the record then
     // keeps the density before it in chunk 7 for the terms evaluated before
@@ -1268,6 +1277,22 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
             }
         }
         if (K.plasticity == 3) S.fail[a] = fail;
+    }
+    if (K.plasticity == 2) {
+        // plastic_flow_rule2 (m_particles.f90:3636-3697): at every point and
+        // without the sign test; its rates act in the return mapping
+        const double j2r = sxxa * sxxa + 2.0 * (sxya * sxya) + syya * syya + (sxxa + syya) * (sxxa + syya);
+        const double exx = txx * 0.5 + vcc * (1.0 / 3.0);
+        const double exy = txy * 0.5;
+        const double eyy = tyy * 0.5 + vcc * (1.0 / 3.0);
+        const double sde = sxxa * exx + 2.0 * sxya * exy + syya * eyy;
+        const double sj = __builtin_sqrt(j2r / 2.0 + 1.0e-10);
+        const double gs = G / sj;
+        const double dl = (3.0 * alpha * Kb * vcc + gs * sde) / (9.0 * (alpha * alpha) * Kb + G);
+        S.dsxx2[a] = -dl * (gs * sxxa);
+        S.dsxy2[a] = -dl * (gs * sxya);
+        S.dsyy2[a] = -dl * (gs * syya);
+        S.dp2[a] = 3.0 * Kb * alpha * dl;
     }
     dsxx = dsxx + G * txx + 2.0 * sxya * wxy;
     dsxy = dsxy + G * txy - (sxxa - syya) * wxy;

@@ -28,6 +28,8 @@ struct DPhase {
     double *sxx, *sxy, *syy, *eps_p;
     double *dsxx, *dsxy, *dsyy, *dp;
     double *repx, *repy, *drgx, *drgy, *porx, *pory;
+    double *sxx_min, *sxy_min, *syy_min, *p_min;   // stress_integration = 2
+    double *dsxx2, *dsxy2, *dsyy2, *dp2;           // plastic_flow_rule2
     int *cell_count, *cell_start, *order;
 };
 
@@ -43,6 +45,7 @@ struct KParams {
     int soil_artificial_stress, water_artificial_volume, volume_fraction;
     int plasticity, yield_criterion, two_phase;
     int ip1, ip2;               // p1, p2 when they are integers in [0, 64], else -1
+    int stress_integration;     // 1: sigma += dt dsigma; 2: leapfrog levels sigma_min
     // Mohr-Coulomb criterion (yield_criterion = 1): cohesion cos(phi), sin(phi)
     // and the tension cut-off pressure -cohesion tan(phi)**(-1.0), host-evaluated
     double mc_ccos, mc_sin, mc_pcut;
@@ -495,10 +498,25 @@ __global__ void k_final_soil(DPhase P, KParams K, int first)
     if (K.volume_fraction && K.two_phase) P.vof[i] = P.rho[i] / K.s_rho0;
     if (first) P.rho[i] = P.rho[i] + h * P.drho[i];
     else P.rho[i] = P.rho_min[i] + dt * P.drho[i];
-    double sxx = P.sxx[i] + dt * P.dsxx[i];
-    double sxy = P.sxy[i] + dt * P.dsxy[i];
-    double syy = P.syy[i] + dt * P.dsyy[i];
-    double p = P.p[i] + dt * P.dp[i];
+    double sxx, sxy, syy, p;
+    if (K.stress_integration == 2) {   // :700-705, :744-749
+        if (first) {
+            sxx = P.sxx[i] + h * P.dsxx[i];
+            sxy = P.sxy[i] + h * P.dsxy[i];
+            syy = P.syy[i] + h * P.dsyy[i];
+            p = P.p[i] + h * P.dp[i];
+        } else {
+            sxx = P.sxx_min[i] + dt * P.dsxx[i];
+            sxy = P.sxy_min[i] + dt * P.dsxy[i];
+            syy = P.syy_min[i] + dt * P.dsyy[i];
+            p = P.p_min[i] + dt * P.dp[i];
+        }
+    } else {
+        sxx = P.sxx[i] + dt * P.dsxx[i];
+        sxy = P.sxy[i] + dt * P.dsxy[i];
+        syy = P.syy[i] + dt * P.dsyy[i];
+        p = P.p[i] + dt * P.dp[i];
+    }
     if (P.itype[i] > 0) {
         double vx, vy;
         if (first) {
@@ -512,6 +530,26 @@ __global__ void k_final_soil(DPhase P, KParams K, int first)
         P.vy[i] = vy;
         P.x[i] = P.x[i] + dt * vx;
         P.y[i] = P.y[i] + dt * vy;
+    }
+    if (K.plasticity == 2 && K.two_phase) {
+        // plastic_or_not + return_mapping (time_integration :576-579, :594-597,
+        // :764-789) on the updated stress
+        const double I1 = 3.0 * p;
+        const double j2r = sxx * sxx + 2.0 * (sxy * sxy) + syy * syy + (sxx + syy) * (sxx + syy);
+        const double J2 = __builtin_sqrt(j2r / 2.0) + 1.0e-6;
+        if (fabs(J2 - K.s_alpha1 * I1) < SPH_YIELD_TOL_F) {
+            if (K.stress_integration == 1) {
+                sxx = sxx + dt * P.dsxx2[i];
+                sxy = sxy + dt * P.dsxy2[i];
+                syy = syy + dt * P.dsyy2[i];
+                p = p + dt * P.dp2[i];
+            } else {
+                sxx = P.sxx_min[i] + dt * P.dsxx[i];
+                sxy = P.sxy_min[i] + dt * P.dsxy[i];
+                syy = P.syy_min[i] + dt * P.dsyy[i];
+                p = P.p_min[i] + dt * P.dp[i];
+            }
+        }
     }
     P.fail[i] = dp_project(K.s_alpha1, p, sxx, sxy, syy);
     P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy; P.p[i] = p;

@@ -268,9 +268,11 @@ int reorder(sphg_ctx* c, int k)
         return SPHG_OK;
     };
     RC(pi(d.id)); RC(pi(d.itype)); RC(pi(d.fail));
-    // av persists across steps when average_velocity is off
+    // av persists across steps when average_velocity is off; the stress
+    // levels of stress_integration = 2 persist too
     double** f[] = {&d.x, &d.y, &d.vx, &d.vy, &d.mass, &d.hsml, &d.rho, &d.vof, &d.p, &d.c,
-                    &d.drho, &d.ax, &d.ay, &d.dvof, &d.sxx, &d.sxy, &d.syy, &d.eps_p, &d.avx, &d.avy};
+                    &d.drho, &d.ax, &d.ay, &d.dvof, &d.sxx, &d.sxy, &d.syy, &d.eps_p, &d.avx, &d.avy,
+                    &d.sxx_min, &d.sxy_min, &d.syy_min, &d.p_min};
     for (double** a : f) RC(pd(*a));
     RC(launch(c, "iota", k_iota, nblk(b.n), TPB, d.order, b.n));
     return SPHG_OK;
@@ -303,6 +305,7 @@ void set_kparams(sphg_ctx* c)
     K.mc_ccos = P.soil_cohesion * std::cos(P.soil_phi);
     K.mc_sin = std::sin(P.soil_phi);
     K.mc_pcut = -P.soil_cohesion * std::pow(std::tan(P.soil_phi), -1.0);
+    K.stress_integration = P.stress_integration;
     K.ip1 = small_int(P.p1);
     K.ip2 = small_int(P.p2);
     K.delta = P.delta; K.epsilon = P.epsilon;
@@ -549,6 +552,18 @@ int upload(sphg_ctx* c, int k, uint64_t m)
         RC(up_scalar(c, b, v.rho_min, d.rho_min));
         RC(up_vec(c, b, v.v_min, d.vx_min, d.vy_min));
         RC(up_scalar(c, b, v.vof_min, d.vof_min));
+        if (b.d.material == SPHG_SOIL) {
+            RC(up_scalar(c, b, v.sxx_min, d.sxx_min));
+            RC(up_scalar(c, b, v.sxy_min, d.sxy_min));
+            RC(up_scalar(c, b, v.syy_min, d.syy_min));
+            RC(up_scalar(c, b, v.p_min, d.p_min));
+        }
+    }
+    if ((m & SPHG_F_DSTRESS) && b.d.material == SPHG_SOIL) {
+        RC(up_scalar(c, b, v.dsxx2, d.dsxx2));
+        RC(up_scalar(c, b, v.dsxy2, d.dsxy2));
+        RC(up_scalar(c, b, v.dsyy2, d.dsyy2));
+        RC(up_scalar(c, b, v.dp2, d.dp2));
     }
     return SPHG_OK;
 }
@@ -596,6 +611,18 @@ int download(sphg_ctx* c, int k, uint64_t m)
         RC(down_scalar(c, b, v.rho_min, d.rho_min));
         RC(down_vec(c, b, v.v_min, d.vx_min, d.vy_min));
         RC(down_scalar(c, b, v.vof_min, d.vof_min));
+        if (b.d.material == SPHG_SOIL) {
+            RC(down_scalar(c, b, v.sxx_min, d.sxx_min));
+            RC(down_scalar(c, b, v.sxy_min, d.sxy_min));
+            RC(down_scalar(c, b, v.syy_min, d.syy_min));
+            RC(down_scalar(c, b, v.p_min, d.p_min));
+        }
+    }
+    if ((m & SPHG_F_DSTRESS) && b.d.material == SPHG_SOIL) {
+        RC(down_scalar(c, b, v.dsxx2, d.dsxx2));
+        RC(down_scalar(c, b, v.dsxy2, d.dsxy2));
+        RC(down_scalar(c, b, v.dsyy2, d.dsyy2));
+        RC(down_scalar(c, b, v.dp2, d.dp2));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // Water stress is sigma = mu * t (newtonian_fluid, m_particles.f90:2496-2498).
@@ -644,7 +671,8 @@ int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
                        &d.drho, &d.ax, &d.ay, &d.avx, &d.avy, &d.vx_min, &d.vy_min,
                        &d.txx, &d.txy, &d.tyy, &d.vcc, &d.wxy, &d.sxx, &d.sxy, &d.syy, &d.eps_p,
                        &d.dsxx, &d.dsxy, &d.dsyy, &d.dp, &d.repx, &d.repy, &d.drgx, &d.drgy,
-                       &d.porx, &d.pory};
+                       &d.porx, &d.pory, &d.sxx_min, &d.sxy_min, &d.syy_min, &d.p_min,
+                       &d.dsxx2, &d.dsxy2, &d.dsyy2, &d.dp2};
     for (double** p : dbls) {
         RC(dalloc(c, b, p, n));
         RC(fill_dbl(c, *p, b.n, 0.0));
@@ -898,13 +926,18 @@ int rhs_water(sphg_ctx* c)
 int check_fused_switches(const sphg_ctx* c)
 {
     const sphg_params& P = c->P;
-    if (P.summation_density || P.stress_integration != 1) return SPHG_ERR_UNSUPPORTED;
+    if (P.summation_density || (P.stress_integration != 1 && P.stress_integration != 2))
+        return SPHG_ERR_UNSUPPORTED;
+    // stress_integration = 2 and plasticity = 2 (return mapping) in the FP64 fused steps
+    if ((P.stress_integration == 2 || P.plasticity == 2) && (P.precision != SPHG_FP64 || c->legacy))
+        return SPHG_ERR_UNSUPPORTED;
     // the Verlet scheme is the reference's single-phase water integrator
     if (P.integrate_scheme == 2 && !P.single_phase) return SPHG_ERR_UNSUPPORTED;
     if (soil_only(c)) {
         // time_integration_for_soil: leapfrog only; Drucker-Prager with flow rule 3
         if (P.integrate_scheme == 2 || P.precision != SPHG_FP64) return SPHG_ERR_UNSUPPORTED;
-        if ((P.yield_criterion != 1 && P.yield_criterion != 2) || P.plasticity == 2) return SPHG_ERR_UNSUPPORTED;
+        if ((P.yield_criterion != 1 && P.yield_criterion != 2) || P.plasticity < 0 || P.plasticity > 3)
+            return SPHG_ERR_UNSUPPORTED;
         return SPHG_OK;
     }
     // the mixed-precision form exists for the fused two-phase step with the
@@ -919,7 +952,7 @@ int check_fused_switches(const sphg_ctx* c)
             return SPHG_ERR_UNSUPPORTED;
         // plastic_flow_rule (plasticity 1) in the FP64 fused step only
         if (P.plasticity != 3 && P.plasticity != 0 &&
-            !(P.plasticity == 1 && P.precision == SPHG_FP64 && !c->legacy))
+            !((P.plasticity == 1 || P.plasticity == 2) && P.precision == SPHG_FP64 && !c->legacy))
             return SPHG_ERR_UNSUPPORTED;
     }
     if (!c->ph[0].bound) return SPHG_ERR_INVALID;

@@ -46,6 +46,8 @@ type, bind(C) :: sphg_phase_view_t
    type(c_ptr) :: sxx, syy, sxy, txx, tyy, txy, wxy, vcc, epsilon_p
    type(c_ptr) :: dsxx, dsyy, dsxy, dp
    type(c_ptr) :: w, dwdx
+   type(c_ptr) :: sxx_min, sxy_min, syy_min, p_min   ! NULL here: the device keeps them
+   type(c_ptr) :: dsxx2, dsxy2, dsyy2, dp2
 end type
 
 ! The per-step subset of the reference's type(particles) (m_particles.f90:62-384).
@@ -208,6 +210,8 @@ subroutine make_view(pl)
    v%wxy = c_loc(pl%wxy); v%vcc = c_loc(pl%vcc); v%epsilon_p = c_loc(pl%epsilon_p)
    v%dsxx = c_loc(pl%dsxx); v%dsyy = c_loc(pl%dsyy); v%dsxy = c_loc(pl%dsxy); v%dp = c_loc(pl%dp)
    v%w = c_loc(pl%w); v%dwdx = c_loc(pl%dwdx)
+   v%sxx_min = c_null_ptr; v%sxy_min = c_null_ptr; v%syy_min = c_null_ptr; v%p_min = c_null_ptr
+   v%dsxx2 = c_null_ptr; v%dsxy2 = c_null_ptr; v%dsyy2 = c_null_ptr; v%dp2 = c_null_ptr
    pl%view = v
 end subroutine
 

@@ -339,6 +339,39 @@ def test_plastic_flow_rule_1():
     s.close()
 
 
+@pytest.mark.parametrize("si,plasticity", [(2, 3), (1, 2), (2, 2)])
+def test_stress_integration_and_return_mapping(si, plasticity):
+    """stress_integration = 2 (leapfrog stress levels) and plasticity = 2
+    (plastic_flow_rule2 + plastic_or_not + return_mapping after the update),
+    two-phase; and the same switches on the soil-only path (whose integrator
+    has no return mapping)."""
+    def make():
+        sc = _column()
+        sc.params.stress_integration = si
+        sc.params.plasticity = plasticity
+        return sc
+    sc, ref = _both(make)
+    s = Solver(sc.params, sc.water, sc.soil)
+    for nsteps, done, tol in ((1, 0, FP64_TOL_1STEP), (5, 1, 1e-10)):
+        print(f"si={si} plasticity={plasticity} steps {done}->{nsteps}", flush=True)
+        s.time_integration(nsteps - done, sync=True)
+        oracle.time_integration(ref.params, ref.water, ref.soil, done, nsteps - done)
+        assert_fields(sc.water, ref.water, tol, label="water")
+        es = assert_fields(sc.soil, ref.soil, tol, label="soil")
+        es2 = field_errors(sc.soil, ref.soil, ("sxx_min", "p_min", "dsxx2", "dp2"))
+        assert max(es2.values()) <= tol, es2
+        print(nsteps, "soil", max(es.values()), es2)
+    s.close()
+    sc, ref = _both(lambda: scenarios.soil_collapse(dx=0.005, jitter=0.05, seed=9, stress_integration=si,
+                                                    plasticity=plasticity))
+    s = Solver(sc.params, None, sc.soil)
+    s.time_integration(4, sync=True)
+    oracle.time_integration(ref.params, None, ref.soil, 0, 4)
+    es = assert_fields(sc.soil, ref.soil, 1e-11, label="soil only")
+    print("soil only", max(es.values()))
+    s.close()
+
+
 def test_two_phase_without_delta_and_xsph():
     """artificial_density and average_velocity off: no delta-SPH on rho, and
     av keeps its value (the reference does not touch it)."""

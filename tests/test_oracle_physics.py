@@ -325,3 +325,78 @@ def test_plastic_flow_rule_is_rule3_at_every_point():
     k = a.dsxx[moved] / a.sxx[moved]
     np.testing.assert_allclose(a.dsxy[moved] / a.sxy[moved], k, rtol=1e-12)
     assert np.all(a.fail == 0)   # rule 1 leaves the flags alone
+
+
+def test_plastic_flow_rule2_rates():
+    # plastic_flow_rule2 (m_particles.f90:3636-3697): rule 1's increments as
+    # return-mapping rates, at every point and without the sign test
+    from sources_amd.state import default_params
+    P = default_params(soil_phi=math.radians(30.0))
+    _, ph = lattice_phase(n=10, material=abi.SPHG_SOIL)
+    rng = np.random.default_rng(21)
+    n = ph.maxn
+    for f, sc in (("sxx", 200), ("syy", 200), ("sxy", 100), ("txx", 1), ("tyy", 1), ("txy", 1), ("vcc", 1)):
+        getattr(ph, f)[:] = rng.normal(scale=sc, size=n)
+    one = ph.copy()
+    for q in (ph, one):
+        for f in ("dsxx", "dsxy", "dsyy", "dp", "epsilon_p", "dsxx2", "dsxy2", "dsyy2", "dp2"):
+            getattr(q, f)[:] = 0.0
+    oracle.call_phase("plastic_flow_rule2", P, ph)
+    oracle.call_phase("plastic_flow_rule", P, one)
+    pos = one.dp != 0                      # dlambda >= 0 (rule 1 acted)
+    assert 10 < pos.sum() < n               # both signs occur
+    for a, b in (("dsxx2", "dsxx"), ("dsxy2", "dsxy"), ("dsyy2", "dsyy"), ("dp2", "dp")):
+        assert np.array_equal(getattr(ph, a)[pos], getattr(one, b)[pos]), a
+    assert np.all(ph.dp2[~pos] < 0)        # negative dlambda kept
+    assert np.all(ph.dsxx == 0) and np.all(ph.epsilon_p == 0)
+
+
+@pytest.mark.parametrize("si", [1, 2])
+def test_stress_levels_under_constant_rates(si):
+    # first_half / second_half / first_step / return_mapping (declarations_sph.f90:643-789)
+    # with constant rates r: stress_integration 1 leaves the stress alone at the
+    # half step and adds dt r at the end; 2 adds dt/2 r at the half step (saving
+    # the level) and ends at level + dt r; first_step adds dt r (1) or dt/2 r (2).
+    from sources_amd.state import Phase, default_params
+    dt = 1e-3
+    P = default_params(dt=dt, stress_integration=si, summation_density=0)
+    ph = Phase(abi.SPHG_SOIL, 6, 2, max_interaction=10, dspp=0.005)
+    rng = np.random.default_rng(4)
+    n = ph.n
+    s0 = {f: rng.normal(size=n) * 100 for f in ("sxx", "sxy", "syy", "p")}
+    r = {f: rng.normal(size=n) for f in ("dsxx", "dsxy", "dsyy", "dp")}
+    pairs = (("sxx", "dsxx"), ("sxy", "dsxy"), ("syy", "dsyy"), ("p", "dp"))
+    ph.itype[:6], ph.itype[6:] = 3, -3
+    for f, v in s0.items():
+        getattr(ph, f)[:] = v
+    for f, v in r.items():
+        getattr(ph, f)[:] = v
+    oracle.call_phase("first_half", P, ph)
+    for s, d in pairs:
+        half = s0[s] + (dt / 2.0) * r[d] if si == 2 else s0[s]
+        np.testing.assert_array_equal(getattr(ph, s), half)
+        if si == 2:
+            np.testing.assert_array_equal(getattr(ph, s + "_min"), s0[s])
+    oracle.call_phase("second_half", P, ph)
+    for s, d in pairs:
+        np.testing.assert_allclose(getattr(ph, s), s0[s] + dt * r[d], rtol=1e-13, atol=1e-13)
+    # first step from s0
+    for f, v in s0.items():
+        getattr(ph, f)[:] = v
+    oracle.call_phase("first_step", P, ph)
+    for s, d in pairs:
+        np.testing.assert_allclose(getattr(ph, s), s0[s] + (dt if si == 1 else dt / 2.0) * r[d], rtol=1e-13)
+    # return mapping of the flagged points
+    for f, v in s0.items():
+        getattr(ph, f)[:] = v
+    for f in ("dsxx2", "dsxy2", "dsyy2", "dp2"):
+        getattr(ph, f)[:] = rng.normal(size=n)
+    mins = {s: getattr(ph, s + "_min").copy() for s in ("sxx", "sxy", "syy", "p")}
+    ph.fail[:] = [1, 0, 1, 0, 1, 0, 1, 0]
+    oracle.call_phase("return_mapping", P, ph)
+    f1 = ph.fail[:n] == 1
+    for s, d in pairs:
+        got = getattr(ph, s)
+        exp = s0[s] + dt * getattr(ph, d + "2") if si == 1 else mins[s] + dt * r[d]
+        np.testing.assert_allclose(got[f1], exp[f1], rtol=1e-13)
+        np.testing.assert_array_equal(got[~f1], s0[s][~f1])
