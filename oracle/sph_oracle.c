@@ -886,7 +886,8 @@ static int phase_rhs(const sphg_params *P, sphg_phase_view *pl, int itimestep)
 int oracle_single_step(const sphg_params *P, sphg_phase_view *wa, sphg_phase_view *so, int itimestep)
 {
     int rc, i;
-    if (P->summation_density) return SPHG_ERR_UNSUPPORTED;
+    /* summation_density: single_step keeps con_density (its sum_density call
+       is commented out, declarations_sph.f90:1190-1199) */
     rc = phase_rhs(P, wa, itimestep);
     if (rc) return rc;
     if (P->single_phase) return SPHG_OK;
@@ -1022,8 +1023,8 @@ int oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int i
 }
 
 /* ------------------------------------------------------------------------ */
-/* time_integration, declarations_sph.f90:525-791 (summation_density =       */
-/* .false.).  first_half / first_step / second_half / return_mapping are the */
+/* time_integration, declarations_sph.f90:525-791.  first_half / first_step */
+/* / second_half / return_mapping are the                                    */
 /* routines contained there (:643-789).  Pinned by the free-fall recurrence  */
 /* of an isolated particle and by closed forms of the stress levels under    */
 /* constant rates, both stress_integration values and the return mapping     */
@@ -1034,8 +1035,10 @@ void oracle_first_half(const sphg_params *P, sphg_phase_view *pl)
     int n = pl->ntotal + pl->nvirt, i, d;
     double dt = P->dt;
     for (i = 0; i < n; i++) {
-        pl->rho_min[i] = pl->rho[i];
-        pl->rho[i] = pl->rho[i] + (dt / 2.0) * pl->drho[i];
+        if (!P->summation_density) {
+            pl->rho_min[i] = pl->rho[i];
+            pl->rho[i] = pl->rho[i] + (dt / 2.0) * pl->drho[i];
+        }
         if (pl->material == SPHG_WATER && P->volume_fraction) {
             pl->vof_min[i] = pl->vof[i];
             pl->vof[i] = pl->vof[i] + (dt / 2.0) * pl->dvof[i];
@@ -1063,7 +1066,7 @@ void oracle_first_step(const sphg_params *P, sphg_phase_view *pl)
     int n = pl->ntotal + pl->nvirt, i, d;
     double dt = P->dt;
     for (i = 0; i < n; i++) {
-        pl->rho[i] = pl->rho[i] + (dt / 2.0) * pl->drho[i];
+        if (!P->summation_density) pl->rho[i] = pl->rho[i] + (dt / 2.0) * pl->drho[i];
         if (pl->material == SPHG_WATER && P->volume_fraction)
             pl->vof[i] = pl->vof[i] + (dt / 2.0) * pl->dvof[i];
         if (pl->material == SPHG_SOIL && P->stress_integration == 2) {   /* :700-705 */
@@ -1090,7 +1093,7 @@ void oracle_second_half(const sphg_params *P, sphg_phase_view *pl)
     int n = pl->ntotal + pl->nvirt, i, d;
     double dt = P->dt;
     for (i = 0; i < n; i++) {
-        pl->rho[i] = pl->rho_min[i] + dt * pl->drho[i];
+        if (!P->summation_density) pl->rho[i] = pl->rho_min[i] + dt * pl->drho[i];
         if (pl->material == SPHG_WATER && P->volume_fraction)
             pl->vof[i] = pl->vof_min[i] + dt * pl->dvof[i];
         if (pl->material == SPHG_SOIL && P->stress_integration == 2) {   /* :744-749 */
@@ -1114,8 +1117,7 @@ void oracle_second_half(const sphg_params *P, sphg_phase_view *pl)
 
 static int check_switches(const sphg_params *P)
 {
-    if (P->summation_density || (P->stress_integration != 1 && P->stress_integration != 2))
-        return SPHG_ERR_UNSUPPORTED;
+    if (P->stress_integration != 1 && P->stress_integration != 2) return SPHG_ERR_UNSUPPORTED;
     return SPHG_OK;
 }
 
@@ -1203,13 +1205,14 @@ int oracle_time_integration_for_water(const sphg_params *P, sphg_phase_view *wa,
 {
     int it, i, d, rc;
     double dt = P->dt;
-    if (P->summation_density) return SPHG_ERR_UNSUPPORTED;
     for (it = 0; it < nsteps; it++) {
         int step = ++(*itimestep), n = wa->ntotal + wa->nvirt;
         if (step != 1) {
             for (i = 0; i < n; i++) {
-                wa->rho_min[i] = wa->rho[i];
-                wa->rho[i] = wa->rho[i] + (dt / 2.0) * wa->drho[i];
+                if (!P->summation_density) {   /* :814-817 */
+                    wa->rho_min[i] = wa->rho[i];
+                    wa->rho[i] = wa->rho[i] + (dt / 2.0) * wa->drho[i];
+                }
                 if (wa->itype[i] < 0) continue;
                 for (d = 0; d < 2; d++) {
                     wa->v_min[2 * i + d] = wa->vx[2 * i + d];
@@ -1220,7 +1223,8 @@ int oracle_time_integration_for_water(const sphg_params *P, sphg_phase_view *wa,
         rc = oracle_single_step_for_water(P, wa, step);
         if (rc) return rc;
         for (i = 0; i < n; i++) {
-            if (step == 1) wa->rho[i] = wa->rho[i] + (dt / 2.0) * wa->drho[i];
+            if (P->summation_density) { /* rho left alone, :835-856 */ }
+            else if (step == 1) wa->rho[i] = wa->rho[i] + (dt / 2.0) * wa->drho[i];
             else wa->rho[i] = wa->rho_min[i] + dt * wa->drho[i];
             if (wa->itype[i] < 0) continue;
             for (d = 0; d < 2; d++) {

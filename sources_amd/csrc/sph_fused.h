@@ -293,9 +293,11 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
     const int it = P.itype[i];
     if (do_half) {
         const double h = K.dt / 2.0;
-        P.rho_min[i] = rho;
-        rho = rho + h * P.drho[i];
-        P.rho[i] = rho;
+        if (!K.summation_density) {   // first_half's rho update (declarations_sph.f90:647-650)
+            P.rho_min[i] = rho;
+            rho = rho + h * P.drho[i];
+            P.rho[i] = rho;
+        }
         if (K.volume_fraction && K.two_phase) {
             P.vof_min[i] = vof;
             vof = vof + h * P.dvof[i];
@@ -348,9 +350,11 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
     const int it = P.itype[i];
     if (do_half) {
         const double h = K.dt / 2.0;
-        P.rho_min[i] = rho;
-        rho = rho + h * P.drho[i];
-        P.rho[i] = rho;
+        if (!K.summation_density) {   // first_half's rho update (declarations_sph.f90:647-650)
+            P.rho_min[i] = rho;
+            rho = rho + h * P.drho[i];
+            P.rho[i] = rho;
+        }
         if (it > 0) {
             P.vx_min[i] = vx;
             P.vy_min[i] = vy;

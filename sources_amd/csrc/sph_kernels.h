@@ -46,6 +46,7 @@ struct KParams {
     int plasticity, yield_criterion, two_phase;
     int ip1, ip2;               // p1, p2 when they are integers in [0, 64], else -1
     int stress_integration;     // 1: sigma += dt dsigma; 2: leapfrog levels sigma_min
+    int summation_density;      // the leapfrog leaves rho alone (declarations_sph.f90:647-650)
     // Mohr-Coulomb criterion (yield_criterion = 1): cohesion cos(phi), sin(phi)
     // and the tension cut-off pressure -cohesion tan(phi)**(-1.0), host-evaluated
     double mc_ccos, mc_sin, mc_pcut;
@@ -372,9 +373,11 @@ __global__ void k_first_half_water(DPhase P, KParams K)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const double h = K.dt / 2.0;
-    const double r = P.rho[i];
-    P.rho_min[i] = r;
-    P.rho[i] = r + h * P.drho[i];
+    if (!K.summation_density) {
+        const double r = P.rho[i];
+        P.rho_min[i] = r;
+        P.rho[i] = r + h * P.drho[i];
+    }
     if (K.volume_fraction && K.two_phase) {
         const double v = P.vof[i];
         P.vof_min[i] = v;
@@ -397,9 +400,11 @@ __global__ void k_first_half_soil(DPhase P, KParams K, int do_half, int ndp)
     if (i >= P.n) return;
     if (do_half) {
         const double h = K.dt / 2.0;
-        const double r = P.rho[i];
-        P.rho_min[i] = r;
-        P.rho[i] = r + h * P.drho[i];
+        if (!K.summation_density) {
+            const double r = P.rho[i];
+            P.rho_min[i] = r;
+            P.rho[i] = r + h * P.drho[i];
+        }
         if (P.itype[i] > 0) {
             const double vx = P.vx[i], vy = P.vy[i];
             P.vx_min[i] = vx;
@@ -425,10 +430,10 @@ __global__ void k_final_water(DPhase P, KParams K, int first, int with_vof)
     if (i >= P.n) return;
     const double dt = K.dt, h = dt / 2.0;
     if (first) {
-        P.rho[i] = P.rho[i] + h * P.drho[i];
+        if (!K.summation_density) P.rho[i] = P.rho[i] + h * P.drho[i];
         if (with_vof) P.vof[i] = P.vof[i] + h * P.dvof[i];
     } else {
-        P.rho[i] = P.rho_min[i] + dt * P.drho[i];
+        if (!K.summation_density) P.rho[i] = P.rho_min[i] + dt * P.drho[i];
         if (with_vof) P.vof[i] = P.vof_min[i] + dt * P.dvof[i];
     }
     if (P.itype[i] < 0) return;
@@ -496,8 +501,10 @@ __global__ void k_final_soil(DPhase P, KParams K, int first)
     if (i >= P.n) return;
     const double dt = K.dt, h = dt / 2.0;
     if (K.volume_fraction && K.two_phase) P.vof[i] = P.rho[i] / K.s_rho0;
-    if (first) P.rho[i] = P.rho[i] + h * P.drho[i];
-    else P.rho[i] = P.rho_min[i] + dt * P.drho[i];
+    if (!K.summation_density) {
+        if (first) P.rho[i] = P.rho[i] + h * P.drho[i];
+        else P.rho[i] = P.rho_min[i] + dt * P.drho[i];
+    }
     double sxx, sxy, syy, p;
     if (K.stress_integration == 2) {   // :700-705, :744-749
         if (first) {

@@ -306,6 +306,7 @@ void set_kparams(sphg_ctx* c)
     K.mc_sin = std::sin(P.soil_phi);
     K.mc_pcut = -P.soil_cohesion * std::pow(std::tan(P.soil_phi), -1.0);
     K.stress_integration = P.stress_integration;
+    K.summation_density = P.summation_density;
     K.ip1 = small_int(P.p1);
     K.ip2 = small_int(P.p2);
     K.delta = P.delta; K.epsilon = P.epsilon;
@@ -926,8 +927,11 @@ int rhs_water(sphg_ctx* c)
 int check_fused_switches(const sphg_ctx* c)
 {
     const sphg_params& P = c->P;
-    if (P.summation_density || (P.stress_integration != 1 && P.stress_integration != 2))
-        return SPHG_ERR_UNSUPPORTED;
+    if (P.stress_integration != 1 && P.stress_integration != 2) return SPHG_ERR_UNSUPPORTED;
+    // summation_density: the two-phase single_step keeps con_density (its
+    // sum_density call is commented out, declarations_sph.f90:1190-1199) and
+    // the leapfrog leaves rho alone; single_step_for_soil would call sum_density
+    if (P.summation_density && soil_only(c)) return SPHG_ERR_UNSUPPORTED;
     // stress_integration = 2 and plasticity = 2 (return mapping) in the FP64 fused steps
     if ((P.stress_integration == 2 || P.plasticity == 2) && (P.precision != SPHG_FP64 || c->legacy))
         return SPHG_ERR_UNSUPPORTED;

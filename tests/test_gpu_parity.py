@@ -372,6 +372,34 @@ def test_stress_integration_and_return_mapping(si, plasticity):
     s.close()
 
 
+def test_summation_density_switch():
+    """summation_density = 1: two-phase single_step keeps con_density and the
+    leapfrog leaves rho alone; the single-phase water integrator likewise
+    (its right-hand side always uses summation density)."""
+    def make():
+        sc = _column()
+        sc.params.summation_density = 1
+        return sc
+    sc, ref = _both(make)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(4, sync=True)
+    oracle.time_integration(ref.params, ref.water, ref.soil, 0, 4)
+    assert_fields(sc.water, ref.water, 1e-11, label="water")
+    assert_fields(sc.soil, ref.soil, 1e-11, label="soil")
+    s.close()
+    make1 = lambda: scenarios.dam_break(dx=0.01, skf=1, jitter=0.05, seed=5, summation_density=1)  # noqa: E731
+    sc, ref = _both(make1)
+    s = Solver(sc.params, sc.water)
+    s.time_integration(5, sync=True)
+    oracle.time_integration(ref.params, ref.water, None, 0, 5)
+    fields = ("x", "vx", "rho", "p", "dvx", "av")
+    errs = field_errors(sc.water, ref.water, fields)
+    sens = _sensitivity(make1, 5, fields)
+    bad = {f: (errs[f], sens[f]) for f in fields if not (errs[f] <= max(1e-12, 16.0 * sens[f]))}
+    assert not bad, bad
+    s.close()
+
+
 def test_two_phase_without_delta_and_xsph():
     """artificial_density and average_velocity off: no delta-SPH on rho, and
     av keeps its value (the reference does not touch it)."""

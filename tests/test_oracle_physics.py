@@ -400,3 +400,31 @@ def test_stress_levels_under_constant_rates(si):
         exp = s0[s] + dt * getattr(ph, d + "2") if si == 1 else mins[s] + dt * r[d]
         np.testing.assert_allclose(got[f1], exp[f1], rtol=1e-13)
         np.testing.assert_array_equal(got[~f1], s0[s][~f1])
+
+
+@pytest.mark.parametrize("summation", [0, 1])
+def test_integrators_and_summation_density(summation):
+    # with summation_density the leapfrog leaves rho (and rho_min) alone
+    # (declarations_sph.f90:647-650,688-690,733-735); otherwise rho_min saves
+    # the level and rho advances by dt/2 and dt drho
+    from sources_amd.state import Phase, default_params
+    dt = 1e-3
+    P = default_params(dt=dt, summation_density=summation)
+    ph = Phase(abi.SPHG_WATER, 5, 1, max_interaction=10, dspp=0.005)
+    rng = np.random.default_rng(8)
+    r0 = rng.uniform(990, 1010, ph.n)
+    q = rng.normal(size=ph.n)
+    ph.rho[:] = r0
+    ph.drho[:] = q
+    ph.rho_min[:] = -1.0
+    oracle.call_phase("first_half", P, ph)
+    if summation:
+        assert np.array_equal(ph.rho, r0) and np.all(ph.rho_min == -1.0)
+    else:
+        assert np.array_equal(ph.rho_min, r0)
+        np.testing.assert_array_equal(ph.rho, r0 + (dt / 2.0) * q)
+    oracle.call_phase("second_half", P, ph)
+    np.testing.assert_allclose(ph.rho, r0 if summation else r0 + dt * q, rtol=1e-15)
+    ph.rho[:] = r0
+    oracle.call_phase("first_step", P, ph)
+    np.testing.assert_allclose(ph.rho, r0 if summation else r0 + (dt / 2.0) * q, rtol=1e-15)
