@@ -54,8 +54,7 @@ struct PhaseBuf {
     double* stage = nullptr;
     // fused path: cell-sorted neighbour records and same-phase masks
     double* rec = nullptr;
-    Masks masks{};               // same-phase partners
-    Masks xmasks{};              // partners in the other phase
+    Masks masks{};               // same-phase partners of water stage A, reused by stage B
     int* spare_i = nullptr;      // permutation scratch
     double* spare_d = nullptr;
     double hmin = 0.0, hmax = 0.0;
@@ -684,10 +683,10 @@ int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
     RC(fill_int(c, d.countiac, b.n, 0));
     // fused-path buffers
     RC(dalloc(c, b, &b.rec, (size_t)R_STRIDE * n));
-    RC(dalloc(c, b, &b.masks.m, 3 * n));
-    RC(dalloc(c, b, &b.masks.ovf, n));
-    RC(dalloc(c, b, &b.xmasks.m, 3 * n));
-    RC(dalloc(c, b, &b.xmasks.ovf, n));
+    if (k == 0) {   // water; soil runs both stages in one pass and stores no masks
+        RC(dalloc(c, b, &b.masks.m, 3 * n));
+        RC(dalloc(c, b, &b.masks.ovf, n));
+    }
     b.hmin = b.hmax = v->hsml[0];
     for (int i = 1; i < b.n; ++i) {
         b.hmin = std::fmin(b.hmin, v->hsml[i]);
