@@ -115,7 +115,7 @@ struct Graph {
     int n;
     const float* nf;   // [n][FN]
     const float* raw;  // [n][RAW]
-    const int* nb;     // [n][DEG]
+    const int* nb;     // [DEG][n]: slot-major, so a wave's 32 nodes read one slot contiguously
     const int* nd;     // [n]
 };
 
@@ -201,11 +201,13 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
 #pragma unroll
     for (int q = 0; q < 8; ++q) w1[q] = (__bf16)W[O_W1 + (h * 8 + q) * HE + p];
 
-    auto edge_operand = [&](int s) {
-        const int j = (s < deg) ? A.G.nb[(size_t)i * DEG + s] : -1;
-        float rj[RAW];
+    // neighbour index of slot s (slot-major table), -1 past the degree
+    auto nbr = [&](int s) { return (s < deg) ? A.G.nb[(size_t)s * A.G.n + i] : -1; };
+    auto load_raw = [&](int j, float* rj) {
 #pragma unroll
         for (int f = 0; f < RAW; ++f) rj[f] = (j >= 0) ? A.G.raw[(size_t)j * RAW + f] : 0.0f;
+    };
+    auto edge_from = [&](int j, const float* rj) {
         float e[FE];
         edge_features(ri, rj, j >= 0, e);
         bf16x8 av;
@@ -213,12 +215,20 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
         for (int q = 0; q < 8; ++q) av[q] = (__bf16)(h ? e[8 + q] : e[q]);
         return av;
     };
+    auto edge_operand = [&](int j) {
+        float rj[RAW];
+        load_raw(j, rj);
+        return edge_from(j, rj);
+    };
 
     // ---- edge layer and sum over neighbours
     f32x16 hsum = zero16();
+    int jn = nbr(0);
 #pragma unroll 1
     for (int s = 0; s < maxdeg; ++s) {
-        const bf16x8 av = edge_operand(s);
+        const int j = jn;
+        jn = nbr(s + 1);   // next slot's index in flight during this slot
+        const bf16x8 av = edge_operand(j);
         f32x16 z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, w1, zero16(), 0, 0, 0);
 #pragma unroll
         for (int v = 0; v < 16; ++v) hsum[v] += fmaxf(z[v], 0.0f);
@@ -371,7 +381,7 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
         f32x16 g1 = zero16();
 #pragma unroll 1
         for (int s = 0; s < maxdeg; ++s) {
-            const bf16x8 av = edge_operand(s);
+            const bf16x8 av = edge_operand(nbr(s));
 #pragma unroll
             for (int q = 0; q < 8; ++q) E[p][h * 8 + q] = (float)av[q];
             f32x16 z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, w1, zero16(), 0, 0, 0);
@@ -415,6 +425,7 @@ struct GatherOut {
     float* raw;
     int* nb;
     int* nd;
+    int n_nodes;       // nodes of both phases: the slot stride of nb
     float* tw;         // [n]
     int* node_of_id;   // [n of the phase] (phase-local ids)
     int* overflow;     // [1]
@@ -431,7 +442,7 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
     const double rho0 = phase == 0 ? K.w_rho0 : K.s_rho0;
     const bool virt = P.itype[a] < 0;
     const double kdt = S.kdt;
-    float* nf = O.nf + (size_t)node * FN;
+    float nf[FN];
     nf[0] = 1.0f;
     nf[1] = (float)phase;
     nf[2] = virt ? 1.0f : 0.0f;
@@ -451,23 +462,21 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
         nf[10] = nf[11] = nf[12] = nf[13] = 0.0f;
     }
     nf[14] = nf[15] = 0.0f;
-    float* rw = O.raw + (size_t)node * RAW;
-    rw[0] = (float)x;
-    rw[1] = (float)y;
-    rw[2] = nf[5];
-    rw[3] = nf[6];
-    rw[4] = nf[7];
-    rw[5] = nf[3];
-    rw[6] = nf[4];
-    rw[7] = (float)h;
-    rw[8] = nf[2];
-    rw[9] = rw[10] = rw[11] = 0.0f;
+    // rows of 16 and 12 floats, written as 16-byte vectors
+    float4* nf4 = reinterpret_cast<float4*>(O.nf + (size_t)node * FN);
+#pragma unroll
+    for (int q = 0; q < FN / 4; ++q) nf4[q] = make_float4(nf[4 * q], nf[4 * q + 1], nf[4 * q + 2], nf[4 * q + 3]);
+    float4* rw4 = reinterpret_cast<float4*>(O.raw + (size_t)node * RAW);
+    rw4[0] = make_float4((float)x, (float)y, nf[5], nf[6]);
+    rw4[1] = make_float4(nf[7], nf[3], nf[4], (float)h);
+    rw4[2] = make_float4(nf[2], 0.0f, 0.0f, 0.0f);
     O.tw[node] = virt ? 0.0f : 1.0f;
     O.node_of_id[P.id[a]] = node;
     // neighbours (cell order of the same phase)
     const Rows R = block_rows(g, P.cell_start, x, y);
     int cnt = 0;
-    int* nb = O.nb + (size_t)node * DEG;
+    int* nb = O.nb + node;   // slot s at nb[s * n_nodes]
+    const size_t ns = (size_t)O.n_nodes;
 #pragma unroll 1
     for (int r = 0; r < 3; ++r) {
 #pragma unroll 1
@@ -480,14 +489,14 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
 #pragma clang fp contract(off)
                 dr = dx * dx + dy * dy;
             }
-            if (!pred(uni != 0, thr, dr, h, P.hsml[b], K)) continue;
-            if (cnt < DEG) nb[cnt] = off + k;
+            if (!pred(uni != 0, thr, dr, h, uni ? h : P.hsml[b], K)) continue;
+            if (cnt < DEG) nb[(size_t)cnt * ns] = off + k;
             ++cnt;
         }
     }
     if (cnt > DEG) atomicOr(O.overflow, 1);
     O.nd[node] = min(cnt, DEG);
-    for (int s = min(cnt, DEG); s < DEG; ++s) nb[s] = -1;
+    for (int s = min(cnt, DEG); s < DEG; ++s) nb[(size_t)s * ns] = -1;
 }
 
 // Targets: fine state (K plain steps, current) minus coarse state (one step

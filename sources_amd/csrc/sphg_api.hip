@@ -1633,7 +1633,7 @@ int build_graph(sphg_gnn* g, sphg_gnn::Graph& G)
     RC(fill_int(c, g->overflow, 1, 0));
     for (int k = 0; k < 2; ++k) {
         PhaseBuf& b = c->ph[k];
-        gnn::GatherOut O{G.nf, G.raw, G.nb, G.nd, G.tw, g->node_of_id[k], g->overflow};
+        gnn::GatherOut O{G.nf, G.raw, G.nb, G.nd, G.n, G.tw, g->node_of_id[k], g->overflow};
         RC(launch(c, "gnn_gather", gnn::k_gnn_gather, nblk(b.n), TPB, b.d, c->gl[k], c->K,
                   k == 0 ? c->F.thr_w : c->F.thr_s, k == 0 ? c->F.uni_w : c->F.uni_s, k == 0 ? 0 : Wb.n, k,
                   g->S, O));
@@ -1977,8 +1977,14 @@ int sphg_gnn_debug(sphg_gnn* g, int sample, float* nf, float* raw, int* nb, int*
     };
     RC(dl(nf, G.nf, n * gnn::FN * sizeof(float)));
     RC(dl(raw, G.raw, n * gnn::RAW * sizeof(float)));
-    RC(dl(nb, G.nb, n * gnn::DEG * sizeof(int)));
     RC(dl(nd, G.nd, n * sizeof(int)));
+    if (nb) {   // the device table is slot-major; the caller gets [n][DEG]
+        std::vector<int> t(n * gnn::DEG);
+        HIPCHK(c, hipMemcpyAsync(t.data(), G.nb, t.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < n; ++i)
+            for (int s = 0; s < gnn::DEG; ++s) nb[i * gnn::DEG + (size_t)s] = t[(size_t)s * n + i];
+    }
     RC(dl(tgt, G.tgt, n * gnn::NO * sizeof(float)));
     RC(dl(tw, G.tw, n * sizeof(float)));
     RC(dl(y, g->y, n * gnn::NO * sizeof(float)));
