@@ -937,7 +937,7 @@ int check_fused_switches(const sphg_ctx* c)
     // the Verlet scheme is the reference's single-phase water integrator
     if (P.integrate_scheme == 2 && !P.single_phase) return SPHG_ERR_UNSUPPORTED;
     if (soil_only(c)) {
-        // time_integration_for_soil: leapfrog only; Drucker-Prager with flow rule 3
+        // time_integration_for_soil: leapfrog, FP64; either criterion, flow rules 0-3
         if (P.integrate_scheme == 2 || P.precision != SPHG_FP64) return SPHG_ERR_UNSUPPORTED;
         if ((P.yield_criterion != 1 && P.yield_criterion != 2) || P.plasticity < 0 || P.plasticity > 3)
             return SPHG_ERR_UNSUPPORTED;

@@ -87,3 +87,18 @@ def test_mixed_rejects_single_phase():
         s.time_integration(1, sync=True)
     assert ei.value.code == abi.SPHG_ERR_UNSUPPORTED
     s.close()
+
+
+@pytest.mark.parametrize("switch,value", [("yield_criterion", 1), ("plasticity", 1), ("plasticity", 2),
+                                          ("stress_integration", 2)])
+def test_mixed_rejects_fp64_only_switches(switch, value):
+    # Mohr-Coulomb, flow rules 1/2 and stress_integration = 2 exist in the FP64
+    # fused step only: the mixed form refuses them instead of ignoring them
+    sc = scenarios.savage_column()
+    sc.params.precision = abi.SPHG_MIXED
+    setattr(sc.params, switch, value)
+    s = Solver(sc.params, sc.water, sc.soil)
+    with pytest.raises(abi.SphgError) as ei:
+        s.time_integration(1, sync=True)
+    assert ei.value.code == abi.SPHG_ERR_UNSUPPORTED
+    s.close()
