@@ -666,6 +666,14 @@ __device__ __forceinline__ CatMask cat_masks(const LaneRows& L, uint64_t m0, uin
     return c;
 }
 
+// Value select.  A conditional expression over two lvalues (struct fields)
+// is itself an lvalue: clang then selects between their addresses and loads
+// through the result, which keeps the struct on the stack and puts a
+// scratch load on every partner iteration.  Arguments taken by value make
+// it a register select.
+template <class T>
+__device__ __forceinline__ T pick(bool c, T x, T y) { return c ? x : y; }
+
 // Iterator over the set bits of a CatMask; branch-free apart from the loop.
 struct CatIter {
     uint64_t lo, hi;
@@ -675,15 +683,15 @@ struct CatIter {
     __device__ __forceinline__ int next(int& i)
     {
         const bool in_lo = lo != 0;
-        const uint64_t m = in_lo ? lo : hi;
+        const uint64_t m = pick(in_lo, lo, hi);
         if (m == 0) return -1;
         const int c = __builtin_ctzll(m) + (in_lo ? 0 : 64);
         const uint64_t rest = m & (m - 1);
-        lo = in_lo ? rest : lo;
-        hi = in_lo ? hi : rest;
+        lo = pick(in_lo, rest, lo);
+        hi = pick(in_lo, hi, rest);
         const bool a = c < s1, b = c < s2;
-        const int k = c + (a ? e0 : (b ? e1 : e2));
-        i = k + (a ? d0 : (b ? d1 : d2));
+        const int k = c + pick(a, e0, pick(b, e1, e2));
+        i = k + pick(a, d0, pick(b, d1, d2));
         return k;
     }
 };

@@ -23,7 +23,7 @@
 // faster than the compiler's default of four).
 #define STAGE_ATTR_MX __launch_bounds__(STAGE_BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)))
 #ifndef SPHG_MX_AB_WAVES
-#define SPHG_MX_AB_WAVES 4   // the one-pass soil kernel: 4 waves (a few spills) beat 3
+#define SPHG_MX_AB_WAVES 0   // the one-pass soil kernel: compiler's choice (no spills) beats 4 waves with spills
 #endif
 #if SPHG_MX_AB_WAVES > 0
 #define STAGE_ATTR_MX_AB __launch_bounds__(STAGE_BLOCK) __attribute__((amdgpu_waves_per_eu(SPHG_MX_AB_WAVES, 8)))
