@@ -40,3 +40,36 @@ def canonical_pairs(pi, pj):
     a = np.stack([np.minimum(pi, pj), np.maximum(pi, pj)], 1)
     order = np.lexsort((a[:, 1], a[:, 0]))
     return a[order]
+
+
+def crop_phase(ph, idx):
+    """The particles idx (sorted; real ones stay before virtual ones) of a
+    phase as a phase of their own: every per-particle array is taken at idx,
+    the pair arrays sized for the crop."""
+    from sources_amd.state import Phase
+    idx = np.asarray(idx, np.int64)
+    q = Phase.__new__(Phase)
+    m = ph.maxn
+    for k, v in ph.__dict__.items():
+        if k in ("pair_i", "pair_j", "w", "dwdx", "_view"):
+            continue
+        if isinstance(v, np.ndarray) and v.shape[:1] == (m,):
+            setattr(q, k, np.ascontiguousarray(v[idx]))
+        elif isinstance(v, np.ndarray):
+            setattr(q, k, v.copy())
+        else:
+            setattr(q, k, v)
+    q._view = None
+    q.ntotal = int(np.count_nonzero(idx < ph.ntotal))
+    q.nvirt = len(idx) - q.ntotal
+    q.maxn = max(len(idx), 1)
+    q.niac = 0
+    q.set_max_interaction(64 * max(len(idx), 1))
+    return q
+
+
+def window_indices(ph, lo, hi):
+    """Indices of the particles of ph inside the box [lo, hi) (sorted)."""
+    x = ph.x[:ph.n]
+    inside = (x[:, 0] >= lo[0]) & (x[:, 0] < hi[0]) & (x[:, 1] >= lo[1]) & (x[:, 1] < hi[1])
+    return np.nonzero(inside)[0]

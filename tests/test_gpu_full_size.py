@@ -1,0 +1,251 @@
+"""Parity at full size: the 1M-particle two-phase slope (dx = 0.001, ~563k
+soil and ~420k water particles plus walls) and the submerged column (soil
+0.2 x 0.1 m in a 0.8 x 0.3 m tank at dx = 0.0005, ~1.04M particles), run
+whole on the GPU, checked by the oracle on seeded windows cut from the same
+input.
+
+The oracle's pair search is O(N^2), so it cannot run the whole state.  Within
+one step a particle's result depends only on particles a few support radii
+away (stage B reads neighbours' stage-A results), so a window of the input
+with a margin of 10 h around it gives its interior particles exactly the
+inputs they see in the full run.  The CPU test of the window method
+(test_oracle_windows.py) shows that the oracle's interior results on a crop
+are bit-identical to its results on the whole state.  Here the GPU's full
+run is compared with the oracle's crops: pair sets (and their W, dW) bit
+for bit, FP64 fields after two steps within 1e-12 norm-wise over each
+window."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import STATE_FIELDS, canonical_pairs, crop_phase, field_errors, window_indices
+from sources_amd import abi, scenarios
+from sources_amd.sph import Solver
+from sources_amd.state import params_copy
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+NSTEPS = 2
+HALF = 10      # window half-width, in dx
+MARGIN = 10    # margin around the window, in h (6 h suffices for two steps)
+
+
+def _slope():
+    return scenarios.slope(max_interaction=0)
+
+
+def _column():
+    # soil column 0.2 x 0.1 m in a 0.8 x 0.3 m water tank at dx = 0.0005:
+    # ~80k soil and ~960k water particles plus walls
+    return scenarios.submerged_column(max_interaction=0)
+
+
+MAKERS = {"slope_1m": (_slope, 0.001), "submerged_column": (_column, 0.0005)}
+
+
+def _centres(sc, seed):
+    """Seeded window centres: two real soil particles, two real water
+    particles, one virtual (wall) particle."""
+    rng = np.random.default_rng(seed)
+    w, s = sc.water, sc.soil
+    return [s.x[rng.integers(s.ntotal)], s.x[rng.integers(s.ntotal)],
+            w.x[rng.integers(w.ntotal)], w.x[rng.integers(w.ntotal)],
+            s.x[s.ntotal + rng.integers(s.nvirt)]]
+
+
+def _crop(sc, c, a, m):
+    iw = window_indices(sc.water, c - a - m, c + a + m)
+    is_ = window_indices(sc.soil, c - a - m, c + a + m)
+    return iw, is_, crop_phase(sc.water, iw), crop_phase(sc.soil, is_)
+
+
+def _inner(ph, idx, c, a):
+    x = ph.x[idx]
+    return np.nonzero(np.all((x >= c - a) & (x < c + a), axis=1))[0]
+
+
+def _window_results(sc, inp, dx, nsteps, seed):
+    """The GPU's results (already in sc) and the oracle's on each window, per
+    phase: lists of (gpu, oracle) phases over the windows' interiors."""
+    h = float(max(np.max(inp.water.hsml[:inp.water.n]), np.max(inp.soil.hsml[:inp.soil.n])))
+    a, m = HALF * dx, MARGIN * h
+    out = {"water": [], "soil": []}
+    for k, c in enumerate(_centres(inp, seed)):
+        iw, is_, cw, cs = _crop(inp, c, a, m)
+        P = params_copy(inp.params)
+        P.precision = abi.SPHG_FP64
+        oracle.time_integration(P, cw, cs, 0, nsteps)
+        for name, full_in, full_out, idx, crop in (("water", inp.water, sc.water, iw, cw),
+                                                  ("soil", inp.soil, sc.soil, is_, cs)):
+            inner = _inner(full_in, idx, c, a)
+            if len(inner):
+                out[name].append((k, c, crop_phase(full_out, idx[inner]), crop_phase(crop, inner)))
+    return out
+
+
+def _union(parts, which):
+    """One phase holding the interiors of every window (gpu: which = 2,
+    oracle: which = 3), for norm-wise errors over all windows together."""
+    ph = parts[0][which]
+    q = ph.copy()
+    for f, v in ph.__dict__.items():
+        if isinstance(v, np.ndarray) and v.shape[:1] == (ph.maxn,):
+            setattr(q, f, np.concatenate([getattr(p[which], f) for p in parts]))
+    q.ntotal = sum(p[which].ntotal for p in parts)
+    q.nvirt = sum(p[which].nvirt for p in parts)
+    q.maxn = q.ntotal + q.nvirt
+    return q
+
+
+def _check_windows(res, tol_of, fields_of, label):
+    checked = 0
+    for name, parts in res.items():
+        if not parts:
+            continue
+        for k, c, got, ref in parts:
+            e = field_errors(got, ref, fields_of(ref.material))
+            worst = max(e, key=e.get)
+            print(f"  {label} window {k} at ({c[0]:.4f}, {c[1]:.4f}) {name}: {got.n} particles, "
+                  f"max err {e[worst]:.2e} ({worst})", flush=True)
+            checked += got.n
+        got, ref = _union(parts, 2), _union(parts, 3)
+        errs = field_errors(got, ref, fields_of(ref.material))
+        print(f"  {label} {name}, all windows: " + " ".join(f"{f}={v:.1e}" for f, v in errs.items()), flush=True)
+        bad = {f: v for f, v in errs.items() if not v <= tol_of(f)}
+        assert not bad, (label, name, bad)
+    assert checked > 1000
+
+
+@pytest.mark.parametrize("case", list(MAKERS))
+def test_full_size_steps_on_windows(case):
+    make, dx = MAKERS[case]
+    sc = make()
+    inp = sc.copy()
+    print(f"case={case} particles={sc.n_particles} steps={NSTEPS}", flush=True)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(NSTEPS, sync=True)
+    s.close()
+    res = _window_results(sc, inp, dx, NSTEPS, seed=2026)
+    _check_windows(res, lambda f: FP64_TOL, lambda mat: STATE_FIELDS[mat], case)
+    for parts in res.values():
+        for _, _, got, ref in parts:
+            np.testing.assert_array_equal(got.fail[:got.n], ref.fail[:ref.n])
+
+
+MIXED_STATE = {1: ("x", "vx", "rho", "p", "vof"), 2: ("x", "vx", "rho", "p", "sxx", "sxy", "syy")}
+MIXED_RATES = {1: ("drho", "dvx", "dvof"), 2: ("drho", "dvx", "dsxx", "dsxy", "dsyy", "dp")}
+
+
+def _mixed_tol(f):
+    # State: the small-case bounds, 2e-6 (velocity 1e-3: after one step it is
+    # dt/2 times a small residual acceleration; sxy starts at zero and is dt
+    # times its rate).  Rates: after one step from the hydrostatic / K0 state
+    # drho and the stress rates are small residuals of FP32 velocity
+    # differences (water drho ~2.5e-2 norm-wise while rho is within ~3e-9),
+    # so they are held to 0.1, a bound on gross errors only.
+    if f in ("drho", "dvx", "dvof", "dsxx", "dsxy", "dsyy", "dp"):
+        return 0.1
+    return 1e-3 if f in ("vx", "sxy") else 2e-6
+
+
+def test_full_size_mixed_slope_on_windows():
+    """The 1M slope in its own precision (FP32 Kahan pair sums, FP64
+    integration and projection): one step against the FP64 oracle."""
+    sc = _slope()
+    inp = sc.copy()
+    sc.params.precision = abi.SPHG_MIXED
+    print(f"slope_1m mixed particles={sc.n_particles}", flush=True)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(1, sync=True)
+    s.close()
+    res = _window_results(sc, inp, 0.001, 1, seed=5)
+    _check_windows(res, _mixed_tol, lambda mat: MIXED_STATE[mat] + MIXED_RATES[mat], "mixed")
+
+
+def _pairs_touching(pi, pj, w, dwdx, glob_idx, inner_glob):
+    """Pairs (1-based, as the pair search lists them) with at least one end
+    in inner_glob, ends mapped to window-local indices via glob_idx (sorted),
+    canonical order; W and dW in that order."""
+    pi, pj = pi - 1, pj - 1
+    sel = np.isin(pi, inner_glob) | np.isin(pj, inner_glob)
+    a, b = pi[sel], pj[sel]
+    la, lb = np.searchsorted(glob_idx, a), np.searchsorted(glob_idx, b)
+    assert np.array_equal(glob_idx[la], a) and np.array_equal(glob_idx[lb], b), "pair reaches outside the crop"
+    assert np.all(la < lb)   # both searches list i < j; the crop keeps the index order
+    order = np.lexsort((lb, la))
+    return canonical_pairs(la, lb), w[sel][order], dwdx[sel][order]
+
+
+@pytest.mark.parametrize("phase", ["water", "soil"])
+def test_full_size_pair_sets_on_windows(phase):
+    sc = scenarios.slope()
+    inp = sc.copy()
+    print(f"slope_1m phase={phase} particles={sc.n_particles}", flush=True)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.direct_find(phase)
+    g = getattr(sc, phase)
+    pi, pj, w, dwdx = g.pairs()
+    s.close()
+    assert g.niac > 0 and np.all(pi < pj)
+    # every pair once: countiac counts both ends
+    assert int(np.sum(g.countiac[:g.n], dtype=np.int64)) == 2 * g.niac
+    full_in = getattr(inp, phase)
+    h = float(np.max(full_in.hsml[:full_in.n]))
+    a, m = HALF * 0.001, 4 * h
+    checked = 0
+    for k, c in enumerate(_centres(inp, seed=7)):
+        idx = window_indices(full_in, c - a - m, c + a + m)
+        crop = crop_phase(full_in, idx)
+        oracle.call_phase("direct_find", params_copy(inp.params), crop)
+        inner = _inner(full_in, idx, c, a)
+        if len(inner) == 0:
+            continue
+        kg, wg, dg = _pairs_touching(pi, pj, w, dwdx, idx, idx[inner])
+        r = crop.pairs()
+        kr, wr, dr = _pairs_touching(r[0], r[1], r[2], r[3], np.arange(crop.n), inner)
+        print(f"  window {k}: {len(inner)} particles, {len(kg)} pairs", flush=True)
+        np.testing.assert_array_equal(kg, kr)
+        assert np.array_equal(wg, wr), np.max(np.abs(wg - wr))
+        assert np.array_equal(dg, dr), np.max(np.abs(dg - dr))
+        checked += len(kg)
+    assert checked > 1000
+
+
+def test_full_size_cross_pairs_on_windows():
+    """direct_find_2 (water i x soil j, listed in the water phase's pair
+    arrays) at 1M against the oracle's on the windows."""
+    sc = scenarios.slope()
+    inp = sc.copy()
+    print(f"slope_1m cross particles={sc.n_particles}", flush=True)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.direct_find_2()
+    pi, pj, w, dwdx = sc.water.pairs()
+    s.close()
+    k = sc.water.niac
+    assert k > 0
+    assert int(np.sum(sc.water.countiac[:sc.water.n], dtype=np.int64)) == k
+    assert int(np.sum(sc.soil.countiac[:sc.soil.n], dtype=np.int64)) == k
+    h = float(max(np.max(inp.water.hsml[:inp.water.n]), np.max(inp.soil.hsml[:inp.soil.n])))
+    a, m = HALF * 0.001, 4 * h
+    pi0, pj0 = pi - 1, pj - 1
+    checked = 0
+    for n, c in enumerate(_centres(inp, seed=11)):
+        iw, is_, cw, cs = _crop(inp, c, a, m)
+        oracle.call_cross("direct_find_2", params_copy(inp.params), cw, cs)
+        inw, ins = _inner(inp.water, iw, c, a), _inner(inp.soil, is_, c, a)
+        sel = np.isin(pi0, iw[inw]) | np.isin(pj0, is_[ins])
+        lw, ls = np.searchsorted(iw, pi0[sel]), np.searchsorted(is_, pj0[sel])
+        assert np.array_equal(iw[lw], pi0[sel]) and np.array_equal(is_[ls], pj0[sel]), "pair reaches outside the crop"
+        og = np.lexsort((ls, lw))
+        r = cw.pairs()
+        ri, rj = r[0] - 1, r[1] - 1
+        rs = np.isin(ri, inw) | np.isin(rj, ins)
+        orr = np.lexsort((rj[rs], ri[rs]))
+        print(f"  window {n}: {len(inw)} water + {len(ins)} soil particles, {int(sel.sum())} pairs", flush=True)
+        np.testing.assert_array_equal(lw[og], ri[rs][orr])
+        np.testing.assert_array_equal(ls[og], rj[rs][orr])
+        assert np.array_equal(w[sel][og], r[2][rs][orr])
+        assert np.array_equal(dwdx[sel][og], r[3][rs][orr])
+        checked += int(sel.sum())
+    assert checked > 1000
