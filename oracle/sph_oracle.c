@@ -794,8 +794,8 @@ void oracle_volume_fraction_water(const sphg_params *P, sphg_phase_view *wa, sph
 
 /* ------------------------------------------------------------------------ */
 /* single_step, declarations_sph.f90:1131-1435 (serial path).                */
-/* Parity unpinned beyond the per-routine checks: the composed step is        */
-/* checked for conservation / rest-state invariants and against the GPU.      */
+/* Pinned beyond the per-routine checks by bit-exact mirror equivariance and  */
+/* Galilean invariance of the composed two-phase step.                        */
 /* ------------------------------------------------------------------------ */
 static int phase_rhs(const sphg_params *P, sphg_phase_view *pl, int itimestep)
 {

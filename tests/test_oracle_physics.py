@@ -428,3 +428,163 @@ def test_integrators_and_summation_density(summation):
     ph.rho[:] = r0
     oracle.call_phase("first_step", P, ph)
     np.testing.assert_allclose(ph.rho, r0 if summation else r0 + (dt / 2.0) * q, rtol=1e-15)
+
+
+def _dp_consistency(ph, P, scale_plastic=1.0):
+    """Rate of the Drucker-Prager yield function f = sqrt(J2) - alpha I1
+    (I1 = 3p, plane strain: s_zz = -(s_xx + s_yy)) along the elastic rates
+    (ds = G t, dp = -K div v) plus the flow rule's correction (already in
+    ph.dsxx.. with zero elastic part): (df, df_elastic, |terms|)."""
+    tp = math.tan(P.soil_phi)
+    alpha = tp / math.sqrt(9.0 + 12.0 * tp * tp)
+    G = P.soil_E / (2.0 * (1.0 + P.soil_niu))
+    K = P.soil_k
+    sxx, sxy, syy = ph.sxx, ph.sxy, ph.syy
+    sj = np.sqrt((sxx * sxx + 2.0 * sxy * sxy + syy * syy + (sxx + syy) ** 2) / 2.0)
+
+    def rate(dxx, dxy, dyy, dp):
+        ds = (sxx * dxx + syy * dyy + (sxx + syy) * (dxx + dyy) + 2.0 * sxy * dxy) / (2.0 * sj)
+        return ds - 3.0 * alpha * dp, np.abs(ds) + np.abs(3.0 * alpha * dp)
+
+    e = (G * ph.txx, G * ph.txy, G * ph.tyy, -K * ph.vcc)
+    pl = (ph.dsxx, ph.dsxy, ph.dsyy, ph.dp)
+    df_e, mag_e = rate(*e)
+    df, _ = rate(*[a + scale_plastic * b for a, b in zip(e, pl)])
+    return df, df_e, mag_e
+
+
+def _flow_rule3_state(seed=21):
+    from sources_amd.state import default_params
+    E, nu = 1.8e6, 0.3
+    P = default_params(soil_phi=math.radians(30.0), soil_E=E, soil_niu=nu, soil_k=E / (3.0 * (1.0 - 2.0 * nu)))
+    _, ph = lattice_phase(n=12, material=abi.SPHG_SOIL)
+    rng = np.random.default_rng(seed)
+    n = ph.maxn
+    ph.sxx[:] = rng.normal(scale=2000, size=n)
+    ph.syy[:] = rng.normal(scale=2000, size=n)
+    ph.sxy[:] = rng.normal(scale=1000, size=n)
+    # strain rates of a velocity gradient [[a, b], [c, d]] as the reference
+    # forms them: t = 2/3 (2 du/dx - dv/dy), ..., vcc = div v
+    a, b, c, d = (rng.normal(size=n) for _ in range(4))
+    ph.txx[:] = 2.0 / 3.0 * (2.0 * a - d)
+    ph.tyy[:] = 2.0 / 3.0 * (2.0 * d - a)
+    ph.txy[:] = b + c
+    ph.vcc[:] = a + d
+    ph.fail[:] = 1
+    for f in ("dsxx", "dsxy", "dsyy", "dp", "epsilon_p"):
+        getattr(ph, f)[:] = 0.0
+    oracle.call_phase("plastic_flow_rule3", P, ph)
+    return P, ph
+
+
+def test_plastic_flow_rule3_magnitude_restores_consistency():
+    # The plastic multiplier of plastic_flow_rule3 (m_particles.f90:3733) is
+    # the consistency condition of Drucker-Prager with the correction along
+    # the deviator and 3 K alpha on p: with it the yield function's rate
+    # vanishes, df = df_elastic - dlambda (G + 9 alpha^2 K) = 0.  Where the
+    # rule leaves a point alone (dlambda < 0) the elastic rate unloads.
+    P, ph = _flow_rule3_state()
+    df, df_e, mag = _dp_consistency(ph, P)
+    moved = ph.dp != 0
+    assert 50 < np.count_nonzero(moved) < ph.maxn - 50
+    assert np.all(np.abs(df[moved]) <= 1e-12 * mag[moved]), np.max(np.abs(df[moved]) / mag[moved])
+    assert np.all(df_e[~moved] < 0)
+    assert np.all(df_e[moved] > 0)
+
+
+def test_plastic_flow_rule3_consistency_check_bites():
+    # the check above fails for a multiplier off by one part in a million
+    P, ph = _flow_rule3_state()
+    df, _, mag = _dp_consistency(ph, P, scale_plastic=1.0 + 1e-6)
+    moved = ph.dp != 0
+    assert np.max(np.abs(df[moved]) / mag[moved]) > 1e-8
+
+
+def _saturated_lattice(n=24, dx=0.005, por=0.4):
+    """Co-located water and soil lattices of a saturated soil (water vof =
+    porosity, soil vof = 1 - porosity, bulk densities rho0 * vof) with the
+    cross pairs of direct_find_2."""
+    P, wa = lattice_phase(n=n, dx=dx)
+    _, so = lattice_phase(n=n, dx=dx, material=abi.SPHG_SOIL)
+    P.single_phase = 0
+    P.soil_rho0 = 2650.0
+    vol = dx * dx
+    wa.rho[:] = 1000.0 * por
+    wa.mass[:] = vol * wa.rho
+    wa.vof[:] = por
+    so.rho[:] = P.soil_rho0 * (1.0 - por)
+    so.mass[:] = vol * so.rho
+    so.vof[:] = 1.0 - por
+    oracle.call_cross("direct_find_2", P, wa, so)
+    return P, wa, so, interior(wa, n)
+
+
+def test_volume_fraction_water_is_the_porosity():
+    # volume_fraction_water (m_particles.f90:3912-3937): 1 - sum m_s W / rho_s0,
+    # i.e. one minus the soil fraction's kernel estimate: the porosity
+    P, wa, so, inn = _saturated_lattice(por=0.4)
+    oracle.call_cross("volume_fraction_water", P, wa, so)
+    v = wa.vof2[:wa.n][inn]
+    assert np.all(np.abs(v - 0.4) < 1e-3), (v.min(), v.max())
+
+
+def _pore_pressure_factor(P, wa, so, inn):
+    # pore_water_pressure (m_particles.f90:3877-3909) on a linear pore
+    # pressure p = p0 + g.x: a_s = -(vof_s / rho_s) grad p, times the
+    # lattice's gradient factor, the same in x and y
+    g = np.array([1200.0, -3400.0])
+    wa.p[:] = 5000.0 + wa.x @ g
+    so.dvx[:] = 0.0
+    oracle.call_cross("pore_water_pressure", P, wa, so)
+    a = so.dvx[:so.n][inn]
+    k = a / (-(so.vof[0] / so.rho[0]) * g)
+    return k
+
+
+def test_pore_water_pressure_is_the_pressure_gradient():
+    P, wa, so, inn = _saturated_lattice()
+    k = _pore_pressure_factor(P, wa, so, inn)
+    np.testing.assert_allclose(k, k[0, 0], rtol=1e-9)
+    assert abs(k[0, 0] - 1.0) < 0.02
+
+
+def test_volume_fraction_water2_is_the_soil_divergence():
+    # volume_fraction_water2 (m_particles.f90:3940-3967): water vof rate
+    # -sum m_s (v_w - v_s).grad W / rho_s0 = (1 - porosity) div v_s when the
+    # water is at rest, with the lattice's gradient factor of the pore
+    # pressure force; zero for a common uniform velocity (Galilean invariant)
+    P, wa, so, inn = _saturated_lattice(por=0.4)
+    k = _pore_pressure_factor(P, wa, so, inn)[0, 0]
+    A = np.array([[0.3, -0.2], [0.5, 0.7]])
+    so.vx[:] = so.x @ A.T
+    wa.vx[:] = 0.0
+    wa.dvof[:] = 0.0
+    oracle.call_cross("volume_fraction_water2", P, wa, so)
+    d = wa.dvof[:wa.n][inn]
+    np.testing.assert_allclose(d, k * 0.6 * np.trace(A), rtol=1e-9)
+    so.vx[:] = [0.4, -1.1]
+    wa.vx[:] = [0.4, -1.1]
+    wa.dvof[:] = 0.0
+    oracle.call_cross("volume_fraction_water2", P, wa, so)
+    assert np.all(wa.dvof[:wa.n] == 0.0)
+
+
+def test_newtonian_fluid_is_mu_times_the_strain_rate():
+    # newtonian_fluid (m_particles.f90:2484-2500): sigma = mu t with mu the
+    # dynamic viscosity: in a simple shear u = gamma y it is mu gamma times
+    # the strain rate's lattice factor, whatever the density
+    out = []
+    for rho in (1000.0, 3000.0):
+        P, ph = lattice_phase()
+        P.water_viscosity = 1.0e-3
+        ph.rho[:] = rho
+        ph.mass[:] = rho * 0.005 ** 2
+        ph.vx[:, 0] = 2.5 * ph.x[:, 1]
+        ph.vx[:, 1] = 0.0
+        oracle.call_phase("shear_strain_rate", P, ph)
+        oracle.call_phase("newtonian_fluid", P, ph)
+        inn = interior(ph, 24)
+        out.append(ph.sxy[:ph.n][inn].copy())
+        assert np.all(np.abs(ph.sxx[:ph.n][inn]) < 1e-12) and np.all(np.abs(ph.syy[:ph.n][inn]) < 1e-12)
+    np.testing.assert_allclose(out[0], 1.0e-3 * 2.5, rtol=0.02)
+    np.testing.assert_allclose(out[0], out[1], rtol=1e-13)   # m/rho rounds differently
