@@ -11,6 +11,7 @@ beyond the per-routine checks:
   rates unchanged up to rounding (every velocity enters through
   differences)."""
 import numpy as np
+import pytest
 
 import oracle
 from helpers import normwise
@@ -20,19 +21,31 @@ RATES = ("drho", "dvx", "txx", "txy", "tyy", "vcc", "wxy", "dsxx", "dsxy", "dsyy
 ODD = ("txy", "wxy", "dsxy")   # shear components: change sign under x -> -x
 
 
-def _case(seed=3):
-    sc = scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3, tank_h=0.12, jitter=0.05,
-                                    seed=7)
+MAKERS = {
+    "two_phase": lambda: scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3, tank_h=0.12,
+                                                    jitter=0.05, seed=7),
+    "water_only": lambda: scenarios.dam_break(dx=0.01, jitter=0.05, seed=5),
+    "soil_only": lambda: scenarios.soil_collapse(jitter=0.05, seed=4),
+}
+
+
+def _phases(sc):
+    return [ph for ph in (sc.water, sc.soil) if ph is not None]
+
+
+def _case(seed=3, kind="two_phase"):
+    sc = MAKERS[kind]()
     rng = np.random.default_rng(seed)
-    for ph in (sc.water, sc.soil):
+    for ph in _phases(sc):
         ph.vx[:ph.n] = rng.normal(scale=0.05, size=(ph.n, 2))
-    sc.soil.sxy[:sc.soil.n] = rng.normal(scale=50.0, size=sc.soil.n)
+    if sc.soil is not None:
+        sc.soil.sxy[:sc.soil.n] = rng.normal(scale=50.0, size=sc.soil.n)
     return sc
 
 
 def _mirror(sc, flip_velocity=True):
     b = sc.copy()
-    for ph in (b.water, b.soil):
+    for ph in _phases(b):
         ph.x[:, 0] = -ph.x[:, 0]
         if flip_velocity:
             ph.vx[:, 0] = -ph.vx[:, 0]
@@ -45,6 +58,8 @@ def _mirror_mismatch(a, b):
     bad = []
     for name in ("water", "soil"):
         pa, pb = getattr(a, name), getattr(b, name)
+        if pa is None:
+            continue
         n = pa.n
         for f in RATES:
             A = getattr(pa, f)[:n]
@@ -58,14 +73,19 @@ def _mirror_mismatch(a, b):
     return bad
 
 
-def test_single_step_is_mirror_equivariant_bit_for_bit():
-    a = _case()
+@pytest.mark.parametrize("kind", list(MAKERS))
+def test_single_step_is_mirror_equivariant_bit_for_bit(kind):
+    # two-phase single_step, single_step_for_water and single_step_for_soil
+    a = _case(kind=kind)
     b = _mirror(a)
     oracle.single_step(a.params, a.water, a.soil, 3)
     oracle.single_step(b.params, b.water, b.soil, 3)
     assert _mirror_mismatch(a, b) == []
     # the check is not vacuous: the rates are far from zero
-    assert np.max(np.abs(a.soil.dsxy[:a.soil.n])) > 1e5 and np.max(np.abs(a.water.dvx[:a.water.n])) > 1.0
+    for ph in _phases(a):
+        assert np.max(np.abs(ph.dvx[:ph.n])) > 1.0
+    if a.soil is not None:
+        assert np.max(np.abs(a.soil.dsxy[:a.soil.n])) > 1e5
 
 
 def test_mirror_check_fails_for_an_unmirrored_input():
