@@ -118,6 +118,30 @@ __global__ void k_gather(const T* src, T* dst, const int* idx, int n)
     if (t < n) dst[t] = src[idx[t]];
 }
 
+// Several arrays through one permutation in one pass: dst_a[t] = src_a[idx[t]]
+// for the nd double and ni int arrays (idx == nullptr: a plain copy).
+constexpr int PERM_MAXD = 28, PERM_MAXI = 4;
+struct PermArgs {
+    const double* sd[PERM_MAXD];
+    double* dd[PERM_MAXD];
+    const int* si[PERM_MAXI];
+    int* di[PERM_MAXI];
+    int nd, ni;
+};
+
+__global__ void k_permute_multi(PermArgs A, const int* idx, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int s = idx ? idx[t] : t;
+#pragma unroll
+    for (int a = 0; a < PERM_MAXD; ++a)
+        if (a < A.nd) A.dd[a][t] = A.sd[a][s];
+#pragma unroll
+    for (int a = 0; a < PERM_MAXI; ++a)
+        if (a < A.ni) A.di[a][t] = A.si[a][s];
+}
+
 // dst[idx[t]] = src[t]: device -> host order.
 template <class T>
 __global__ void k_scatter(const T* src, T* dst, const int* idx, int n)

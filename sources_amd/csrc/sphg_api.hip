@@ -33,6 +33,9 @@ struct PendingEvent {
     hipEvent_t a, b;
 };
 
+// numbers of double and int arrays the reorder permutes
+constexpr int REORDER_ND = 24, REORDER_NI = 3;
+
 struct PhaseBuf {
     bool bound = false;
     sphg_phase_view view{};
@@ -55,8 +58,10 @@ struct PhaseBuf {
     // fused path: cell-sorted neighbour records and same-phase masks
     double* rec = nullptr;
     Masks masks{};               // same-phase partners of water stage A, reused by stage B
-    int* spare_i = nullptr;      // permutation scratch
-    double* spare_d = nullptr;
+    // second buffers of the arrays the reorder permutes: the permuted copy is
+    // written there and the pointers swap
+    double* shadow_d[PERM_MAXD] = {};
+    int* shadow_i[PERM_MAXI] = {};
     double hmin = 0.0, hmax = 0.0;
 };
 
@@ -250,29 +255,33 @@ int reorder(sphg_ctx* c, int k)
     PhaseBuf& b = c->ph[k];
     DPhase& d = b.d;
     const int* ord = d.order;
-    auto pd = [&](double*& arr) -> int {
-        RC(launch(c, "permute", k_gather<double>, nblk(b.n), TPB, (const double*)arr, b.spare_d, ord, b.n));
-        if (c->stable_ptrs)
-            RC(launch(c, "permute", k_copy<double>, nblk(b.n), TPB, (const double*)b.spare_d, arr, b.n));
-        else
-            std::swap(arr, b.spare_d);
-        return SPHG_OK;
-    };
-    auto pi = [&](int*& arr) -> int {
-        RC(launch(c, "permute", k_gather<int>, nblk(b.n), TPB, (const int*)arr, b.spare_i, ord, b.n));
-        if (c->stable_ptrs)
-            RC(launch(c, "permute", k_copy<int>, nblk(b.n), TPB, (const int*)b.spare_i, arr, b.n));
-        else
-            std::swap(arr, b.spare_i);
-        return SPHG_OK;
-    };
-    RC(pi(d.id)); RC(pi(d.itype)); RC(pi(d.fail));
+    int** fi[] = {&d.id, &d.itype, &d.fail};
     // av persists across steps when average_velocity is off; the stress
     // levels of stress_integration = 2 persist too
     double** f[] = {&d.x, &d.y, &d.vx, &d.vy, &d.mass, &d.hsml, &d.rho, &d.vof, &d.p, &d.c,
                     &d.drho, &d.ax, &d.ay, &d.dvof, &d.sxx, &d.sxy, &d.syy, &d.eps_p, &d.avx, &d.avy,
                     &d.sxx_min, &d.sxy_min, &d.syy_min, &d.p_min};
-    for (double** a : f) RC(pd(*a));
+    constexpr int ND = (int)(sizeof(f) / sizeof(f[0])), NI = (int)(sizeof(fi) / sizeof(fi[0]));
+    static_assert(ND == REORDER_ND && NI == REORDER_NI, "reorder array lists and shadow counts differ");
+    // one pass gathers every array into its shadow
+    PermArgs A{};
+    A.nd = ND;
+    A.ni = NI;
+    for (int a = 0; a < ND; ++a) { A.sd[a] = *f[a]; A.dd[a] = b.shadow_d[a]; }
+    for (int a = 0; a < NI; ++a) { A.si[a] = *fi[a]; A.di[a] = b.shadow_i[a]; }
+    RC(launch(c, "permute", k_permute_multi, nblk(b.n), TPB, A, ord, b.n));
+    if (c->stable_ptrs) {
+        // captured graphs hold the arrays' addresses: copy back instead
+        PermArgs B{};
+        B.nd = ND;
+        B.ni = NI;
+        for (int a = 0; a < ND; ++a) { B.sd[a] = b.shadow_d[a]; B.dd[a] = *f[a]; }
+        for (int a = 0; a < NI; ++a) { B.si[a] = b.shadow_i[a]; B.di[a] = *fi[a]; }
+        RC(launch(c, "permute", k_permute_multi, nblk(b.n), TPB, B, (const int*)nullptr, b.n));
+    } else {
+        for (int a = 0; a < ND; ++a) std::swap(*f[a], b.shadow_d[a]);
+        for (int a = 0; a < NI; ++a) std::swap(*fi[a], b.shadow_i[a]);
+    }
     RC(launch(c, "iota", k_iota, nblk(b.n), TPB, d.order, b.n));
     return SPHG_OK;
 }
@@ -658,8 +667,10 @@ int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
     const size_t n = (size_t)b.n;
     RC(dalloc(c, b, &d.id, n));
     RC(launch(c, "iota", k_iota, nblk(b.n), TPB, d.id, b.n));
-    RC(dalloc(c, b, &b.spare_i, n));
-    RC(dalloc(c, b, &b.spare_d, n));
+    // shadows of the reordered arrays (allocated here, never inside a step, so
+    // that a step can be captured into a graph)
+    for (int a = 0; a < REORDER_ND; ++a) RC(dalloc(c, b, &b.shadow_d[a], n));
+    for (int a = 0; a < REORDER_NI; ++a) RC(dalloc(c, b, &b.shadow_i[a], n));
     RC(dalloc(c, b, &d.itype, n));
     RC(dalloc(c, b, &d.fail, n));
     RC(dalloc(c, b, &d.countiac, n));
@@ -1518,6 +1529,11 @@ struct sphg_gnn {
     double tsum[gnn::NO] = {0, 0, 0, 0}, tsq[gnn::NO] = {0, 0, 0, 0};
     double tcount = 0.0;
     StateCopy snap[2][2];           // [slot][phase]
+    // the network's forward pass runs on its own stream beside the coarse
+    // step (both read the state the advance starts from; neither writes what
+    // the other reads)
+    hipStream_t fstream = nullptr;
+    hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr;
 };
 
 namespace {
@@ -1675,6 +1691,11 @@ int sphg_gnn_create(sphg_ctx* c, int ksteps, int max_samples, uint64_t seed, sph
         (rc = galloc(g, &g->y, (size_t)n * gnn::NO)) || (rc = galloc(g, &g->overflow, 1)) ||
         (rc = galloc(g, &g->err, 1)) || (rc = galloc(g, &g->errsq, 2)) || (rc = graph_alloc(g, g->cur, n)))
         return fail(rc);
+    // default-priority stream and fork / join events of the forward pass
+    if (hipStreamCreateWithFlags(&g->fstream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_f0, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_f1, hipEventDisableTiming) != hipSuccess)
+        return fail(SPHG_ERR_HIP);
     for (int k = 0; k < 2; ++k) {
         if ((rc = galloc(g, &g->node_of_id[k], c->ph[k].n)) || (rc = galloc(g, &g->slot_of_id[k], c->ph[k].n)))
             return fail(rc);
@@ -1727,6 +1748,12 @@ void sphg_gnn_destroy(sphg_gnn* g)
 {
     if (!g) return;
     if (g->c) hipStreamSynchronize(g->c->stream);
+    if (g->fstream) {
+        hipStreamSynchronize(g->fstream);
+        hipStreamDestroy(g->fstream);
+    }
+    if (g->ev_f0) hipEventDestroy(g->ev_f0);
+    if (g->ev_f1) hipEventDestroy(g->ev_f1);
     for (void* p : g->allocs) hipFree(p);
     delete g;
 }
@@ -1836,12 +1863,26 @@ int sphg_gnn_step(sphg_gnn* g, int use_net)
     if (!g) return SPHG_ERR_INVALID;
     sphg_ctx* c = g->c;
     RC(check_fused_switches(c));
+    // the forward pass overlaps the coarse step unless kernels are being
+    // timed one by one
+    const bool ovl = use_net && side_on(c);
     if (use_net) {
         RC(build_graph(g, g->cur));
-        RC(forward(g, g->cur));
+        if (ovl) {
+            HIPCHK(c, hipEventRecord(g->ev_f0, c->stream));
+            HIPCHK(c, hipStreamWaitEvent(g->fstream, g->ev_f0, 0));
+            c->lstream = g->fstream;
+            const int rc = forward(g, g->cur);
+            c->lstream = nullptr;
+            RC(rc);
+            HIPCHK(c, hipEventRecord(g->ev_f1, g->fstream));
+        } else {
+            RC(forward(g, g->cur));
+        }
     }
     RC(coarse_step(c, g->K));
     if (use_net) {
+        if (ovl) HIPCHK(c, hipStreamWaitEvent(c->stream, g->ev_f1, 0));
         for (int k = 0; k < 2; ++k) {
             PhaseBuf& b = c->ph[k];
             RC(launch(c, "gnn_correct", gnn::k_gnn_correct, nblk(b.n), TPB, b.d, c->K,
