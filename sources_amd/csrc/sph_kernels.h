@@ -316,6 +316,27 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_cell_sort_t(DPhase P, const int*
     const int e = c + 1 < ncell ? part[c + 1] + tp[(c + 1) / SCAN_TILE] : tp[ntiles];
     P.cell_start[c] = s;
     if (c + 1 == ncell) P.cell_start[ncell] = e;
+    constexpr int CS = 16;   // cells up to this size sort in registers
+    if (e - s <= CS) {
+        // the cell's entries in registers (independent loads), an unrolled
+        // odd-even transposition network, one store each
+        int v[CS];
+#pragma unroll
+        for (int q = 0; q < CS; ++q) v[q] = s + q < e ? P.order[s + q] : INT_MAX;
+#pragma unroll
+        for (int r = 0; r < CS; ++r) {
+#pragma unroll
+            for (int q = r & 1; q + 1 < CS; q += 2) {
+                const int a = v[q], b = v[q + 1];
+                v[q] = min(a, b);
+                v[q + 1] = max(a, b);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CS; ++q)
+            if (s + q < e) P.order[s + q] = v[q];
+        return;
+    }
     for (int k = s + 1; k < e; ++k) {
         const int v = P.order[k];
         int m = k - 1;
