@@ -399,8 +399,12 @@ def main():
             "workload": "two-phase 30-degree slope, partially submerged, dx=0.001, FP64 leapfrog time_integration",
             "particles": ntot, "water": nw, "soil": ns,
             "water_real": sc.water.ntotal, "soil_real": sc.soil.ntotal,
-            "cross_pairs": pairs_x, "skf": sc.params.skf, "precision": args.precision,
+            "water_pairs": pairs_w, "soil_pairs": pairs_s, "cross_pairs": pairs_x,
+            "skf": sc.params.skf, "precision": args.precision,
         },
+        # distinct interacting pairs (water-water, soil-soil, water-soil) per
+        # second; the gather visits each same-phase pair from both ends
+        "pair_interactions_per_s": (pairs_w + pairs_s + pairs_x) / (ms_step * 1e-3),
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -233,7 +233,8 @@ int sphg_profile_reset(sphg_ctx *ctx);
 /* Fills launches and total milliseconds for kernel `name`; returns
    SPHG_ERR_INVALID if the name is unknown. */
 int sphg_profile_get(sphg_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
-/* Number of same-phase / cross pairs found by the most recent step. */
+/* Number of same-phase / cross pairs found by the most recent fused step
+   (each pair once). */
 int sphg_pair_counts(sphg_ctx *ctx, int64_t *water_pairs, int64_t *soil_pairs, int64_t *cross_pairs);
 
 /* ---- learned multi-step surrogate ---------------------------------------

@@ -966,6 +966,7 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     } else {
         W.countiac[a] = cnt;
     }
+    W.nsame[a] = cnt;
     // p and the new c (chunks 3 and 4) come from the pack
     double2* WP = reinterpret_cast<double2*>(recWw) + t;
     const size_t nw = (size_t)W.n;
@@ -1260,6 +1261,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     } else {
         S.countiac[a] = cnt;
     }
+    S.nsame[a] = cnt;
     S.c_next[a] = cn;
 
     const double alpha = K.s_alpha1, G = K.s_G, Kb = K.s_k;

@@ -19,6 +19,7 @@ struct DPhase {
     double dspp;
     int *id;   // reference particle number (0-based) of each device slot
     int *itype, *fail, *countiac, *key, *slot;
+    int *nsame;   // same-phase partners of the last fused step (countiac holds the cross count in two-phase runs)
     double *x, *y, *vx, *vy, *mass, *hsml;
     double *rho, *rho_min, *rho_next;
     double *vof, *vof_min, *vof2, *dvof, *dvof_x;

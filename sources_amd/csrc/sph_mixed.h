@@ -312,6 +312,7 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
     } else {
         W.countiac[a] = cnt;
     }
+    W.nsame[a] = cnt;
     // p (q2.w) and the new c (q3.x) come from the pack
     float* Q3 = reinterpret_cast<float*>(reinterpret_cast<float4*>(qWw) + 3 * (size_t)W.n + t);
     Q3[1] = (float)(vofa_d * (mu * txx_d));
@@ -562,6 +563,7 @@ __global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* q
     } else {
         S.countiac[a] = cnt;
     }
+    S.nsame[a] = cnt;
     S.c_next[a] = K.s_c;
     const double p = S.p[a], sxx = S.sxx[a], sxy = S.sxy[a], syy = S.syy[a];
     const double alpha_d = K.s_alpha1, G = K.s_G, Kb = K.s_k;

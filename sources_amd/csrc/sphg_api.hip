@@ -674,6 +674,7 @@ int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
     RC(dalloc(c, b, &d.itype, n));
     RC(dalloc(c, b, &d.fail, n));
     RC(dalloc(c, b, &d.countiac, n));
+    RC(dalloc(c, b, &d.nsame, n));
     RC(dalloc(c, b, &d.key, n));
     RC(dalloc(c, b, &d.slot, n));
     RC(dalloc(c, b, &d.order, n));
@@ -692,6 +693,7 @@ int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
     RC(fill_int(c, d.itype, b.n, 0));
     RC(fill_int(c, d.fail, b.n, 0));
     RC(fill_int(c, d.countiac, b.n, 0));
+    RC(fill_int(c, d.nsame, b.n, 0));
     // fused-path buffers
     RC(dalloc(c, b, &b.rec, (size_t)R_STRIDE * n));
     if (k == 0) {   // water; soil runs both stages in one pass and stores no masks
@@ -1455,17 +1457,23 @@ int sphg_pair_counts(sphg_ctx* c, int64_t* wp, int64_t* sp, int64_t* xp)
 {
     if (!c) return SPHG_ERR_INVALID;
     // Pairs seen by the most recent fused step: half the same-phase neighbour
-    // counts (countiac holds them in single-phase mode) or the cross counts.
+    // counts, and the cross counts (countiac of water in two-phase runs).
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     int64_t out[3] = {0, 0, 0};
+    auto sum = [&](const int* d, int n, int64_t& s) -> int {
+        std::vector<int> cnt((size_t)n);
+        HIPCHK(c, hipMemcpy(cnt.data(), d, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+        s = 0;
+        for (int v : cnt) s += v;
+        return SPHG_OK;
+    };
     for (int k = 0; k < 2; ++k) {
         PhaseBuf& b = c->ph[k];
         if (!b.bound) continue;
-        std::vector<int> cnt((size_t)b.n);
-        HIPCHK(c, hipMemcpy(cnt.data(), b.d.countiac, sizeof(int) * (size_t)b.n, hipMemcpyDeviceToHost));
         int64_t s = 0;
-        for (int v : cnt) s += v;
-        if (c->P.single_phase) out[k] = s / 2;
-        else if (k == 0) out[2] = s;
+        RC(sum(b.d.nsame, b.n, s));
+        out[k] = s / 2;
+        if (!c->P.single_phase && k == 0) RC(sum(b.d.countiac, b.n, out[2]));
     }
     if (wp) *wp = out[0];
     if (sp) *sp = out[1];

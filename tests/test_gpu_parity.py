@@ -523,3 +523,27 @@ def test_too_many_interactions_is_reported():
         s.direct_find("water")
     assert e.value.code == abi.SPHG_ERR_TOO_MANY_INTERACTIONS
     s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pair_counts_match_the_pair_searches(precision):
+    # sphg_pair_counts after one fused step: the same-phase and cross pairs of
+    # the state the step started from, each pair once -- direct_find's and
+    # direct_find_2's niac
+    sc = _column()
+    refs = [sc.copy() for _ in range(3)]
+    if precision == "mixed":
+        sc.params.precision = abi.SPHG_MIXED
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(1, sync=True)
+    nw, ns, nx = s.pair_counts()
+    s.close()
+    oracle.call_phase("direct_find", refs[0].params, refs[0].water)
+    oracle.call_phase("direct_find", refs[1].params, refs[1].soil)
+    oracle.call_cross("direct_find_2", refs[2].params, refs[2].water, refs[2].soil)
+    got, want = (nw, ns, nx), (refs[0].water.niac, refs[1].soil.niac, refs[2].water.niac)
+    if precision == "fp64":
+        assert got == want
+    else:
+        # FP32 separations may flip pairs at r = 2h, where W and grad W vanish
+        assert all(abs(g - w) <= 1e-4 * w for g, w in zip(got, want)), (got, want)
