@@ -292,11 +292,13 @@ def main():
     ap.add_argument("--mixed-error-steps", type=int, default=1000,
                     help="mixed vs FP64 error is reported after 1 step and after this many (0: skip)")
     ap.add_argument("--no-pcie", action="store_true")
-    ap.add_argument("--learned-k", type=int, default=2)
+    # 3 dt is the largest coarse step that stays stable on the slope (4 dt diverges)
+    ap.add_argument("--learned-k", type=int, default=3)
     ap.add_argument("--learned-samples", type=int, default=8)
     ap.add_argument("--learned-iters", type=int, default=400)
-    ap.add_argument("--learned-advances", type=int, default=40)
-    ap.add_argument("--learned-judge", type=int, default=10, help="plain FP64 check every this many advances")
+    ap.add_argument("--learned-advances", type=int, default=42)
+    ap.add_argument("--learned-judge", type=int, default=7,
+                    help="plain FP64 check every this many advances (7 x 3 steps: about every 20 steps)")
     ap.add_argument("--learned-scale", type=float, default=0.5, help="slope size for the learned section")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="fp64: reference arithmetic; mixed: FP32 Kahan pair sums, FP64 integration")
