@@ -1,5 +1,6 @@
 """The C-ABI library loads and exports every entry point the header declares
 (no compute calls: this runs without a GPU)."""
+import pytest
 import ctypes as C
 import os
 import re
@@ -43,3 +44,9 @@ def test_strerror_names_reference_stops():
     lib = abi.load_library()
     assert b"Too many interactions" in lib.sphg_strerror(abi.SPHG_ERR_TOO_MANY_INTERACTIONS)
     assert b"out of range" in lib.sphg_strerror(abi.SPHG_ERR_OUT_OF_RANGE)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    # no CPU fallback: without the HIP library the product path refuses to run
+    with pytest.raises(RuntimeError, match="missing"):
+        abi.load_library(str(tmp_path / "libsphg.so"))
