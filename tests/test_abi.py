@@ -1,10 +1,11 @@
 """The C-ABI library loads and exports every entry point the header declares
 (no compute calls: this runs without a GPU)."""
-import pytest
 import ctypes as C
 import os
 import re
 import subprocess
+
+import pytest
 
 from sources_amd import abi
 
