@@ -479,19 +479,37 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
     const size_t ns = (size_t)O.n_nodes;
 #pragma unroll 1
     for (int r = 0; r < 3; ++r) {
+        const int k1 = R.k1[r];
+        // candidates in groups of four: the index loads, then the position
+        // loads of all four, then the predicates in candidate order (the slot
+        // order stays ascending in k)
 #pragma unroll 1
-        for (int k = R.k0[r]; k < R.k1[r]; ++k) {
-            if (k == t) continue;
-            const int b = P.order[k];
-            const double dx = x - P.x[b], dy = y - P.y[b];
-            double dr;
-            {
-#pragma clang fp contract(off)
-                dr = dx * dx + dy * dy;
+        for (int kb = R.k0[r]; kb < k1; kb += 4) {
+            int bq[4];
+            double xq[4], yq[4], hq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bq[q] = kb + q < k1 ? P.order[kb + q] : -1;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int b = bq[q] >= 0 ? bq[q] : a;
+                xq[q] = P.x[b];
+                yq[q] = P.y[b];
+                hq[q] = uni ? h : P.hsml[b];
             }
-            if (!pred(uni != 0, thr, dr, h, uni ? h : P.hsml[b], K)) continue;
-            if (cnt < DEG) nb[(size_t)cnt * ns] = off + k;
-            ++cnt;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = kb + q;
+                if (bq[q] < 0 || k == t) continue;
+                const double dx = x - xq[q], dy = y - yq[q];
+                double dr;
+                {
+#pragma clang fp contract(off)
+                    dr = dx * dx + dy * dy;
+                }
+                if (!pred(uni != 0, thr, dr, h, hq[q], K)) continue;
+                if (cnt < DEG) nb[(size_t)cnt * ns] = off + k;
+                ++cnt;
+            }
         }
     }
     if (cnt > DEG) atomicOr(O.overflow, 1);
