@@ -90,3 +90,27 @@ def test_judge_falls_back_when_tolerance_is_exceeded():
         assert np.isfinite(ph.rho[:ph.n]).all() and np.isfinite(ph.x[:ph.n]).all()
     g.close()
     s.close()
+
+
+def test_broken_state_falls_back_to_plain_steps():
+    # A correction that throws the particles out of the grid (output bias of
+    # the velocity channel 1e30, times the recorded target scale): the sanity
+    # test after the advance catches it
+    # before anything advances that state, the state returns to the last good
+    # snapshot and plain steps cover the span, and the rollout goes on.
+    sc, s, g = setup(nwarm=10, nsamples=1)
+    w = g.weights()
+    w[G.O_B5] = 1.0e30
+    g.set_weights(w)
+    nfb, err, _ = g.rollout(3, judge_every=0, tol=0.01, use_net=True)
+    assert nfb == 3 and err >= 1.0        # every advance broke and was replaced
+    s.sync_all()
+    for ph in (sc.water, sc.soil):
+        n = ph.n
+        assert np.isfinite(ph.x[:n]).all() and np.isfinite(ph.rho[:n]).all()
+        assert np.all(np.abs(ph.x[:n]) < 1.0)   # back inside the tank
+    # the same rollout with the network off is the plain coarse step: no fallback
+    nfb, _, _ = g.rollout(3, judge_every=0, tol=0.01, use_net=False)
+    assert nfb == 0
+    g.close()
+    s.close()
