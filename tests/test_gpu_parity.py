@@ -38,8 +38,20 @@ def _column_fractional_repulsion():
     return sc
 
 
+def _column_large_h():
+    """h = 2.5 dx: about 25 particles per cell and 75 candidates per cell row,
+    so the cell sort takes its path for cells over 16 entries and the partner
+    loops walk rows over 64 candidates directly."""
+    sc = _column()
+    for ph in (sc.water, sc.soil):
+        ph.hsml[:] *= 2.5 / 1.2
+        ph.set_max_interaction(120 * ph.n)
+    return sc
+
+
 def small_cases():
     return {
+        "column_large_h": _column_large_h,
         "savage_skf4": lambda: scenarios.savage_column(),
         "savage_skf1": lambda: scenarios.savage_column(skf=1),
         "column_jitter": _column,
