@@ -20,7 +20,18 @@ CASES = {
     "column_jitter": lambda: scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3,
                                                         tank_h=0.12, jitter=0.05, seed=7),
     "slope_small": lambda: scenarios.slope(dx=0.01, scale=0.25, jitter=0.1, seed=3),
+    "column_large_h": lambda: _large_h(scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3,
+                                                                  tank_h=0.12, jitter=0.05, seed=7)),
 }
+
+
+def _large_h(sc):
+    # h = 2.5 dx: cells over 16 entries and candidate rows over 64 (the
+    # mixed kernels' direct row walk)
+    for ph in (sc.water, sc.soil):
+        ph.hsml[:] *= 2.5 / 1.2
+        ph.set_max_interaction(120 * ph.n)
+    return sc
 
 
 def _run(make, precision, nsteps):
