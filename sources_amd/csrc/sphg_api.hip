@@ -1471,9 +1471,14 @@ int sphg_pair_counts(sphg_ctx* c, int64_t* wp, int64_t* sp, int64_t* xp)
         PhaseBuf& b = c->ph[k];
         if (!b.bound) continue;
         int64_t s = 0;
-        RC(sum(b.d.nsame, b.n, s));
+        if (c->P.single_phase) {
+            // single-phase steps keep the same-phase counts in countiac
+            RC(sum(b.d.countiac, b.n, s));
+        } else {
+            RC(sum(b.d.nsame, b.n, s));
+            if (k == 0) RC(sum(b.d.countiac, b.n, out[2]));
+        }
         out[k] = s / 2;
-        if (!c->P.single_phase && k == 0) RC(sum(b.d.countiac, b.n, out[2]));
     }
     if (wp) *wp = out[0];
     if (sp) *sp = out[1];

@@ -573,3 +573,23 @@ def test_pair_counts_match_the_pair_searches(precision):
     else:
         # FP32 separations may flip pairs at r = 2h, where W and grad W vanish
         assert all(abs(g - w) <= 1e-4 * w for g, w in zip(got, want)), (got, want)
+
+
+@pytest.mark.parametrize("kind", ["water_only", "soil_only"])
+def test_pair_counts_single_phase(kind):
+    # the single-phase water step and the single-phase soil step count their
+    # same-phase pairs too (each pair once; no cross pairs)
+    if kind == "water_only":
+        sc = scenarios.dam_break(dx=0.01, jitter=0.05, seed=5)
+    else:
+        sc = scenarios.soil_collapse(jitter=0.05, seed=4)
+    ref = sc.copy()
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(1, sync=True)
+    nw, ns, nx = s.pair_counts()
+    s.close()
+    ph = ref.water if kind == "water_only" else ref.soil
+    oracle.call_phase("direct_find", ref.params, ph)
+    got = nw if kind == "water_only" else ns
+    assert got == ph.niac > 0
+    assert nx == 0 and (ns if kind == "water_only" else nw) == 0
