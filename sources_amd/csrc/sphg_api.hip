@@ -46,6 +46,7 @@ struct PhaseBuf {
     bool cells_valid = false;
     bool counts_zero = false;   // d.cell_count known to be all zero
     int* cell_part = nullptr;   // tile-local exclusive prefix of the cell counts
+    int* tile_sums = nullptr;   // the scan's tile sums (per phase: the two builds may run concurrently)
     // the other velocity and density levels of the Verlet scheme (water)
     double *lvx = nullptr, *lvy = nullptr, *lrho = nullptr;
     // materialised pairs
@@ -236,13 +237,13 @@ int build_cells(sphg_ctx* c, int k)
     // per-cell sort add the prefix of the tile sums themselves
     const int ntiles = std::max((c->ncell + SCAN_TILE - 1) / SCAN_TILE, 1);
     if (ntiles > SCAN_TILE) return SPHG_ERR_UNSUPPORTED;
-    RC(launch(c, "scan", k_scan_tile, ntiles, SCAN_BLOCK, (const int*)b.d.cell_count, b.cell_part, c->ncell, c->d_scan,
+    RC(launch(c, "scan", k_scan_tile, ntiles, SCAN_BLOCK, (const int*)b.d.cell_count, b.cell_part, c->ncell, b.tile_sums,
               b.d.cell_count));
     b.counts_zero = true;
     RC(launch(c, "cell_scatter", k_cell_scatter_t, nblk(b.n, SCAN_BLOCK), SCAN_BLOCK, b.d, (const int*)b.cell_part,
-              (const int*)c->d_scan, ntiles));
+              (const int*)b.tile_sums, ntiles));
     RC(launch(c, "cell_sort", k_cell_sort_t, nblk(c->ncell, SCAN_BLOCK), SCAN_BLOCK, b.d, (const int*)b.cell_part,
-              (const int*)c->d_scan, ntiles, c->ncell));
+              (const int*)b.tile_sums, ntiles, c->ncell));
     b.cells_valid = true;
     return SPHG_OK;
 }
@@ -437,6 +438,7 @@ int setup_grid(sphg_ctx* c)
             if (!b.bound) continue;
             RC(dalloc(c, b, &b.d.cell_count, (size_t)ncell));
             RC(dalloc(c, b, &b.cell_part, (size_t)ncell));
+            RC(dalloc(c, b, &b.tile_sums, (size_t)(ncell + SCAN_TILE - 1) / SCAN_TILE + 2));
             b.counts_zero = false;
             RC(dalloc(c, b, &b.d.cell_start, (size_t)ncell + 1));
             b.cells_valid = false;
@@ -893,14 +895,28 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
 {
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
-    RC(build_cells(c, 0));
-    RC(build_cells(c, 1));
-    if (c->P.precision == SPHG_MIXED) {
-        float* qW = reinterpret_cast<float*>(W.rec);
-        float* qS = reinterpret_cast<float*>(S.rec);
-        RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, c->gl[0], c->cs));
+    const bool mixed = c->P.precision == SPHG_MIXED;
+    float* qW = mixed ? reinterpret_cast<float*>(W.rec) : nullptr;
+    float* qS = mixed ? reinterpret_cast<float*>(S.rec) : nullptr;
+    // The two phases' cell lists and packs are independent: the soil chain
+    // runs on the side stream beside the water chain (the stages after them
+    // read both phases' records).
+    const bool side = side_on(c);
+    if (side) {
+        RC(fork_side(c));
+        SideScope sc(c);
+        RC(build_cells(c, 1));
         RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
                   c->cs));
+    } else {
+        RC(build_cells(c, 1));
+    }
+    RC(build_cells(c, 0));
+    RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, c->gl[0], c->cs));
+    if (side) RC(join_side(c));
+    else RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
+                   c->cs));
+    if (mixed) {
         switch (c->P.skf) {
         case 1: return stages_mixed<1>(c);
         case 2: return stages_mixed<2>(c);
@@ -908,10 +924,6 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
         default: return stages_mixed<4>(c);
         }
     }
-    RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, (float*)nullptr, c->gl[0],
-              c->cs));
-    RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, (float*)nullptr,
-              c->gl[1], c->cs));
     switch (c->P.skf) {
     case 1: return stages_fused<1>(c);
     case 2: return stages_fused<2>(c);
