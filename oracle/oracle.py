@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, as the checker.  The product package never
 imports this module.
 """
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -21,53 +22,71 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def load(path):
+    """Open an oracle library at `path` and declare its entry points."""
+    L = C.CDLL(path)
+    P = C.POINTER(abi.Params)
+    V = C.POINTER(abi.PhaseView)
+    D = C.POINTER(C.c_double)
+    L.oracle_kernel.argtypes = [C.c_int, C.c_double, D, C.c_double, D, D]
+    L.oracle_kernel.restype = None
+    for name in ("oracle_direct_find", "oracle_link_list", "oracle_find_pairs"):
+        getattr(L, name).argtypes = [P, V]
+    L.oracle_direct_find_2.argtypes = [P, V, V]
+    for name in ("oracle_con_density", "oracle_sum_density", "oracle_shear_strain_rate",
+                 "oracle_velocity_divergence", "oracle_pressure", "oracle_newtonian_fluid",
+                 "oracle_drucker_prager", "oracle_mohr_coulomb", "oracle_plastic_or_not",
+                 "oracle_plastic_flow_rule3", "oracle_plastic_flow_rule",
+                 "oracle_plastic_flow_rule2", "oracle_first_half", "oracle_first_step",
+                 "oracle_second_half", "oracle_return_mapping", "oracle_jaumann_rate", "oracle_art_visc",
+                 "oracle_repulsive_force", "oracle_av_vel"):
+        getattr(L, name).argtypes = [P, V]
+        getattr(L, name).restype = None
+    L.oracle_delta_sph.argtypes = [P, V, D, D]
+    L.oracle_delta_sph.restype = None
+    for name in ("oracle_df", "oracle_df2", "oracle_df3"):
+        getattr(L, name).argtypes = [V, D, C.c_int, D]
+        getattr(L, name).restype = None
+    for name in ("oracle_darcy_law", "oracle_pore_water_pressure",
+                 "oracle_volume_fraction_water2", "oracle_volume_fraction_water"):
+        getattr(L, name).argtypes = [P, V, V]
+        getattr(L, name).restype = None
+    L.oracle_volume_fraction_soil.argtypes = [P, V]
+    L.oracle_volume_fraction_soil.restype = None
+    L.oracle_single_step.argtypes = [P, V, V, C.c_int]
+    L.oracle_single_step_for_water.argtypes = [P, V, C.c_int]
+    L.oracle_time_integration.argtypes = [P, V, V, C.POINTER(C.c_int), C.c_int]
+    L.oracle_time_integration_for_water.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
+    L.oracle_time_integration_for_water_by_verlet.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
+    L.oracle_verlet_update.argtypes = [P, V, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    L.oracle_verlet_update.restype = None
+    L.oracle_single_step_for_soil.argtypes = [P, V, C.c_int]
+    L.oracle_time_integration_for_soil.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
+    L.oracle_init_grid.argtypes = [P, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int), D, D]
+    L.oracle_init_grid.restype = None
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        P = C.POINTER(abi.Params)
-        V = C.POINTER(abi.PhaseView)
-        D = C.POINTER(C.c_double)
-        L.oracle_kernel.argtypes = [C.c_int, C.c_double, D, C.c_double, D, D]
-        L.oracle_kernel.restype = None
-        for name in ("oracle_direct_find", "oracle_link_list", "oracle_find_pairs"):
-            getattr(L, name).argtypes = [P, V]
-        L.oracle_direct_find_2.argtypes = [P, V, V]
-        for name in ("oracle_con_density", "oracle_sum_density", "oracle_shear_strain_rate",
-                     "oracle_velocity_divergence", "oracle_pressure", "oracle_newtonian_fluid",
-                     "oracle_drucker_prager", "oracle_mohr_coulomb", "oracle_plastic_or_not",
-                     "oracle_plastic_flow_rule3", "oracle_plastic_flow_rule",
-                     "oracle_plastic_flow_rule2", "oracle_first_half", "oracle_first_step",
-                     "oracle_second_half", "oracle_return_mapping", "oracle_jaumann_rate", "oracle_art_visc",
-                     "oracle_repulsive_force", "oracle_av_vel"):
-            getattr(L, name).argtypes = [P, V]
-            getattr(L, name).restype = None
-        L.oracle_delta_sph.argtypes = [P, V, D, D]
-        L.oracle_delta_sph.restype = None
-        for name in ("oracle_df", "oracle_df2", "oracle_df3"):
-            getattr(L, name).argtypes = [V, D, C.c_int, D]
-            getattr(L, name).restype = None
-        for name in ("oracle_darcy_law", "oracle_pore_water_pressure",
-                     "oracle_volume_fraction_water2", "oracle_volume_fraction_water"):
-            getattr(L, name).argtypes = [P, V, V]
-            getattr(L, name).restype = None
-        L.oracle_volume_fraction_soil.argtypes = [P, V]
-        L.oracle_volume_fraction_soil.restype = None
-        L.oracle_single_step.argtypes = [P, V, V, C.c_int]
-        L.oracle_single_step_for_water.argtypes = [P, V, C.c_int]
-        L.oracle_time_integration.argtypes = [P, V, V, C.POINTER(C.c_int), C.c_int]
-        L.oracle_time_integration_for_water.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
-        L.oracle_time_integration_for_water_by_verlet.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
-        L.oracle_verlet_update.argtypes = [P, V, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
-        L.oracle_verlet_update.restype = None
-        L.oracle_single_step_for_soil.argtypes = [P, V, C.c_int]
-        L.oracle_time_integration_for_soil.argtypes = [P, V, C.POINTER(C.c_int), C.c_int]
-        L.oracle_init_grid.argtypes = [P, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int), D, D]
-        L.oracle_init_grid.restype = None
-        _lib = L
+        _lib = load(LIB)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Route every call of this module to the library at `path` (a mutated
+    build, for the tests that show a pin can fail) inside the block."""
+    global _lib
+    saved = _lib
+    _lib = load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def _dp(a):

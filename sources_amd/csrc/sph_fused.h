@@ -46,9 +46,6 @@ enum : int {
 #define SPHG_WPB 4           // waves per workgroup of the stage kernels (consecutive particles share a CU)
 #endif
 constexpr int STAGE_BLOCK = 64 * SPHG_WPB;
-#ifndef SPHG_PIPE
-#define SPHG_PIPE 0          // 1: software-pipelined partner loads in the dense loops
-#endif
 #ifndef SPHG_STAGE_WAVES
 #define SPHG_STAGE_WAVES 0   // 0: compiler's choice; n: ask for at least n waves per SIMD
 #endif
@@ -605,28 +602,8 @@ __device__ __forceinline__ void visit_win(IT it, const LaneRows& L, int ovf, int
                                           B&& body)
 {
     int i = 0;
-#if SPHG_PIPE
-    // next partner's loads issued before the current partner's terms
-    int k = it.next(i);
-    if (k >= 0) {
-        auto b = load(i, k);
-#pragma unroll 1
-        for (;;) {
-            int i2 = 0;
-            const int k2 = it.next(i2);
-            if (k2 < 0) {
-                body(b);
-                break;
-            }
-            auto b2 = load(i2, k2);
-            body(b);
-            b = b2;
-        }
-    }
-#else
 #pragma unroll 1
     for (int k = it.next(i); k >= 0; k = it.next(i)) body(load(i, k));
-#endif
     if (ovf) {
 #pragma unroll 1
         for (int r = 0; r < 3; ++r) {

@@ -468,13 +468,18 @@ __global__ void k_first_half_soil(DPhase P, KParams K, int do_half, int ndp)
     }
 }
 
-// first_step (:686-723) / second_half (:725-762) for water.
-__global__ void k_final_water(DPhase P, KParams K, int first, int with_vof)
+// first_step (:686-723) / second_half (:725-762) for water.  renorm: the
+// volume-fraction renormalisation that single_step applies every 40th step
+// (declarations_sph.f90:1414-1420) before the update, rho <- rho / vof * vof2
+// with the half-step vof; second_half then recomputes vof (and, unless
+// summation_density is set, rho), so only a summation-density run keeps it.
+__global__ void k_final_water(DPhase P, KParams K, int first, int with_vof, int renorm)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const double dt = K.dt, h = dt / 2.0;
+    if (renorm) P.rho[i] = P.rho[i] / P.vof[i] * P.vof2[i];
     if (first) {
         if (!K.summation_density) P.rho[i] = P.rho[i] + h * P.drho[i];
         if (with_vof) P.vof[i] = P.vof[i] + h * P.dvof[i];
@@ -495,6 +500,18 @@ __global__ void k_final_water(DPhase P, KParams K, int first, int with_vof)
     P.vy[i] = vy;
     P.x[i] = P.x[i] + dt * vx;
     P.y[i] = P.y[i] + dt * vy;
+}
+
+// The renormalisation as single_step itself leaves it (declarations_sph.f90:
+// 1414-1420), for the bare right-hand side: rho <- rho / vof * vof2, vof <- vof2.
+__global__ void k_renorm_water(DPhase P)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const double v2 = P.vof2[i];
+    P.rho[i] = P.rho[i] / P.vof[i] * v2;
+    P.vof[i] = v2;
 }
 
 // time_integration_for_water_by_verlet (declarations_sph.f90:889-974), the

@@ -987,6 +987,17 @@ int check_fused_switches(const sphg_ctx* c)
     return SPHG_OK;
 }
 
+// The water volume-fraction renormalisation of single_step
+// (declarations_sph.f90:1414-1420) acts on every 40th step; it changes the
+// state only when the leapfrog leaves rho alone (summation_density), since
+// second_half otherwise recomputes rho and vof from their saved levels.
+bool renorm_active(const sphg_params& P)
+{
+    return !P.single_phase && P.volume_fraction && P.volume_fraction_renorm && P.summation_density;
+}
+
+bool renorm_step(const sphg_ctx* c, int step) { return renorm_active(c->P) && step % 40 == 0; }
+
 int one_step(sphg_ctx* c)
 {
     const int step = ++c->itimestep;
@@ -1011,7 +1022,7 @@ int one_step(sphg_ctx* c)
     if (c->P.single_phase) {
         if (step != 1) RC(launch(c, "first_half", k_first_half_water, nblk(W.n), TPB, W.d, c->K));
         RC(rhs_water(c));
-        RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, 0));
+        RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, 0, 0));
         W.cells_valid = false;
         return SPHG_OK;
     }
@@ -1035,7 +1046,8 @@ int one_step(sphg_ctx* c)
         else RC(rhs_two_phase_fused(c, 0, 0, 1));
     }
     const int vf = c->P.volume_fraction ? 1 : 0;
-    RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, vf));
+    RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, vf,
+              renorm_step(c, step) ? 1 : 0));
     RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, step == 1 ? 1 : 0));
     W.cells_valid = false;
     S.cells_valid = false;
@@ -1389,6 +1401,9 @@ int sphg_single_step(sphg_ctx* c, int sync)
         } else {
             RC(rhs_two_phase_fused(c, 0, 0, 1));
         }
+        const sphg_params& P = c->P;
+        if (P.volume_fraction && P.volume_fraction_renorm && c->itimestep % 40 == 0)
+            RC(launch(c, "renorm", k_renorm_water, nblk(c->ph[0].n), TPB, c->ph[0].d));
     }
     if (sync) {
         const uint64_t m = SPHG_F_ALL & ~SPHG_F_X & ~SPHG_F_VX & ~SPHG_F_MIN;
@@ -2166,7 +2181,10 @@ int sphg_batch_create(sphg_ctx** ctxs, int n, sphg_batch** out)
     sphg_batch* b = new sphg_batch();
     for (int i = 0; i < n; ++i) {
         sphg_ctx* c = ctxs[i];
-        if (!c || c->legacy || c->P.single_phase || c->prof) {
+        // a captured period replays fixed step numbers, so the every-40-step
+        // renormalisation (which persists under summation_density) cannot be
+        // part of it
+        if (!c || c->legacy || c->P.single_phase || c->prof || renorm_active(c->P)) {
             delete b;
             return c ? SPHG_ERR_UNSUPPORTED : SPHG_ERR_INVALID;
         }

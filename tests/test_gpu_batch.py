@@ -4,7 +4,7 @@ contexts stepped alone."""
 import numpy as np
 import pytest
 
-from sources_amd import scenarios
+from sources_amd import abi, scenarios
 from sources_amd.batch import Batch
 from sources_amd.sph import Solver
 
@@ -43,6 +43,48 @@ def test_batch_is_bit_identical_to_solo(nsteps):
     b.close()
     for s in ss + sb:
         s.close()
+
+
+def test_full_sweep_is_bit_identical_to_solo():
+    """The whole 64-case sweep (N ~ logU[5k, 120k], 2.41M particles, seed 11):
+    33 steps batched (eager to the period boundary, two captured periods
+    replayed, eager remainder) equal the same 33 steps of each case alone."""
+    nsteps = 33
+    solo = scenarios.sweep(64, seed=11)
+    bat = scenarios.sweep(64, seed=11)
+    print("cases", len(solo), "particles", sum(c.water.n + c.soil.n for c in solo), flush=True)
+    for c in solo:
+        s = Solver(c.params, c.water, c.soil)
+        s.time_integration(nsteps, sync=True)
+        s.close()
+    sb = [Solver(c.params, c.water, c.soil) for c in bat]
+    b = Batch(sb)
+    b.run(nsteps)
+    for s in sb:
+        s.sync_all()
+    for i, (a, c) in enumerate(zip(solo, bat)):
+        for ph in ("water", "soil"):
+            pa, pc = getattr(a, ph), getattr(c, ph)
+            n = pa.n
+            for f in FIELDS:
+                x, y = getattr(pa, f)[:n], getattr(pc, f)[:n]
+                assert np.array_equal(x, y), (i, ph, f, float(np.max(np.abs(x - y))))
+    assert all(s.itimestep == nsteps for s in sb)
+    b.close()
+    for s in sb:
+        s.close()
+
+
+def test_renormalising_contexts_are_refused():
+    """A captured period replays fixed step numbers, so a context whose
+    every-40-step renormalisation persists (summation_density) is refused."""
+    c = cases()[0]
+    c.params.summation_density = 1
+    s = Solver(c.params, c.water, c.soil)
+    with pytest.raises(abi.SphgError) as e:
+        Batch([s])
+    assert e.value.code == abi.SPHG_ERR_UNSUPPORTED
+    s.close()
 
 
 def test_autotuner_picks_a_stream_count():

@@ -426,6 +426,117 @@ def test_summation_density_switch():
     s.close()
 
 
+def test_summation_density_with_vof_renormalisation_over_45_steps():
+    """summation_density = 1 with volume_fraction_renorm = 1: the leapfrog
+    leaves rho alone, so the water renormalisation of step 40
+    (declarations_sph.f90:1414-1420) persists; compared before, at and after
+    it."""
+    def make():
+        sc = _column()
+        sc.params.summation_density = 1
+        sc.params.volume_fraction_renorm = 1
+        return sc
+    sc, ref = _both(make)
+    s = Solver(sc.params, sc.water, sc.soil)
+    done = 0
+    for upto in (39, 40, 45):
+        print(f"steps {done}->{upto}", flush=True)
+        s.time_integration(upto - done, sync=True)
+        oracle.time_integration(ref.params, ref.water, ref.soil, done, upto - done)
+        done = upto
+        ew = assert_fields(sc.water, ref.water, 1e-10, label=f"water after {upto}")
+        es = assert_fields(sc.soil, ref.soil, 1e-10, label=f"soil after {upto}")
+        print(upto, "water", max(ew.values()), "soil", max(es.values()))
+    n = ref.water.n
+    assert np.max(np.abs(ref.water.vof2[:n] - ref.water.vof[:n])) > 1e-3   # the renormalisation acted
+    s.close()
+
+
+def test_single_step_renormalises_on_the_40th_step():
+    """The bare right-hand side at itimestep 40: single_step itself rescales
+    water rho by vof2 / vof and sets vof = vof2."""
+    sc, ref = _both(_column)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.itimestep = 40
+    s.single_step(sync=True)
+    oracle.single_step(ref.params, ref.water, ref.soil, 40)
+    assert_fields(sc.water, ref.water, FP64_TOL_1STEP, label="water")
+    assert_fields(sc.soil, ref.soil, FP64_TOL_1STEP, label="soil")
+    n = ref.water.n
+    assert np.array_equal(ref.water.vof[:n], ref.water.vof2[:n])
+    s.close()
+
+
+@pytest.mark.parametrize("skf", [2, 3])
+def test_gaussian_and_quintic_kernels(skf):
+    """skf 2 (Gaussian, m_particles.f90:1524-1538) and 3 (quintic,
+    :1539-1574) have a 3h support (get_scale_k, :916-934): wider cells and
+    candidate rows.  Pair lists bit-exact, one step within 1e-12."""
+    def make():
+        sc = _column()
+        sc.params.skf = skf
+        for ph in (sc.water, sc.soil):
+            ph.set_max_interaction(40 * ph.n)
+        return sc
+    for phase in ("water", "soil"):
+        sc, ref = _both(make)
+        s = Solver(sc.params, sc.water, sc.soil)
+        s.direct_find(phase)
+        g, r = getattr(sc, phase), getattr(ref, phase)
+        oracle.call_phase("direct_find", ref.params, r)
+        k = r.niac
+        assert g.niac == k > 0
+        np.testing.assert_array_equal(g.pair_i[:k], r.pair_i[:k])
+        np.testing.assert_array_equal(g.pair_j[:k], r.pair_j[:k])
+        np.testing.assert_array_equal(g.w[:k], r.w[:k])
+        np.testing.assert_array_equal(g.dwdx[:k], r.dwdx[:k])
+        s.close()
+    sc, ref = _both(make)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.direct_find_2()
+    oracle.call_cross("direct_find_2", ref.params, ref.water, ref.soil)
+    k = ref.water.niac
+    assert sc.water.niac == k > 0
+    np.testing.assert_array_equal(sc.water.pair_j[:k], ref.water.pair_j[:k])
+    np.testing.assert_array_equal(sc.water.w[:k], ref.water.w[:k])
+    s.close()
+    sc, ref = _both(make)
+    s = Solver(sc.params, sc.water, sc.soil)
+    for nsteps, done, tol in ((1, 0, FP64_TOL_1STEP), (3, 1, 1e-11)):
+        s.time_integration(nsteps - done, sync=True)
+        oracle.time_integration(ref.params, ref.water, ref.soil, done, nsteps - done)
+        ew = assert_fields(sc.water, ref.water, tol, label=f"skf {skf} water")
+        es = assert_fields(sc.soil, ref.soil, tol, label=f"skf {skf} soil")
+        print(skf, nsteps, "water", max(ew.values()), "soil", max(es.values()))
+    s.close()
+
+
+def test_dam_break_config_as_stated():
+    """The single-phase dam break exactly as configured: 50 x 100 water
+    lattice at dx = 0.005 (no jitter), 3 wall layers in a 1.0 x 0.6 tank,
+    cubic spline, h = 1.2 dx, c0 = 10 sqrt(2 g H), dt = 0.2 h / c0; 100 steps
+    against the oracle, each field within 1e-12 norm-wise or within 16x the
+    oracle's own change under a 1-ulp perturbation of its input density."""
+    make = lambda: scenarios.dam_break(dx=0.005, skf=1, jitter=0.0)  # noqa: E731
+    sc, ref = _both(make)
+    assert sc.water.ntotal == 5000
+    fields = ("x", "vx", "rho", "p", "drho", "dvx", "av", "txx", "txy", "tyy")
+    s = Solver(sc.params, sc.water)
+    done = 0
+    for nsteps in (1, 100):
+        print(f"steps {done}->{nsteps}", flush=True)
+        s.time_integration(nsteps - done, sync=True)
+        oracle.time_integration(ref.params, ref.water, None, done, nsteps - done)
+        done = nsteps
+        sens = _sensitivity(make, nsteps, fields)
+        errs = field_errors(sc.water, ref.water, fields)
+        bad = {f: (errs[f], sens[f]) for f in fields if not (errs[f] <= max(1e-12, 16.0 * sens[f]))}
+        print(nsteps, "errs", {k: f"{v:.1e}" for k, v in errs.items()},
+              "sens", {k: f"{v:.1e}" for k, v in sens.items()})
+        assert not bad, f"{nsteps} steps: {bad}"
+    s.close()
+
+
 def test_two_phase_without_delta_and_xsph():
     """artificial_density and average_velocity off: no delta-SPH on rho, and
     av keeps its value (the reference does not touch it)."""

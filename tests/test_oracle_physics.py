@@ -2,8 +2,8 @@
 exact zeros for uniform fields, first-order consistency on a lattice
 (continuity, strain rate, divergence, spin of a linear velocity field),
 conservation of antisymmetric pair terms, EOS / Drucker-Prager / Jaumann
-properties, the leapfrog free-fall recurrence and the effect-free vof
-renormalisation of the reference's step."""
+properties, the leapfrog free-fall recurrence and the vof renormalisation
+of the reference's step (effect-free unless summation_density is set)."""
 import math
 
 import numpy as np
@@ -143,7 +143,6 @@ def test_eos_rest_state_and_monotonicity():
     np.testing.assert_allclose(ph.c, P.water_c, rtol=1e-15)
     ph.rho[:] = P.water_rho0 * ph.vof * np.linspace(0.99, 1.01, ph.maxn)
     oracle.call_phase("pressure", P, ph)
-    assert np.all(np.diff(ph.p / 1.0) * 0 == 0)
     order = np.argsort(ph.rho / (P.water_rho0 * ph.vof))
     assert np.all(np.diff(ph.p[order]) >= 0)
 
@@ -235,10 +234,11 @@ def test_free_fall_leapfrog_recurrence(two_phase):
         assert np.all(ph.vx[:, 0] == 0.0)
 
 
-def test_vof_renormalisation_has_no_effect_on_the_step():
+def test_vof_renormalisation_has_no_effect_without_summation_density():
     # single_step renormalises water rho/vof every 40 steps
-    # (declarations_sph.f90:1414-1420), but second_half then recomputes both
-    # from rho_min / vof_min (:733-739): the state is bit-identical either way.
+    # (declarations_sph.f90:1414-1420); with summation_density = 0,
+    # second_half then recomputes both from rho_min / vof_min (:733-739), so
+    # the state is bit-identical either way.
     a = scenarios.savage_column(volume_fraction_renorm=1)
     b = scenarios.savage_column(volume_fraction_renorm=0)
     oracle.time_integration(a.params, a.water, a.soil, 0, 42)
@@ -246,6 +246,27 @@ def test_vof_renormalisation_has_no_effect_on_the_step():
     for f in ("x", "vx", "rho", "vof", "p"):
         assert np.array_equal(getattr(a.water, f), getattr(b.water, f)), f
         assert np.array_equal(getattr(a.soil, f), getattr(b.soil, f)), f
+
+
+def test_vof_renormalisation_keeps_the_intrinsic_density():
+    # With summation_density = 1 the leapfrog leaves rho alone, so the
+    # renormalisation of step 40 persists: it replaces the water volume
+    # fraction by the one the soil leaves (vof2 = 1 - sum m_s W / rho0_s) and
+    # keeps the intrinsic density rho / vof of every water particle, virtual
+    # ones included.  The vof it divides by is the half-step value
+    # vof_39 + dt/2 dvof_39 that first_half leaves.
+    a = scenarios.savage_column(volume_fraction_renorm=1, summation_density=1)
+    b = scenarios.savage_column(volume_fraction_renorm=0, summation_density=1)
+    oracle.time_integration(b.params, b.water, b.soil, 0, 39)
+    n = b.water.n
+    rho0 = b.water.rho[:n].copy()
+    vof_half = b.water.vof[:n] + b.params.dt / 2.0 * b.water.dvof[:n]
+    oracle.time_integration(a.params, a.water, a.soil, 0, 40)
+    oracle.time_integration(b.params, b.water, b.soil, 39, 1)
+    assert np.array_equal(b.water.rho[:n], rho0)          # without it rho never moves
+    vof2 = a.water.vof2[:n]
+    assert np.max(np.abs(vof2 - vof_half)) > 1e-3          # the renormalisation acts
+    np.testing.assert_allclose(a.water.rho[:n] / vof2, rho0 / vof_half, rtol=1e-14)
 
 
 def test_mohr_coulomb_projection():
