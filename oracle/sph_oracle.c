@@ -976,14 +976,16 @@ int oracle_single_step_for_soil(const sphg_params *P, sphg_phase_view *pl, int i
     int n = pl->ntotal + pl->nvirt, i, rc;
     double *a, *b, *c3;
     (void)itimestep;
-    if (P->summation_density || P->plasticity < 0 || P->plasticity > 3) return SPHG_ERR_UNSUPPORTED;
+    if (P->plasticity < 0 || P->plasticity > 3) return SPHG_ERR_UNSUPPORTED;
     for (i = 0; i < 2 * n; i++) pl->dvx[i] = 0.0;
     for (i = 0; i < n; i++) {
         pl->drho[i] = 0.0; pl->dsxx[i] = 0.0; pl->dsxy[i] = 0.0; pl->dsyy[i] = 0.0; pl->dp[i] = 0.0;
     }
     rc = oracle_find_pairs(P, pl);
     if (rc) return rc;
-    oracle_con_density(P, pl);
+    /* density approximation or change rate, :1558-1562 */
+    if (P->summation_density) oracle_sum_density(P, pl);
+    else oracle_con_density(P, pl);
     if (P->artificial_density) oracle_delta_sph(P, pl, pl->rho, pl->drho);
     oracle_shear_strain_rate(P, pl);
     oracle_velocity_divergence(P, pl);
@@ -1183,8 +1185,7 @@ int oracle_time_integration(const sphg_params *P, sphg_phase_view *wa, sphg_phas
 int oracle_time_integration_for_soil(const sphg_params *P, sphg_phase_view *so, int *itimestep, int nsteps)
 {
     int it, rc;
-    if (P->summation_density || (P->stress_integration != 1 && P->stress_integration != 2))
-        return SPHG_ERR_UNSUPPORTED;
+    if (P->stress_integration != 1 && P->stress_integration != 2) return SPHG_ERR_UNSUPPORTED;
     for (it = 0; it < nsteps; it++) {
         int step = ++(*itimestep);
         if (step != 1) {

@@ -1227,6 +1227,9 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     wxy *= 0.5;
     // delta_sph(rho) and av_vel only under their switches (artificial_density,
     // average_velocity); without av_vel the reference leaves av as it was
+    // single-phase soil under summation_density: sum_density replaced rho and
+    // drho keeps only the delta-SPH term (declarations_sph.f90:1558-1570)
+    if (SOLO && K.summation_density) con = 0.0;
     S.drho[a] = K.artificial_density ? con + (K.delta * ha * ca) * drd : con;
     S.txx[a] = txx; S.txy[a] = txy; S.tyy[a] = tyy; S.vcc[a] = vcc; S.wxy[a] = wxy;
     if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }

@@ -1110,7 +1110,9 @@ __global__ __launch_bounds__(256) void k_soil_b(DPhase S, Grid g, KParams K, int
 
 // sum_density (m_particles.f90:2243-2298): Shepard-normalised summation
 // density from the current (old) rho; writes rho_next.
-__global__ __launch_bounds__(256) void k_sum_density(DPhase W, Grid g, KParams K)
+// sum_density (m_particles.f90:2243-2298) of one phase; h1 = hsml(1) of the
+// phase (the link_list predicate).
+__global__ __launch_bounds__(256) void k_sum_density(DPhase W, Grid g, KParams K, double h1)
 {
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= W.n) return;
@@ -1124,7 +1126,7 @@ __global__ __launch_bounds__(256) void k_sum_density(DPhase W, Grid g, KParams K
     double rs = self * ma;
     for_each_candidate(g, cw, xa, ya, [&](int b) {
         if (b == a) return;
-        const double mh = link ? K.h1_water : sph_mhsml(ha, W.hsml[b]);
+        const double mh = link ? h1 : sph_mhsml(ha, W.hsml[b]);
         double dx, dy, r;
         if (!sph_pair_test(xa, ya, W.x[b], W.y[b], K.scale_k * mh, dx, dy, r)) return;
         if (link && !ll_reach(g, xa, ya, W.x[b], W.y[b])) return;

@@ -855,6 +855,14 @@ int rhs_soil_only(sphg_ctx* c, int soil_half, int soil_ndp)
 {
     PhaseBuf& S = c->ph[1];
     RC(build_cells(c, 1));
+    if (c->P.summation_density) {
+        // single_step_for_soil's density approximation (declarations_sph.f90:
+        // 1558-1562): rho <- sum_density.  first_half and the Drucker-Prager
+        // projection of the pack leave rho and x alone under the switch, so
+        // the summation can run before them; the pack then records the new rho.
+        RC(launch(c, "sum_density", k_sum_density, nblk(S.n), TPB, S.d, c->gl[1], c->K, c->K.h1_soil));
+        std::swap(S.d.rho, S.d.rho_next);
+    }
     RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, (float*)nullptr,
               c->gl[1], c->cs));
     switch (c->P.skf) {
@@ -937,7 +945,7 @@ int rhs_water(sphg_ctx* c)
 {
     PhaseBuf& W = c->ph[0];
     RC(build_cells(c, 0));
-    RC(launch(c, "sum_density", k_sum_density, nblk(W.n), TPB, W.d, c->gl[0], c->K));
+    RC(launch(c, "sum_density", k_sum_density, nblk(W.n), TPB, W.d, c->gl[0], c->K, c->K.h1_water));
     std::swap(W.d.rho, W.d.rho_next);
     int ma = M_DRHO_DELTA | M_XSPH;
     if (!c->P.artificial_density) ma &= ~M_DRHO_DELTA;
@@ -954,8 +962,7 @@ int check_fused_switches(const sphg_ctx* c)
     if (P.stress_integration != 1 && P.stress_integration != 2) return SPHG_ERR_UNSUPPORTED;
     // summation_density: the two-phase single_step keeps con_density (its
     // sum_density call is commented out, declarations_sph.f90:1190-1199) and
-    // the leapfrog leaves rho alone; single_step_for_soil would call sum_density
-    if (P.summation_density && soil_only(c)) return SPHG_ERR_UNSUPPORTED;
+    // the leapfrog leaves rho alone; single_step_for_soil calls sum_density
     // stress_integration = 2 and plasticity = 2 (return mapping) in the FP64 fused steps
     if ((P.stress_integration == 2 || P.plasticity == 2) && (P.precision != SPHG_FP64 || c->legacy))
         return SPHG_ERR_UNSUPPORTED;
@@ -1332,7 +1339,8 @@ int sphg_sum_density(sphg_ctx* c, int phase)
     RC(upload(c, k, SPHG_F_X | SPHG_F_RHO));
     RC(build_cells(c, k));
     PhaseBuf& b = c->ph[k];
-    RC(launch(c, "sum_density", k_sum_density, nblk(b.n), TPB, b.d, c->gl[k], c->K));
+    RC(launch(c, "sum_density", k_sum_density, nblk(b.n), TPB, b.d, c->gl[k], c->K,
+              k == 0 ? c->K.h1_water : c->K.h1_soil));
     std::swap(b.d.rho, b.d.rho_next);
     RC(download(c, k, SPHG_F_RHO));
     return check_flags(c);

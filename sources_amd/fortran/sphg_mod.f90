@@ -120,6 +120,16 @@ interface
       type(c_ptr), value :: c
       integer(c_int), value :: phase
    end function
+   integer(c_int) function sphg_delta_sph_density(c, phase) bind(C, name='sphg_delta_sph_density')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: phase
+   end function
+   integer(c_int) function sphg_sum_density(c, phase) bind(C, name='sphg_sum_density')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: phase
+   end function
    integer(c_int) function sphg_internal_force(c, phase) bind(C, name='sphg_internal_force')
       import :: c_int, c_ptr
       type(c_ptr), value :: c
@@ -264,6 +274,21 @@ end subroutine
 subroutine con_density(pl)
    type(particles), intent(inout), target :: pl
    call check(sphg_con_density(ctx, int(pl%material, c_int)), 'con_density')
+end subroutine
+
+! delta_sph(pl, pl%rho, pl%drho), m_particles.f90:2658-2695, as single_step
+! calls it after con_density (declarations_sph.f90:1207): adds the density
+! diffusion term to drho
+subroutine delta_sph_density(pl)
+   type(particles), intent(inout), target :: pl
+   call check(sphg_delta_sph_density(ctx, int(pl%material, c_int)), 'delta_sph')
+end subroutine
+
+! sum_density(parts), m_particles.f90:2243-2298 (single_step_for_water,
+! declarations_sph.f90:1465): Shepard-normalised summation density into rho
+subroutine sum_density(pl)
+   type(particles), intent(inout), target :: pl
+   call check(sphg_sum_density(ctx, int(pl%material, c_int)), 'sum_density')
 end subroutine
 
 ! Internal-force group of single_step, declarations_sph.f90:1222-1308

@@ -88,3 +88,30 @@ def test_steering_driver_writes_reference_tecplot_zones(tmp_path):
         assert np.max(np.abs(got - want[:s.n])) <= 1e-12 * max(np.max(np.abs(want[:s.n])), 1e-300), name
     np.testing.assert_allclose(st[1]["sigma_yy"], -s.p[:s.n] + s.syy[:s.n], rtol=0,
                                atol=1e-12 * np.max(np.abs(s.syy[:s.n])))
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built (flang absent at build time)")
+def test_fortran_density_procedures_match_oracle(tmp_path):
+    """The granular wrappers con_density, delta_sph_density and sum_density
+    of module sphg, called by the Fortran host at the reference's call sites
+    (declarations_sph.f90:1195-1207, :1465), against the oracle's
+    con_density, delta_sph(rho, drho) and sum_density on the same state."""
+    sc = scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3, tank_h=0.12,
+                                    jitter=0.05, seed=7)
+    # a state with motion: two oracle steps from the column
+    oracle.time_integration(sc.params, sc.water, sc.soil, 0, 2)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    snapshot.write(fin, sc.params, [sc.water, sc.soil], itimestep=2)
+    print("sph_gpu density", flush=True)
+    r = subprocess.run([EXE, fin, fout, "density"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    _, (w, s), _ = snapshot.read(fout)
+    ref = sc.copy()
+    for ph in (ref.water, ref.soil):
+        oracle.call_phase("find_pairs", ref.params, ph)
+        oracle.call_phase("con_density", ref.params, ph)
+        oracle.delta_sph(ref.params, ph, ph.rho, ph.drho)
+    oracle.call_phase("sum_density", ref.params, ref.water)
+    assert_fields(w, ref.water, 1e-12, ("drho", "rho"), "fortran water density")
+    assert_fields(s, ref.soil, 1e-12, ("drho", "rho"), "fortran soil density")

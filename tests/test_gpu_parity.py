@@ -279,6 +279,26 @@ def test_soil_only_collapse(skf, nnps):
     s.close()
 
 
+def test_soil_only_summation_density():
+    """Single-phase soil under summation_density: single_step_for_soil
+    replaces rho by sum_density (declarations_sph.f90:1558-1562) and the
+    leapfrog leaves it alone; 10 steps against the oracle."""
+    def make():
+        sc = scenarios.soil_collapse(dx=0.005, skf=1, nnps=1, jitter=0.05, seed=9)
+        sc.params.summation_density = 1
+        return sc
+    sc, ref = _both(make)
+    s = Solver(sc.params, None, sc.soil)
+    for nsteps, done, tol in ((1, 0, FP64_TOL_1STEP), (10, 1, 1e-10)):
+        print(f"soil only summation density steps {done}->{nsteps}", flush=True)
+        s.time_integration(nsteps - done, sync=True)
+        oracle.time_integration(ref.params, None, ref.soil, done, nsteps - done)
+        es = assert_fields(sc.soil, ref.soil, tol, label=f"soil only summation {nsteps} steps")
+        print(nsteps, "soil", max(es.values()))
+        _assert_fail_flags(sc.soil, ref.soil, ref.params)
+    s.close()
+
+
 def _assert_fail_flags(g, r, P):
     """The Drucker-Prager flags agree, except where the oracle's state sits on
     the yield surface to rounding (sqrt(J2) = alpha1 I1, as a projection
@@ -471,7 +491,17 @@ def test_single_step_renormalises_on_the_40th_step():
 def test_gaussian_and_quintic_kernels(skf):
     """skf 2 (Gaussian, m_particles.f90:1524-1538) and 3 (quintic,
     :1539-1574) have a 3h support (get_scale_k, :916-934): wider cells and
-    candidate rows.  Pair lists bit-exact, one step within 1e-12."""
+    candidate rows.  Pair lists bit-exact, one step within 1e-12.  The
+    Gaussian's W and grad W hold exp(-q^2): the device exp and the host libm
+    exp are both faithfully rounded but not always to the same double, so
+    for skf 2 W and grad W agree to a few ulp (held to 1e-14 relative to the
+    largest value) instead of bit for bit; the pair lists stay bit-exact."""
+    def same_values(a, b):
+        if skf == 2:
+            np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-14 * float(np.max(np.abs(b))))
+        else:
+            np.testing.assert_array_equal(a, b)
+
     def make():
         sc = _column()
         sc.params.skf = skf
@@ -488,8 +518,8 @@ def test_gaussian_and_quintic_kernels(skf):
         assert g.niac == k > 0
         np.testing.assert_array_equal(g.pair_i[:k], r.pair_i[:k])
         np.testing.assert_array_equal(g.pair_j[:k], r.pair_j[:k])
-        np.testing.assert_array_equal(g.w[:k], r.w[:k])
-        np.testing.assert_array_equal(g.dwdx[:k], r.dwdx[:k])
+        same_values(g.w[:k], r.w[:k])
+        same_values(g.dwdx[:k], r.dwdx[:k])
         s.close()
     sc, ref = _both(make)
     s = Solver(sc.params, sc.water, sc.soil)
@@ -497,8 +527,10 @@ def test_gaussian_and_quintic_kernels(skf):
     oracle.call_cross("direct_find_2", ref.params, ref.water, ref.soil)
     k = ref.water.niac
     assert sc.water.niac == k > 0
+    np.testing.assert_array_equal(sc.water.pair_i[:k], ref.water.pair_i[:k])
     np.testing.assert_array_equal(sc.water.pair_j[:k], ref.water.pair_j[:k])
-    np.testing.assert_array_equal(sc.water.w[:k], ref.water.w[:k])
+    same_values(sc.water.w[:k], ref.water.w[:k])
+    same_values(sc.water.dwdx[:k], ref.water.dwdx[:k])
     s.close()
     sc, ref = _both(make)
     s = Solver(sc.params, sc.water, sc.soil)

@@ -12,6 +12,7 @@ import pytest
 import oracle
 import test_oracle_physics as phys
 import test_oracle_pins as pins
+import test_oracle_soil_only as soil
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(oracle.__file__))
 SRC = os.path.join(ORACLE_DIR, "sph_oracle.c")
@@ -59,6 +60,10 @@ MUTANTS = {
         "if (fabs(J2 - alpha1 * I1) < EPS_YIELD_F) { k++; ph->fail[i] = 1; }",
         "if (fabs(J2 - alpha1 * I1) < 1.0e-3) { k++; ph->fail[i] = 1; }",
         [pins.test_plastic_or_not_tolerance_band]),
+    "soil_only_con_density": (
+        "if (P->summation_density) oracle_sum_density(P, pl);\n    else oracle_con_density(P, pl);",
+        "oracle_con_density(P, pl);",
+        [soil.test_soil_only_step_uses_summation_density]),
     "renorm_skipped": (
         "if (P->volume_fraction_renorm && itimestep % 40 == 0) {",
         "if (P->volume_fraction_renorm && itimestep % 40 == 0 && 0) {",

@@ -114,3 +114,51 @@ def test_free_fall_of_isolated_soil_particles():
     np.testing.assert_allclose(so.vx[:, 1], (nsteps - 0.5) * g * dt, rtol=1e-12)
     np.testing.assert_allclose(so.x[:, 1], 1.0 + g * dt * dt * nsteps * nsteps / 2, rtol=1e-12)
     np.testing.assert_array_equal(so.vx[:, 0], 0.0)
+
+
+def _shepard_numpy(so, skf):
+    """Shepard-normalised summation density (m_particles.f90:2243-2298) by
+    brute force over all pairs with numpy, the old density in the
+    denominator: rho_i = (W0 m_i + sum_j m_j W_ij) / (W0 m_i/rho_i + sum_j m_j/rho_j W_ij)."""
+    n = so.n
+    x = so.x[:n]
+    h = so.hsml[:n]
+    m, rho = so.mass[:n], so.rho[:n]
+    d = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1))
+    mh = (h[:, None] + h[None, :]) / 2.0
+    q = d / mh
+    assert skf == 1
+    f = 15.0 / (7.0 * np.pi * mh * mh)
+    c23 = float(np.float32(2.0 / 3.0))   # the reference's default-real literal 2./3.
+    w = np.where(q <= 1.0, f * (c23 - q * q + q ** 3 / 2.0),
+                 np.where(q < 2.0, f * (2.0 - q) ** 3 / 6.0, 0.0))
+    num = (w * m[None, :]).sum(1)
+    den = (w * (m / rho)[None, :]).sum(1)
+    return num / den
+
+
+def test_soil_only_step_uses_summation_density():
+    """single_step_for_soil under summation_density (declarations_sph.f90:
+    1558-1562): the density is the Shepard summation of the state the step
+    starts from, and the leapfrog leaves it alone; drho carries only the
+    delta-SPH term.  Checked against a brute-force numpy summation over all
+    particle pairs (the kernel's FP32 literal 2./3. as the reference has it;
+    its FP32 pi scales numerator and denominator alike and cancels)."""
+    sc = scenarios.soil_collapse(dx=0.01, width=0.08, height=0.05, tank_w=0.2, tank_h=0.1, jitter=0.1, seed=4)
+    P, so = sc.params, sc.soil
+    P.summation_density = 1
+    rng = np.random.default_rng(5)
+    so.rho[:so.n] *= 1.0 + 0.05 * rng.uniform(-1.0, 1.0, so.n)   # rho off its rest value
+    want = _shepard_numpy(so, P.skf)
+    before = so.rho[:so.n].copy()
+    oracle.time_integration(P, None, so, 0, 1)
+    np.testing.assert_allclose(so.rho[:so.n], want, rtol=1e-13)
+    assert np.max(np.abs(so.rho[:so.n] - before) / before) > 1e-3   # the summation changed rho
+    # a second step: the summation of the moved state, again untouched by the leapfrog
+    want2 = _shepard_numpy(so, P.skf)
+    oracle.time_integration(P, None, so, 1, 1)
+    np.testing.assert_allclose(so.rho[:so.n], want2, rtol=1e-13)
+    # without artificial density, drho stays zero (no con_density)
+    P.artificial_density = 0
+    oracle.time_integration(P, None, so, 2, 1)
+    np.testing.assert_array_equal(so.drho[:so.n], 0.0)
