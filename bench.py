@@ -10,8 +10,9 @@ The timed region is K steps bracketed by device synchronisation, timed with
 HIP events recorded on the library's stream.  A second, shorter pass with
 per-launch HIP events gives each kernel's average duration for the roofline.
 The CPU baseline times the oracle restatement of the reference algorithm
-(pair list, O(N^2) direct_find, O(N_w N_s) direct_find_2, scatter passes) on
-a bounded sample of the same scenario on one host core.
+(pair lists by link_list, O(N_w N_s) direct_find_2, scatter passes; searches
+and passes split over up to 8 OpenMP threads, the reference's cap) on a
+bounded sample of the same scenario.
 """
 import argparse
 import ctypes as C
@@ -74,23 +75,51 @@ class Hip:
         assert self.h.hipDeviceSynchronize() == 0
 
 
-def cpu_baseline(scale, steps):
-    """Oracle restatement of the reference loop on a bounded sample."""
+def cpu_baseline(args):
+    """The oracle's threaded timing build on a bounded sample, in a child
+    process (its OpenMP thread count is fixed before the runtime starts)."""
+    import subprocess
+    threads = max(1, min(8, len(os.sched_getaffinity(0))))   # the reference caps OpenMP at 8 threads
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", "--cpu-steps", str(args.cpu_steps),
+           "--cpu-dx", str(args.cpu_dx)]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed: {r.stderr[-2000:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["cores"] = threads
+    return out
+
+
+def cpu_child(args):
+    """Oracle restatement of the reference loop (liboracle_omp.so: the pair
+    searches and pair passes split over OMP_NUM_THREADS threads, as the
+    reference's OpenMP build splits them) with nnps = 2 (link_list) on the
+    slope at a bounded size."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import subprocess
     import oracle
     from sources_amd import scenarios
-    sc = scenarios.slope(dx=0.005, scale=scale)
+    lib = os.path.join(ROOT, "oracle", "liboracle_omp.so")
+    if not os.path.exists(lib):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle_omp.so"], check=True)
+    sc = scenarios.slope(dx=args.cpu_dx, scale=1.0)
+    sc.params.nnps = 2
     n = sc.n_particles
-    it = oracle.time_integration(sc.params, sc.water, sc.soil, 0, 1)  # warm-up step (step 1 path)
-    t0 = time.perf_counter()
-    oracle.time_integration(sc.params, sc.water, sc.soil, it, steps)
-    dt = time.perf_counter() - t0
-    return {
-        "value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-        "sample": f"oracle (serial FP64 restatement, nnps=1 direct_find + direct_find_2) on the same slope "
-                  f"scenario at dx=0.005 ({n} particles), {steps} steps after 1 warm-up step, "
-                  f"{dt:.1f} s wall",
-    }
+    steps = args.cpu_steps
+    with oracle.using(lib):
+        it = oracle.time_integration(sc.params, sc.water, sc.soil, 0, 1)  # warm-up step (step 1 path)
+        t0 = time.perf_counter()
+        oracle.time_integration(sc.params, sc.water, sc.soil, it, steps)
+        dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0)))))
+    print(json.dumps({
+        "value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        "sample": f"oracle restatement of the reference loop, FP64, nnps=2 link_list + O(Nw*Ns) direct_find_2, "
+                  f"pair searches and pair passes split over {threads} OpenMP threads (oracle/liboracle_omp.so), "
+                  f"on the same slope scenario at dx={args.cpu_dx} ({n} particles), {steps} steps after 1 "
+                  f"warm-up step, {dt:.1f} s wall",
+    }), flush=True)
 
 
 def learned_section(args, hip):
@@ -284,7 +313,9 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--dx", type=float, default=0.001)
     ap.add_argument("--prof-steps", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-dx", type=float, default=0.0025, help="slope spacing of the CPU baseline sample")
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the mixed and learned-surrogate section")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched scenario sweep section")
@@ -306,6 +337,9 @@ def main():
                     help="only run a plain device copy of MIB MiB (read MIB, write MIB) five times: "
                          "the known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args)
+        return
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4
@@ -430,7 +464,7 @@ def main():
     if not args.no_pcie:
         out.update(pcie_section(args, hip))
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(1.0, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -114,11 +114,148 @@ void oracle_kernel(int skf, double r, const double dx[2], double hsml,
 /* Pinned by: numpy brute force with the identical predicate                 */
 /* (tests/test_oracle_pairs.py).                                             */
 /* ------------------------------------------------------------------------ */
+#ifdef _OPENMP
+/* Timing build (cpu_baseline) only: the pair searches split over threads by
+ * i, as the reference's threaded search splits its particles
+ * (link_list_omp, m_particles.f90:1269-1443; get_niac_start_end :4219-4251).
+ * Chunks of consecutive i collect their pairs in the serial visiting order
+ * and are concatenated in order, so the result is the serial search's pair
+ * list; the checker build (no OpenMP) runs the serial loops below. */
+static int pair_gen = 1;   /* bumped by every pair search: invalidates the pass schedules */
+
+/* Pairs found for a range of i, in the serial order. */
+typedef struct {
+    int *pi, *pj;
+    double *w, *dw;
+    int n, cap;
+} pair_buf;
+
+static void pbuf_add(pair_buf *B, int i, int j, double w, const double dw[2])
+{
+    if (B->n == B->cap) {
+        B->cap = B->cap ? 2 * B->cap : 1024;
+        B->pi = (int *)realloc(B->pi, sizeof(int) * (size_t)B->cap);
+        B->pj = (int *)realloc(B->pj, sizeof(int) * (size_t)B->cap);
+        B->w = (double *)realloc(B->w, sizeof(double) * (size_t)B->cap);
+        B->dw = (double *)realloc(B->dw, 2 * sizeof(double) * (size_t)B->cap);
+    }
+    B->pi[B->n] = i + 1;
+    B->pj[B->n] = j + 1;
+    B->w[B->n] = w;
+    B->dw[2 * B->n] = dw[0];
+    B->dw[2 * B->n + 1] = dw[1];
+    B->n++;
+}
+
+typedef void (*pair_visit_fn)(const sphg_params *P, const sphg_phase_view *a, const sphg_phase_view *b, int i,
+                              int scale_k, const void *aux, pair_buf *out);
+
+/* The i range [0, na) in many chunks, taken dynamically; each chunk's pairs
+ * are kept in order, and the chunks are concatenated in order. */
+static int omp_pairs(const sphg_params *P, const sphg_phase_view *a, const sphg_phase_view *b, int na,
+                     int scale_k, const void *aux, pair_visit_fn visit, sphg_phase_view *out)
+{
+    enum { NCH = 512 };
+    pair_buf *bufs = (pair_buf *)calloc(NCH, sizeof(pair_buf));
+    long long total = 0;
+    int c, rc = SPHG_OK;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (c = 0; c < NCH; c++) {
+        int i, i0 = (int)((long long)na * c / NCH), i1 = (int)((long long)na * (c + 1) / NCH);
+        for (i = i0; i < i1; i++) visit(P, a, b, i, scale_k, aux, &bufs[c]);
+    }
+    for (c = 0; c < NCH; c++) total += bufs[c].n;
+    if (total > out->max_interaction) {
+        out->niac = out->max_interaction;
+        rc = SPHG_ERR_TOO_MANY_INTERACTIONS;
+    } else {
+        long long o = 0;
+        for (c = 0; c < NCH; c++) {
+            pair_buf *B = &bufs[c];
+            if (B->n) {
+                memcpy(out->pair_i + o, B->pi, sizeof(int) * (size_t)B->n);
+                memcpy(out->pair_j + o, B->pj, sizeof(int) * (size_t)B->n);
+                memcpy(out->w + o, B->w, sizeof(double) * (size_t)B->n);
+                memcpy(out->dwdx + 2 * o, B->dw, 2 * sizeof(double) * (size_t)B->n);
+            }
+            o += B->n;
+        }
+        out->niac = (int)total;
+    }
+    for (c = 0; c < NCH; c++) { free(bufs[c].pi); free(bufs[c].pj); free(bufs[c].w); free(bufs[c].dw); }
+    free(bufs);
+    return rc;
+}
+
+/* partners j > i of direct_find for one i */
+static void visit_direct(const sphg_params *P, const sphg_phase_view *ph, const sphg_phase_view *b, int i,
+                         int scale_k, const void *aux, pair_buf *out)
+{
+    int n = ph->ntotal + ph->nvirt, j;
+    (void)b; (void)aux;
+    for (j = i + 1; j < n; j++) {
+        double dxiac[2], driac, mhsml;
+        dxiac[0] = ph->x[2 * i] - ph->x[2 * j];
+        driac = dxiac[0] * dxiac[0];
+        dxiac[1] = ph->x[2 * i + 1] - ph->x[2 * j + 1];
+        driac = driac + dxiac[1] * dxiac[1];
+        mhsml = (ph->hsml[i] + ph->hsml[j]) / 2.0;
+        if (sqrt(driac) < scale_k * mhsml) {
+            double ww, tdw[2];
+            oracle_kernel(P->skf, sqrt(driac), dxiac, mhsml, &ww, tdw);
+            pbuf_add(out, i, j, ww, tdw);
+        }
+    }
+}
+
+/* all soil partners j of water particle i (direct_find_2) */
+static void visit_cross(const sphg_params *P, const sphg_phase_view *wa, const sphg_phase_view *so, int i,
+                        int scale_k, const void *aux, pair_buf *out)
+{
+    int ns = so->ntotal + so->nvirt, j;
+    (void)aux;
+    for (j = 0; j < ns; j++) {
+        double dxiac[2], driac, mhsml;
+        dxiac[0] = wa->x[2 * i] - so->x[2 * j];
+        driac = dxiac[0] * dxiac[0];
+        dxiac[1] = wa->x[2 * i + 1] - so->x[2 * j + 1];
+        driac = driac + dxiac[1] * dxiac[1];
+        mhsml = (wa->hsml[i] + so->hsml[j]) / 2.0;
+        if (sqrt(driac) < scale_k * mhsml) {
+            double ww, tdw[2];
+            oracle_kernel(P->skf, sqrt(driac), dxiac, mhsml, &ww, tdw);
+            pbuf_add(out, i, j, ww, tdw);
+        }
+    }
+}
+
+/* countiac of the pairs of a list (both ends) */
+static void count_ends(const sphg_phase_view *lst, int *ci, int ni, int *cj, int nj)
+{
+    int k;
+    for (k = 0; k < ni; k++) ci[k] = 0;
+    if (cj && cj != ci) for (k = 0; k < nj; k++) cj[k] = 0;
+    for (k = 0; k < lst->niac; k++) {
+        ci[lst->pair_i[k] - 1]++;
+        (cj ? cj : ci)[lst->pair_j[k] - 1]++;
+    }
+}
+#endif
+
 int oracle_direct_find(const sphg_params *P, sphg_phase_view *ph)
 {
     int n = ph->ntotal + ph->nvirt, i, j, niac = 0;
     int scale_k = oracle_scale_k(P->skf);
     if (scale_k < 0) return SPHG_ERR_BAD_KERNEL;
+#ifdef _OPENMP
+    {
+        int rc;
+        pair_gen++;
+        rc = omp_pairs(P, ph, ph, n > 0 ? n - 1 : 0, scale_k, NULL, visit_direct, ph);
+        if (rc == SPHG_OK) count_ends(ph, ph->countiac, n, NULL, 0);
+        return rc;
+    }
+#endif
     for (i = 0; i < n; i++) ph->countiac[i] = 0;
     for (i = 0; i < n - 1; i++) {
         for (j = i + 1; j < n; j++) {
@@ -156,6 +293,15 @@ int oracle_direct_find_2(const sphg_params *P, sphg_phase_view *wa, sphg_phase_v
     int nw = wa->ntotal + wa->nvirt, ns = so->ntotal + so->nvirt, i, j, niac = 0;
     int scale_k = oracle_scale_k(P->skf);
     if (scale_k < 0) return SPHG_ERR_BAD_KERNEL;
+#ifdef _OPENMP
+    {
+        int rc;
+        pair_gen++;
+        rc = omp_pairs(P, wa, so, nw, scale_k, NULL, visit_cross, wa);
+        if (rc == SPHG_OK) count_ends(wa, wa->countiac, nw, so->countiac, ns);
+        return rc;
+    }
+#endif
     for (i = 0; i < nw; i++) wa->countiac[i] = 0;
     for (j = 0; j < ns; j++) so->countiac[j] = 0;
     for (i = 0; i < nw; i++) {
@@ -221,6 +367,45 @@ int oracle_grid_geom(const sphg_params *P, const int ngridx[2], const double min
     return SPHG_OK;
 }
 
+#ifdef _OPENMP
+struct ll_aux {
+    const int *ngridx, *ghsmlx, *grid, *celldata, *xg;
+    double hsml;
+};
+
+/* partners of i in link_list's cell walk, in its order */
+static void visit_link(const sphg_params *P, const sphg_phase_view *ph, const sphg_phase_view *b, int i,
+                       int scale_k, const void *aux, pair_buf *out)
+{
+    const struct ll_aux *A = (const struct ll_aux *)aux;
+    int mn[2], mx[2], xc, yc, d, j;
+    (void)b;
+    for (d = 0; d < 2; d++) {
+        mn[d] = A->xg[2 * i + d] - A->ghsmlx[d]; if (mn[d] < 1) mn[d] = 1;
+        mx[d] = A->xg[2 * i + d] + A->ghsmlx[d]; if (mx[d] > A->ngridx[d]) mx[d] = A->ngridx[d];
+    }
+    for (yc = mn[1]; yc <= mx[1]; yc++) {
+        for (xc = mn[0]; xc <= mx[0]; xc++) {
+            j = A->grid[(yc - 1) * A->ngridx[0] + (xc - 1)];
+            while (j > i + 1) {
+                int jj = j - 1;
+                double dx[2], dr;
+                dx[0] = ph->x[2 * i] - ph->x[2 * jj];
+                dr = dx[0] * dx[0];
+                dx[1] = ph->x[2 * i + 1] - ph->x[2 * jj + 1];
+                dr = dr + dx[1] * dx[1];
+                if (sqrt(dr) < scale_k * A->hsml) {
+                    double ww, tdw[2];
+                    oracle_kernel(P->skf, sqrt(dr), dx, A->hsml, &ww, tdw);
+                    pbuf_add(out, i, jj, ww, tdw);
+                }
+                j = A->celldata[jj];
+            }
+        }
+    }
+}
+#endif
+
 int oracle_link_list(const sphg_params *P, sphg_phase_view *ph)
 {
     int n = ph->ntotal + ph->nvirt, i, j, niac = 0, rc;
@@ -243,6 +428,17 @@ int oracle_link_list(const sphg_params *P, sphg_phase_view *ph)
         celldata[i] = grid[(g[1] - 1) * ngridx[0] + (g[0] - 1)];
         grid[(g[1] - 1) * ngridx[0] + (g[0] - 1)] = i + 1;
     }
+#ifdef _OPENMP
+    {
+        struct ll_aux A;
+        A.ngridx = ngridx; A.ghsmlx = ghsmlx; A.grid = grid; A.celldata = celldata; A.xg = xg; A.hsml = hsml;
+        pair_gen++;
+        rc = omp_pairs(P, ph, ph, n > 0 ? n - 1 : 0, scale_k, &A, visit_link, ph);
+        if (rc == SPHG_OK) count_ends(ph, ph->countiac, n, NULL, 0);
+        free(grid); free(celldata); free(xg);
+        return rc;
+    }
+#endif
     for (i = 0; i < n - 1; i++) {
         int mn[2], mx[2], xc, yc, d;
         for (d = 0; d < 2; d++) {
@@ -295,7 +491,101 @@ int oracle_find_pairs(const sphg_params *P, sphg_phase_view *ph)
 
 /* ------------------------------------------------------------------------ */
 /* Pair loops.  All scatter over the phase's current pair list.              */
+/*                                                                           */
+/* PAIR_LOOP(lst, nj, body) runs body for every pair k of the list lst with  */
+/* i = pair_i(k) - 1, j = pair_j(k) - 1 and side = 3: both ends are updated, */
+/* pair after pair in list order, as the reference's serial loops do (the    */
+/* checker build).  The threaded timing build (_OPENMP) splits the pairs     */
+/* over threads by owner instead, the data-race-free form of the reference's */
+/* even pair split with per-thread slabs (get_niac_start_end,                */
+/* m_particles.f90:4219-4251): body runs with side = 1 over chunks of        */
+/* consecutive i (the lists are i-major) updating only i, then with side = 2 */
+/* over chunks of a j-sorted order updating only j.  nj: particles the       */
+/* j ends index.                                                             */
 /* ------------------------------------------------------------------------ */
+#ifdef _OPENMP
+#include <omp.h>
+typedef struct {
+    const int *pair_i, *pair_j;
+    int niac, nj, nt, gen;
+    int *kj;        /* pairs sorted by j (stable) */
+    int ib[65], jb[65];
+} pair_sched;
+
+static pair_sched sched_cache[4];
+
+static const pair_sched *sched_get(const sphg_phase_view *lst, int nj)
+{
+    int c, t, k, *cnt;
+    pair_sched *S = NULL;
+    for (c = 0; c < 4; c++) {
+        pair_sched *q = &sched_cache[c];
+        if (q->gen == pair_gen && q->pair_i == lst->pair_i && q->niac == lst->niac && q->nj == nj) return q;
+    }
+    for (c = 0; c < 4 && !S; c++)
+        if (sched_cache[c].gen != pair_gen) S = &sched_cache[c];
+    if (!S) S = &sched_cache[0];
+    free(S->kj);
+    S->pair_i = lst->pair_i; S->pair_j = lst->pair_j; S->niac = lst->niac; S->nj = nj; S->gen = pair_gen;
+    S->nt = omp_get_max_threads();
+    if (S->nt > 64) S->nt = 64;
+    S->kj = (int *)malloc(sizeof(int) * (size_t)(S->niac > 0 ? S->niac : 1));
+    cnt = (int *)calloc((size_t)nj + 1, sizeof(int));
+    for (k = 0; k < S->niac; k++) cnt[lst->pair_j[k]]++;          /* pair_j is 1-based */
+    for (k = 1; k <= nj; k++) cnt[k] += cnt[k - 1];
+    for (k = 0; k < S->niac; k++) S->kj[cnt[lst->pair_j[k] - 1]++] = k;
+    free(cnt);
+    /* chunk bounds on particle boundaries of each order */
+    S->ib[0] = 0; S->jb[0] = 0;
+    for (t = 1; t < S->nt; t++) {
+        int a = (int)((long long)S->niac * t / S->nt), b = a;
+        if (a < S->ib[t - 1]) a = S->ib[t - 1];
+        while (a > 0 && a < S->niac && lst->pair_i[a] == lst->pair_i[a - 1]) a++;
+        S->ib[t] = a;
+        if (b < S->jb[t - 1]) b = S->jb[t - 1];
+        while (b > 0 && b < S->niac && lst->pair_j[S->kj[b]] == lst->pair_j[S->kj[b - 1]]) b++;
+        S->jb[t] = b;
+    }
+    S->ib[S->nt] = S->niac; S->jb[S->nt] = S->niac;
+    return S;
+}
+
+#define PAIR_LOOP(lst, nj, ...)                                                          \
+    do {                                                                                 \
+        const pair_sched *S_ = sched_get((lst), (nj));                                   \
+        int t_;                                                                          \
+        _Pragma("omp parallel for schedule(static, 1) private(i, j, k)")                 \
+        for (t_ = 0; t_ < S_->nt; t_++) {                                                \
+            const int side = 1;                                                          \
+            int q_;                                                                      \
+            for (q_ = S_->ib[t_]; q_ < S_->ib[t_ + 1]; q_++) {                           \
+                k = q_; i = (lst)->pair_i[k] - 1; j = (lst)->pair_j[k] - 1;              \
+                __VA_ARGS__                                                              \
+            }                                                                            \
+        }                                                                                \
+        _Pragma("omp parallel for schedule(static, 1) private(i, j, k)")                 \
+        for (t_ = 0; t_ < S_->nt; t_++) {                                                \
+            const int side = 2;                                                          \
+            int q_;                                                                      \
+            for (q_ = S_->jb[t_]; q_ < S_->jb[t_ + 1]; q_++) {                           \
+                k = S_->kj[q_]; i = (lst)->pair_i[k] - 1; j = (lst)->pair_j[k] - 1;      \
+                __VA_ARGS__                                                              \
+            }                                                                            \
+        }                                                                                \
+    } while (0)
+#else
+#define PAIR_LOOP(lst, nj, ...)                                                          \
+    do {                                                                                 \
+        const int side = 3;                                                              \
+        for (k = 0; k < (lst)->niac; k++) {                                              \
+            i = (lst)->pair_i[k] - 1; j = (lst)->pair_j[k] - 1;                          \
+            __VA_ARGS__                                                                  \
+        }                                                                                \
+    } while (0)
+#endif
+#define I_END (side & 1)
+#define J_END (side & 2)
+
 #define PI_(k) (ph->pair_i[k] - 1)
 #define PJ_(k) (ph->pair_j[k] - 1)
 
@@ -305,16 +595,15 @@ void oracle_con_density(const sphg_params *P, sphg_phase_view *ph)
     int n = ph->ntotal + ph->nvirt, i, j, k;
     (void)P;
     for (i = 0; i < n; i++) ph->drho[i] = 0.0;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, n, {
         double dvx[2], vcc;
-        i = PI_(k); j = PJ_(k);
         dvx[0] = ph->vx[2 * i] - ph->vx[2 * j];
         dvx[1] = ph->vx[2 * i + 1] - ph->vx[2 * j + 1];
         vcc = dvx[0] * ph->dwdx[2 * k];
         vcc = vcc + dvx[1] * ph->dwdx[2 * k + 1];
-        ph->drho[i] = ph->drho[i] + ph->mass[j] * vcc;
-        ph->drho[j] = ph->drho[j] + ph->mass[i] * vcc;
-    }
+        if (I_END) ph->drho[i] = ph->drho[i] + ph->mass[j] * vcc;
+        if (J_END) ph->drho[j] = ph->drho[j] + ph->mass[i] * vcc;
+    });
 }
 
 /* sum_density, m_particles.f90:2243-2298 */
@@ -327,20 +616,18 @@ void oracle_sum_density(const sphg_params *P, sphg_phase_view *ph)
         oracle_kernel(P->skf, r, hv, ph->hsml[i], &selfdens, dw);
         wi[i] = selfdens * ph->mass[i] / ph->rho[i];
     }
-    for (k = 0; k < ph->niac; k++) {
-        i = PI_(k); j = PJ_(k);
-        wi[i] = wi[i] + ph->mass[j] / ph->rho[j] * ph->w[k];
-        wi[j] = wi[j] + ph->mass[i] / ph->rho[i] * ph->w[k];
-    }
+    PAIR_LOOP(ph, n, {
+        if (I_END) wi[i] = wi[i] + ph->mass[j] / ph->rho[j] * ph->w[k];
+        if (J_END) wi[j] = wi[j] + ph->mass[i] / ph->rho[i] * ph->w[k];
+    });
     for (i = 0; i < n; i++) {
         oracle_kernel(P->skf, r, hv, ph->hsml[i], &selfdens, dw);
         ph->rho[i] = selfdens * ph->mass[i];
     }
-    for (k = 0; k < ph->niac; k++) {
-        i = PI_(k); j = PJ_(k);
-        ph->rho[i] = ph->rho[i] + ph->mass[j] * ph->w[k];
-        ph->rho[j] = ph->rho[j] + ph->mass[i] * ph->w[k];
-    }
+    PAIR_LOOP(ph, n, {
+        if (I_END) ph->rho[i] = ph->rho[i] + ph->mass[j] * ph->w[k];
+        if (J_END) ph->rho[j] = ph->rho[j] + ph->mass[i] * ph->w[k];
+    });
     for (i = 0; i < n; i++) ph->rho[i] = ph->rho[i] / wi[i];
     free(wi);
 }
@@ -350,9 +637,8 @@ void oracle_delta_sph(const sphg_params *P, sphg_phase_view *ph, const double *f
 {
     int i, j, k;
     double delta = P->delta;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, ph->ntotal + ph->nvirt, {
         double dx[2], rr, muv, h;
-        i = PI_(k); j = PJ_(k);
         rr = 0.0;
         dx[0] = ph->x[2 * i] - ph->x[2 * j];
         rr = rr + dx[0] * dx[0];
@@ -362,9 +648,9 @@ void oracle_delta_sph(const sphg_params *P, sphg_phase_view *ph, const double *f
         h = 0.0;
         h = h + dx[0] * muv * ph->dwdx[2 * k];
         h = h + dx[1] * muv * ph->dwdx[2 * k + 1];
-        df[i] = df[i] + delta * ph->hsml[i] * ph->c[i] * ph->mass[j] * h / ph->rho[j];
-        df[j] = df[j] - delta * ph->hsml[j] * ph->c[j] * ph->mass[i] * h / ph->rho[i];
-    }
+        if (I_END) df[i] = df[i] + delta * ph->hsml[i] * ph->c[i] * ph->mass[j] * h / ph->rho[j];
+        if (J_END) df[j] = df[j] - delta * ph->hsml[j] * ph->c[j] * ph->mass[i] * h / ph->rho[i];
+    });
 }
 
 /* shear_strain_rate, m_particles.f90:2855-2932 (dim = 2) */
@@ -373,9 +659,8 @@ void oracle_shear_strain_rate(const sphg_params *P, sphg_phase_view *ph)
     int n = ph->ntotal + ph->nvirt, i, j, k;
     (void)P;
     for (i = 0; i < n; i++) { ph->txx[i] = 0.0; ph->tyy[i] = 0.0; ph->txy[i] = 0.0; }
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, n, {
         double dvx[2], hxx, hxy, hyy, d1, d2;
-        i = PI_(k); j = PJ_(k);
         dvx[0] = ph->vx[2 * j] - ph->vx[2 * i];
         dvx[1] = ph->vx[2 * j + 1] - ph->vx[2 * i + 1];
         d1 = ph->dwdx[2 * k]; d2 = ph->dwdx[2 * k + 1];
@@ -384,13 +669,13 @@ void oracle_shear_strain_rate(const sphg_params *P, sphg_phase_view *ph)
         hyy = 2.0 * dvx[1] * d2 - dvx[0] * d1;
         hxx = C23_F * hxx;
         hyy = C23_F * hyy;
-        ph->txx[i] = ph->txx[i] + ph->mass[j] * hxx / ph->rho[j];
-        ph->txx[j] = ph->txx[j] + ph->mass[i] * hxx / ph->rho[i];
-        ph->txy[i] = ph->txy[i] + ph->mass[j] * hxy / ph->rho[j];
-        ph->txy[j] = ph->txy[j] + ph->mass[i] * hxy / ph->rho[i];
-        ph->tyy[i] = ph->tyy[i] + ph->mass[j] * hyy / ph->rho[j];
-        ph->tyy[j] = ph->tyy[j] + ph->mass[i] * hyy / ph->rho[i];
-    }
+        if (I_END) ph->txx[i] = ph->txx[i] + ph->mass[j] * hxx / ph->rho[j];
+        if (J_END) ph->txx[j] = ph->txx[j] + ph->mass[i] * hxx / ph->rho[i];
+        if (I_END) ph->txy[i] = ph->txy[i] + ph->mass[j] * hxy / ph->rho[j];
+        if (J_END) ph->txy[j] = ph->txy[j] + ph->mass[i] * hxy / ph->rho[i];
+        if (I_END) ph->tyy[i] = ph->tyy[i] + ph->mass[j] * hyy / ph->rho[j];
+        if (J_END) ph->tyy[j] = ph->tyy[j] + ph->mass[i] * hyy / ph->rho[i];
+    });
 }
 
 /* velocity_divergence, m_particles.f90:1996-2037 */
@@ -399,16 +684,15 @@ void oracle_velocity_divergence(const sphg_params *P, sphg_phase_view *ph)
     int n = ph->ntotal + ph->nvirt, i, j, k;
     (void)P;
     for (i = 0; i < n; i++) ph->vcc[i] = 0.0;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, n, {
         double dvx[2], hvcc;
-        i = PI_(k); j = PJ_(k);
         dvx[0] = ph->vx[2 * j] - ph->vx[2 * i];
         dvx[1] = ph->vx[2 * j + 1] - ph->vx[2 * i + 1];
         hvcc = dvx[0] * ph->dwdx[2 * k];
         hvcc = hvcc + dvx[1] * ph->dwdx[2 * k + 1];
-        ph->vcc[i] = ph->vcc[i] + ph->mass[j] * hvcc / ph->rho[j];
-        ph->vcc[j] = ph->vcc[j] + ph->mass[i] * hvcc / ph->rho[i];
-    }
+        if (I_END) ph->vcc[i] = ph->vcc[i] + ph->mass[j] * hvcc / ph->rho[j];
+        if (J_END) ph->vcc[j] = ph->vcc[j] + ph->mass[i] * hvcc / ph->rho[i];
+    });
 }
 
 /* pressure, m_particles.f90:2133-2175 */
@@ -589,15 +873,14 @@ void oracle_jaumann_rate(const sphg_params *P, sphg_phase_view *ph)
     }
     for (i = 0; i < n; i++) ph->dp[i] = ph->dp[i] - P->soil_k * ph->vcc[i];
     for (i = 0; i < n; i++) ph->wxy[i] = 0.0;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, n, {
         double dvx[2], hxy;
-        i = PI_(k); j = PJ_(k);
         dvx[0] = ph->vx[2 * j] - ph->vx[2 * i];
         dvx[1] = ph->vx[2 * j + 1] - ph->vx[2 * i + 1];
         hxy = 0.5 * (dvx[0] * ph->dwdx[2 * k + 1] - dvx[1] * ph->dwdx[2 * k]);
-        ph->wxy[i] = ph->wxy[i] + ph->mass[j] * hxy / ph->rho[j];
-        ph->wxy[j] = ph->wxy[j] + ph->mass[i] * hxy / ph->rho[i];
-    }
+        if (I_END) ph->wxy[i] = ph->wxy[i] + ph->mass[j] * hxy / ph->rho[j];
+        if (J_END) ph->wxy[j] = ph->wxy[j] + ph->mass[i] * hxy / ph->rho[i];
+    });
     for (i = 0; i < n; i++) {
         ph->dsxx[i] = ph->dsxx[i] + 2.0 * ph->sxy[i] * ph->wxy[i];
         ph->dsxy[i] = ph->dsxy[i] - (ph->sxx[i] - ph->syy[i]) * ph->wxy[i];
@@ -608,11 +891,11 @@ void oracle_jaumann_rate(const sphg_params *P, sphg_phase_view *ph)
 /* art_visc, m_particles.f90:2504-2564 */
 void oracle_art_visc(const sphg_params *P, sphg_phase_view *ph)
 {
-    int i, j, k, d;
+    int i, j, k;
     double alpha = P->alpha, beta = P->beta, etq = P->etq;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, ph->ntotal + ph->nvirt, {
         double dvx[2], dx, vr, rr, mhsml;
-        i = PI_(k); j = PJ_(k);
+        int d;
         mhsml = (ph->hsml[i] + ph->hsml[j]) / 2.0;
         vr = 0.0; rr = 0.0;
         for (d = 0; d < 2; d++) {
@@ -629,21 +912,21 @@ void oracle_art_visc(const sphg_params *P, sphg_phase_view *ph)
             piv = (beta * muv - alpha * mc) * muv / mrho;
             for (d = 0; d < 2; d++) {
                 double h = -piv * ph->dwdx[2 * k + d];
-                ph->dvx[2 * i + d] = ph->dvx[2 * i + d] + ph->mass[j] * h;
-                ph->dvx[2 * j + d] = ph->dvx[2 * j + d] - ph->mass[i] * h;
+                if (I_END) ph->dvx[2 * i + d] = ph->dvx[2 * i + d] + ph->mass[j] * h;
+                if (J_END) ph->dvx[2 * j + d] = ph->dvx[2 * j + d] - ph->mass[i] * h;
             }
         }
-    }
+    });
 }
 
 /* repulsive_force, m_particles.f90:1605-1642 */
 void oracle_repulsive_force(const sphg_params *P, sphg_phase_view *ph)
 {
-    int i, j, k, d, ii;
+    int i, j, k;
     double rr0 = ph->dspp, dd = P->dd, p1 = P->p1, p2 = P->p2;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, ph->ntotal + ph->nvirt, {
         double dx[2], rr, f;
-        i = PI_(k); j = PJ_(k);
+        int d, ii;
         if (ph->itype[i] * ph->itype[j] > 0) continue;
         rr = 0.0;
         for (d = 0; d < 2; d++) {
@@ -655,9 +938,10 @@ void oracle_repulsive_force(const sphg_params *P, sphg_phase_view *ph)
         f = (pow(rr0 / rr, p1) - pow(rr0 / rr, p2)) / (rr * rr);
         ii = i;
         if (ph->itype[i] < 0) ii = j;
-        for (d = 0; d < 2; d++)
-            ph->dvx[2 * ii + d] = ph->dvx[2 * ii + d] + dd * dx[d] * f;
-    }
+        if ((ii == i && I_END) || (ii == j && J_END))
+            for (d = 0; d < 2; d++)
+                ph->dvx[2 * ii + d] = ph->dvx[2 * ii + d] + dd * dx[d] * f;
+    });
 }
 
 /* av_vel, m_particles.f90:2761-2793 */
@@ -666,16 +950,16 @@ void oracle_av_vel(const sphg_params *P, sphg_phase_view *ph)
     int n = ph->ntotal + ph->nvirt, i, j, k, d;
     double epsilon = P->epsilon;
     for (i = 0; i < 2 * n; i++) ph->av[i] = 0.0;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, n, {
         double mrho;
-        i = PI_(k); j = PJ_(k);
+        int e;
         mrho = (ph->rho[i] + ph->rho[j]) / 2.0;
-        for (d = 0; d < 2; d++) {
-            double dvx = ph->vx[2 * i + d] - ph->vx[2 * j + d];
-            ph->av[2 * i + d] = ph->av[2 * i + d] - ph->mass[j] * dvx / mrho * ph->w[k];
-            ph->av[2 * j + d] = ph->av[2 * j + d] + ph->mass[i] * dvx / mrho * ph->w[k];
+        for (e = 0; e < 2; e++) {
+            double dvx = ph->vx[2 * i + e] - ph->vx[2 * j + e];
+            if (I_END) ph->av[2 * i + e] = ph->av[2 * i + e] - ph->mass[j] * dvx / mrho * ph->w[k];
+            if (J_END) ph->av[2 * j + e] = ph->av[2 * j + e] + ph->mass[i] * dvx / mrho * ph->w[k];
         }
-    }
+    });
     for (i = 0; i < ph->ntotal; i++)
         for (d = 0; d < 2; d++) ph->av[2 * i + d] = epsilon * ph->av[2 * i + d];
 }
@@ -685,13 +969,12 @@ void oracle_df(sphg_phase_view *ph, const double *f, int d, double *out)
 {
     int n = ph->ntotal + ph->nvirt, i, j, k;
     for (i = 0; i < n; i++) out[i] = 0.0;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, n, {
         double fwx;
-        i = PI_(k); j = PJ_(k);
         fwx = (f[i] + f[j]) * ph->dwdx[2 * k + d];
-        out[i] = out[i] + ph->mass[j] / ph->rho[j] * fwx;
-        out[j] = out[j] - ph->mass[i] / ph->rho[i] * fwx;
-    }
+        if (I_END) out[i] = out[i] + ph->mass[j] / ph->rho[j] * fwx;
+        if (J_END) out[j] = out[j] - ph->mass[i] / ph->rho[i] * fwx;
+    });
 }
 
 /* df2, m_particles.f90:1802-1831: (1/rho_i) sum m_j (f_j-f_i) dW/dx_d */
@@ -699,13 +982,12 @@ void oracle_df2(sphg_phase_view *ph, const double *f, int d, double *out)
 {
     int n = ph->ntotal + ph->nvirt, i, j, k;
     for (i = 0; i < n; i++) out[i] = 0.0;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, n, {
         double fwx;
-        i = PI_(k); j = PJ_(k);
         fwx = (f[j] - f[i]) * ph->dwdx[2 * k + d];
-        out[i] = out[i] + ph->mass[j] * fwx;
-        out[j] = out[j] + ph->mass[i] * fwx;
-    }
+        if (I_END) out[i] = out[i] + ph->mass[j] * fwx;
+        if (J_END) out[j] = out[j] + ph->mass[i] * fwx;
+    });
     for (i = 0; i < n; i++) out[i] = out[i] / ph->rho[i];
 }
 
@@ -714,13 +996,12 @@ void oracle_df3(sphg_phase_view *ph, const double *f, int d, double *out)
 {
     int n = ph->ntotal + ph->nvirt, i, j, k;
     for (i = 0; i < n; i++) out[i] = 0.0;
-    for (k = 0; k < ph->niac; k++) {
+    PAIR_LOOP(ph, n, {
         double fwx;
-        i = PI_(k); j = PJ_(k);
         fwx = ((f[i] / (ph->rho[i] * ph->rho[i])) + (f[j] / (ph->rho[j] * ph->rho[j]))) * ph->dwdx[2 * k + d];
-        out[i] = out[i] + ph->mass[j] * fwx;
-        out[j] = out[j] - ph->mass[i] * fwx;
-    }
+        if (I_END) out[i] = out[i] + ph->mass[j] * fwx;
+        if (J_END) out[j] = out[j] - ph->mass[i] * fwx;
+    });
     for (i = 0; i < n; i++) out[i] = out[i] * ph->rho[i];
 }
 #undef PI_
@@ -733,31 +1014,32 @@ void oracle_df3(sphg_phase_view *ph, const double *f, int d, double *out)
 /* darcy_law, m_particles.f90:3832-3874 */
 void oracle_darcy_law(const sphg_params *P, sphg_phase_view *wa, sphg_phase_view *so)
 {
-    int k, d;
+    int i, j, k;
     double gravity = P->numeric_gravity, ks = P->soil_permeability;
-    for (k = 0; k < wa->niac; k++) {
-        int i = wa->pair_i[k] - 1, j = wa->pair_j[k] - 1;
+    PAIR_LOOP(wa, so->ntotal + so->nvirt, {
         double rrw = wa->w[k] / (wa->rho[i] * so->rho[j]);
         double cf = wa->vof[i] * so->vof[j] * wa->rho[i] * (-gravity) / ks;
+        int d;
         for (d = 0; d < 2; d++) {
             double sp = cf * (wa->vx[2 * i + d] - so->vx[2 * j + d]) * rrw;
-            wa->dvx[2 * i + d] = wa->dvx[2 * i + d] - so->mass[j] * sp;
-            so->dvx[2 * j + d] = so->dvx[2 * j + d] + wa->mass[i] * sp;
+            if (I_END) wa->dvx[2 * i + d] = wa->dvx[2 * i + d] - so->mass[j] * sp;
+            if (J_END) so->dvx[2 * j + d] = so->dvx[2 * j + d] + wa->mass[i] * sp;
         }
-    }
+    });
 }
 
 /* pore_water_pressure, m_particles.f90:3877-3909 */
 void oracle_pore_water_pressure(const sphg_params *P, sphg_phase_view *wa, sphg_phase_view *so)
 {
-    int k, d;
+    int i, j, k;
     (void)P;
-    for (k = 0; k < wa->niac; k++) {
-        int i = wa->pair_i[k] - 1, j = wa->pair_j[k] - 1;
+    PAIR_LOOP(wa, so->ntotal + so->nvirt, {
         double mprr = wa->mass[i] * wa->p[i] / (wa->rho[i] * so->rho[j]);
-        for (d = 0; d < 2; d++)
-            so->dvx[2 * j + d] = so->dvx[2 * j + d] + mprr * wa->dwdx[2 * k + d] * so->vof[j];
-    }
+        int d;
+        if (J_END)
+            for (d = 0; d < 2; d++)
+                so->dvx[2 * j + d] = so->dvx[2 * j + d] + mprr * wa->dwdx[2 * k + d] * so->vof[j];
+    });
 }
 
 /* volume_fraction_soil, m_particles.f90:3970-3988 */
@@ -770,25 +1052,23 @@ void oracle_volume_fraction_soil(const sphg_params *P, sphg_phase_view *so)
 /* volume_fraction_water2, m_particles.f90:3940-3967 */
 void oracle_volume_fraction_water2(const sphg_params *P, sphg_phase_view *wa, sphg_phase_view *so)
 {
-    int k;
-    for (k = 0; k < wa->niac; k++) {
-        int i = wa->pair_i[k] - 1, j = wa->pair_j[k] - 1;
+    int i, j, k;
+    PAIR_LOOP(wa, so->ntotal + so->nvirt, {
         double dvx0 = wa->vx[2 * i] - so->vx[2 * j];
         double dvx1 = wa->vx[2 * i + 1] - so->vx[2 * j + 1];
         double tmp = dvx0 * wa->dwdx[2 * k] + dvx1 * wa->dwdx[2 * k + 1];
-        wa->dvof[i] = wa->dvof[i] - so->mass[j] * tmp / P->soil_rho0;
-    }
+        if (I_END) wa->dvof[i] = wa->dvof[i] - so->mass[j] * tmp / P->soil_rho0;
+    });
 }
 
 /* volume_fraction_water, m_particles.f90:3912-3937 */
 void oracle_volume_fraction_water(const sphg_params *P, sphg_phase_view *wa, sphg_phase_view *so)
 {
-    int n = wa->ntotal + wa->nvirt, i, k;
+    int n = wa->ntotal + wa->nvirt, i, j, k;
     for (i = 0; i < n; i++) wa->vof2[i] = 0.0;
-    for (k = 0; k < wa->niac; k++) {
-        int ii = wa->pair_i[k] - 1, j = wa->pair_j[k] - 1;
-        wa->vof2[ii] = wa->vof2[ii] + so->mass[j] * wa->w[k];
-    }
+    PAIR_LOOP(wa, so->ntotal + so->nvirt, {
+        if (I_END) wa->vof2[i] = wa->vof2[i] + so->mass[j] * wa->w[k];
+    });
     for (i = 0; i < n; i++) wa->vof2[i] = 1.0 - wa->vof2[i] / P->soil_rho0;
 }
 

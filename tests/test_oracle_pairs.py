@@ -104,3 +104,46 @@ def test_pair_capacity_and_range_errors():
     with pytest.raises(abi.SphgError) as e:
         oracle.call_phase("link_list", sc.params, sc.water)
     assert e.value.code == abi.SPHG_ERR_OUT_OF_RANGE
+
+
+def test_threaded_timing_build_gives_the_serial_pairs_and_step():
+    """liboracle_omp.so (the bench's CPU baseline: pair searches and pair
+    passes split over threads) returns the serial oracle's pair lists in the
+    same order, and a two-step result that differs from the serial one only
+    by the summation order of the passes (1e-11)."""
+    import os
+    import subprocess
+    from sources_amd import scenarios
+    lib = os.path.join(os.path.dirname(oracle.__file__), "liboracle_omp.so")
+    if not os.path.exists(lib):
+        subprocess.run(["make", "-s", "-C", os.path.dirname(lib), "liboracle_omp.so"], check=True)
+    for nnps in (1, 2):
+        sc = scenarios.submerged_column(dx=0.005, soil_w=0.1, soil_h=0.05, tank_w=0.3, tank_h=0.12,
+                                        jitter=0.05, seed=3)
+        sc.params.nnps = nnps
+        a, b = sc.copy(), sc.copy()
+        for ph in ("water", "soil"):
+            oracle.call_phase("find_pairs", a.params, getattr(a, ph))
+            with oracle.using(lib):
+                oracle.call_phase("find_pairs", b.params, getattr(b, ph))
+            pa, pb = getattr(a, ph), getattr(b, ph)
+            assert pa.niac == pb.niac > 0
+            for f in ("pair_i", "pair_j", "w", "dwdx"):
+                np.testing.assert_array_equal(getattr(pa, f)[:pa.niac], getattr(pb, f)[:pb.niac])
+            np.testing.assert_array_equal(pa.countiac[:pa.n], pb.countiac[:pb.n])
+        oracle.call_cross("direct_find_2", a.params, a.water, a.soil)
+        with oracle.using(lib):
+            oracle.call_cross("direct_find_2", b.params, b.water, b.soil)
+        k = a.water.niac
+        assert k == b.water.niac > 0
+        for f in ("pair_i", "pair_j", "w", "dwdx"):
+            np.testing.assert_array_equal(getattr(a.water, f)[:k], getattr(b.water, f)[:k])
+        np.testing.assert_array_equal(a.soil.countiac[:a.soil.n], b.soil.countiac[:b.soil.n])
+        a, b = sc.copy(), sc.copy()
+        oracle.time_integration(a.params, a.water, a.soil, 0, 2)
+        with oracle.using(lib):
+            oracle.time_integration(b.params, b.water, b.soil, 0, 2)
+        for ph in ("water", "soil"):
+            for f in ("x", "vx", "rho", "p", "drho", "dvx"):
+                u, v = getattr(getattr(a, ph), f), getattr(getattr(b, ph), f)
+                assert np.max(np.abs(u - v)) <= 1e-11 * np.max(np.abs(u)), (ph, f)
