@@ -37,24 +37,24 @@ MUTANTS = {
         "wi[i] = selfdens * ph->mass[i] / P->water_rho0;",
         [pins.test_sum_density_reproduces_a_constant_density, pins.test_sum_density_scales_with_the_old_density]),
     "sum_density_partner_rho0": (
-        "wi[i] = wi[i] + ph->mass[j] / ph->rho[j] * ph->w[k];",
-        "wi[i] = wi[i] + ph->mass[j] / P->water_rho0 * ph->w[k];",
+        "if (I_END) wi[i] = wi[i] + ph->mass[j] / ph->rho[j] * ph->w[k];",
+        "if (I_END) wi[i] = wi[i] + ph->mass[j] / P->water_rho0 * ph->w[k];",
         [pins.test_sum_density_reproduces_a_constant_density]),
     "repulsion_on_virtual": (
         "if (ph->itype[i] < 0) ii = j;",
         "if (ph->itype[i] > 0) ii = j;",
         [pins.test_repulsive_force_real_virtual_pair]),
     "darcy_sign": (
-        "wa->dvx[2 * i + d] = wa->dvx[2 * i + d] - so->mass[j] * sp;\n"
-        "            so->dvx[2 * j + d] = so->dvx[2 * j + d] + wa->mass[i] * sp;",
-        "wa->dvx[2 * i + d] = wa->dvx[2 * i + d] + so->mass[j] * sp;\n"
-        "            so->dvx[2 * j + d] = so->dvx[2 * j + d] - wa->mass[i] * sp;",
+        "if (I_END) wa->dvx[2 * i + d] = wa->dvx[2 * i + d] - so->mass[j] * sp;\n"
+        "            if (J_END) so->dvx[2 * j + d] = so->dvx[2 * j + d] + wa->mass[i] * sp;",
+        "if (I_END) wa->dvx[2 * i + d] = wa->dvx[2 * i + d] + so->mass[j] * sp;\n"
+        "            if (J_END) so->dvx[2 * j + d] = so->dvx[2 * j + d] - wa->mass[i] * sp;",
         [pins.test_darcy_drag_two_particle_closed_form]),
     "xsph_sign": (
-        "ph->av[2 * i + d] = ph->av[2 * i + d] - ph->mass[j] * dvx / mrho * ph->w[k];\n"
-        "            ph->av[2 * j + d] = ph->av[2 * j + d] + ph->mass[i] * dvx / mrho * ph->w[k];",
-        "ph->av[2 * i + d] = ph->av[2 * i + d] + ph->mass[j] * dvx / mrho * ph->w[k];\n"
-        "            ph->av[2 * j + d] = ph->av[2 * j + d] - ph->mass[i] * dvx / mrho * ph->w[k];",
+        "if (I_END) ph->av[2 * i + e] = ph->av[2 * i + e] - ph->mass[j] * dvx / mrho * ph->w[k];\n"
+        "            if (J_END) ph->av[2 * j + e] = ph->av[2 * j + e] + ph->mass[i] * dvx / mrho * ph->w[k];",
+        "if (I_END) ph->av[2 * i + e] = ph->av[2 * i + e] + ph->mass[j] * dvx / mrho * ph->w[k];\n"
+        "            if (J_END) ph->av[2 * j + e] = ph->av[2 * j + e] - ph->mass[i] * dvx / mrho * ph->w[k];",
         [pins.test_xsph_two_particle_closed_form]),
     "plastic_or_not_tolerance": (
         "if (fabs(J2 - alpha1 * I1) < EPS_YIELD_F) { k++; ph->fail[i] = 1; }",
