@@ -222,6 +222,8 @@ int sphg_time_integration(sphg_ctx *ctx, int nsteps, int sync);
 
 /* ---- diagnostics / measurement ---------------------------------------- */
 const char *sphg_strerror(int code);
+/* The HIP error behind the last SPHG_ERR_HIP of ctx; ctx NULL: of the
+   context-free entry point sphg_direct_find_noalloc. */
 int sphg_last_hip_error(const sphg_ctx *ctx);
 /* HIP stream (hipStream_t) all work of the context is issued on. */
 void *sphg_stream(sphg_ctx *ctx);
@@ -302,6 +304,32 @@ int sphg_batch_set_streams(sphg_batch *batch, int nstreams);
    streams, and keeps the fastest (best_streams = 0: eager).  The contexts
    advance during tuning; numerics do not depend on the choice. */
 int sphg_batch_tune(sphg_batch *batch, int trial_periods, int *best_streams, double *best_ms_per_step);
+
+/* ---- pair search without allocation -------------------------------------
+   No counterpart in the reference.  For callers that cannot allocate device
+   memory: code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc.  The pair search of sphg_direct_find (direct_find,
+   m_particles.f90:938-1001: all i<j pairs with sqrt(dx^2+dy^2) <
+   scale_k*(h_i+h_j)/2, i-major / j-ascending, 1-based, with w, dwdx and
+   countiac) on caller-owned DEVICE arrays: x is x(2,n) interleaved, hsml has
+   n entries; pair_i, pair_j, w (max_interaction) and dwdx
+   (2*max_interaction) receive the pairs, countiac (n) the partner counts.
+   niac (host) receives the pair count, also when it exceeds max_interaction
+   (then SPHG_ERR_TOO_MANY_INTERACTIONS and no pair is written).  The grid box
+   of params is the geometry box; a particle outside it gives
+   SPHG_ERR_OUT_OF_RANGE.  stream: a hipStream_t (NULL: the null stream); the
+   call returns after its work is complete.
+   The library allocates nothing for it: its scratch is one static device
+   array per GPU (8.0 MiB, held until the process ends), which
+   limits n to SPHG_NOALLOC_MAX_N (larger n: SPHG_ERR_INVALID) and the cell
+   grid to SPHG_NOALLOC_MAX_CELLS cells (coarser cells are used above that;
+   the pairs do not change).  Every call on one GPU uses that GPU's scratch:
+   calls that overlap on one GPU race. */
+#define SPHG_NOALLOC_MAX_N 131072
+#define SPHG_NOALLOC_MAX_CELLS 262144
+int sphg_direct_find_noalloc(const sphg_params *params, int n, const double *x, const double *hsml,
+                             int max_interaction, int32_t *pair_i, int32_t *pair_j, double *w, double *dwdx,
+                             int32_t *countiac, int32_t *niac, void *stream);
 
 #ifdef __cplusplus
 }

@@ -181,6 +181,9 @@ def load_library(path=None):
     lib.sphg_profile_reset.argtypes = [vp]
     lib.sphg_profile_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     lib.sphg_pair_counts.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    # device pointers are passed as integers (c_void_p)
+    lib.sphg_direct_find_noalloc.argtypes = [C.POINTER(Params), C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp,
+                                             C.POINTER(C.c_int32), vp]
     if path is None:
         _lib = lib
     return lib
@@ -200,4 +203,9 @@ EXPORTED_SYMBOLS = (
     "sphg_gnn_record", "sphg_gnn_train", "sphg_gnn_step", "sphg_gnn_rollout", "sphg_gnn_sample_size",
     "sphg_gnn_debug",
     "sphg_batch_create", "sphg_batch_destroy", "sphg_batch_run", "sphg_batch_set_streams", "sphg_batch_tune",
+    "sphg_direct_find_noalloc",
 )
+
+# Limits of sphg_direct_find_noalloc (include/sphg.h)
+NOALLOC_MAX_N = 131072
+NOALLOC_MAX_CELLS = 262144

@@ -9,6 +9,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <atomic>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +22,10 @@
 namespace {
 
 constexpr int TPB = 256;
+
+// HIP error of the last failing call of the context-free entry point
+// sphg_direct_find_noalloc (sphg_last_hip_error(NULL) returns it).
+std::atomic<int> noalloc_hip_error{0};
 
 inline int nblk(int n, int b = TPB) { return (n + b - 1) / b; }
 
@@ -1453,7 +1459,7 @@ const char* sphg_strerror(int code)
     }
 }
 
-int sphg_last_hip_error(const sphg_ctx* c) { return c ? c->last_hip : 0; }
+int sphg_last_hip_error(const sphg_ctx* c) { return c ? c->last_hip : noalloc_hip_error.load(); }
 
 void* sphg_stream(sphg_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
@@ -2355,6 +2361,210 @@ int sphg_batch_tune(sphg_batch* b, int trial_periods, int* best_streams, double*
     if (!b->eager) RC(batch_set_streams(b, best));
     if (best_streams) *best_streams = best;
     if (best_ms_per_step) *best_ms_per_step = best_ms;
+    return SPHG_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// direct_find on caller-owned device arrays without any allocation.
+//
+// The scratch of this entry point (deinterleaved positions, the cell list,
+// the per-particle partner counts and their prefix, the scan's tile sums and
+// the flags) lives in one file-scope device array per GPU, `scratch`, of
+// NOALLOC_SCRATCH_BYTES.  Host code never passes the array's name to a kernel
+// (in host code that name is a stand-in, not a device address): the first
+// call on each device resolves the array's address with
+// hipGetSymbolAddress, with that device current, and keeps it per device.
+// Every call on one GPU shares that GPU's array and overwrites it; calls that
+// overlap on one GPU race.  What must start at zero (cell counts, flags) is
+// cleared by fill kernels at every call.
+// ===========================================================================
+namespace {
+
+constexpr int NOALLOC_MAX_N = SPHG_NOALLOC_MAX_N;
+constexpr int NOALLOC_MAX_CELLS = SPHG_NOALLOC_MAX_CELLS;
+constexpr int NOALLOC_MAX_TILES = (NOALLOC_MAX_CELLS > NOALLOC_MAX_N ? NOALLOC_MAX_CELLS : NOALLOC_MAX_N) / SCAN_TILE + 2;
+
+// offsets (bytes) of the scratch regions; each 256-byte aligned
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NA_X = 0;
+constexpr size_t NA_Y = NA_X + na_align(sizeof(double) * NOALLOC_MAX_N);
+constexpr size_t NA_HMAX = NA_Y + na_align(sizeof(double) * NOALLOC_MAX_N);
+constexpr size_t NA_ID = NA_HMAX + 256;
+constexpr size_t NA_KEY = NA_ID + na_align(sizeof(int) * NOALLOC_MAX_N);
+constexpr size_t NA_SLOT = NA_KEY + na_align(sizeof(int) * NOALLOC_MAX_N);
+constexpr size_t NA_ORDER = NA_SLOT + na_align(sizeof(int) * NOALLOC_MAX_N);
+constexpr size_t NA_FWD = NA_ORDER + na_align(sizeof(int) * NOALLOC_MAX_N);
+constexpr size_t NA_OFF = NA_FWD + na_align(sizeof(int) * NOALLOC_MAX_N);
+constexpr size_t NA_COUNT = NA_OFF + na_align(sizeof(int) * (NOALLOC_MAX_N + 1));
+constexpr size_t NA_PART = NA_COUNT + na_align(sizeof(int) * NOALLOC_MAX_CELLS);
+constexpr size_t NA_START = NA_PART + na_align(sizeof(int) * NOALLOC_MAX_CELLS);
+constexpr size_t NA_TILES = NA_START + na_align(sizeof(int) * (NOALLOC_MAX_CELLS + 1));
+constexpr size_t NA_FLAGS = NA_TILES + na_align(sizeof(int) * (NOALLOC_MAX_TILES + 1));
+constexpr size_t NOALLOC_SCRATCH_BYTES = NA_FLAGS + 256;
+static_assert(NOALLOC_SCRATCH_BYTES < (size_t)16 << 20, "the no-allocation scratch stays modest");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+int na_fail(hipError_t e)
+{
+    noalloc_hip_error.store((int)e);
+    return SPHG_ERR_HIP;
+}
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+unsigned char* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of this GPU's scratch array (resolved once per device).
+int noalloc_scratch(unsigned char** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return SPHG_ERR_HIP;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return na_fail(e);
+        if (!p) return na_fail(hipErrorInvalidSymbol);
+        noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = noalloc_ptr[dev];
+    return SPHG_OK;
+}
+
+// Largest of n doubles into out[0]: one workgroup (n <= NOALLOC_MAX_N).
+__global__ __launch_bounds__(256) void k_max_dbl(const double* a, int n, double* out)
+{
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, a[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+#define NACHK(expr)                                   \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) return na_fail(e_);     \
+    } while (0)
+
+template <typename... KArgs, typename... Args>
+int na_launch(hipStream_t st, void (*kern)(KArgs...), int grid, int block, Args... args)
+{
+    if (grid <= 0) return SPHG_OK;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, args...);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SPHG_OK : na_fail(e);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sphg_direct_find_noalloc(const sphg_params* params, int n, const double* x, const double* hsml,
+                             int max_interaction, int32_t* pair_i, int32_t* pair_j, double* w, double* dwdx,
+                             int32_t* countiac, int32_t* niac, void* stream)
+{
+    if (!params || !x || !hsml || !countiac || !niac) return SPHG_ERR_INVALID;
+    *niac = 0;
+    RC(check_params(*params));
+    if (n < 0 || n > NOALLOC_MAX_N || max_interaction < 0) return SPHG_ERR_INVALID;
+    if (n == 0) return SPHG_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    (void)hipGetLastError();   // an earlier call's error is not this call's
+    unsigned char* base = nullptr;
+    RC(noalloc_scratch(&base));
+    double* xs = reinterpret_cast<double*>(base + NA_X);
+    double* ys = reinterpret_cast<double*>(base + NA_Y);
+    double* hmax_d = reinterpret_cast<double*>(base + NA_HMAX);
+    int* id = reinterpret_cast<int*>(base + NA_ID);
+    int* fwd = reinterpret_cast<int*>(base + NA_FWD);
+    int* off = reinterpret_cast<int*>(base + NA_OFF);
+    int* cell_part = reinterpret_cast<int*>(base + NA_PART);
+    int* tiles = reinterpret_cast<int*>(base + NA_TILES);
+    int* flags = reinterpret_cast<int*>(base + NA_FLAGS);
+
+    // the grid: cells of at least the largest support radius over the
+    // geometry box, coarsened when the box would need more cells than the
+    // scratch holds (the pair list does not depend on the cell size)
+    RC(na_launch(st, k_max_dbl, 1, 256, hsml, n, hmax_d));
+    double hmax = 0.0;
+    NACHK(hipMemcpyAsync(&hmax, hmax_d, sizeof(double), hipMemcpyDeviceToHost, st));
+    NACHK(hipStreamSynchronize(st));
+    if (!(hmax > 0.0)) return SPHG_ERR_INVALID;
+    const sphg_params& P = *params;
+    KParams K{};
+    K.skf = P.skf;
+    K.nnps = 1;
+    K.scale_k = (P.skf == 1 || P.skf == 4) ? 2.0 : 3.0;
+    const double dgx = P.x_maxgeom - P.x_mingeom, dgy = P.y_maxgeom - P.y_mingeom;
+    double cs = K.scale_k * hmax * (1.0 + 1.0e-9);
+    double fx = std::floor(dgx / cs) + 1.0, fy = std::floor(dgy / cs) + 1.0;
+    while (fx * fy > (double)NOALLOC_MAX_CELLS) {
+        cs *= 1.25;
+        fx = std::floor(dgx / cs) + 1.0;
+        fy = std::floor(dgy / cs) + 1.0;
+    }
+    Grid g{};
+    g.x0 = P.x_mingeom; g.y0 = P.y_mingeom; g.x1 = P.x_maxgeom; g.y1 = P.y_maxgeom;
+    g.inv_cs = 1.0 / cs;
+    g.ncx = (int)fx;
+    g.ncy = (int)fy;
+    g.link_list = 0;
+    const int ncell = g.ncx * g.ncy;
+    const int ntiles = std::max((ncell + SCAN_TILE - 1) / SCAN_TILE, 1);
+    const int ptiles = std::max((n + SCAN_TILE - 1) / SCAN_TILE, 1);
+
+    DPhase d{};
+    d.n = n;
+    d.ntotal = n;
+    d.x = xs;
+    d.y = ys;
+    d.hsml = const_cast<double*>(hsml);
+    d.id = id;
+    d.key = reinterpret_cast<int*>(base + NA_KEY);
+    d.slot = reinterpret_cast<int*>(base + NA_SLOT);
+    d.order = reinterpret_cast<int*>(base + NA_ORDER);
+    d.cell_count = reinterpret_cast<int*>(base + NA_COUNT);
+    d.cell_start = reinterpret_cast<int*>(base + NA_START);
+    d.countiac = countiac;
+
+    // positions in slot order (slot t holds particle t), cleared counts and flags
+    RC(na_launch(st, k_iota, nblk(n), TPB, id, n));
+    RC(na_launch(st, k_split2, nblk(n), TPB, x, xs, ys, (const int*)id, n));
+    RC(na_launch(st, k_fill_int, nblk(ncell), TPB, d.cell_count, ncell, 0));
+    RC(na_launch(st, k_fill_int, 1, TPB, flags, 1, 0));
+    // cell list (the kernels of the context's build_cells)
+    RC(na_launch(st, k_cell_key, nblk(n), TPB, d, g, flags));
+    RC(na_launch(st, k_scan_tile, ntiles, SCAN_BLOCK, (const int*)d.cell_count, cell_part, ncell, tiles,
+                 d.cell_count));
+    RC(na_launch(st, k_cell_scatter_t, nblk(n, SCAN_BLOCK), SCAN_BLOCK, d, (const int*)cell_part,
+                 (const int*)tiles, ntiles));
+    RC(na_launch(st, k_cell_sort_t, nblk(ncell, SCAN_BLOCK), SCAN_BLOCK, d, (const int*)cell_part,
+                 (const int*)tiles, ntiles, ncell));
+    // partner counts, their prefix, the total (the kernels of materialise)
+    RC(na_launch(st, k_pair_count, nblk(n), TPB, d, d, g, K, 1, 0, 0.0, fwd, countiac));
+    RC(na_launch(st, k_scan_tile, ptiles, SCAN_BLOCK, (const int*)fwd, off, n, tiles, (int*)nullptr));
+    RC(na_launch(st, k_scan_sums, 1, SCAN_BLOCK, tiles, ptiles));
+    RC(na_launch(st, k_scan_add, nblk(n), TPB, off, n, (const int*)tiles, ptiles));
+    int host[2] = {0, 0};
+    NACHK(hipMemcpyAsync(&host[0], off + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    NACHK(hipMemcpyAsync(&host[1], flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    NACHK(hipStreamSynchronize(st));
+    if (host[1] & 1) return SPHG_ERR_OUT_OF_RANGE;
+    *niac = host[0];
+    if (host[0] > max_interaction) return SPHG_ERR_TOO_MANY_INTERACTIONS;
+    if (host[0] > 0 && (!pair_i || !pair_j || !w || !dwdx)) return SPHG_ERR_INVALID;
+    RC(na_launch(st, k_pair_write, nblk(n), TPB, d, d, g, K, 1, 0, 0.0, (const int*)off, (int*)pair_i,
+                 (int*)pair_j, w, dwdx));
+    NACHK(hipStreamSynchronize(st));
     return SPHG_OK;
 }
 
