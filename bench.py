@@ -395,7 +395,7 @@ def main():
     hip.sync()
     kern = {}
     for name in ("cell_key", "cell_scatter", "cell_sort", "scan", "fill", "first_half", "pack", "permute", "iota",
-                 "water_a", "soil_a", "soil_ab", "water_b", "soil_b", "final_water", "final_soil"):
+                 "water_x", "water_a", "soil_x", "soil_a", "soil_ab", "water_b", "soil_b", "final_water", "final_soil"):
         n, t = s.profile_get(name)
         if n:
             kern[name] = {"launches": n, "avg_us": 1e3 * t / n, "total_ms_per_step": t / args.prof_steps}

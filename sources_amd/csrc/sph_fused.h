@@ -31,17 +31,6 @@ enum : int {
     R_STRIDE = 16
 };
 
-// Record chunks staged in the per-wave LDS window (bit c = chunk c) for the
-// same-phase (LA) and cross (LX) loops of stage A and for stage B (LB).
-#ifndef SPHG_LA
-#define SPHG_LA 0
-#endif
-#ifndef SPHG_LX
-#define SPHG_LX 0
-#endif
-#ifndef SPHG_LB
-#define SPHG_LB 0
-#endif
 #ifndef SPHG_WPB
 #define SPHG_WPB 4           // waves per workgroup of the stage kernels (consecutive particles share a CU)
 #endif
@@ -422,166 +411,39 @@ struct Masks {
 };
 
 // ------------------------------------------------------------------------
-// Wave-cooperative LDS windows.
-//
-// A workgroup is one wavefront holding 64 consecutive cell-sorted
-// particles.  The union of their 3 x 3 cell blocks is at most six contiguous
-// ranges of the cell order (one or two particle rows, each needing three
-// neighbour rows).  The wave copies those records into LDS with coalesced
-// loads -- eight lanes per 128-byte record, so one load instruction covers
-// 1 KB of consecutive memory -- and every candidate test and partner read
-// afterwards is an LDS access.  A per-lane gather would instead touch 64
-// cache lines per load instruction, which is what bounds a naive gather.
-// Waves whose window does not fit (more than two particle rows or more than
-// WIN_N records) use the same code with global loads.
+// Partner iteration.  A particle's candidates are the three cell rows of
+// its 3 x 3 block, each a contiguous range [k0, k1) of the cell-sorted
+// records.  A scan tests every candidate with the exact predicate and keeps
+// one 64-bit mask per row; the three masks are concatenated into one
+// 128-bit mask over row0 ++ row1 ++ row2, and the dense loops visit its set
+// bits.  A row over 64 candidates, or more than 128 in all, sets the
+// overflow flag and that particle walks its rows directly.
 // ------------------------------------------------------------------------
-constexpr int WIN_N = 256;
-
-struct Seg {
-    int row, klo, khi, off;
-};
-
-struct Win {
-    Seg s[6];
-    int ns, total, ok;
-};
-
-__device__ __forceinline__ void win_add(Win& w, const Grid& g, const int* start, int row, int c0, int c1)
-{
-    if (row < 0 || row >= g.ncy) return;
-    c0 = max(c0, 0);
-    c1 = min(c1, g.ncx - 1);
-    Seg& sg = w.s[w.ns++];
-    sg.row = row;
-    sg.klo = start[row * g.ncx + c0];
-    sg.khi = start[row * g.ncx + c1 + 1];
-    sg.off = w.total;
-    w.total += sg.khi - sg.klo;
-}
-
-// Window of the wave; cx, cy are the lane's (clamped) cell; `last` is the
-// highest valid lane.  All lanes compute the same window.
-__device__ __forceinline__ Win make_window(const Grid& g, const int* start, int cx, int cy, int last, bool valid,
-                                           bool use = true)
-{
-    Win w;
-    w.ns = 0;
-    w.total = 0;
-    w.ok = 0;
-    if (!use) return w;
-    const int cyf = __shfl(cy, 0), cyl = __shfl(cy, last);
-    const int cxf = __shfl(cx, 0), cxl = __shfl(cx, last);
-    if (cyl == cyf) {
-        for (int r = cyf - 1; r <= cyf + 1; ++r) win_add(w, g, start, r, cxf - 1, cxl + 1);
-    } else if (cyl == cyf + 1) {
-        const uint64_t mb = __ballot(valid && cy == cyl);
-        const int fb = __builtin_ctzll(mb);
-        const int cxa = __shfl(cx, fb - 1), cxb = __shfl(cx, fb);
-        // row cyf holds the wave's first lanes (high columns a0..a1), row cyl
-        // its last lanes (low columns b0..b1)
-        const int a0 = cxf - 1, a1 = cxa + 1, b0 = cxb - 1, b1 = cxl + 1;
-        const bool merge = b1 >= a0 - 1;   // the two column ranges touch or overlap
-        win_add(w, g, start, cyf - 1, a0, a1);
-        if (merge) {
-            win_add(w, g, start, cyf, min(a0, b0), max(a1, b1));
-            win_add(w, g, start, cyl, min(a0, b0), max(a1, b1));
-        } else {
-            win_add(w, g, start, cyf, b0, b1);
-            win_add(w, g, start, cyf, a0, a1);
-            win_add(w, g, start, cyl, b0, b1);
-            win_add(w, g, start, cyl, a0, a1);
-        }
-        win_add(w, g, start, cyl + 1, b0, b1);
-    } else {
-        return w;
-    }
-    w.ok = w.total <= WIN_N;
-    return w;
-}
-
-// LDS index base of a lane's neighbour row: the segment of that row which
-// contains the lane's candidate range [k0, k1).
-__device__ __forceinline__ int win_base(const Win& w, int row, int k0, int k1)
-{
-    int b = 0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        if (i < w.ns && w.s[i].row == row && w.s[i].klo <= k0 && k1 <= w.s[i].khi) b = w.s[i].off - w.s[i].klo;
-    }
-    return b;
-}
-
-// Number of LDS chunks for the chunk mask LM, and the LDS slot of chunk c.
-template <int LM>
-struct Lds {
-    static constexpr int n = __builtin_popcount(LM);
-    static constexpr int slot(int c) { return __builtin_popcount(LM & ((1 << c) - 1)); }
-};
-
-// Copy the chunks (16-byte pairs of record slots) selected by LM of every
-// window record into lds[slot][i].
-template <int LM>
-__device__ __forceinline__ void win_load(const Win& w, const RecV& rec, double2 (*lds)[WIN_N], int lane)
-{
-    constexpr int NC = Lds<LM>::n;
-    if constexpr (NC == 0) return;
-    int CH[NC > 0 ? NC : 1];
-    {
-        int q = 0;
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (LM & (1 << c)) CH[q++] = c;
-    }
-    constexpr int NCD = NC > 0 ? NC : 1;
-    const int n = w.total * NC;
-#pragma unroll 2
-    for (int idx = lane; idx < n; idx += 64) {
-        const int i = idx / NCD, c = idx - i * NCD;
-        int sg = 0;
-#pragma unroll
-        for (int q = 1; q < 6; ++q)
-            if (q < w.ns && i >= w.s[q].off) sg = q;
-        const int k = w.s[sg].klo + (i - w.s[sg].off);
-        int ch = CH[0];
-#pragma unroll
-        for (int q = 1; q < NC; ++q)
-            if (c == q) ch = CH[q];
-        lds[c][i] = rec(k, ch);
-    }
-    __syncthreads();
-}
-
-// The lane's three candidate rows with LDS (window) or global (fallback) bases.
 struct LaneRows {
-    int k0[3], k1[3], base[3];
+    int k0[3], k1[3], base[3];   // base: index offset of a staged copy (0: the records themselves)
 };
 
-__device__ __forceinline__ LaneRows lane_rows(const Rows& R, const Win& w, int cy)
+__device__ __forceinline__ LaneRows lane_rows(const Rows& R)
 {
     LaneRows L;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         L.k0[r] = R.k0[r];
         L.k1[r] = R.k1[r];
-        L.base[r] = w.ok ? win_base(w, cy - 1 + r, R.k0[r], R.k1[r]) : 0;
+        L.base[r] = 0;
     }
     return L;
 }
 
-
-// Scan of one lane row: exact predicate on the candidates' positions
-// (LDS chunk 0 of the window, or the record).  accept(i, k, xy) gets the
-// window index i and the record index k.
+// Scan of one candidate row: accept(k, xy) on the candidates' positions.
 template <class P>
-__device__ __forceinline__ uint64_t scan_lane_row(int k0, int k1, int base, int self, bool win, const double2* ldsxy,
-                                                  const RecV& rec, P&& accept)
+__device__ __forceinline__ uint64_t scan_row(int k0, int k1, int self, const RecV& rec, P&& accept)
 {
     uint64_t m = 0;
     if (k1 - k0 > 64) return 0;
 #pragma unroll 4
     for (int k = k0; k < k1; ++k) {
-        const double2 b = win ? ldsxy[k + base] : rec(k, 0);
-        const bool ok = k != self && accept(k + base, k, b);
+        const bool ok = k != self && accept(k, rec(k, 0));
         m |= (uint64_t)ok << (k - k0);
     }
     return m;
@@ -598,21 +460,19 @@ __device__ __forceinline__ int ovf_of(const LaneRows& L)
 
 // Dense visit of the masked partners, then the direct walk of overflowed rows.
 template <class IT, class LD, class B, class P>
-__device__ __forceinline__ void visit_win(IT it, const LaneRows& L, int ovf, int self, LD&& load, P&& accept,
-                                          B&& body)
+__device__ __forceinline__ void visit(IT it, const LaneRows& L, int ovf, int self, LD&& load, P&& accept, B&& body)
 {
     int i = 0;
 #pragma unroll 1
-    for (int k = it.next(i); k >= 0; k = it.next(i)) body(load(i, k));
+    for (int k = it.next(i); k >= 0; k = it.next(i)) body(load(k));
     if (ovf) {
 #pragma unroll 1
         for (int r = 0; r < 3; ++r) {
             if (!(ovf & (1 << r))) continue;
 #pragma unroll 1
             for (int kk = L.k0[r]; kk < L.k1[r]; ++kk) {
-                const int i = kk + L.base[r];
-                if (kk == self || !accept(i, kk)) continue;
-                body(load(i, kk));
+                if (kk == self || !accept(kk)) continue;
+                body(load(kk));
             }
         }
     }
@@ -652,11 +512,13 @@ template <class T>
 __device__ __forceinline__ T pick(bool c, T x, T y) { return c ? x : y; }
 
 // Iterator over the set bits of a CatMask; branch-free apart from the loop.
+// next() returns the record index of the next partner (-1 at the end) and
+// i = the index in the staged copy (record index + base of its row).
 struct CatIter {
     uint64_t lo, hi;
     int s1, s2;        // start of row 1 and row 2 in the concatenated list
     int e0, e1, e2;    // record index - list index, per row
-    int d0, d1, d2;    // window index - record index, per row
+    int d0, d1, d2;    // staged index - record index, per row
     __device__ __forceinline__ int next(int& i)
     {
         const bool in_lo = lo != 0;
@@ -714,19 +576,16 @@ struct Nb {
 constexpr int NEED_IV = 1 << 8;   // recompute 1/rho and m/rho of the partner
 constexpr int CH_H = 1 << 6;      // chunk holding h
 
-// Loads the chunks NEED of partner record k (window index i) -- plus the h
-// chunk when with_h (non-uniform smoothing length): chunks in LM come from
-// the LDS window when it is in use, the others from the record.
-template <int LM, int NEED>
-__device__ __forceinline__ Nb load_nb(const double2 (*lds)[WIN_N], const RecV& rec, bool win, int i, int k,
-                                      bool with_h)
+// Loads the chunks NEED of partner record k -- plus the h chunk when with_h
+// (non-uniform smoothing length).
+template <int NEED>
+__device__ __forceinline__ Nb load_nb(const RecV& rec, int k, bool with_h)
 {
     Nb b{};
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
         if (!(NEED & (1 << c)) && !(c == 6 && with_h)) continue;
-        if ((LM & (1 << c)) && win) b.set(c, lds[Lds<LM>::slot(c)][i]);
-        else b.set(c, rec(k, c));
+        b.set(c, rec(k, c));
     }
     if (NEED & 4) {
         // chunk 2 carries m with the sign of itype (virtual < 0)
@@ -740,7 +599,7 @@ __device__ __forceinline__ Nb load_nb(const double2 (*lds)[WIN_N], const RecV& r
     return b;
 }
 
-// Exact pair predicate on a candidate at window/global index i.
+// Exact pair predicate on a candidate.
 struct Pred {
     double xa, ya, ha, thr;
     int uni, link;
@@ -775,29 +634,16 @@ __device__ __forceinline__ WaveCtx wave_ctx(int n)
     return c;
 }
 
-__device__ __forceinline__ void lane_cell(const Grid& g, double x, double y, int& cx, int& cy)
+// Mask of a particle over its candidate rows in the records rec: scan of the
+// three rows with the predicate P (self: own record index, or -1).
+template <class A>
+__device__ __forceinline__ CatMask scan_mask(const LaneRows& L, int self, const RecV& rec, A&& acc)
 {
-    grid_cell(g, x, y, cx, cy);
-    cx = min(max(cx, 0), g.ncx - 1);
-    cy = min(max(cy, 0), g.ncy - 1);
+    const uint64_t m0 = scan_row(L.k0[0], L.k1[0], self, rec, acc);
+    const uint64_t m1 = scan_row(L.k0[1], L.k1[1], self, rec, acc);
+    const uint64_t m2 = scan_row(L.k0[2], L.k1[2], self, rec, acc);
+    return cat_masks(L, m0, m1, m2);
 }
-
-
-template <int LM>
-__device__ __forceinline__ double2 xy_of(const double2 (*lds)[WIN_N], const RecV& rec, bool win, int i, int k)
-{
-    if ((LM & 1) && win) return lds[0][i];
-    return rec(k, 0);
-}
-
-template <int LM>
-__device__ __forceinline__ double h_of(const double2 (*lds)[WIN_N], const RecV& rec, bool win, int i, int k)
-{
-    if ((LM & CH_H) && win) return lds[Lds<LM>::slot(6)][i].x;
-    return rec(k, 6).x;
-}
-
-constexpr int lds_rows(int a, int b) { return (a > b ? a : b) > 0 ? (a > b ? a : b) : 1; }
 
 // Chunks read per partner by each loop (h only when it is not uniform).
 constexpr int NEED_SAME_A = 0x07 | NEED_IV;          // x,y v rho,m(itype in its sign); 1/rho, m/rho
@@ -806,61 +652,152 @@ constexpr int NEED_SX_A = 0x0F | NEED_IV;            // water partners of soil: 
 constexpr int NEED_B = 0x3F | NEED_IV;               // x,y v rho,m vof,p s3,c s1,s2
 
 // ------------------------------------------------------------------------
-// Stage A, water: same-phase terms (con_density, delta_sph(rho),
-// shear_strain_rate, av_vel, repulsive_force) and cross terms from soil
-// partners (darcy_law water side, volume_fraction_water2 / _water), then
-// pressure() per particle.  Writes a-space outputs, the same-phase masks
-// and the record slots p, c_new, vof*sigma.  LA / LX select the record
-// chunks staged in the LDS window for the same-phase and cross loops.
+// Cross terms, water side (stage A of single_step's interaction block,
+// declarations_sph.f90:1380-1410): darcy_law's drag on the water,
+// volume_fraction_water2's dvof and volume_fraction_water's vof2 from the
+// soil partners (direct_find_2's pairs, virtual x virtual and r = 0
+// included).  Its own kernel: the loop needs few registers, so it runs at
+// twice the occupancy of the same-phase stage.
 // ------------------------------------------------------------------------
-template <int SKF, int LA, int LX, bool U>
-__global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, double* recWw,
-                                                 const double* recS, Masks MW, Grid g, KParams K, FusedK F,
-                                                 int cross)
+template <int SKF, bool U>
+__global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, const double* recS, Grid g,
+                                      KParams K, FusedK F)
 {
-    static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
-    static_assert(SPHG_WPB == 1 || (LA == 0 && LX == 0), "windows need one wave per workgroup");
-    __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
     const RecV RW = recv(recW, W.n), RS = recv(recS, S.n);
+    const WaveCtx wc = wave_ctx(W.n);
+    const double2 xy = RW(wc.tt, 0), va = RW(wc.tt, 1), rm = RW(wc.tt, 2);
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, vofa = RW(wc.tt, 3).x;
+    const double ha = U ? K.h1_water : RW(wc.tt, 6).x;
+    const double irhoa = 1.0 / rhoa;
+    const int a = W.order[wc.tt];
+    const bool ux = U || F.uni_x;
+    const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya));
+    const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
+    auto acc = [&](int k, double2 b) { return PX(b, ux ? ha : RS(k, 6).x, g, K); };
+    auto acc_k = [&](int k) { return acc(k, RS(k, 0)); };
+    const CatMask cm = scan_mask(L, -1, RS, acc);
+    auto load = [&](int k) { return load_nb<NEED_WX_A>(RS, k, !ux); };
+    const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
+    const KC kx = make_kc(SKF, sph_mhsml(ha, ux ? RS(0, 6).x : ha));
+    double dg_x = 0.0, dg_y = 0.0, dvof_x = 0.0, vof2 = 0.0;
+    int cntx = 0;
+    visit(cat_iter(cm, L), L, cm.ovf, -1, load, acc_k, [&](const Nb& b) {
+        const double dx = xa - b.xy.x, dy = ya - b.xy.y;
+        const double dr = dx * dx + dy * dy;
+        const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
+        const double r = dr * ir;
+        double ww, gx, gy;
+        if (ux) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, ir, dx, dy, ww, gx, gy);
+        ++cntx;
+        const double mb = b.rm.y;
+        const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
+        const double sc = mb * (cfa * b.hv.y) * (ww * irhoa * b.iv.x);
+        dg_x += sc * dvx;
+        dg_y += sc * dvy;
+        dvof_x += mb * (dvx * gx + dvy * gy);
+        vof2 += mb * ww;
+    });
+    if (!wc.valid) return;
+    W.drgx[a] = dg_x;
+    W.drgy[a] = dg_y;
+    W.dvof_x[a] = dvof_x * (1.0 / K.s_rho0);
+    W.vof2[a] = 1.0 - vof2 / K.s_rho0;
+    W.countiac[a] = cntx;
+}
+
+// ------------------------------------------------------------------------
+// Cross terms, soil side: darcy_law's drag on the soil and
+// pore_water_pressure (m_particles.f90:3832-3909) from the water partners
+// (whose pressure is the new one the water pack evaluated).
+// ------------------------------------------------------------------------
+template <int SKF, bool U>
+__global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, const double* recW, Grid g,
+                                     KParams K, FusedK F)
+{
+    const RecV RS = recv(recS, S.n), RW = recv(recW, W.n);
+    const WaveCtx wc = wave_ctx(S.n);
+    const double2 xy = RS(wc.tt, 0), va = RS(wc.tt, 1), rm = RS(wc.tt, 2);
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, vofa = RS(wc.tt, 3).x;
+    const double ha = U ? K.h1_soil : RS(wc.tt, 6).x;
+    const double irhoa = 1.0 / rhoa;
+    const int a = S.order[wc.tt];
+    const bool ux = U || F.uni_x;
+    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya));
+    const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
+    auto acc = [&](int k, double2 b) { return PX(b, ux ? ha : RW(k, 6).x, g, K); };
+    auto acc_k = [&](int k) { return acc(k, RW(k, 0)); };
+    const CatMask cm = scan_mask(L, -1, RW, acc);
+    auto load = [&](int k) { return load_nb<NEED_SX_A>(RW, k, !ux); };
+    const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
+    const KC kx = make_kc(SKF, sph_mhsml(ux ? RW(0, 6).x : ha, ha));
+    double dg_x = 0.0, dg_y = 0.0, po_x = 0.0, po_y = 0.0;
+    int cntx = 0;
+    visit(cat_iter(cm, L), L, cm.ovf, -1, load, acc_k, [&](const Nb& b) {
+        const double dx = b.xy.x - xa, dy = b.xy.y - ya;   // water - soil
+        const double dr = dx * dx + dy * dy;
+        const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
+        const double r = dr * ir;
+        double ww, gx, gy;
+        if (ux) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+        else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(b.hv.x, ha)), r, ir, dx, dy, ww, gx, gy);
+        ++cntx;
+        const double mw = b.rm.y, rhow = b.rm.x;
+        const double sc = mw * (b.hv.y * rhow * cfs) * (ww * b.iv.x * irhoa);
+        dg_x += sc * (b.v.x - vxa);
+        dg_y += sc * (b.v.y - vya);
+        const double mprr = mw * b.pc.x * (b.iv.x * irhoa);
+        po_x += mprr * gx;
+        po_y += mprr * gy;
+    });
+    if (!wc.valid) return;
+    S.drgx[a] = dg_x;
+    S.drgy[a] = dg_y;
+    S.porx[a] = vofa * po_x;
+    S.pory[a] = vofa * po_y;
+    S.countiac[a] = cntx;
+}
+
+// ------------------------------------------------------------------------
+// Stage A, water: same-phase terms (con_density, delta_sph(rho),
+// shear_strain_rate, av_vel, repulsive_force), then pressure() per particle.
+// Writes a-space outputs, the same-phase masks and the record slots
+// vof*sigma for stage B.  cross: two-phase run (countiac then counts the
+// soil partners, which k_water_x4 writes).
+// ------------------------------------------------------------------------
+template <int SKF, bool U>
+__global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recWw, Masks MW, Grid g, KParams K,
+                                      FusedK F, int cross)
+{
+    const RecV RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(W.n);
     const int t = wc.t;
     auto RA = [&](int c) { return RW(wc.tt, c); };
     const double2 xy = RA(0), va = RA(1), rm = RA(2), hi = RA(6);
-    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = RA(3).x;
-    const double irhoa = 1.0 / rhoa;
+    const bool link = K.nnps == 2;
+    // U (checked by the host): h uniform in the phase (hsml(1) everywhere, a
+    // wave-uniform value), integer repulsion exponents
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, vofa = RA(3).x;
+    const double ha = U ? K.h1_water : hi.x;
     const int ita = (int)hi.y;
     const int a = W.order[wc.tt];
     const double ca = W.c[a];
-    const bool link = K.nnps == 2;
-    // U (checked by the host): h uniform in both phases, integer repulsion exponents
     const bool uni = U || link || F.uni_w;
-    const bool ux = U || F.uni_x;
     const KC kc = make_kc(SKF, link ? K.h1_water : ha);
-    int cx, cy;
-    lane_cell(g, xa, ya, cx, cy);
-
-    // ---- same-phase window, scan, masks
-    const Win w = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
-    if (w.ok) win_load<LA>(w, RW, lds, wc.lane);
-    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya), w, cy);
+    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya));
     const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
-    auto acc_b = [&](int i, int k, double2 b) {
-        return P(b, uni ? ha : h_of<LA>(lds, RW, w.ok, i, k), g, K);
-    };
-    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, RW, w.ok, i, k)); };
-    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], RW, acc_b);
-    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], RW, acc_b);
-    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], RW, acc_b);
-    const CatMask cm = cat_masks(L, m0, m1, m2);
+    auto acc = [&](int k, double2 b) { return P(b, uni ? ha : RW(k, 6).x, g, K); };
+    auto acc_k = [&](int k) { return acc(k, RW(k, 0)); };
+    const CatMask cm = scan_mask(L, t, RW, acc);
     if (wc.valid) {
         MW.m[t] = cm.lo;
         MW.m[W.n + t] = cm.hi;
         MW.ovf[t] = cm.ovf;
     }
-    auto load = [&](int i, int k) { return load_nb<LA, NEED_SAME_A>(lds, RW, w.ok, i, k, !uni); };
+    auto load = [&](int k) { return load_nb<NEED_SAME_A>(RW, k, !uni); };
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
     int cnt = 0;
-    visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
+    visit(cat_iter(cm, L), L, cm.ovf, t, load, acc_k, [&](const Nb& b) {
         const double dx = xa - b.xy.x, dy = ya - b.xy.y;
         const double dr = dx * dx + dy * dy;
         const Dist D = dist_of(dr);
@@ -886,45 +823,6 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
             repy += K.dd * dy * fr;
         }
     });
-
-    // ---- cross window (soil partners)
-    double dg_x = 0.0, dg_y = 0.0, dvof_x = 0.0, vof2 = 0.0;
-    int cntx = 0;
-    if (cross) {
-        if (LA != 0) __syncthreads();
-        const Win wx = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
-        if (wx.ok) win_load<LX>(wx, RS, lds, wc.lane);
-        const LaneRows LXr = lane_rows(block_rows(g, S.cell_start, xa, ya), wx, cy);
-        const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
-        auto accx_b = [&](int i, int k, double2 b) {
-            return PX(b, ux ? ha : h_of<LX>(lds, RS, wx.ok, i, k), g, K);
-        };
-        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, RS, wx.ok, i, k)); };
-        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], RS, accx_b);
-        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], RS, accx_b);
-        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], RS, accx_b);
-        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_WX_A>(lds, RS, wx.ok, i, k, !ux); };
-        const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
-        const KC kx = make_kc(SKF, sph_mhsml(ha, ux ? RS(0, 6).x : ha));
-        const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
-        visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
-            const double dx = xa - b.xy.x, dy = ya - b.xy.y;
-            const double dr = dx * dx + dy * dy;
-            const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
-            const double r = dr * ir;
-            double ww, gx, gy;
-            if (ux) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
-            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(ha, b.hv.x)), r, ir, dx, dy, ww, gx, gy);
-            ++cntx;
-            const double mb = b.rm.y;
-            const double dvx = vxa - b.v.x, dvy = vya - b.v.y;
-            const double sc = mb * (cfa * b.hv.y) * (ww * irhoa * b.iv.x);
-            dg_x += sc * dvx;
-            dg_y += sc * dvy;
-            dvof_x += mb * (dvx * gx + dvy * gy);
-            vof2 += mb * ww;
-        });
-    }
     if (!wc.valid) return;
     const double mu = K.w_mu;
     txx *= SPH_C23_F;
@@ -936,13 +834,7 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
     if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
     if (K.average_velocity) { W.avx[a] = avx; W.avy[a] = avy; }
     W.repx[a] = repx; W.repy[a] = repy;
-    if (cross) {
-        W.drgx[a] = dg_x; W.drgy[a] = dg_y; W.dvof_x[a] = dvof_x * (1.0 / K.s_rho0);
-        W.vof2[a] = 1.0 - vof2 / K.s_rho0;
-        W.countiac[a] = cntx;
-    } else {
-        W.countiac[a] = cnt;
-    }
+    if (!cross) W.countiac[a] = cnt;
     W.nsame[a] = cnt;
     // p and the new c (chunks 3 and 4) come from the pack
     double2* WP = reinterpret_cast<double2*>(recWw) + t;
@@ -956,35 +848,27 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, DPhase S, const double* recW, do
 // art_visc, delta_sph(vof, dvof); /rho, + repulsion, + gravity, - Darcy drag.
 // The partner masks come from stage A.
 // ------------------------------------------------------------------------
-template <int SKF, int LB>
-__global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Grid g, KParams K,
-                                                 FusedK F, int mask)
+template <int SKF, bool U>
+__global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Grid g, KParams K, FusedK F, int mask)
 {
-    static_assert(LB == 0 || (LB & 1), "window needs positions");
-    static_assert(SPHG_WPB == 1 || LB == 0, "windows need one wave per workgroup");
-    __shared__ double2 lds[lds_rows(Lds<LB>::n, 0)][WIN_N];
     const RecV RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(W.n);
     const int t = wc.t;
     auto RA = [&](int c) { return RW(wc.tt, c); };
     const double2 axy = RA(0), av = RA(1), arm = RA(2), a3 = RA(3), apc = RA(4), a12 = RA(5);
-    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, ha = RA(6).x, vofa = a3.x;
+    // U: h uniform in the phase (checked by the host)
+    const double xa = axy.x, ya = axy.y, vxa = av.x, vya = av.y, rhoa = arm.x, vofa = a3.x;
+    const double ha = U ? K.h1_water : RA(6).x;
     const double pa = a3.y, ca = apc.y, fxx_a = a12.x, fxy_a = a12.y, fyy_a = apc.x;
     const double irhoa = 1.0 / rhoa;
     const int a = W.order[wc.tt];
     const bool link = K.nnps == 2;
-    const bool uni = link || F.uni_w;
+    const bool uni = U || link || F.uni_w;
     const KC kc = make_kc(SKF, link ? K.h1_water : ha);
-    int cx, cy;
-    lane_cell(g, xa, ya, cx, cy);
-    const Win w = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LB != 0);
-    if (w.ok) win_load<LB>(w, RW, lds, wc.lane);
-    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya), w, cy);
+    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya));
     const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
-    auto acc_i = [&](int i, int k) {
-        return P(xy_of<LB>(lds, RW, w.ok, i, k), uni ? ha : h_of<LB>(lds, RW, w.ok, i, k), g, K);
-    };
-    auto load = [&](int i, int k) { return load_nb<LB, NEED_B>(lds, RW, w.ok, i, k, !uni); };
+    auto acc_k = [&](int k) { return P(RW(k, 0), uni ? ha : RW(k, 6).x, g, K); };
+    auto load = [&](int k) { return load_nb<NEED_B>(RW, k, !uni); };
     CatMask cm{0, 0, 0};
     if (wc.valid) {
         cm.lo = MW.m[t];
@@ -994,7 +878,7 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
     const double et2 = K.etq * K.etq;
     double Px = 0.0, Py = 0.0, Sxx = 0.0, Sxyy = 0.0, Sxyx = 0.0, Syy = 0.0;
     double artx = 0.0, arty = 0.0, dvd = 0.0;
-    visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
+    visit(cat_iter(cm, L), L, cm.ovf, t, load, acc_k, [&](const Nb& b) {
         const double dx = xa - b.xy.x, dy = ya - b.xy.y;
         const double dr = dx * dx + dy * dy;
         const Dist D = dist_of(dr);
@@ -1056,24 +940,23 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
 //               velocity_divergence, Jaumann spin, av_vel, repulsive_force
 //               (stage A); -df3(vof*p) + df3(vof*sigma) (:1270-1277),
 //               art_visc, delta_sph on p, sxx, sxy, syy (stage B);
-//   cross       Darcy drag and pore-water pressure from the water partners
-//               (new water p);
 //   tail        plastic_or_not + plastic_flow_rule3 + Jaumann_rate, then
-//               /rho, + repulsion, + gravity, + Darcy drag, + pore pressure.
+//               /rho, + repulsion, + gravity, + Darcy drag, + pore pressure
+//               (the cross terms k_soil_x4 wrote; cross: two-phase run).
 // ------------------------------------------------------------------------
-template <int SKF, int LA, int LX, bool U, bool SOLO>
-__global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS, const double* recW, Grid g,
-                                      KParams K, FusedK F, int cross, int mask)
+template <int SKF, bool U, bool SOLO>
+__global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, const double* recS, Grid g, KParams K, FusedK F, int cross,
+                                         int mask)
 {
-    static_assert((LA == 0 || (LA & 1)) && (LX == 0 || (LX & 1)), "window needs positions");
-    static_assert(SPHG_WPB == 1 || (LA == 0 && LX == 0), "windows need one wave per workgroup");
-    __shared__ double2 lds[lds_rows(Lds<LA>::n, Lds<LX>::n)][WIN_N];
-    const RecV RS = recv(recS, S.n), RW = recv(recW, W.n);
+    const RecV RS = recv(recS, S.n);
     const WaveCtx wc = wave_ctx(S.n);
     const int t = wc.t;
     auto RA = [&](int c) { return RS(wc.tt, c); };
     const double2 xy = RA(0), va = RA(1), rm = RA(2), a3 = RA(3), a4 = RA(4), a12 = RA(5), hi = RA(6);
-    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, ha = hi.x, vofa = a3.x;
+    // U (checked by the host): h uniform in the phase (hsml(1) everywhere, a
+    // wave-uniform value), integer repulsion exponents, Drucker-Prager
+    const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, vofa = a3.x;
+    const double ha = U ? K.h1_soil : hi.x;
     const double pa = a3.y, sxxa = a12.x, sxya = a12.y, syya = a4.x;
     const double ca = a4.y;          // sound speed before this step (delta_sph of rho)
     const double cn = K.s_c;         // sound speed after it (stage B)
@@ -1088,66 +971,15 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     const int ita = (int)hi.y;
     const int a = S.order[wc.tt];
     const bool link = K.nnps == 2;
-    // U (checked by the host): h uniform in both phases, integer repulsion exponents
     const bool uni = U || link || F.uni_s;
-    const bool ux = U || F.uni_x;
     const KC kc = make_kc(SKF, link ? K.h1_soil : ha);
-    int cx, cy;
-    lane_cell(g, xa, ya, cx, cy);
-
-    double dg_x = 0.0, dg_y = 0.0, po_x = 0.0, po_y = 0.0;
-    int cntx = 0;
-    if (!SOLO && cross) {
-        const Win wx = make_window(g, W.cell_start, cx, cy, wc.last, wc.valid, LX != 0);
-        if (wx.ok) win_load<LX>(wx, RW, lds, wc.lane);
-        const LaneRows LXr = lane_rows(block_rows(g, W.cell_start, xa, ya), wx, cy);
-        const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
-        auto accx_b = [&](int i, int k, double2 b) {
-            return PX(b, ux ? ha : h_of<LX>(lds, RW, wx.ok, i, k), g, K);
-        };
-        auto accx_i = [&](int i, int k) { return accx_b(i, k, xy_of<LX>(lds, RW, wx.ok, i, k)); };
-        const uint64_t c0 = scan_lane_row(LXr.k0[0], LXr.k1[0], LXr.base[0], -1, wx.ok, lds[0], RW, accx_b);
-        const uint64_t c1 = scan_lane_row(LXr.k0[1], LXr.k1[1], LXr.base[1], -1, wx.ok, lds[0], RW, accx_b);
-        const uint64_t c2 = scan_lane_row(LXr.k0[2], LXr.k1[2], LXr.base[2], -1, wx.ok, lds[0], RW, accx_b);
-        auto loadx = [&](int i, int k) { return load_nb<LX, NEED_SX_A>(lds, RW, wx.ok, i, k, !ux); };
-        const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
-        const KC kx = make_kc(SKF, sph_mhsml(ux ? RW(0, 6).x : ha, ha));
-        const CatMask cx3 = cat_masks(LXr, c0, c1, c2);
-        visit_win(cat_iter(cx3, LXr), LXr, cx3.ovf, -1, loadx, accx_i, [&](const Nb& b) {
-            const double dx = b.xy.x - xa, dy = b.xy.y - ya;   // water - soil
-            const double dr = dx * dx + dy * dy;
-            const double ir = (dr > 0.0) ? fast_rsq(dr) : 0.0;
-            const double r = dr * ir;
-            double ww, gx, gy;
-            if (ux) kern_fast<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
-            else kern_fast<SKF, true>(make_kc(SKF, sph_mhsml(b.hv.x, ha)), r, ir, dx, dy, ww, gx, gy);
-            ++cntx;
-            const double mw = b.rm.y, rhow = b.rm.x;
-            const double sc = mw * (b.hv.y * rhow * cfs) * (ww * b.iv.x * irhoa);
-            dg_x += sc * (b.v.x - vxa);
-            dg_y += sc * (b.v.y - vya);
-            const double mprr = mw * b.pc.x * (b.iv.x * irhoa);
-            po_x += mprr * gx;
-            po_y += mprr * gy;
-        });
-        po_x *= vofa;
-        po_y *= vofa;
-    }
-    if (!SOLO && cross && LX != 0) __syncthreads();
-    const Win w = make_window(g, S.cell_start, cx, cy, wc.last, wc.valid, LA != 0);
-    if (w.ok) win_load<LA>(w, RS, lds, wc.lane);
-    const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya), w, cy);
+    const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya));
     const Pred P{xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0};
-    auto acc_b = [&](int i, int k, double2 b) {
-        return P(b, uni ? ha : h_of<LA>(lds, RS, w.ok, i, k), g, K);
-    };
-    auto acc_i = [&](int i, int k) { return acc_b(i, k, xy_of<LA>(lds, RS, w.ok, i, k)); };
-    const uint64_t m0 = scan_lane_row(L.k0[0], L.k1[0], L.base[0], t, w.ok, lds[0], RS, acc_b);
-    const uint64_t m1 = scan_lane_row(L.k0[1], L.k1[1], L.base[1], t, w.ok, lds[0], RS, acc_b);
-    const uint64_t m2 = scan_lane_row(L.k0[2], L.k1[2], L.base[2], t, w.ok, lds[0], RS, acc_b);
-    const CatMask cm = cat_masks(L, m0, m1, m2);
-    auto load = [&](int i, int k) {
-        Nb b = load_nb<LA, NEED_B>(lds, RS, w.ok, i, k, !uni);
+    auto acc = [&](int k, double2 b) { return P(b, uni ? ha : RS(k, 6).x, g, K); };
+    auto acc_k = [&](int k) { return acc(k, RS(k, 0)); };
+    const CatMask cm = scan_mask(L, t, RS, acc);
+    auto load = [&](int k) {
+        Nb b = load_nb<NEED_B>(RS, k, !uni);
         if (mc) b.rpre = RS(k, 7).x;
         return b;
     };
@@ -1157,7 +989,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     double Fx = 0.0, Fy = 0.0;
     double artx = 0.0, arty = 0.0, dP = 0.0, dXX = 0.0, dXY = 0.0, dYY = 0.0;
     int cnt = 0;
-    visit_win(cat_iter(cm, L), L, cm.ovf, t, load, acc_i, [&](const Nb& b) {
+    visit(cat_iter(cm, L), L, cm.ovf, t, load, acc_k, [&](const Nb& b) {
         const double dx = xa - b.xy.x, dy = ya - b.xy.y;
         const double dr = dx * dx + dy * dy;
         const Dist D = dist_of(dr);
@@ -1235,12 +1067,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     if (ita > 0) { avx *= K.epsilon; avy *= K.epsilon; }
     if (K.average_velocity) { S.avx[a] = avx; S.avy[a] = avy; }
     S.repx[a] = repx; S.repy[a] = repy;
-    if (cross) {
-        S.drgx[a] = dg_x; S.drgy[a] = dg_y; S.porx[a] = po_x; S.pory[a] = po_y;
-        S.countiac[a] = cntx;
-    } else {
-        S.countiac[a] = cnt;
-    }
+    if (!cross) S.countiac[a] = cnt;
     S.nsame[a] = cnt;
     S.c_next[a] = cn;
 
@@ -1301,8 +1128,8 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, DPhase W, const double* recS,
     ay = ay + repy;
     ay = ay + K.gravity;
     if (mask & M_CROSS) {
-        ax = ax + dg_x; ay = ay + dg_y;
-        ax = ax + po_x; ay = ay + po_y;
+        ax = ax + S.drgx[a]; ay = ay + S.drgy[a];
+        ax = ax + S.porx[a]; ay = ay + S.pory[a];
     }
     if (mask & M_DELTA_FIELD) {
         const double cf = K.delta * ha * cn;

@@ -801,9 +801,10 @@ int rhs_two_phase_legacy(sphg_ctx* c)
 }
 
 // The fused step: pack (first_half + soil Drucker-Prager projections +
-// records), partner masks, stage A, stage B.  water_half / soil_half /
-// soil_ndp select the leapfrog variant (step 1, later steps, or a bare
-// single_step).
+// records), cross terms, partner masks, stage A, stage B.  The soil kernels
+// read only the packed records of both phases (the water pressure comes from
+// the pack): they run on the side stream while the water kernels run on the
+// main stream.
 template <int SKF>
 int stages_fused(sphg_ctx* c)
 {
@@ -811,31 +812,44 @@ int stages_fused(sphg_ctx* c)
     PhaseBuf& S = c->ph[1];
     const int cross = c->P.single_phase ? 0 : 1;
     const bool rep_int = c->K.ip1 >= 0 && c->K.ip2 >= 0;
-    // The soil stage reads only the packed records of both phases (the water
-    // pressure comes from the pack): it runs on the side stream while the
-    // water stages run on the main stream.
+    const double* rW = W.rec;
+    const double* rS = S.rec;
+    // U instantiations: h uniform in the phase(s) the kernel reads (then
+    // hsml(1) is every h), integer repulsion exponents, Drucker-Prager; the
+    // general ones read h per particle and partner
+    const bool us = c->ph[1].hmin == c->ph[1].hmax && rep_int && c->K.yield_criterion != 1;
+    const bool uw = c->ph[0].hmin == c->ph[0].hmax && rep_int;
+    const bool ux = c->F.uni_x != 0;
     if (side_on(c)) RC(fork_side(c));
     {
         SideScope side(c);
-        // U: uniform h in both phases, integer repulsion exponents, Drucker-Prager
-        if ((c->K.nnps == 2 || c->F.uni_s) && c->F.uni_x && rep_int && c->K.yield_criterion != 1)
-            RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d,
-                      W.d, (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
-        else
-            RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, false, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
-                      S.d, W.d, (const double*)S.rec, (const double*)W.rec, c->gl[1], c->K, c->F, cross,
-                      mask_b(c, 1)));
+        if (cross) {
+            if (ux) RC(launch(c, "soil_x", k_soil_x4<SKF, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, rS, rW,
+                              c->gl[1], c->K, c->F));
+            else RC(launch(c, "soil_x", k_soil_x4<SKF, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, rS, rW,
+                           c->gl[1], c->K, c->F));
+        }
+        if (us) RC(launch(c, "soil_ab", k_soil_ab4<SKF, true, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rS,
+                          c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
+        else RC(launch(c, "soil_ab", k_soil_ab4<SKF, false, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rS,
+                       c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
     }
-    if ((c->K.nnps == 2 || c->F.uni_w) && c->F.uni_x && rep_int)
-        RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d,
-                  (const double*)W.rec, W.rec, (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
-    else
-        RC(launch(c, "water_a", k_water_a4<SKF, SPHG_LA, SPHG_LX, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d,
-                  S.d, (const double*)W.rec, W.rec, (const double*)S.rec, W.masks, c->gl[0], c->K, c->F, cross));
+    if (cross) {
+        if (ux) RC(launch(c, "water_x", k_water_x4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, rW, rS,
+                          c->gl[0], c->K, c->F));
+        else RC(launch(c, "water_x", k_water_x4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, rW, rS,
+                       c->gl[0], c->K, c->F));
+    }
+    if (uw) RC(launch(c, "water_a", k_water_a4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
+                      W.masks, c->gl[0], c->K, c->F, cross));
+    else RC(launch(c, "water_a", k_water_a4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
+                   W.masks, c->gl[0], c->K, c->F, cross));
     swap_c(W);
     swap_c(S);
-    RC(launch(c, "water_b", k_water_b4<SKF, SPHG_LB>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, (const double*)W.rec,
-              W.masks, c->gl[0], c->K, c->F, mask_b(c, 0)));
+    if (uw) RC(launch(c, "water_b", k_water_b4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.masks,
+                      c->gl[0], c->K, c->F, mask_b(c, 0)));
+    else RC(launch(c, "water_b", k_water_b4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.masks,
+                   c->gl[0], c->K, c->F, mask_b(c, 0)));
     if (side_on(c)) RC(join_side(c));
     return SPHG_OK;
 }
@@ -847,12 +861,12 @@ int stages_soil_only(sphg_ctx* c)
 {
     PhaseBuf& S = c->ph[1];
     const double* rs = S.rec;
-    if ((c->K.nnps == 2 || c->F.uni_s) && c->K.yield_criterion != 1)
-        RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, true, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
-                  S.d, S.d, rs, rs, c->gl[1], c->K, c->F, 0, mask_b(c, 1)));
+    if (S.hmin == S.hmax && c->K.yield_criterion != 1)
+        RC(launch(c, "soil_ab", k_soil_ab4<SKF, true, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rs, c->gl[1],
+                  c->K, c->F, 0, mask_b(c, 1)));
     else
-        RC(launch(c, "soil_ab", k_soil_ab4<SKF, SPHG_LA, SPHG_LX, false, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK,
-                  S.d, S.d, rs, rs, c->gl[1], c->K, c->F, 0, mask_b(c, 1)));
+        RC(launch(c, "soil_ab", k_soil_ab4<SKF, false, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rs, c->gl[1],
+                  c->K, c->F, 0, mask_b(c, 1)));
     swap_c(S);
     return SPHG_OK;
 }
