@@ -50,7 +50,7 @@ STEP_BYTES = {"fp64": {"water": 368 + 28, "soil": 284 + 28},
 
 class Hip:
     def __init__(self):
-        self.h = C.CDLL("libamdhip64.so")
+        self.h = C.CDLL("libamdhip64.so.7")   # the runtime libsphg.so links
         self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
         self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
         self.h.hipEventSynchronize.argtypes = [C.c_void_p]

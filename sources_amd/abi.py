@@ -135,7 +135,12 @@ def load_library(path=None):
         raise RuntimeError(
             f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
         )
-    lib = C.CDLL(p)
+    # RTLD_DEEPBIND: the library's HIP calls bind to the HIP runtime it links
+    # (ROCm's libamdhip64.so.7).  Without it, a process that imported torch
+    # first has torch's own bundled libamdhip64.so in the global symbol scope,
+    # and every HIP symbol of the library (kernel registration, streams,
+    # graph capture) would bind to that other runtime instead.
+    lib = C.CDLL(p, mode=os.RTLD_LAZY | os.RTLD_DEEPBIND)
     vp = C.c_void_p
     lib.sphg_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
     lib.sphg_destroy.argtypes = [vp]

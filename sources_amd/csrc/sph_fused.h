@@ -460,7 +460,7 @@ __device__ __forceinline__ int ovf_of(const LaneRows& L)
 
 // Dense visit of the masked partners, then the direct walk of overflowed rows.
 template <class IT, class LD, class B, class P>
-__device__ __forceinline__ void visit(IT it, const LaneRows& L, int ovf, int self, LD&& load, P&& accept, B&& body)
+__device__ __forceinline__ void visit(IT it, const LaneRows& L, int ovf, int self, const LD& load, P&& accept, B&& body)
 {
     int i = 0;
 #pragma unroll 1
@@ -577,13 +577,13 @@ constexpr int NEED_IV = 1 << 8;   // recompute 1/rho and m/rho of the partner
 constexpr int CH_H = 1 << 6;      // chunk holding h
 
 // Loads the chunks NEED of partner record k -- plus the h chunk when with_h
-// (non-uniform smoothing length).
-template <int NEED>
+// (non-uniform smoothing length); FROM = 1: all but the positions (chunk 0).
+template <int NEED, int FROM = 0>
 __device__ __forceinline__ Nb load_nb(const RecV& rec, int k, bool with_h)
 {
     Nb b{};
 #pragma unroll
-    for (int c = 0; c < 7; ++c) {
+    for (int c = FROM; c < 7; ++c) {
         if (!(NEED & (1 << c)) && !(c == 6 && with_h)) continue;
         b.set(c, rec(k, c));
     }
@@ -598,6 +598,29 @@ __device__ __forceinline__ Nb load_nb(const RecV& rec, int k, bool with_h)
     }
     return b;
 }
+
+// Partner loader of a dense loop: the chunks NEED of record k (chunk 7, the
+// density before the Mohr-Coulomb criterion, when mc), whole or as positions
+// plus the rest.
+template <int NEED>
+struct Loader {
+    RecV rec;
+    bool with_h, mc;
+    __device__ __forceinline__ double2 xy(int k) const { return rec(k, 0); }
+    __device__ __forceinline__ Nb rest(int k, double2 p) const
+    {
+        Nb b = load_nb<NEED, 1>(rec, k, with_h);
+        b.xy = p;
+        if (mc) b.rpre = rec(k, 7).x;
+        return b;
+    }
+    __device__ __forceinline__ Nb operator()(int k) const
+    {
+        Nb b = load_nb<NEED>(rec, k, with_h);
+        if (mc) b.rpre = rec(k, 7).x;
+        return b;
+    }
+};
 
 // Exact pair predicate on a candidate.
 struct Pred {
@@ -676,7 +699,7 @@ __global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, co
     auto acc = [&](int k, double2 b) { return PX(b, ux ? ha : RS(k, 6).x, g, K); };
     auto acc_k = [&](int k) { return acc(k, RS(k, 0)); };
     const CatMask cm = scan_mask(L, -1, RS, acc);
-    auto load = [&](int k) { return load_nb<NEED_WX_A>(RS, k, !ux); };
+    const Loader<NEED_WX_A> load{RS, !ux, false};
     const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
     const KC kx = make_kc(SKF, sph_mhsml(ha, ux ? RS(0, 6).x : ha));
     double dg_x = 0.0, dg_y = 0.0, dvof_x = 0.0, vof2 = 0.0;
@@ -728,7 +751,7 @@ __global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, con
     auto acc = [&](int k, double2 b) { return PX(b, ux ? ha : RW(k, 6).x, g, K); };
     auto acc_k = [&](int k) { return acc(k, RW(k, 0)); };
     const CatMask cm = scan_mask(L, -1, RW, acc);
-    auto load = [&](int k) { return load_nb<NEED_SX_A>(RW, k, !ux); };
+    const Loader<NEED_SX_A> load{RW, !ux, false};
     const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
     const KC kx = make_kc(SKF, sph_mhsml(ux ? RW(0, 6).x : ha, ha));
     double dg_x = 0.0, dg_y = 0.0, po_x = 0.0, po_y = 0.0;
@@ -794,7 +817,7 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recW
         MW.m[W.n + t] = cm.hi;
         MW.ovf[t] = cm.ovf;
     }
-    auto load = [&](int k) { return load_nb<NEED_SAME_A>(RW, k, !uni); };
+    const Loader<NEED_SAME_A> load{RW, !uni, false};
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;
     int cnt = 0;
     visit(cat_iter(cm, L), L, cm.ovf, t, load, acc_k, [&](const Nb& b) {
@@ -868,7 +891,7 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
     const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya));
     const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
     auto acc_k = [&](int k) { return P(RW(k, 0), uni ? ha : RW(k, 6).x, g, K); };
-    auto load = [&](int k) { return load_nb<NEED_B>(RW, k, !uni); };
+    const Loader<NEED_B> load{RW, !uni, false};
     CatMask cm{0, 0, 0};
     if (wc.valid) {
         cm.lo = MW.m[t];
@@ -978,11 +1001,7 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, const double* recS, Grid g, K
     auto acc = [&](int k, double2 b) { return P(b, uni ? ha : RS(k, 6).x, g, K); };
     auto acc_k = [&](int k) { return acc(k, RS(k, 0)); };
     const CatMask cm = scan_mask(L, t, RS, acc);
-    auto load = [&](int k) {
-        Nb b = load_nb<NEED_B>(RS, k, !uni);
-        if (mc) b.rpre = RS(k, 7).x;
-        return b;
-    };
+    const Loader<NEED_B> load{RS, !uni, mc};
     const double et2 = K.etq * K.etq;
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;
     double avx = 0.0, avy = 0.0, repx = 0.0, repy = 0.0;

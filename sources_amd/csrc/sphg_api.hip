@@ -773,6 +773,32 @@ struct SideScope {
     ~SideScope() { c->lstream = nullptr; }
 };
 
+// The side stream forked off the main stream for the life of the object and
+// joined back by join() or, at the latest, by the destructor -- also when an
+// error returns early -- so that every fork is joined.  (In a captured graph
+// an unjoined fork invalidates the capture and leaves the side stream in it.)
+struct SideFork {
+    sphg_ctx* c;
+    bool open = false;
+    int rc = SPHG_OK;
+    explicit SideFork(sphg_ctx* ctx) : c(ctx)
+    {
+        if (side_on(c)) {
+            rc = fork_side(c);
+            open = rc == SPHG_OK;
+        }
+    }
+    int join()
+    {
+        if (!open) return SPHG_OK;
+        open = false;
+        return join_side(c);
+    }
+    ~SideFork() { join(); }
+    SideFork(const SideFork&) = delete;
+    SideFork& operator=(const SideFork&) = delete;
+};
+
 int mask_b(const sphg_ctx* c, int phase)
 {
     int m = MASK_B;
@@ -820,7 +846,8 @@ int stages_fused(sphg_ctx* c)
     const bool us = c->ph[1].hmin == c->ph[1].hmax && rep_int && c->K.yield_criterion != 1;
     const bool uw = c->ph[0].hmin == c->ph[0].hmax && rep_int;
     const bool ux = c->F.uni_x != 0;
-    if (side_on(c)) RC(fork_side(c));
+    SideFork fork(c);
+    RC(fork.rc);
     {
         SideScope side(c);
         if (cross) {
@@ -850,8 +877,7 @@ int stages_fused(sphg_ctx* c)
                       c->gl[0], c->K, c->F, mask_b(c, 0)));
     else RC(launch(c, "water_b", k_water_b4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.masks,
                    c->gl[0], c->K, c->F, mask_b(c, 0)));
-    if (side_on(c)) RC(join_side(c));
-    return SPHG_OK;
+    return fork.join();
 }
 
 // Single-phase soil (single_step_for_soil, declarations_sph.f90:1530-1655):
@@ -903,7 +929,8 @@ int stages_mixed(sphg_ctx* c)
     PhaseBuf& S = c->ph[1];
     float* qW = reinterpret_cast<float*>(W.rec);
     float* qS = reinterpret_cast<float*>(S.rec);
-    if (side_on(c)) RC(fork_side(c));   // soil stage beside the water stages
+    SideFork fork(c);   // soil stage beside the water stages
+    RC(fork.rc);
     {
         SideScope side(c);
         RC(launch(c, "soil_ab", k_soil_ab_mx<SKF>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, (const float*)qS,
@@ -915,8 +942,7 @@ int stages_mixed(sphg_ctx* c)
     swap_c(S);
     RC(launch(c, "water_b", k_water_b_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, (const float*)qW, W.masks,
               c->gl[0], c->K, c->F, c->X, mask_b(c, 0)));
-    if (side_on(c)) RC(join_side(c));
-    return SPHG_OK;
+    return fork.join();
 }
 
 int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
@@ -929,21 +955,23 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
     // The two phases' cell lists and packs are independent: the soil chain
     // runs on the side stream beside the water chain (the stages after them
     // read both phases' records).
-    const bool side = side_on(c);
-    if (side) {
-        RC(fork_side(c));
-        SideScope sc(c);
-        RC(build_cells(c, 1));
-        RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
-                  c->cs));
-    } else {
-        RC(build_cells(c, 1));
+    {
+        SideFork fork(c);
+        RC(fork.rc);
+        if (fork.open) {
+            SideScope sc(c);
+            RC(build_cells(c, 1));
+            RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
+                      c->cs));
+        } else {
+            RC(build_cells(c, 1));
+        }
+        RC(build_cells(c, 0));
+        RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, c->gl[0], c->cs));
+        if (fork.open) RC(fork.join());
+        else RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
+                       c->cs));
     }
-    RC(build_cells(c, 0));
-    RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, c->gl[0], c->cs));
-    if (side) RC(join_side(c));
-    else RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
-                   c->cs));
     if (mixed) {
         switch (c->P.skf) {
         case 1: return stages_mixed<1>(c);
@@ -2121,7 +2149,6 @@ struct sphg_batch {
     bool eager = false;                 // autotuner's choice: per-context launches, no graph
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
-    std::vector<char> side;             // the contexts' side-stream setting, restored on destroy
 };
 
 namespace {
@@ -2134,13 +2161,21 @@ void batch_release_graph(sphg_batch* b)
     b->graph = nullptr;
 }
 
-int batch_set_streams(sphg_batch* b, int nstreams)
+// Streams and events of the batch are released only after their work has
+// drained: a replay may still be running on them.
+void batch_free_streams(sphg_batch* b)
 {
+    for (hipStream_t s : b->streams) hipStreamSynchronize(s);
     batch_release_graph(b);
     for (hipStream_t s : b->streams) hipStreamDestroy(s);
     for (hipEvent_t e : b->join) hipEventDestroy(e);
     b->streams.clear();
     b->join.clear();
+}
+
+int batch_set_streams(sphg_batch* b, int nstreams)
+{
+    batch_free_streams(b);
     b->nstreams = std::max(1, std::min(nstreams, (int)b->ctx.size()));
     for (int i = 0; i < b->nstreams; ++i) {
         hipStream_t s = nullptr;
@@ -2151,6 +2186,16 @@ int batch_set_streams(sphg_batch* b, int nstreams)
         b->join.push_back(e);
     }
     return SPHG_OK;
+}
+
+// True when a stream of the batch is still in a stream capture.
+bool batch_capturing(const sphg_batch* b)
+{
+    for (hipStream_t s : b->streams) {
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return true;
+    }
+    return false;
 }
 
 // Eager steps of one context until its next step starts a reorder period.
@@ -2164,6 +2209,11 @@ int batch_align(sphg_ctx* c, int* left)
 }
 
 // Capture `period` steps of every context, one stream per group of contexts.
+// While captured, a context's main stream is a batch stream and its side
+// stream is off: the graph overlaps whole contexts across the streams.  (With
+// each context's side stream in the graph as well, the autotuner's best for
+// the 64-case sweep was 8.6 ms per step, on one stream, against 3.8 ms
+// without.)  The context's own streams never enter a capture.
 int batch_capture(sphg_batch* b)
 {
     batch_release_graph(b);
@@ -2176,10 +2226,13 @@ int batch_capture(sphg_batch* b)
     for (size_t r = 0; r < b->order.size() && rc == SPHG_OK; ++r) {
         sphg_ctx* c = b->ctx[(size_t)b->order[r]];
         hipStream_t own = c->stream;
+        const bool own_side = c->use_side;
         const int it0 = c->itimestep;
         c->stream = b->streams[r % (size_t)b->nstreams];
+        c->use_side = false;
         for (int s = 0; s < b->period && rc == SPHG_OK; ++s) rc = one_step(c);
         c->stream = own;
+        c->use_side = own_side;
         c->itimestep = it0;   // the capture issued no work: the step count advances on replay
     }
     for (int i = 1; i < b->nstreams && rc == SPHG_OK; ++i) {
@@ -2189,11 +2242,16 @@ int batch_capture(sphg_batch* b)
     }
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(origin, &g);
-    if (rc != SPHG_OK) {
+    if (rc != SPHG_OK || ec != hipSuccess || !g || batch_capturing(b)) {
         if (g) hipGraphDestroy(g);
-        return rc;
+        // streams left in a failed capture are not reused
+        if (batch_capturing(b)) {
+            const int ns = b->nstreams;
+            const int rs = batch_set_streams(b, ns);
+            if (rs != SPHG_OK) return rs;
+        }
+        return rc != SPHG_OK ? rc : SPHG_ERR_HIP;
     }
-    if (ec != hipSuccess) return SPHG_ERR_HIP;
     b->graph = g;
     if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) return SPHG_ERR_HIP;
     return SPHG_OK;
@@ -2240,13 +2298,7 @@ int sphg_batch_create(sphg_ctx** ctxs, int n, sphg_batch** out)
             return SPHG_ERR_HIP;
         }
     }
-    for (sphg_ctx* c : b->ctx) {
-        c->stable_ptrs = true;
-        // a context in a batch keeps every kernel on its own (captured)
-        // stream: the batch overlaps whole contexts instead
-        b->side.push_back(c->use_side ? 1 : 0);
-        c->use_side = false;
-    }
+    for (sphg_ctx* c : b->ctx) c->stable_ptrs = true;
     if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
         delete b;
         return SPHG_ERR_HIP;
@@ -2263,15 +2315,9 @@ int sphg_batch_create(sphg_ctx** ctxs, int n, sphg_batch** out)
 void sphg_batch_destroy(sphg_batch* b)
 {
     if (!b) return;
-    for (hipStream_t s : b->streams) hipStreamSynchronize(s);
-    batch_release_graph(b);
-    for (hipStream_t s : b->streams) hipStreamDestroy(s);
-    for (hipEvent_t e : b->join) hipEventDestroy(e);
+    batch_free_streams(b);
     if (b->fork) hipEventDestroy(b->fork);
-    for (size_t i = 0; i < b->ctx.size(); ++i) {
-        b->ctx[i]->stable_ptrs = false;
-        if (i < b->side.size()) b->ctx[i]->use_side = b->side[i];
-    }
+    for (sphg_ctx* c : b->ctx) c->stable_ptrs = false;
     delete b;
 }
 

@@ -1,9 +1,10 @@
 """Pins of the CPU reference of the surrogate network: its hand-written
 backward against torch autograd (float64) and central finite differences on
-a tiny random graph."""
+a tiny random graph.  torch is imported inside the tests: a module-level
+import would load torch's own HIP runtime into every test process at
+collection, GPU runs included."""
 import numpy as np
 import pytest
-import torch
 
 import gnn_ref as G
 
@@ -34,6 +35,7 @@ def tiny_graph(n=40, seed=0):
 
 
 def torch_loss(w, nf, e, ii, n, tgt, tw):
+    import torch
     p = {k: torch.tensor(v, dtype=torch.float64) for k, v in G.unpack(w).items()}
     for v in p.values():
         v.requires_grad_(True)
