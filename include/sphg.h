@@ -50,6 +50,8 @@ extern "C" {
 #define SPHG_ERR_UNSUPPORTED         (-6)  /* switch combination outside the ported path */
 #define SPHG_ERR_NONFINITE           (-7)  /* NaN/Inf detected in a checked field */
 #define SPHG_ERR_OUT_OF_MEMORY       (-8)
+#define SPHG_ERR_MOVED_TOO_FAR       (-9)  /* a particle moved half the neighbour-list skin between
+                                              two list rebuilds: a pair may have been missed */
 
 /* ---- phases ------------------------------------------------------------ */
 #define SPHG_WATER 1   /* `parts`, itype +-2 (m_particles.f90:896-913) */
@@ -238,6 +240,16 @@ int sphg_profile_get(sphg_ctx *ctx, const char *name, int64_t *launches, double 
 /* Number of same-phase / cross pairs found by the most recent fused step
    (each pair once). */
 int sphg_pair_counts(sphg_ctx *ctx, int64_t *water_pairs, int64_t *soil_pairs, int64_t *cross_pairs);
+/* Neighbour lists of the FP64 two-phase step (no counterpart in the
+   reference, whose pair search runs every step): rebuilds so far, spells of
+   fast motion with the lists off, and the skin (length).  A rebuild comes
+   every SPHG_NL_EVERY steps (default 64; a batch captures that period) and
+   when a particle has moved a quarter of the skin or would pass 0.45 of it
+   within four steps at its speed; rebuilds within 8 steps of each other turn
+   the lists off for 256 steps (not in a batch).  The skin is SPHG_NL_SKIN
+   (default 0.1) times the largest support radius.  SPHG_NLIST=0 turns the
+   lists off (row scans every step).  All three are read at sphg_create. */
+int sphg_nlist_stats(const sphg_ctx *ctx, int64_t *rebuilds, int64_t *off_spells, double *skin);
 
 /* ---- learned multi-step surrogate ---------------------------------------
    No counterpart in the reference.  A learned advance of K steps is one

@@ -19,6 +19,7 @@ SPHG_ERR_HIP = -5
 SPHG_ERR_UNSUPPORTED = -6
 SPHG_ERR_NONFINITE = -7
 SPHG_ERR_OUT_OF_MEMORY = -8
+SPHG_ERR_MOVED_TOO_FAR = -9
 
 SPHG_WATER = 1
 SPHG_SOIL = 2
@@ -55,6 +56,7 @@ ERROR_TEXT = {
     SPHG_ERR_UNSUPPORTED: "switch combination not supported",
     SPHG_ERR_NONFINITE: "non-finite value detected",
     SPHG_ERR_OUT_OF_MEMORY: "out of device memory",
+    SPHG_ERR_MOVED_TOO_FAR: "a particle moved farther than the neighbour-list skin allows",
 }
 
 
@@ -186,6 +188,7 @@ def load_library(path=None):
     lib.sphg_profile_reset.argtypes = [vp]
     lib.sphg_profile_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     lib.sphg_pair_counts.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.sphg_nlist_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     # device pointers are passed as integers (c_void_p)
     lib.sphg_direct_find_noalloc.argtypes = [C.POINTER(Params), C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, vp,
                                              C.POINTER(C.c_int32), vp]
@@ -204,6 +207,7 @@ EXPORTED_SYMBOLS = (
     "sphg_single_step", "sphg_time_integration",
     "sphg_strerror", "sphg_last_hip_error", "sphg_stream",
     "sphg_profile_enable", "sphg_profile_reset", "sphg_profile_get", "sphg_pair_counts",
+    "sphg_nlist_stats",
     "sphg_gnn_nparams", "sphg_gnn_create", "sphg_gnn_destroy", "sphg_gnn_weights", "sphg_gnn_num_samples",
     "sphg_gnn_record", "sphg_gnn_train", "sphg_gnn_step", "sphg_gnn_rollout", "sphg_gnn_sample_size",
     "sphg_gnn_debug",

@@ -206,13 +206,10 @@ struct Rows {
     int k0[3], k1[3];
 };
 
-__device__ __forceinline__ Rows block_rows(const Grid& g, const int* cell_start, double x, double y)
+// Row ranges of the 3 x 3 block around cell (cx, cy).
+__device__ __forceinline__ Rows block_rows_at(const Grid& g, const int* cell_start, int cx, int cy)
 {
     Rows R;
-    int cx, cy;
-    grid_cell(g, x, y, cx, cy);
-    cx = min(max(cx, 0), g.ncx - 1);
-    cy = min(max(cy, 0), g.ncy - 1);
     const int c0 = max(cx - 1, 0), c1 = min(cx + 1, g.ncx - 1);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -226,6 +223,22 @@ __device__ __forceinline__ Rows block_rows(const Grid& g, const int* cell_start,
         }
     }
     return R;
+}
+
+// Cell of a position, clamped into the grid (a particle outside it has
+// raised the out-of-range flag at the cell-list build).
+__device__ __forceinline__ void clamped_cell(const Grid& g, double x, double y, int& cx, int& cy)
+{
+    grid_cell(g, x, y, cx, cy);
+    cx = min(max(cx, 0), g.ncx - 1);
+    cy = min(max(cy, 0), g.ncy - 1);
+}
+
+__device__ __forceinline__ Rows block_rows(const Grid& g, const int* cell_start, double x, double y)
+{
+    int cx, cy;
+    clamped_cell(g, x, y, cx, cy);
+    return block_rows_at(g, cell_start, cx, cy);
 }
 
 constexpr int Q_STRIDE = 16;   // floats per record: four float4 chunk planes
@@ -268,14 +281,49 @@ __device__ __forceinline__ void write_q01(float4* R, size_t pn, const Grid& g, d
 // pack: half-step update (first_half, declarations_sph.f90:643-684, and the
 // soil Drucker-Prager projections) fused with writing the neighbour record.
 // ------------------------------------------------------------------------
+// Displacement since the neighbour-list rebuild, per record slot t (the
+// record order is fixed between rebuilds).  A rebuild (rebuild = 1) records
+// the position.  Later steps raise bit 0 of flag (rebuild) when the
+// displacement d passes trig, or when d plus `look` times the speed (look =
+// a few steps' dt) passes near, so that a rebuild that takes effect two
+// steps after the flag is raised still comes before d reaches half the
+// skin; and bit 1 when d is past half the skin (d^2 > hard2): the lists may
+// then have missed a pair.  One atomic per wave.  x == nullptr: no lists.
+struct NRef {
+    double *x, *y;
+    int* flag;
+    double trig, near, hard2, look;
+    int rebuild;
+};
+
+__device__ __forceinline__ void track_displacement(const NRef& R, int t, double x, double y, double vx, double vy)
+{
+    if (!R.x) return;
+    if (R.rebuild) {
+        R.x[t] = x;
+        R.y[t] = y;
+        return;
+    }
+    const double dx = x - R.x[t], dy = y - R.y[t];
+    const double d2 = dx * dx + dy * dy;
+    const double d = sqrt(d2);
+    const bool hard = !(d2 <= R.hard2);
+    const bool due = hard || !(d <= R.trig) || !(d + R.look * sqrt(vx * vx + vy * vy) <= R.near);
+    const uint64_t any_due = __ballot(due), any_hard = __ballot(hard);
+    const uint64_t active = __ballot(1);
+    if (any_due && (int)(threadIdx.x & 63) == __ffsll((unsigned long long)active) - 1)
+        atomicOr(R.flag, any_hard ? 3 : 1);
+}
+
 // q != nullptr selects the FP32 record of the mixed-precision form.
-__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs)
+__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs, NRef NR)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
     double rho = P.rho[i], vof = P.vof[i], vx = P.vx[i], vy = P.vy[i];
+    track_displacement(NR, t, P.x[i], P.y[i], vx, vy);
     const int it = P.itype[i];
     if (do_half) {
         const double h = K.dt / 2.0;
@@ -326,13 +374,15 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
     R[6 * pn] = make_double2(P.hsml[i], (double)it);
 }
 
-__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs)
+__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs,
+                            NRef NR)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
     double rho = P.rho[i], vx = P.vx[i], vy = P.vy[i];
+    track_displacement(NR, t, P.x[i], P.y[i], vx, vy);
     const int it = P.itype[i];
     if (do_half) {
         const double h = K.dt / 2.0;
@@ -411,6 +461,22 @@ struct Masks {
 };
 
 // ------------------------------------------------------------------------
+// Neighbour lists with a skin (FP64 fused step).  At a rebuild the cell
+// list is built on a grid of cell size >= R + skin (R the largest support
+// radius), the records are written in its order, and every particle keeps
+// (a) the key of its cell, (b) a 128-bit mask over its candidate rows of the
+// candidates within R + skin (same phase and cross), and (c) its position.
+// The record order and the cell list then stay fixed until the next rebuild.
+// While no particle has moved more than skin / 2 from its rebuild position,
+// every pair within R lies in these masks, so each step tests only the
+// listed candidates with the exact predicate instead of scanning the rows.
+// ------------------------------------------------------------------------
+struct NList {
+    Masks same, cross;   // candidates within R + skin: same phase, other phase
+    const int* cell;     // [n] key of the particle's cell at the rebuild (record order)
+};
+
+// ------------------------------------------------------------------------
 // Partner iteration.  A particle's candidates are the three cell rows of
 // its 3 x 3 block, each a contiguous range [k0, k1) of the cell-sorted
 // records.  A scan tests every candidate with the exact predicate and keeps
@@ -466,14 +532,17 @@ __device__ __forceinline__ void visit(IT it, const LaneRows& L, int ovf, int sel
 #pragma unroll 1
     for (int k = it.next(i); k >= 0; k = it.next(i)) body(load(k));
     if (ovf) {
+        // one pass over the concatenated rows (row bounds as scalars: an
+        // indexed LaneRows would be kept in scratch memory)
+        const int s1 = L.k1[0] - L.k0[0], s2 = s1 + L.k1[1] - L.k0[1], tot = s2 + L.k1[2] - L.k0[2];
+        const int e0 = L.k0[0], e1 = L.k0[1] - s1, e2 = L.k0[2] - s2;
 #pragma unroll 1
-        for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < tot; ++c) {
+            const int r = c < s1 ? 0 : (c < s2 ? 1 : 2);
             if (!(ovf & (1 << r))) continue;
-#pragma unroll 1
-            for (int kk = L.k0[r]; kk < L.k1[r]; ++kk) {
-                if (kk == self || !accept(kk)) continue;
-                body(load(kk));
-            }
+            const int kk = c + (r == 0 ? e0 : (r == 1 ? e1 : e2));
+            if (kk == self || !accept(kk)) continue;
+            body(load(kk));
         }
     }
 }
@@ -668,6 +737,112 @@ __device__ __forceinline__ CatMask scan_mask(const LaneRows& L, int self, const 
     return cat_masks(L, m0, m1, m2);
 }
 
+// The exact partner mask from a neighbour-list mask: the listed candidates
+// (bits of vm over the rows L) whose current positions pass accept(k, xy).
+// Four candidates' positions are loaded back to back.  A list that
+// overflowed (ovf) stays overflowed: the visit then walks the rows with the
+// exact predicate.
+template <class A>
+__device__ __forceinline__ CatMask filter_mask(const LaneRows& L, const CatMask& vm, const RecV& rec, A&& acc)
+{
+    if (vm.ovf) return vm;
+    const int s1 = L.k1[0] - L.k0[0], s2 = s1 + L.k1[1] - L.k0[1];
+    const int e0 = L.k0[0], e1 = L.k0[1] - s1, e2 = L.k0[2] - s2;
+    uint64_t lo = vm.lo, hi = vm.hi, olo = 0, ohi = 0;
+#pragma unroll 1
+    for (;;) {
+        int c[4], k[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {   // next listed bit: position c, record index k (-1: none)
+            const bool in_lo = lo != 0;
+            const uint64_t m = in_lo ? lo : hi;
+            c[j] = m ? __builtin_ctzll(m) + (in_lo ? 0 : 64) : -1;
+            const uint64_t rest = m & (m - 1);
+            lo = in_lo ? rest : lo;
+            hi = in_lo ? hi : rest;
+            k[j] = c[j] < 0 ? -1 : c[j] + (c[j] < s1 ? e0 : (c[j] < s2 ? e1 : e2));
+        }
+        if (k[0] < 0) break;
+        double2 p[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) p[j] = rec(k[j] >= 0 ? k[j] : k[0], 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (k[j] >= 0 && acc(k[j], p[j])) {
+                if (c[j] < 64) olo |= 1ull << c[j];
+                else ohi |= 1ull << (c[j] - 64);
+            }
+        }
+        if (k[3] < 0) break;
+    }
+    CatMask out;
+    out.lo = olo;
+    out.hi = ohi;
+    out.ovf = 0;
+    return out;
+}
+
+__device__ __forceinline__ CatMask mask_of(const Masks& M, int n, int t)
+{
+    CatMask c;
+    c.lo = M.m[t];
+    c.hi = M.m[n + t];
+    c.ovf = M.ovf[t];
+    return c;
+}
+
+// Candidate rows of particle tt: the 3 x 3 block of its cell at the list
+// rebuild (row-major key NL.cell[tt]), or (no lists: NL.cell == nullptr) of
+// its current cell.
+__device__ __forceinline__ LaneRows cand_rows(const Grid& g, const int* cell_start, const NList& NL, int tt,
+                                              double xa, double ya)
+{
+    int cx, cy;
+    if (NL.cell) {
+        const int key = NL.cell[tt];
+        cy = key / g.ncx;
+        cx = key - cy * g.ncx;
+    } else {
+        clamped_cell(g, xa, ya, cx, cy);
+    }
+    return lane_rows(block_rows_at(g, cell_start, cx, cy));
+}
+
+// Partner mask of particle tt over the rows L: the list M filtered by the
+// exact predicate, or (no lists: M.m == nullptr) the scan of the rows.
+template <class A>
+__device__ __forceinline__ CatMask cand_mask(const LaneRows& L, const Masks& M, int n, int tt, int self,
+                                             const RecV& rec, A&& acc)
+{
+    if (M.m) return filter_mask(L, mask_of(M, n, tt), rec, acc);
+    return scan_mask(L, self, rec, acc);
+}
+
+// Neighbour lists of phase A (records recA) against the records of phase B
+// (cell list startB on grid g): candidates with dr < thr (thr = (R + skin)^2
+// of the pair type), the partner itself excluded when same.  Also records
+// each particle's cell key.
+__global__ __launch_bounds__(256) void k_nlist(const double* recA, int nA, const double* recB, int nB,
+                                               const int* startB, Grid g, double thr, int same, Masks VM, int* cell)
+{
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= nA) return;
+    const RecV RA = recv(recA, nA), RB = recv(recB, nB);
+    const double2 p = RA(t, 0);
+    int cx, cy;
+    clamped_cell(g, p.x, p.y, cx, cy);
+    if (cell) cell[t] = cy * g.ncx + cx;
+    const LaneRows L = lane_rows(block_rows_at(g, startB, cx, cy));
+    auto acc = [&](int, double2 b) {
+        const double dx = p.x - b.x, dy = p.y - b.y;
+        return dx * dx + dy * dy < thr;
+    };
+    const CatMask cm = scan_mask(L, same ? t : -1, RB, acc);
+    VM.m[t] = cm.lo;
+    VM.m[nA + t] = cm.hi;
+    VM.ovf[t] = cm.ovf;
+}
+
 // Chunks read per partner by each loop (h only when it is not uniform).
 constexpr int NEED_SAME_A = 0x07 | NEED_IV;          // x,y v rho,m(itype in its sign); 1/rho, m/rho
 constexpr int NEED_WX_A = 0x0F | NEED_IV;            // soil partners of water: x,y v rho,m vof
@@ -683,8 +858,8 @@ constexpr int NEED_B = 0x3F | NEED_IV;               // x,y v rho,m vof,p s3,c s
 // twice the occupancy of the same-phase stage.
 // ------------------------------------------------------------------------
 template <int SKF, bool U>
-__global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, const double* recS, Grid g,
-                                      KParams K, FusedK F)
+__global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, const double* recS, NList NL,
+                                      Grid g, KParams K, FusedK F)
 {
     const RecV RW = recv(recW, W.n), RS = recv(recS, S.n);
     const WaveCtx wc = wave_ctx(W.n);
@@ -694,11 +869,11 @@ __global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, co
     const double irhoa = 1.0 / rhoa;
     const int a = W.order[wc.tt];
     const bool ux = U || F.uni_x;
-    const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya));
+    const LaneRows L = cand_rows(g, S.cell_start, NL, wc.tt, xa, ya);
     const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
     auto acc = [&](int k, double2 b) { return PX(b, ux ? ha : RS(k, 6).x, g, K); };
     auto acc_k = [&](int k) { return acc(k, RS(k, 0)); };
-    const CatMask cm = scan_mask(L, -1, RS, acc);
+    const CatMask cm = cand_mask(L, NL.cross, W.n, wc.tt, -1, RS, acc);
     const Loader<NEED_WX_A> load{RS, !ux, false};
     const double cfa = vofa * rhoa * (-K.numeric_gravity) / K.s_perm;
     const KC kx = make_kc(SKF, sph_mhsml(ha, ux ? RS(0, 6).x : ha));
@@ -735,8 +910,8 @@ __global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, co
 // (whose pressure is the new one the water pack evaluated).
 // ------------------------------------------------------------------------
 template <int SKF, bool U>
-__global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, const double* recW, Grid g,
-                                     KParams K, FusedK F)
+__global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, const double* recW, NList NL,
+                                     Grid g, KParams K, FusedK F)
 {
     const RecV RS = recv(recS, S.n), RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(S.n);
@@ -746,11 +921,11 @@ __global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, con
     const double irhoa = 1.0 / rhoa;
     const int a = S.order[wc.tt];
     const bool ux = U || F.uni_x;
-    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya));
+    const LaneRows L = cand_rows(g, W.cell_start, NL, wc.tt, xa, ya);
     const Pred PX{xa, ya, ha, F.thr_x, ux ? 1 : 0, 0};
     auto acc = [&](int k, double2 b) { return PX(b, ux ? ha : RW(k, 6).x, g, K); };
     auto acc_k = [&](int k) { return acc(k, RW(k, 0)); };
-    const CatMask cm = scan_mask(L, -1, RW, acc);
+    const CatMask cm = cand_mask(L, NL.cross, S.n, wc.tt, -1, RW, acc);
     const Loader<NEED_SX_A> load{RW, !ux, false};
     const double cfs = vofa * (-K.numeric_gravity) / K.s_perm;
     const KC kx = make_kc(SKF, sph_mhsml(ux ? RW(0, 6).x : ha, ha));
@@ -789,8 +964,8 @@ __global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, con
 // soil partners, which k_water_x4 writes).
 // ------------------------------------------------------------------------
 template <int SKF, bool U>
-__global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recWw, Masks MW, Grid g, KParams K,
-                                      FusedK F, int cross)
+__global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recWw, NList NL, Masks MW, Grid g,
+                                      KParams K, FusedK F, int cross)
 {
     const RecV RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(W.n);
@@ -807,11 +982,11 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recW
     const double ca = W.c[a];
     const bool uni = U || link || F.uni_w;
     const KC kc = make_kc(SKF, link ? K.h1_water : ha);
-    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya));
+    const LaneRows L = cand_rows(g, W.cell_start, NL, wc.tt, xa, ya);
     const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
     auto acc = [&](int k, double2 b) { return P(b, uni ? ha : RW(k, 6).x, g, K); };
     auto acc_k = [&](int k) { return acc(k, RW(k, 0)); };
-    const CatMask cm = scan_mask(L, t, RW, acc);
+    const CatMask cm = cand_mask(L, NL.same, W.n, wc.tt, t, RW, acc);
     if (wc.valid) {
         MW.m[t] = cm.lo;
         MW.m[W.n + t] = cm.hi;
@@ -872,7 +1047,8 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recW
 // The partner masks come from stage A.
 // ------------------------------------------------------------------------
 template <int SKF, bool U>
-__global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Grid g, KParams K, FusedK F, int mask)
+__global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, NList NL, Masks MW, Grid g, KParams K, FusedK F,
+                                      int mask)
 {
     const RecV RW = recv(recW, W.n);
     const WaveCtx wc = wave_ctx(W.n);
@@ -888,7 +1064,7 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
     const bool link = K.nnps == 2;
     const bool uni = U || link || F.uni_w;
     const KC kc = make_kc(SKF, link ? K.h1_water : ha);
-    const LaneRows L = lane_rows(block_rows(g, W.cell_start, xa, ya));
+    const LaneRows L = cand_rows(g, W.cell_start, NL, wc.tt, xa, ya);
     const Pred P{xa, ya, ha, F.thr_w, uni ? 1 : 0, link ? 1 : 0};
     auto acc_k = [&](int k) { return P(RW(k, 0), uni ? ha : RW(k, 6).x, g, K); };
     const Loader<NEED_B> load{RW, !uni, false};
@@ -968,8 +1144,8 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, Masks MW, Gr
 //               (the cross terms k_soil_x4 wrote; cross: two-phase run).
 // ------------------------------------------------------------------------
 template <int SKF, bool U, bool SOLO>
-__global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, const double* recS, Grid g, KParams K, FusedK F, int cross,
-                                         int mask)
+__global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, const double* recS, NList NL, Grid g, KParams K, FusedK F,
+                                         int cross, int mask)
 {
     const RecV RS = recv(recS, S.n);
     const WaveCtx wc = wave_ctx(S.n);
@@ -996,11 +1172,11 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, const double* recS, Grid g, K
     const bool link = K.nnps == 2;
     const bool uni = U || link || F.uni_s;
     const KC kc = make_kc(SKF, link ? K.h1_soil : ha);
-    const LaneRows L = lane_rows(block_rows(g, S.cell_start, xa, ya));
+    const LaneRows L = cand_rows(g, S.cell_start, NL, wc.tt, xa, ya);
     const Pred P{xa, ya, ha, F.thr_s, uni ? 1 : 0, link ? 1 : 0};
     auto acc = [&](int k, double2 b) { return P(b, uni ? ha : RS(k, 6).x, g, K); };
     auto acc_k = [&](int k) { return acc(k, RS(k, 0)); };
-    const CatMask cm = scan_mask(L, t, RS, acc);
+    const CatMask cm = cand_mask(L, NL.same, S.n, wc.tt, t, RS, acc);
     const Loader<NEED_B> load{RS, !uni, mc};
     const double et2 = K.etq * K.etq;
     double con = 0.0, drd = 0.0, txx = 0.0, txy = 0.0, tyy = 0.0, vcc = 0.0, wxy = 0.0;

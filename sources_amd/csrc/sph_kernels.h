@@ -90,6 +90,12 @@ __global__ void k_fill_int(int* a, int n, int v)
     if (i < n) a[i] = v;
 }
 
+// a[0] &= keep (one thread).
+__global__ void k_keep_bits(int* a, int keep)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) a[0] &= keep;
+}
+
 __global__ void k_fill_dbl(double* a, int n, double v)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -23,6 +23,18 @@ namespace {
 
 constexpr int TPB = 256;
 
+// Skin of the neighbour lists as a fraction of the largest support radius R:
+// lists hold candidates within R + skin, a particle that has moved skin / 4
+// since the rebuild triggers the next one, and one that has moved skin / 2
+// (when a pair could be missing) is an error.
+constexpr double NL_SKIN = 0.1;    // neighbour-list skin / largest support radius
+constexpr int NL_EVERY = 64;       // steps between periodic list rebuilds
+constexpr double NL_LOOK = 4.0;    // steps of look-ahead of the displacement trigger
+// a displacement-triggered rebuild within NL_MIN_RUN steps of the previous
+// one turns the lists off for NL_OFF steps (the row scans are then cheaper)
+constexpr int NL_MIN_RUN = 8;
+constexpr int NL_OFF = 256;
+
 // HIP error of the last failing call of the context-free entry point
 // sphg_direct_find_noalloc (sphg_last_hip_error(NULL) returns it).
 std::atomic<int> noalloc_hip_error{0};
@@ -50,6 +62,7 @@ struct PhaseBuf {
     DPhase d{};
     std::vector<void*> allocs;
     bool cells_valid = false;
+    bool cells_nl = false;      // the valid cell list is on the neighbour-list grid
     bool counts_zero = false;   // d.cell_count known to be all zero
     int* cell_part = nullptr;   // tile-local exclusive prefix of the cell counts
     int* tile_sums = nullptr;   // the scan's tile sums (per phase: the two builds may run concurrently)
@@ -65,6 +78,11 @@ struct PhaseBuf {
     // fused path: cell-sorted neighbour records and same-phase masks
     double* rec = nullptr;
     Masks masks{};               // same-phase partners of water stage A, reused by stage B
+    // neighbour lists of the FP64 fused step: candidate masks (same phase,
+    // cross), cell key per record and position per particle at the rebuild
+    Masks nl_same{}, nl_cross{};
+    int* nl_cell = nullptr;
+    double *nl_x = nullptr, *nl_y = nullptr;
     // second buffers of the arrays the reorder permutes: the permuted copy is
     // written there and the pointers swap
     double* shadow_d[PERM_MAXD] = {};
@@ -111,6 +129,28 @@ struct sphg_ctx {
     // is (the reorder copies back instead of swapping buffers), so that a
     // captured sequence of steps can be replayed.
     bool stable_ptrs = false;
+    // Neighbour lists of the FP64 fused two-phase step (rebuilt with the cell
+    // list every reorder period, when a particle has moved skin / 4 from its
+    // rebuild position, and whenever the state changed outside a step)
+    Grid gv{};                  // list grid: cell size >= R + skin
+    Grid gvl[2]{};              // per phase, with the link_list fields
+    int ncell_v = 0;
+    double skin = 0.0;
+    double nl_thr[3] = {0.0, 0.0, 0.0};   // (R + skin)^2: water, soil, cross pairs
+    bool nl_valid = false;      // lists, cell list and records belong to the state
+    bool nl_due = false;        // the displacement check asked for a rebuild
+    int nl_rebuilt_at = 0;      // itimestep of the last rebuild
+    int64_t nl_rebuilds = 0;
+    bool use_nl = true;         // SPHG_NLIST=0: scan the cell rows every step instead
+    double nl_skin = NL_SKIN;   // skin / largest support radius (SPHG_NL_SKIN)
+    int nl_every = NL_EVERY;    // steps between periodic rebuilds (SPHG_NL_EVERY)
+    int nl_off_until = 0;       // lists off (fast-moving particles) before this step
+    int64_t nl_off_spells = 0;
+    bool capturing = false;     // steps are being captured into a graph (no host reads)
+    int* d_nlflag = nullptr;    // bit 0: rebuild due; bit 1: a particle moved past skin / 2
+    int* h_nlflag = nullptr;    // pinned: two slots for the lagged check of sphg_time_integration
+    int h_nlstep[2] = {-1, -1}; // step whose flag each slot holds (-1: none)
+    hipEvent_t ev_nl[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -230,28 +270,42 @@ int exclusive_scan(sphg_ctx* c, const int* in, int* out, int n, int* clear = nul
 
 int phase_index(int phase) { return phase == SPHG_WATER ? 0 : (phase == SPHG_SOIL ? 1 : -1); }
 
-// Cell list of one phase for the current positions.
-int build_cells(sphg_ctx* c, int k)
+// Cell list of one phase for the current positions, on the per-step grid
+// (nl = false) or on the neighbour-list grid.  A per-step build replaces the
+// order the neighbour lists index, so it retires them.
+int build_cells(sphg_ctx* c, int k, bool nl = false)
 {
     PhaseBuf& b = c->ph[k];
-    if (b.cells_valid) return SPHG_OK;
+    if (b.cells_valid && b.cells_nl == nl) return SPHG_OK;
+    if (!nl) c->nl_valid = false;
+    const Grid& g = nl ? c->gvl[k] : c->gl[k];
+    const int ncell = nl ? c->ncell_v : c->ncell;
     // cell_count is all zero between builds: the scan clears what it reads
-    if (!b.counts_zero) RC(fill_int(c, b.d.cell_count, c->ncell, 0));
+    if (!b.counts_zero) RC(fill_int(c, b.d.cell_count, std::max(c->ncell, c->ncell_v), 0));
     b.counts_zero = false;
-    RC(launch(c, "cell_key", k_cell_key, nblk(b.n), TPB, b.d, c->gl[k], c->d_err));
+    RC(launch(c, "cell_key", k_cell_key, nblk(b.n), TPB, b.d, g, c->d_err));
     // one scan pass: tile-local prefixes and tile sums; the scatter and the
     // per-cell sort add the prefix of the tile sums themselves
-    const int ntiles = std::max((c->ncell + SCAN_TILE - 1) / SCAN_TILE, 1);
+    const int ntiles = std::max((ncell + SCAN_TILE - 1) / SCAN_TILE, 1);
     if (ntiles > SCAN_TILE) return SPHG_ERR_UNSUPPORTED;
-    RC(launch(c, "scan", k_scan_tile, ntiles, SCAN_BLOCK, (const int*)b.d.cell_count, b.cell_part, c->ncell, b.tile_sums,
+    RC(launch(c, "scan", k_scan_tile, ntiles, SCAN_BLOCK, (const int*)b.d.cell_count, b.cell_part, ncell, b.tile_sums,
               b.d.cell_count));
     b.counts_zero = true;
     RC(launch(c, "cell_scatter", k_cell_scatter_t, nblk(b.n, SCAN_BLOCK), SCAN_BLOCK, b.d, (const int*)b.cell_part,
               (const int*)b.tile_sums, ntiles));
-    RC(launch(c, "cell_sort", k_cell_sort_t, nblk(c->ncell, SCAN_BLOCK), SCAN_BLOCK, b.d, (const int*)b.cell_part,
-              (const int*)b.tile_sums, ntiles, c->ncell));
+    RC(launch(c, "cell_sort", k_cell_sort_t, nblk(ncell, SCAN_BLOCK), SCAN_BLOCK, b.d, (const int*)b.cell_part,
+              (const int*)b.tile_sums, ntiles, ncell));
     b.cells_valid = true;
+    b.cells_nl = nl;
     return SPHG_OK;
+}
+
+// The state changed outside a step (upload, restore, correction): cell lists
+// and neighbour lists no longer describe it.
+void state_changed(sphg_ctx* c)
+{
+    for (auto& pb : c->ph) pb.cells_valid = false;
+    c->nl_valid = false;
 }
 
 // Permute the persistent state of phase k into the current cell order, so
@@ -293,15 +347,25 @@ int reorder(sphg_ctx* c, int k)
     return SPHG_OK;
 }
 
+// The device flags after the work issued so far: d_err[0] (out of range)
+// and d_nlflag = d_err[1] (neighbour-list displacement).  A raised error
+// flag is cleared, so that the next call starts clean.
 int check_flags(sphg_ctx* c)
 {
-    int err = 0;
-    HIPCHK(c, hipMemcpyAsync(&err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int f[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(f, c->d_err, sizeof(f), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (err & 1) {
-        int z = 0;
+    c->h_nlstep[0] = c->h_nlstep[1] = -1;   // the lagged copies are superseded
+    if (f[0] & 1) {
+        const int z = 0;
         HIPCHK(c, hipMemcpy(c->d_err, &z, sizeof(int), hipMemcpyHostToDevice));
         return SPHG_ERR_OUT_OF_RANGE;
+    }
+    if (f[1] & 2) {
+        const int z = 0;
+        HIPCHK(c, hipMemcpy(c->d_nlflag, &z, sizeof(int), hipMemcpyHostToDevice));
+        c->nl_valid = false;
+        return SPHG_ERR_MOVED_TOO_FAR;
     }
     return SPHG_OK;
 }
@@ -393,6 +457,19 @@ int setup_grid(sphg_ctx* c)
     c->g = g;
     c->cs = cs;
     const int ncell = g.ncx * g.ncy;
+    // neighbour-list grid: cells of at least the largest support radius plus
+    // the skin, so that the lists' candidates within R + skin lie in the
+    // 3 x 3 block of a particle's cell
+    c->skin = c->nl_skin * c->K.scale_k * hmax;
+    {
+        const double csv = (c->K.scale_k * hmax + c->skin) * (1.0 + 1.0e-9);
+        Grid v = g;
+        v.inv_cs = 1.0 / csv;
+        v.ncx = (int)(std::floor((P.x_maxgeom - P.x_mingeom) / csv) + 1.0);
+        v.ncy = (int)(std::floor((P.y_maxgeom - P.y_mingeom) / csv) + 1.0);
+        c->gv = v;
+        c->ncell_v = v.ncx * v.ncy;
+    }
     // per phase: reference link_list sorting grid (init_grid, m_particles.f90:1069-1113)
     for (int k = 0; k < 2; ++k) {
         Grid gk = g;
@@ -414,7 +491,23 @@ int setup_grid(sphg_ctx* c)
             if (k == 0) c->K.h1_water = h1; else c->K.h1_soil = h1;
         }
         c->gl[k] = gk;
+        Grid gvk = c->gv;   // the same link_list fields on the list grid
+        gvk.link_list = gk.link_list;
+        gvk.lmin_x = gk.lmin_x; gvk.lmin_y = gk.lmin_y;
+        gvk.lscale_x = gk.lscale_x; gvk.lscale_y = gk.lscale_y;
+        gvk.lreach_x = gk.lreach_x; gvk.lreach_y = gk.lreach_y;
+        c->gvl[k] = gvk;
     }
+    {
+        // list radii: R + skin per pair type, R from the largest h
+        const double sk = c->K.scale_k;
+        const double hw = c->ph[0].bound ? c->ph[0].hmax : 0.0, hs = c->ph[1].bound ? c->ph[1].hmax : 0.0;
+        const double rw = sk * hw + c->skin, rs = sk * hs + c->skin, rx = sk * 0.5 * (hw + hs) + c->skin;
+        c->nl_thr[0] = rw * rw;
+        c->nl_thr[1] = rs * rs;
+        c->nl_thr[2] = rx * rx;
+    }
+    c->nl_valid = false;
     // sqrt-free pair predicate thresholds (uniform smoothing length)
     {
         FusedK& F = c->F;
@@ -532,7 +625,7 @@ int upload(sphg_ctx* c, int k, uint64_t m)
     PhaseBuf& b = c->ph[k];
     const sphg_phase_view& v = b.view;
     DPhase& d = b.d;
-    if (m & SPHG_F_X) { RC(up_vec(c, b, v.x, d.x, d.y)); b.cells_valid = false; }
+    if (m & SPHG_F_X) { RC(up_vec(c, b, v.x, d.x, d.y)); state_changed(c); }
     if (m & SPHG_F_VX) RC(up_vec(c, b, v.vx, d.vx, d.vy));
     if (m & SPHG_F_RHO) RC(up_scalar(c, b, v.rho, d.rho));
     if (m & SPHG_F_P) RC(up_scalar(c, b, v.p, d.p));
@@ -708,6 +801,14 @@ int alloc_phase(sphg_ctx* c, int k, const sphg_phase_view* v)
         RC(dalloc(c, b, &b.masks.m, 3 * n));
         RC(dalloc(c, b, &b.masks.ovf, n));
     }
+    for (Masks* m : {&b.nl_same, &b.nl_cross}) {
+        RC(dalloc(c, b, &m->m, 2 * n));
+        RC(dalloc(c, b, &m->ovf, n));
+    }
+    RC(dalloc(c, b, &b.nl_cell, n));
+    RC(dalloc(c, b, &b.nl_x, n));
+    RC(dalloc(c, b, &b.nl_y, n));
+    c->nl_valid = false;
     b.hmin = b.hmax = v->hsml[0];
     for (int i = 1; i < b.n; ++i) {
         b.hmin = std::fmin(b.hmin, v->hsml[i]);
@@ -809,6 +910,15 @@ int mask_b(const sphg_ctx* c, int phase)
     return m;
 }
 
+// FP64 fused two-phase steps keep neighbour lists with a skin (nl_config),
+// except during a spell of fast motion (nl_mode).
+bool nl_config(const sphg_ctx* c)
+{
+    return c->use_nl && !c->legacy && !c->P.single_phase && c->P.precision != SPHG_MIXED;
+}
+
+bool nl_mode(const sphg_ctx* c) { return nl_config(c) && c->itimestep >= c->nl_off_until; }
+
 // single_step (declarations_sph.f90:1131-1435) after the soil Drucker-Prager
 // projection of :1244 has been applied (legacy unfused kernels).
 int rhs_two_phase_legacy(sphg_ctx* c)
@@ -840,6 +950,11 @@ int stages_fused(sphg_ctx* c)
     const bool rep_int = c->K.ip1 >= 0 && c->K.ip2 >= 0;
     const double* rW = W.rec;
     const double* rS = S.rec;
+    const bool nl = nl_mode(c);
+    const NList nW = nl ? NList{W.nl_same, W.nl_cross, W.nl_cell} : NList{};
+    const NList nS = nl ? NList{S.nl_same, S.nl_cross, S.nl_cell} : NList{};
+    const Grid& gW = nl ? c->gvl[0] : c->gl[0];
+    const Grid& gS = nl ? c->gvl[1] : c->gl[1];
     // U instantiations: h uniform in the phase(s) the kernel reads (then
     // hsml(1) is every h), integer repulsion exponents, Drucker-Prager; the
     // general ones read h per particle and partner
@@ -852,31 +967,31 @@ int stages_fused(sphg_ctx* c)
         SideScope side(c);
         if (cross) {
             if (ux) RC(launch(c, "soil_x", k_soil_x4<SKF, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, rS, rW,
-                              c->gl[1], c->K, c->F));
+                              nS, gS, c->K, c->F));
             else RC(launch(c, "soil_x", k_soil_x4<SKF, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, rS, rW,
-                           c->gl[1], c->K, c->F));
+                           nS, gS, c->K, c->F));
         }
         if (us) RC(launch(c, "soil_ab", k_soil_ab4<SKF, true, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rS,
-                          c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
+                          nS, gS, c->K, c->F, cross, mask_b(c, 1)));
         else RC(launch(c, "soil_ab", k_soil_ab4<SKF, false, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rS,
-                       c->gl[1], c->K, c->F, cross, mask_b(c, 1)));
+                       nS, gS, c->K, c->F, cross, mask_b(c, 1)));
     }
     if (cross) {
         if (ux) RC(launch(c, "water_x", k_water_x4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, rW, rS,
-                          c->gl[0], c->K, c->F));
+                          nW, gW, c->K, c->F));
         else RC(launch(c, "water_x", k_water_x4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, rW, rS,
-                       c->gl[0], c->K, c->F));
+                       nW, gW, c->K, c->F));
     }
     if (uw) RC(launch(c, "water_a", k_water_a4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
-                      W.masks, c->gl[0], c->K, c->F, cross));
+                      nW, W.masks, gW, c->K, c->F, cross));
     else RC(launch(c, "water_a", k_water_a4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
-                   W.masks, c->gl[0], c->K, c->F, cross));
+                   nW, W.masks, gW, c->K, c->F, cross));
     swap_c(W);
     swap_c(S);
-    if (uw) RC(launch(c, "water_b", k_water_b4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.masks,
-                      c->gl[0], c->K, c->F, mask_b(c, 0)));
-    else RC(launch(c, "water_b", k_water_b4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.masks,
-                   c->gl[0], c->K, c->F, mask_b(c, 0)));
+    if (uw) RC(launch(c, "water_b", k_water_b4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, nW,
+                      W.masks, gW, c->K, c->F, mask_b(c, 0)));
+    else RC(launch(c, "water_b", k_water_b4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, nW,
+                   W.masks, gW, c->K, c->F, mask_b(c, 0)));
     return fork.join();
 }
 
@@ -888,11 +1003,11 @@ int stages_soil_only(sphg_ctx* c)
     PhaseBuf& S = c->ph[1];
     const double* rs = S.rec;
     if (S.hmin == S.hmax && c->K.yield_criterion != 1)
-        RC(launch(c, "soil_ab", k_soil_ab4<SKF, true, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rs, c->gl[1],
-                  c->K, c->F, 0, mask_b(c, 1)));
+        RC(launch(c, "soil_ab", k_soil_ab4<SKF, true, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rs, NList{},
+                  c->gl[1], c->K, c->F, 0, mask_b(c, 1)));
     else
-        RC(launch(c, "soil_ab", k_soil_ab4<SKF, false, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rs, c->gl[1],
-                  c->K, c->F, 0, mask_b(c, 1)));
+        RC(launch(c, "soil_ab", k_soil_ab4<SKF, false, true>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rs, NList{},
+                  c->gl[1], c->K, c->F, 0, mask_b(c, 1)));
     swap_c(S);
     return SPHG_OK;
 }
@@ -910,7 +1025,7 @@ int rhs_soil_only(sphg_ctx* c, int soil_half, int soil_ndp)
         std::swap(S.d.rho, S.d.rho_next);
     }
     RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, (float*)nullptr,
-              c->gl[1], c->cs));
+              c->gl[1], c->cs, NRef{}));
     switch (c->P.skf) {
     case 1: return stages_soil_only<1>(c);
     case 2: return stages_soil_only<2>(c);
@@ -945,6 +1060,32 @@ int stages_mixed(sphg_ctx* c)
     return fork.join();
 }
 
+// Neighbour lists of the fused step at the rebuild: records written in the
+// order of the list-grid cell list; same-phase and cross candidates within
+// R + skin per particle.
+int build_nlists(sphg_ctx* c)
+{
+    PhaseBuf& W = c->ph[0];
+    PhaseBuf& S = c->ph[1];
+    const double* rW = W.rec;
+    const double* rS = S.rec;
+    // the displacement trigger restarts (a past-the-skin bit stays for the check)
+    RC(launch(c, "nlist", k_keep_bits, 1, 1, c->d_nlflag, 2));
+    RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, rW, W.n, rW, W.n, (const int*)W.d.cell_start, c->gv,
+              c->nl_thr[0], 1, W.nl_same, W.nl_cell));
+    RC(launch(c, "nlist", k_nlist, nblk(S.n, 256), 256, rS, S.n, rS, S.n, (const int*)S.d.cell_start, c->gv,
+              c->nl_thr[1], 1, S.nl_same, S.nl_cell));
+    RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, rW, W.n, rS, S.n, (const int*)S.d.cell_start, c->gv,
+              c->nl_thr[2], 0, W.nl_cross, (int*)nullptr));
+    RC(launch(c, "nlist", k_nlist, nblk(S.n, 256), 256, rS, S.n, rW, W.n, (const int*)W.d.cell_start, c->gv,
+              c->nl_thr[2], 0, S.nl_cross, (int*)nullptr));
+    c->nl_valid = true;
+    c->nl_due = false;
+    c->nl_rebuilt_at = c->itimestep;
+    ++c->nl_rebuilds;
+    return SPHG_OK;
+}
+
 int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
 {
     PhaseBuf& W = c->ph[0];
@@ -952,6 +1093,39 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
     const bool mixed = c->P.precision == SPHG_MIXED;
     float* qW = mixed ? reinterpret_cast<float*>(W.rec) : nullptr;
     float* qS = mixed ? reinterpret_cast<float*>(S.rec) : nullptr;
+    // FP64 with lists: a rebuild sorts both phases on the list grid (cell
+    // list, then the state permuted into its order), the packs record the
+    // positions and the lists follow the packs; between rebuilds the cell
+    // lists and the record order stay and the packs track the displacement.
+    const bool nl = nl_mode(c);
+    // (cell lists invalidated since -- a restore, a correction, an upload --
+    // mean a rebuild as well: the lists index their order)
+    const bool rebuild = nl && (!c->nl_valid || !W.cells_valid || !S.cells_valid || !W.cells_nl || !S.cells_nl);
+    NRef nrW{}, nrS{};
+    if (nl) {
+        // rebuild past a quarter of the skin, or when four more steps at the
+        // current speed would pass 0.45 skin; fail past half of it
+        const double hard = 0.5 * c->skin, look = NL_LOOK * c->K.dt;
+        const NRef base{nullptr, nullptr, c->d_nlflag, 0.25 * c->skin, 0.45 * c->skin, hard * hard, look,
+                        rebuild ? 1 : 0};
+        nrW = base;
+        nrW.x = W.nl_x;
+        nrW.y = W.nl_y;
+        nrS = base;
+        nrS.x = S.nl_x;
+        nrS.y = S.nl_y;
+    }
+    if (rebuild) {
+        W.cells_valid = false;
+        S.cells_valid = false;
+        c->since_reorder = 0;
+    }
+    auto soil_chain = [&]() -> int {
+        RC(build_cells(c, 1, nl));
+        if (rebuild) RC(reorder(c, 1));
+        return launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
+                      c->cs, nrS);
+    };
     // The two phases' cell lists and packs are independent: the soil chain
     // runs on the side stream beside the water chain (the stages after them
     // read both phases' records).
@@ -960,18 +1134,15 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
         RC(fork.rc);
         if (fork.open) {
             SideScope sc(c);
-            RC(build_cells(c, 1));
-            RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
-                      c->cs));
-        } else {
-            RC(build_cells(c, 1));
+            RC(soil_chain());
         }
-        RC(build_cells(c, 0));
-        RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, c->gl[0], c->cs));
+        RC(build_cells(c, 0, nl));
+        if (rebuild) RC(reorder(c, 0));
+        RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, c->gl[0], c->cs, nrW));
         if (fork.open) RC(fork.join());
-        else RC(launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
-                       c->cs));
+        else RC(soil_chain());
     }
+    if (rebuild) RC(build_nlists(c));
     if (mixed) {
         switch (c->P.skf) {
         case 1: return stages_mixed<1>(c);
@@ -1053,9 +1224,39 @@ bool renorm_active(const sphg_params& P)
 
 bool renorm_step(const sphg_ctx* c, int step) { return renorm_active(c->P) && step % 40 == 0; }
 
+// Lagged displacement check of the list mode, after each step
+// issued outside a stream capture: the flag word is copied to pinned memory behind the
+// step, and the copy of the step before is read (it has landed by the time
+// the next step is queued, so the queue keeps a step of work).  A particle
+// past the rebuild trigger schedules a rebuild for the next step; one past
+// half the skin fails the run (its step may have missed pairs).
+int nl_poll(sphg_ctx* c)
+{
+    const int s = c->itimestep, slot = s & 1, o = slot ^ 1;
+    HIPCHK(c, hipMemcpyAsync(&c->h_nlflag[slot], c->d_nlflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev_nl[slot], c->stream));
+    c->h_nlstep[slot] = s;
+    if (c->h_nlstep[o] != s - 1) return SPHG_OK;
+    HIPCHK(c, hipEventSynchronize(c->ev_nl[o]));
+    const int f = c->h_nlflag[o];
+    c->h_nlstep[o] = -1;
+    if (f & 2) return check_flags(c);
+    if ((f & 1) && s - 1 >= c->nl_rebuilt_at) {
+        c->nl_due = true;
+        // rebuilds this close together cost more than the row scans: lists
+        // off for a while (never in a batch, whose captured period has them)
+        if (s + 1 - c->nl_rebuilt_at < NL_MIN_RUN && !c->stable_ptrs) {
+            c->nl_off_until = s + 1 + NL_OFF;
+            ++c->nl_off_spells;
+        }
+    }
+    return SPHG_OK;
+}
+
 int one_step(sphg_ctx* c)
 {
     const int step = ++c->itimestep;
+    const bool nl = nl_mode(c);
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
     if (soil_only(c)) {
@@ -1089,6 +1290,12 @@ int one_step(sphg_ctx* c)
             RC(launch(c, "first_half", k_first_half_soil, nblk(S.n), TPB, S.d, c->K, 0, 1));
         }
         RC(rhs_two_phase_legacy(c));
+    } else if (nl) {
+        // the lists are rebuilt (with the reorder) every nl_every steps and
+        // when the displacement check asked for it
+        if (++c->since_reorder >= c->nl_every || c->nl_due) c->nl_valid = false;
+        if (step != 1) RC(rhs_two_phase_fused(c, 1, 1, 2));
+        else RC(rhs_two_phase_fused(c, 0, 0, 1));
     } else {
         if (++c->since_reorder >= c->reorder_every) {
             RC(build_cells(c, 0));
@@ -1104,8 +1311,14 @@ int one_step(sphg_ctx* c)
     RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, vf,
               renorm_step(c, step) ? 1 : 0));
     RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, step == 1 ? 1 : 0));
-    W.cells_valid = false;
-    S.cells_valid = false;
+    if (nl) {
+        // the list mode keeps the rebuild's cell lists and record order
+        if (!c->capturing) RC(nl_poll(c));
+    } else {
+        // the positions moved: the next step builds its cell lists again
+        W.cells_valid = false;
+        S.cells_valid = false;
+    }
     return SPHG_OK;
 }
 
@@ -1195,15 +1408,27 @@ int sphg_create(const sphg_params* params, sphg_ctx** out)
     {
         const char* e = std::getenv("SPHG_SIDE_STREAM");
         if (e && e[0] == '0') c->use_side = false;
+        const char* l = std::getenv("SPHG_NLIST");
+        if (l && l[0] == '0') c->use_nl = false;
+        const char* k = std::getenv("SPHG_NL_SKIN");
+        if (k && std::atof(k) > 0.0) c->nl_skin = std::atof(k);
+        const char* v = std::getenv("SPHG_NL_EVERY");
+        if (v && std::atoi(v) > 0) c->nl_every = std::atoi(v);
     }
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_nl[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_nl[1], hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_nlflag, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&c->d_err, sizeof(int) * 4) != hipSuccess) {
         delete c;
         return SPHG_ERR_HIP;
     }
+    c->d_nlflag = c->d_err + 1;
+    c->h_nlflag[0] = c->h_nlflag[1] = 0;
+    c->h_nlstep[0] = c->h_nlstep[1] = -1;
     const int zeros[4] = {0, 0, 0, 0};
     if (hipMemcpy(c->d_err, zeros, sizeof(zeros), hipMemcpyHostToDevice) != hipSuccess) {
         sphg_destroy(c);
@@ -1228,6 +1453,9 @@ void sphg_destroy(sphg_ctx* c)
     }
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
+    for (hipEvent_t e : c->ev_nl)
+        if (e) hipEventDestroy(e);
+    if (c->h_nlflag) hipHostFree(c->h_nlflag);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1497,6 +1725,7 @@ const char* sphg_strerror(int code)
     case SPHG_ERR_UNSUPPORTED: return "switch combination not supported by this path";
     case SPHG_ERR_NONFINITE: return "non-finite value detected";
     case SPHG_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case SPHG_ERR_MOVED_TOO_FAR: return "a particle moved farther than the neighbour-list skin allows";
     default: return "unknown error";
     }
 }
@@ -1533,6 +1762,15 @@ int sphg_profile_get(sphg_ctx* c, const char* name, int64_t* launches, double* t
     }
     if (launches) *launches = it->second.launches;
     if (total_ms) *total_ms = it->second.ms;
+    return SPHG_OK;
+}
+
+int sphg_nlist_stats(const sphg_ctx* c, int64_t* rebuilds, int64_t* off_spells, double* skin)
+{
+    if (!c || !rebuilds || !off_spells || !skin) return SPHG_ERR_INVALID;
+    *rebuilds = c->nl_rebuilds;
+    *off_spells = c->nl_off_spells;
+    *skin = c->skin;
     return SPHG_OK;
 }
 
@@ -2198,10 +2436,17 @@ bool batch_capturing(const sphg_batch* b)
     return false;
 }
 
-// Eager steps of one context until its next step starts a reorder period.
+// Steps of a captured period: the reorder period, or in the list mode the
+// list rebuild period (each period starts with the reorder or rebuild, the
+// one step whose launches differ; the host cannot act on the displacement
+// check inside a replay, and a particle past half the skin still fails the
+// run at its end).
+int batch_period(const sphg_ctx* c) { return nl_config(c) ? c->nl_every : c->reorder_every; }
+
+// Eager steps of one context until its next step starts a period.
 int batch_align(sphg_ctx* c, int* left)
 {
-    while (*left > 0 && (c->itimestep < 1 || c->since_reorder != c->reorder_every - 1)) {
+    while (*left > 0 && (c->itimestep < 1 || c->since_reorder != batch_period(c) - 1)) {
         RC(one_step(c));
         --*left;
     }
@@ -2230,7 +2475,9 @@ int batch_capture(sphg_batch* b)
         const int it0 = c->itimestep;
         c->stream = b->streams[r % (size_t)b->nstreams];
         c->use_side = false;
+        c->capturing = true;
         for (int s = 0; s < b->period && rc == SPHG_OK; ++s) rc = one_step(c);
+        c->capturing = false;
         c->stream = own;
         c->use_side = own_side;
         c->itimestep = it0;   // the capture issued no work: the step count advances on replay
@@ -2287,9 +2534,9 @@ int sphg_batch_create(sphg_ctx** ctxs, int n, sphg_batch** out)
         return b->ctx[(size_t)x]->ph[0].n + b->ctx[(size_t)x]->ph[1].n >
                b->ctx[(size_t)y]->ph[0].n + b->ctx[(size_t)y]->ph[1].n;
     });
-    b->period = b->ctx[0]->reorder_every;
+    b->period = batch_period(b->ctx[0]);
     for (sphg_ctx* c : b->ctx) {
-        if (c->reorder_every != b->period) {
+        if (batch_period(c) != b->period) {
             delete b;
             return SPHG_ERR_UNSUPPORTED;
         }
@@ -2346,7 +2593,7 @@ int sphg_batch_run(sphg_batch* b, int nsteps)
     int reps = nsteps;
     for (size_t i = 0; i < b->ctx.size(); ++i) {
         sphg_ctx* c = b->ctx[i];
-        const bool aligned = c->itimestep >= 1 && c->since_reorder == c->reorder_every - 1;
+        const bool aligned = c->itimestep >= 1 && c->since_reorder == batch_period(c) - 1;
         reps = std::min(reps, aligned ? left[i] / b->period : 0);
     }
     if (reps > 0) {

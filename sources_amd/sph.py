@@ -190,3 +190,10 @@ class Solver:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         self._check(self.lib.sphg_pair_counts(self._ctx, C.byref(a), C.byref(b), C.byref(c)), "pair_counts")
         return a.value, b.value, c.value
+
+    def nlist_stats(self):
+        """(rebuilds, off_spells, skin) of the neighbour lists of the FP64
+        two-phase step."""
+        n, off, skin = C.c_int64(), C.c_int64(), C.c_double()
+        self._check(self.lib.sphg_nlist_stats(self._ctx, C.byref(n), C.byref(off), C.byref(skin)), "nlist_stats")
+        return n.value, off.value, skin.value

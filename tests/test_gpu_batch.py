@@ -1,6 +1,6 @@
 """Batched scenarios: contexts advanced by replays of one captured hipGraph
-(per reorder period, several streams) give bit-identical states to the same
-contexts stepped alone."""
+(per period: the neighbour-list rebuild period of FP64 contexts, several
+streams) give bit-identical states to the same contexts stepped alone."""
 import numpy as np
 import pytest
 
@@ -17,8 +17,10 @@ def cases():
     return scenarios.sweep(4, seed=5, nmin=2000, nmax=8000)
 
 
-@pytest.mark.parametrize("nsteps", [40, 70])
+@pytest.mark.parametrize("nsteps", [40, 140])
 def test_batch_is_bit_identical_to_solo(nsteps):
+    # 40 steps: eager only; 140: eager to step 64, one replayed period of 64
+    # steps (65-128, starting with the list rebuild), eager remainder
     solo = cases()
     bat = cases()
     print("particles", [c.water.n + c.soil.n for c in solo], flush=True)
@@ -45,10 +47,12 @@ def test_batch_is_bit_identical_to_solo(nsteps):
         s.close()
 
 
-def test_full_sweep_is_bit_identical_to_solo():
+def test_full_sweep_is_bit_identical_to_solo(monkeypatch):
     """The whole 64-case sweep (N ~ logU[5k, 120k], 2.41M particles, seed 11):
-    33 steps batched (eager to the period boundary, two captured periods
-    replayed, eager remainder) equal the same 33 steps of each case alone."""
+    33 steps batched (eager to the period boundary, two captured periods of
+    16 steps replayed, eager remainder) equal the same 33 steps of each case
+    alone (both with the neighbour lists rebuilt every 16 steps)."""
+    monkeypatch.setenv("SPHG_NL_EVERY", "16")
     nsteps = 33
     solo = scenarios.sweep(64, seed=11)
     bat = scenarios.sweep(64, seed=11)

@@ -185,6 +185,7 @@ def test_state_reorder_is_transparent(monkeypatch, every):
     # The device keeps particles in cell order, permuting the state every N
     # steps; host views, pair lists and results stay in reference order.
     monkeypatch.setenv("SPHG_REORDER_EVERY", every)
+    monkeypatch.setenv("SPHG_NL_EVERY", every)     # the list rebuild reorders too
     sc, ref = _both(small_cases()["slope_small"])
     s = Solver(sc.params, sc.water, sc.soil)
     s.time_integration(7, sync=True)
