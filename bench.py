@@ -27,25 +27,20 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at 1M two-phase SPH; learned-vs-plain speedup at ≤1% ρ err"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per particle of each fused pass: the
-# two-neighbour-stage model (state read once per stage, final state written
-# once, stage-A -> stage-B intermediates written and read once), split by stage.
-# FP64:
-#   water: A = 100 state + 40 intermediates = 140, B = 100 + 40 + 88 = 228 (368 total)
-#   soil:  one pass for both stages (its stage B needs nothing from the other
-#          soil particles' stage A): 124 state + 48 + 112 outputs = 284
-# mixed (integrated fields x, v, rho, vof, stress, p in FP64, the rest FP32):
-#   water: A = 76 + 20 = 96,   B = 76 + 20 + 68 = 164  (260 total)
-#   soil:  one pass, 100 + 24 + 92 = 216
-#   cell list: 28 per particle and step
-ALGO_BYTES = {
-    "fp64": {"water_a": ("water", 140), "water_b": ("water", 228),
-             "soil_ab": ("soil", 284)},
-    "mixed": {"water_a": ("water", 96), "water_b": ("water", 164),
-              "soil_ab": ("soil", 216)},
-}
-STEP_BYTES = {"fp64": {"water": 368 + 28, "soil": 284 + 28},
-              "mixed": {"water": 260 + 28, "soil": 216 + 28}}
+# Algorithmic HBM bytes per particle and step, the two-neighbour-stage model
+# of the survey of the reference's loop (state read once per stage, final
+# state written once, stage-A -> stage-B intermediates written and read once;
+# neighbour re-reads on chip): water 368, soil 456, cell list 28 (both
+# phases).  The roofline of the dominant kernel (the soil stage pass,
+# soil_ab) takes the soil figure per soil particle of its launch, as if that
+# pass moved all of the soil's bytes; "frac_split_model" beside it takes the
+# share this build's stage split assigns to that pass: 124 state + 48
+# intermediates + 112 outputs = 284.  Mixed precision (integrated fields x,
+# v, rho, vof, stress, p in FP64, the rest FP32): water 260, soil 344 (split
+# share 216).
+SURVEY_BYTES = {"fp64": {"water": 368, "soil": 456, "cell": 28},
+                "mixed": {"water": 260, "soil": 344, "cell": 28}}
+SPLIT_BYTES = {"fp64": {"soil_ab": 284}, "mixed": {"soil_ab": 216}}
 
 
 class Hip:
@@ -123,8 +118,10 @@ def cpu_child(args):
 
 
 def learned_section(args, hip):
-    """Mixed-precision plain step and the learned K-step surrogate with the
-    FP64 plain step as judge, against FP64 plain steps from the same state."""
+    """The learned K-step surrogate (coarse FP64 step of K dt + the network's
+    correction, the FP64 plain step as judge every J advances), its ablation
+    (the same rollout with the network off: the coarse step alone) and FP64
+    plain steps, all from the same state of the 250k slope."""
     import numpy as np
     from sources_amd import abi, scenarios
     from sources_amd.sph import Solver
@@ -133,7 +130,7 @@ def learned_section(args, hip):
     K, S, M, J = args.learned_k, args.learned_samples, args.learned_advances, args.learned_judge
     # the slope at half the linear size: ~250k particles
     sc = scenarios.slope(dx=args.dx, scale=args.scale * args.learned_scale, max_interaction=0)
-    sc.params.precision = abi.SPHG_MIXED
+    sc.params.precision = abi.SPHG_FP64
     ntot = sc.water.n + sc.soil.n
     b = Solver(sc.params, sc.water, sc.soil)
     b.time_integration(args.warmup, sync=False)
@@ -146,45 +143,74 @@ def learned_section(args, hip):
     l0 = g.train(S, lr=0.0)
     l1 = g.train(args.learned_iters, lr=2e-3)
     t_train = time.perf_counter() - t0
+    gains = [float(x) for x in g.gains()[:3]]
     print(f"# surrogate: {S} samples, {args.learned_iters} Adam steps, loss {l0:.3f} -> {l1:.3f} "
-          f"in {t_train:.1f}s", file=sys.stderr, flush=True)
+          f"in {t_train:.1f}s, gains {gains}", file=sys.stderr, flush=True)
 
     b.sync_all()
-    ref = sc.copy()
-    ref.params.precision = abi.SPHG_FP64
+    base = sc.copy()
+    it0 = b.itimestep
+    ref = base.copy()
     a = Solver(ref.params, ref.water, ref.soil)
-    a.itimestep = b.itimestep
+    a.itimestep = it0
     hip.sync()
     hip.record(e0, a.stream)
     a.time_integration(M * K, sync=False)
     hip.record(e1, a.stream)
     ms_plain = hip.elapsed_ms(e0, e1)
-    print(f"# plain FP64: {M * K} steps in {ms_plain:.1f} ms", file=sys.stderr, flush=True)
-    hip.record(e0, b.stream)
-    nfb, jmax, jrms = g.rollout(M, judge_every=J, tol=0.01, use_net=True)
-    hip.record(e1, b.stream)
-    ms_learned = hip.elapsed_ms(e0, e1)
-    print(f"# learned: {M} advances of {K} steps in {ms_learned:.1f} ms, {nfb} fallbacks, "
-          f"judge max rho err {jmax:.2e}", file=sys.stderr, flush=True)
     a.sync_all()
-    b.sync_all()
-    err = 0.0
-    for pa, pb in ((ref.water, sc.water), (ref.soil, sc.soil)):
-        real = pa.itype[:pa.n] > 0
-        err = max(err, float(np.max(np.abs(pb.rho[:pb.n][real] - pa.rho[:pa.n][real]) / np.abs(pa.rho[:pa.n][real]))))
+    print(f"# plain FP64: {M * K} steps in {ms_plain:.1f} ms", file=sys.stderr, flush=True)
+
+    def rho_err():
+        e, sq, n = 0.0, 0.0, 0
+        for pa, pb in ((ref.water, sc.water), (ref.soil, sc.soil)):
+            real = pa.itype[:pa.n] > 0
+            d = np.abs(pb.rho[:pb.n][real] - pa.rho[:pa.n][real]) / np.abs(pa.rho[:pa.n][real])
+            e, sq, n = max(e, float(np.max(d))), sq + float(np.sum(d * d)), n + d.size
+        return e, (sq / max(n, 1)) ** 0.5
+
+    legs = {}
+    for use_net in (True, False):
+        # both legs start from the recorded state (host copy uploaded)
+        for ph, bp in ((sc.water, base.water), (sc.soil, base.soil)):
+            for k, v in vars(bp).items():
+                if isinstance(v, np.ndarray):
+                    np.copyto(getattr(ph, k), v)
+        b.upload_all()
+        b.itimestep = it0
+        hip.sync()
+        hip.record(e0, b.stream)
+        nfb, jmax, jrms = g.rollout(M, judge_every=J, tol=0.01, use_net=use_net)
+        hip.record(e1, b.stream)
+        ms = hip.elapsed_ms(e0, e1)
+        b.sync_all()
+        emax, erms = rho_err()
+        legs[use_net] = {"ms": ms, "speedup_vs_plain_fp64": ms_plain / ms, "value": ntot * M * K / (ms * 1e-3),
+                         "rho_rel_err_max_vs_fp64": emax, "rho_rel_err_rms_vs_fp64": erms,
+                         "meets_1pct_gate": emax <= 0.01, "fallbacks": nfb, "judge_max_rho_err": jmax,
+                         "judge_rms_rho_err": jrms}
+        print(f"# learned ({'network' if use_net else 'network off'}): {M} advances of {K} steps in {ms:.1f} ms, "
+              f"{nfb} fallbacks, final rho err max {emax:.2e} rms {erms:.2e}", file=sys.stderr, flush=True)
     g.close()
     a.close()
     b.close()
+    net, off = legs[True], legs[False]
     return {
         "learned": {
             "particles": ntot, "k_steps": K, "advances": M, "judge_every": J, "tol": 0.01,
-            "steps_covered": M * K, "ms_plain_fp64": ms_plain, "ms_learned": ms_learned,
-            "value": ntot * M * K / (ms_learned * 1e-3), "unit": "particle-steps/s",
-            "speedup_vs_plain_fp64": ms_plain / ms_learned,
-            "rho_rel_err_max_vs_fp64": err, "meets_1pct_gate": err <= 0.01, "fallbacks": nfb, "judge_max_rho_err": jmax, "judge_rms_rho_err": jrms,
+            "steps_covered": M * K, "ms_plain_fp64": ms_plain, "ms_learned": net["ms"],
+            "value": net["value"], "unit": "particle-steps/s",
+            "speedup_vs_plain_fp64": net["speedup_vs_plain_fp64"],
+            "rho_rel_err_max_vs_fp64": net["rho_rel_err_max_vs_fp64"],
+            "rho_rel_err_rms_vs_fp64": net["rho_rel_err_rms_vs_fp64"],
+            "meets_1pct_gate": net["meets_1pct_gate"], "fallbacks": net["fallbacks"],
+            "judge_max_rho_err": net["judge_max_rho_err"], "judge_rms_rho_err": net["judge_rms_rho_err"],
+            "network_off": off,
+            "network_beats_ablation": (net["rho_rel_err_max_vs_fp64"] < off["rho_rel_err_max_vs_fp64"] and
+                                       net["rho_rel_err_rms_vs_fp64"] < off["rho_rel_err_rms_vs_fp64"]),
             "train": {"samples": S, "iters": args.learned_iters, "loss_before": l0, "loss_after": l1,
-                      "seconds": t_train},
-            "rates_precision": "mixed",
+                      "seconds": t_train, "gains": gains},
+            "rates_precision": "fp64",
         },
     }
 
@@ -324,12 +350,12 @@ def main():
                     help="mixed vs FP64 error is reported after 1 step and after this many (0: skip)")
     ap.add_argument("--no-pcie", action="store_true")
     # 3 dt is the largest coarse step that stays stable on the slope (4 dt diverges)
-    ap.add_argument("--learned-k", type=int, default=3)
+    ap.add_argument("--learned-k", type=int, default=4)
     ap.add_argument("--learned-samples", type=int, default=8)
     ap.add_argument("--learned-iters", type=int, default=400)
-    ap.add_argument("--learned-advances", type=int, default=42)
-    ap.add_argument("--learned-judge", type=int, default=7,
-                    help="plain FP64 check every this many advances (7 x 3 steps: about every 20 steps)")
+    ap.add_argument("--learned-advances", type=int, default=30)
+    ap.add_argument("--learned-judge", type=int, default=5,
+                    help="plain FP64 check every this many advances (5 x 4 steps: every 20 steps)")
     ap.add_argument("--learned-scale", type=float, default=0.5, help="slope size for the learned section")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="fp64: reference arithmetic; mixed: FP32 Kahan pair sums, FP64 integration")
@@ -395,19 +421,21 @@ def main():
     hip.sync()
     kern = {}
     for name in ("cell_key", "cell_scatter", "cell_sort", "scan", "fill", "first_half", "pack", "permute", "iota",
-                 "water_x", "water_a", "soil_x", "soil_a", "soil_ab", "water_b", "soil_b", "final_water", "final_soil"):
+                 "nlist", "water_x", "water_a", "soil_x", "soil_a", "soil_ab", "water_b", "soil_b", "final_water",
+                 "final_soil"):
         n, t = s.profile_get(name)
         if n:
             kern[name] = {"launches": n, "avg_us": 1e3 * t / n, "total_ms_per_step": t / args.prof_steps}
     s.profile(False)
     pairs_w, pairs_s, pairs_x = s.pair_counts()
-    algo_bytes = ALGO_BYTES[args.precision]
-    dom = max((k for k in kern if k in algo_bytes), key=lambda k: kern[k]["total_ms_per_step"])
-    ph, bpp = algo_bytes[dom]
-    npart = nw if ph == "water" else ns
-    algo = bpp * npart
+    nl_rebuilds, nl_off, nl_skin = s.nlist_stats()
+    sb = SURVEY_BYTES[args.precision]
+    dom = "soil_ab"
+    algo = sb["soil"] * ns
+    algo_split = SPLIT_BYTES[args.precision][dom] * ns
     avg_s = kern[dom]["avg_us"] * 1e-6
     achieved = algo / avg_s / 1e9
+    achieved_split = algo_split / avg_s / 1e9
     traffic = None
     # HBM bytes per launch of the dominant kernel at this workload, from the
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated by a copy)
@@ -417,7 +445,7 @@ def main():
             txt = f.read()
         blk = txt.split("```json", 1)[-1].split("```", 1)[0]
         traffic = json.loads(blk).get(dom)
-    step_bytes = STEP_BYTES[args.precision]["water"] * nw + STEP_BYTES[args.precision]["soil"] * ns
+    step_bytes = (sb["water"] + sb["cell"]) * nw + (sb["soil"] + sb["cell"]) * ns
     out = {
         "metric": METRIC,
         "value": value,
@@ -437,6 +465,7 @@ def main():
             "water_real": sc.water.ntotal, "soil_real": sc.soil.ntotal,
             "water_pairs": pairs_w, "soil_pairs": pairs_s, "cross_pairs": pairs_x,
             "skf": sc.params.skf, "precision": args.precision,
+            "neighbour_lists": {"rebuilds": nl_rebuilds, "off_spells": nl_off, "skin": nl_skin},
         },
         # distinct interacting pairs (water-water, soil-soil, water-soil) per
         # second; the gather visits each same-phase pair from both ends
@@ -444,7 +473,10 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "algo_bytes_per_launch": algo, "avg_launch_us": kern[dom]["avg_us"],
+            "algo_bytes_per_launch": algo, "algo_bytes_per_unit": sb["soil"], "units_per_launch": ns,
+            "avg_launch_us": kern[dom]["avg_us"],
+            "achieved_split_model": achieved_split, "frac_split_model": achieved_split / HBM_PEAK_GBS,
+            "algo_bytes_per_launch_split_model": algo_split,
         },
         "step_roofline": {
             "algo_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,

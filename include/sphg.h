@@ -259,9 +259,12 @@ int sphg_nlist_stats(const sphg_ctx *ctx, int64_t *rebuilds, int64_t *off_spells
    FP32 accumulation) predicts from the state the advance starts from.  It is
    trained on samples that the plain FP64 step generates, and the plain step
    judges it: sphg_gnn_rollout runs the plain K steps beside every
-   judge_every-th learned advance and keeps the plain result whenever the
-   learned density deviates by more than tol (relative, max over real
-   particles); the judge always runs in FP64.  Two-phase contexts only
+   judge_every-th learned advance, measures the learned density's deviation
+   from them (relative, max over real particles) and goes on from the plain
+   result; the judge always runs in FP64.  After every other learned advance
+   a sanity test on the device (finite, inside the grid) restores the last
+   judged state when it fails, and the host, which reads its flag one advance
+   later, covers the span with plain steps.  Two-phase contexts only
    (SPHG_ERR_UNSUPPORTED otherwise). */
 typedef struct sphg_gnn sphg_gnn;
 int sphg_gnn_nparams(void);
@@ -278,12 +281,17 @@ int sphg_gnn_num_samples(const sphg_gnn *gnn);
    these); targets = plain result - coarse result. */
 int sphg_gnn_record(sphg_gnn *gnn);
 /* iters Adam steps (one sample each, cycling); loss_out: mean normalised
-   loss over the last pass. */
+   loss over the last pass.  Then the gain of each correction channel is
+   fitted (least squares of the prediction against the targets over every
+   sample, within [0, 1.5]): the applied correction is prediction x gain. */
 int sphg_gnn_train(sphg_gnn *gnn, int iters, float lr, float *loss_out);
+/* The fitted gains of the correction channels (vx, vy, rho, spare). */
+int sphg_gnn_gains(const sphg_gnn *gnn, float *gains);
 /* One learned advance of K steps (use_net=0: the coarse step alone). */
 int sphg_gnn_step(sphg_gnn *gnn, int use_net);
 /* nlearned learned advances with the plain step as judge and fallback:
-   nfallback judged advances replaced by the plain steps; max_err / rms_err:
+   nfallback: judged advances over tol plus advances replaced by plain steps
+   after a failed sanity test; max_err / rms_err:
    largest and root-mean-square relative density deviation of the judged
    learned advances from the plain steps (real particles). */
 int sphg_gnn_rollout(sphg_gnn *gnn, int nlearned, int judge_every, double tol, int use_net, int *nfallback,

@@ -178,6 +178,7 @@ def load_library(path=None):
                                      C.POINTER(C.c_double)]
     lib.sphg_gnn_sample_size.argtypes = [vp, C.c_int]
     lib.sphg_gnn_debug.argtypes = [vp, C.c_int, fp, fp, ip, ip, fp, fp, fp, fp, fp, fp]
+    lib.sphg_gnn_gains.argtypes = [vp, fp]
     lib.sphg_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     lib.sphg_batch_destroy.argtypes = [vp]
     lib.sphg_batch_destroy.restype = None
@@ -210,7 +211,7 @@ EXPORTED_SYMBOLS = (
     "sphg_nlist_stats",
     "sphg_gnn_nparams", "sphg_gnn_create", "sphg_gnn_destroy", "sphg_gnn_weights", "sphg_gnn_num_samples",
     "sphg_gnn_record", "sphg_gnn_train", "sphg_gnn_step", "sphg_gnn_rollout", "sphg_gnn_sample_size",
-    "sphg_gnn_debug",
+    "sphg_gnn_debug", "sphg_gnn_gains",
     "sphg_batch_create", "sphg_batch_destroy", "sphg_batch_run", "sphg_batch_set_streams", "sphg_batch_tune",
     "sphg_direct_find_noalloc",
 )

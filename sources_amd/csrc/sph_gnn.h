@@ -29,7 +29,8 @@ namespace gnn {
 
 constexpr int FE = 16;    // edge input features
 constexpr int HE = 32;    // edge hidden width = message width
-constexpr int FN = 16;    // node input features
+constexpr int FN = 32;    // node input features
+constexpr int FR = 16;    // features FR..FR+2: rate-change estimates of the correction (vx, vy, rho)
 constexpr int FI = FN + HE;  // node MLP input width
 constexpr int D = 64;     // node hidden width
 constexpr int NO = 4;     // outputs used: c_vx, c_vy, c_rho, (spare)
@@ -107,7 +108,8 @@ struct Scales {
     float inv_r;     // 1 / density-deviation scale (relative)
     float inv_s;     // 1 / stress scale
     float kdt;       // K dt
-    float out[NO];   // output scales: c = y * out
+    float out[NO];   // output scales of the training targets (target = y * out)
+    float gain[NO];  // calibrated shrinkage of the applied correction: c = y * out * gain
 };
 
 // One graph: nodes [0, n) = water (cell order) then soil.
@@ -189,7 +191,11 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
     const float* W = A.W;
     if (h == 0) {
 #pragma unroll
-        for (int f = 0; f < FN; ++f) X0[p][f] = valid ? A.G.nf[(size_t)i * FN + f] : 0.0f;
+        for (int f = 0; f < FN; ++f) {
+            float v = valid ? A.G.nf[(size_t)i * FN + f] : 0.0f;
+            if (f >= FR && f < FR + 3) v *= A.inv_out[f - FR];   // physical units -> target units
+            X0[p][f] = v;
+        }
     }
     const int deg = valid ? A.G.nd[i] : 0;
     const int maxdeg = wave_max(deg);
@@ -431,8 +437,14 @@ struct GatherOut {
     int* overflow;     // [1]
 };
 
+// NLM.m = nullptr: candidates are the rows of the particle's current cell (a
+// cell list of the current positions on grid g).  Otherwise the cell list
+// and record order are those of the FP64 step's neighbour lists (grid g is
+// their grid): the candidates are the particle's listed same-phase
+// candidates, tested with the current positions; a list that overflowed
+// walks its rows.
 __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, int off, int phase, Scales S,
-                             GatherOut O)
+                             GatherOut O, Masks NLM, const int* nl_cell)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
@@ -462,6 +474,14 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
         nf[10] = nf[11] = nf[12] = nf[13] = 0.0f;
     }
     nf[14] = nf[15] = 0.0f;
+    // the rates the state holds (of the step before): k_gnn_delta turns them
+    // into the rate-change features once the coarse step has evaluated the
+    // rates of this state
+    nf[FR] = (float)P.ax[a];
+    nf[FR + 1] = (float)P.ay[a];
+    nf[FR + 2] = (float)P.drho[a];
+#pragma unroll
+    for (int f = FR + 3; f < FN; ++f) nf[f] = 0.0f;
     // rows of 16 and 12 floats, written as 16-byte vectors
     float4* nf4 = reinterpret_cast<float4*>(O.nf + (size_t)node * FN);
 #pragma unroll
@@ -472,49 +492,104 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
     rw4[2] = make_float4(nf[2], 0.0f, 0.0f, 0.0f);
     O.tw[node] = virt ? 0.0f : 1.0f;
     O.node_of_id[P.id[a]] = node;
-    // neighbours (cell order of the same phase)
-    const Rows R = block_rows(g, P.cell_start, x, y);
+    // neighbours (cell order of the same phase), in ascending k
     int cnt = 0;
     int* nb = O.nb + node;   // slot s at nb[s * n_nodes]
     const size_t ns = (size_t)O.n_nodes;
-#pragma unroll 1
-    for (int r = 0; r < 3; ++r) {
-        const int k1 = R.k1[r];
-        // candidates in groups of four: the index loads, then the position
-        // loads of all four, then the predicates in candidate order (the slot
-        // order stays ascending in k)
-#pragma unroll 1
-        for (int kb = R.k0[r]; kb < k1; kb += 4) {
-            int bq[4];
-            double xq[4], yq[4], hq[4];
+    // candidates in groups of four: the index loads, then the position loads
+    // of all four, then the predicates in candidate order
+    auto group = [&](int k0, int k1, int k2, int k3) {
+        const int kq[4] = {k0, k1, k2, k3};
+        int bq[4];
+        double xq[4], yq[4], hq[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) bq[q] = kb + q < k1 ? P.order[kb + q] : -1;
+        for (int q = 0; q < 4; ++q) bq[q] = kq[q] >= 0 ? P.order[kq[q]] : -1;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int b = bq[q] >= 0 ? bq[q] : a;
-                xq[q] = P.x[b];
-                yq[q] = P.y[b];
-                hq[q] = uni ? h : P.hsml[b];
-            }
+        for (int q = 0; q < 4; ++q) {
+            const int b = bq[q] >= 0 ? bq[q] : a;
+            xq[q] = P.x[b];
+            yq[q] = P.y[b];
+            hq[q] = uni ? h : P.hsml[b];
+        }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int k = kb + q;
-                if (bq[q] < 0 || k == t) continue;
-                const double dx = x - xq[q], dy = y - yq[q];
-                double dr;
-                {
+        for (int q = 0; q < 4; ++q) {
+            const int k = kq[q];
+            if (bq[q] < 0 || k == t) continue;
+            const double dx = x - xq[q], dy = y - yq[q];
+            double dr;
+            {
 #pragma clang fp contract(off)
-                    dr = dx * dx + dy * dy;
-                }
-                if (!pred(uni != 0, thr, dr, h, hq[q], K)) continue;
-                if (cnt < DEG) nb[(size_t)cnt * ns] = off + k;
-                ++cnt;
+                dr = dx * dx + dy * dy;
+            }
+            if (!pred(uni != 0, thr, dr, h, hq[q], K)) continue;
+            if (cnt < DEG) nb[(size_t)cnt * ns] = off + k;
+            ++cnt;
+        }
+    };
+    int cx, cy;
+    if (nl_cell) {
+        const int key = nl_cell[t];
+        cy = key / g.ncx;
+        cx = key - cy * g.ncx;
+    } else {
+        clamped_cell(g, x, y, cx, cy);
+    }
+    const Rows R = block_rows_at(g, P.cell_start, cx, cy);
+    const CatMask vm = NLM.m ? mask_of(NLM, P.n, t) : CatMask{0, 0, 7};
+    if (!vm.ovf) {
+        // listed candidates: the mask's bits over the rows, ascending
+        const int s1 = R.k1[0] - R.k0[0], s2 = s1 + R.k1[1] - R.k0[1];
+        const int e0 = R.k0[0], e1 = R.k0[1] - s1, e2 = R.k0[2] - s2;
+        uint64_t lo = vm.lo, hi = vm.hi;
+#pragma unroll 1
+        while (lo | hi) {
+            int kq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool in_lo = lo != 0;
+                const uint64_t m = in_lo ? lo : hi;
+                const int c = m ? __builtin_ctzll(m) + (in_lo ? 0 : 64) : -1;
+                const uint64_t rest = m & (m - 1);
+                lo = in_lo ? rest : lo;
+                hi = in_lo ? hi : rest;
+                kq[q] = c < 0 ? -1 : c + (c < s1 ? e0 : (c < s2 ? e1 : e2));
+            }
+            group(kq[0], kq[1], kq[2], kq[3]);
+        }
+    } else {
+#pragma unroll 1
+        for (int r = 0; r < 3; ++r) {
+            const int k0 = r == 0 ? R.k0[0] : (r == 1 ? R.k0[1] : R.k0[2]);
+            const int k1 = r == 0 ? R.k1[0] : (r == 1 ? R.k1[1] : R.k1[2]);
+#pragma unroll 1
+            for (int kb = k0; kb < k1; kb += 4) {
+                int kq[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) kq[q] = kb + q < k1 ? kb + q : -1;
+                group(kq[0], kq[1], kq[2], kq[3]);
             }
         }
     }
     if (cnt > DEG) atomicOr(O.overflow, 1);
     O.nd[node] = min(cnt, DEG);
     for (int s = min(cnt, DEG); s < DEG; ++s) nb[(size_t)s * ns] = -1;
+}
+
+// Rate-change features: after the coarse step P holds the rates evaluated at
+// the state the advance starts from, and features FR..FR+2 hold the rates of
+// the evaluation `gap` steps of dt before it.  Over K steps of dt a rate r
+// changing at dr/dt moves the fine result from the coarse one by about
+// K (K - 1) / 2 dt^2 dr/dt: with dr/dt ~ (r_new - r_old) / (gap dt) the
+// features become coef (r_new - r_old), coef = K (K - 1) / 2 dt / gap, in the
+// units of the correction (velocity, density).
+__global__ void k_gnn_delta(DPhase P, const int* node_of_id, float* nf, double coef)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    float* f = nf + (size_t)node_of_id[P.id[a]] * FN + FR;
+    f[0] = (float)((P.ax[a] - (double)f[0]) * coef);
+    f[1] = (float)((P.ay[a] - (double)f[1]) * coef);
+    f[2] = (float)((P.drho[a] - (double)f[2]) * coef);
 }
 
 // Targets: fine state (K plain steps, current) minus coarse state (one step
@@ -541,9 +616,9 @@ __global__ void k_gnn_correct(DPhase P, KParams K, const int* node_of_id, const 
     if (P.itype[a] <= 0) return;
     const int node = node_of_id[P.id[a]];
     const float* yy = y + (size_t)node * NO;
-    const double cvx = (double)yy[0] * S.out[0];
-    const double cvy = (double)yy[1] * S.out[1];
-    const double crh = (double)yy[2] * S.out[2];
+    const double cvx = (double)yy[0] * S.out[0] * S.gain[0];
+    const double cvy = (double)yy[1] * S.out[1] * S.gain[1];
+    const double crh = (double)yy[2] * S.out[2] * S.gain[2];
     P.vx[a] = P.vx[a] + cvx;
     P.vy[a] = P.vy[a] + cvy;
     P.x[a] = P.x[a] + 0.5 * (double)S.kdt * cvx;
@@ -605,6 +680,49 @@ __global__ void k_sanity(DPhase P, Grid g, int* flag)
     const bool ok = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1 && r > 0.0 && r < 1.0e300 &&
                     P.vx[a] == P.vx[a] && P.vy[a] == P.vy[a];
     if (!ok) atomicOr(flag, 1);
+}
+
+// Least-squares gain of each output channel over one sample (real nodes):
+// acc[o] += sum w y_o t_o / out_o, acc[NO + o] += sum w y_o^2.  The gain
+// sum(y t) / sum(y^2) scales the applied correction so that it removes
+// as much of the target as the network's prediction explains.
+__global__ __launch_bounds__(256) void k_gnn_fit(const float* y, const float* tgt, const float* tw, int n,
+                                                 Scales S, double* acc)
+{
+    __shared__ double part[4][2 * NO];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2 * NO];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+        const bool ok = i < n && tw[i] > 0.0f;
+        const double yo = ok ? (double)y[(size_t)i * NO + o] : 0.0;
+        const double to = ok ? (double)tgt[(size_t)i * NO + o] / (double)S.out[o] : 0.0;
+        v[o] = yo * to;
+        v[NO + o] = yo * yo;
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * NO; ++q)
+        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < 2 * NO; ++q) part[w][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < 2 * NO) {
+        const double sum = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        atomicAdd(&acc[threadIdx.x], sum);
+    }
+}
+
+// The state of a phase copied back from a snapshot (A.s* -> A.d*) when the
+// sanity flag is set: a learned advance that broke the state is undone on
+// the device, before any later kernel reads it.
+__global__ void k_restore_if(const int* flag, PermArgs A, int n)
+{
+    if (*flag == 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = 0; q < A.nd; ++q) A.dd[q][i] = A.sd[q][i];
+    for (int q = 0; q < A.ni; ++q) A.di[q][i] = A.si[q][i];
 }
 
 __global__ void k_slot_of_id(DPhase P, int* slot_of_id)

@@ -139,7 +139,8 @@ struct sphg_ctx {
     double nl_thr[3] = {0.0, 0.0, 0.0};   // (R + skin)^2: water, soil, cross pairs
     bool nl_valid = false;      // lists, cell list and records belong to the state
     bool nl_due = false;        // the displacement check asked for a rebuild
-    int nl_rebuilt_at = 0;      // itimestep of the last rebuild
+    int nl_seq = 0;             // list-mode steps issued (coarse steps of K dt count once)
+    int nl_rebuilt_at = 0;      // nl_seq of the step of the last rebuild
     int64_t nl_rebuilds = 0;
     bool use_nl = true;         // SPHG_NLIST=0: scan the cell rows every step instead
     double nl_skin = NL_SKIN;   // skin / largest support radius (SPHG_NL_SKIN)
@@ -147,9 +148,10 @@ struct sphg_ctx {
     int nl_off_until = 0;       // lists off (fast-moving particles) before this step
     int64_t nl_off_spells = 0;
     bool capturing = false;     // steps are being captured into a graph (no host reads)
+    int last_k = 1;             // the last step was of size last_k dt (a learned coarse step: K)
     int* d_nlflag = nullptr;    // bit 0: rebuild due; bit 1: a particle moved past skin / 2
     int* h_nlflag = nullptr;    // pinned: two slots for the lagged check of sphg_time_integration
-    int h_nlstep[2] = {-1, -1}; // step whose flag each slot holds (-1: none)
+    int h_nlstep[2] = {-1, -1}; // nl_seq of the step whose flag each slot holds (-1: none)
     hipEvent_t ev_nl[2] = {nullptr, nullptr};
 };
 
@@ -1081,7 +1083,7 @@ int build_nlists(sphg_ctx* c)
               c->nl_thr[2], 0, S.nl_cross, (int*)nullptr));
     c->nl_valid = true;
     c->nl_due = false;
-    c->nl_rebuilt_at = c->itimestep;
+    c->nl_rebuilt_at = c->nl_seq + 1;   // the step being issued
     ++c->nl_rebuilds;
     return SPHG_OK;
 }
@@ -1232,7 +1234,7 @@ bool renorm_step(const sphg_ctx* c, int step) { return renorm_active(c->P) && st
 // half the skin fails the run (its step may have missed pairs).
 int nl_poll(sphg_ctx* c)
 {
-    const int s = c->itimestep, slot = s & 1, o = slot ^ 1;
+    const int s = ++c->nl_seq, slot = s & 1, o = slot ^ 1;
     HIPCHK(c, hipMemcpyAsync(&c->h_nlflag[slot], c->d_nlflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(c->ev_nl[slot], c->stream));
     c->h_nlstep[slot] = s;
@@ -1246,7 +1248,7 @@ int nl_poll(sphg_ctx* c)
         // rebuilds this close together cost more than the row scans: lists
         // off for a while (never in a batch, whose captured period has them)
         if (s + 1 - c->nl_rebuilt_at < NL_MIN_RUN && !c->stable_ptrs) {
-            c->nl_off_until = s + 1 + NL_OFF;
+            c->nl_off_until = c->itimestep + 1 + NL_OFF;
             ++c->nl_off_spells;
         }
     }
@@ -1255,6 +1257,7 @@ int nl_poll(sphg_ctx* c)
 
 int one_step(sphg_ctx* c)
 {
+    c->last_k = 1;
     const int step = ++c->itimestep;
     const bool nl = nl_mode(c);
     PhaseBuf& W = c->ph[0];
@@ -1863,11 +1866,13 @@ struct sphg_gnn {
     double tsum[gnn::NO] = {0, 0, 0, 0}, tsq[gnn::NO] = {0, 0, 0, 0};
     double tcount = 0.0;
     StateCopy snap[2][2];           // [slot][phase]
-    // the network's forward pass runs on its own stream beside the coarse
-    // step (both read the state the advance starts from; neither writes what
-    // the other reads)
-    hipStream_t fstream = nullptr;
-    hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr;
+    int64_t snap_rebuilds[2] = {-1, -1};   // list rebuilds when each slot was taken
+    // lagged sanity check of the learned advances: device flag, its pinned
+    // copies (two slots) and their events; gain fit sums
+    int* broken = nullptr;
+    int* h_broken = nullptr;
+    hipEvent_t ev_ok[2] = {nullptr, nullptr};
+    double* fit = nullptr;
 };
 
 namespace {
@@ -1929,8 +1934,15 @@ int snap_copy(sphg_gnn* g, int slot, bool to_snap)
             RC(cp(b.d.itype, s.itype, n * sizeof(int)));
             RC(cp(b.d.fail, s.fail, n * sizeof(int)));
             for (int i = 0; i < N_STATE_FIELDS; ++i) RC(cp(*f[i], s.f[i], n * sizeof(double)));
-            b.cells_valid = false;
         }
+    }
+    if (to_snap) {
+        g->snap_rebuilds[slot] = c->nl_rebuilds;
+    } else if (!(nl_mode(c) && c->nl_valid && c->nl_rebuilds == g->snap_rebuilds[slot])) {
+        // the neighbour lists (and their cell lists and record order) stay
+        // valid for a state of their own rebuild epoch: the slot layout is the
+        // same and the displacement check sees the restored positions
+        state_changed(c);
     }
     return SPHG_OK;
 }
@@ -1957,6 +1969,7 @@ int coarse_step(sphg_ctx* c, int K)
     c->K.dt = dt0;
     RC(rc);
     c->itimestep += K - 1;
+    c->last_k = K;
     return SPHG_OK;
 }
 
@@ -1977,18 +1990,22 @@ int build_graph(sphg_gnn* g, sphg_gnn::Graph& G)
     sphg_ctx* c = g->c;
     PhaseBuf& Wb = c->ph[0];
     PhaseBuf& Sb = c->ph[1];
-    RC(build_cells(c, 0));
-    RC(build_cells(c, 1));
+    // with valid neighbour lists the graph comes from them and their cell
+    // lists stay; otherwise from cell lists of the current positions
+    const bool lists = nl_mode(c) && c->nl_valid && Wb.cells_valid && Sb.cells_valid && Wb.cells_nl && Sb.cells_nl;
+    if (!lists) {
+        RC(build_cells(c, 0));
+        RC(build_cells(c, 1));
+    }
     G.nw = Wb.n;
     RC(fill_int(c, g->overflow, 1, 0));
     for (int k = 0; k < 2; ++k) {
         PhaseBuf& b = c->ph[k];
         gnn::GatherOut O{G.nf, G.raw, G.nb, G.nd, G.n, G.tw, g->node_of_id[k], g->overflow};
-        RC(launch(c, "gnn_gather", gnn::k_gnn_gather, nblk(b.n), TPB, b.d, c->gl[k], c->K,
+        RC(launch(c, "gnn_gather", gnn::k_gnn_gather, nblk(b.n), TPB, b.d, lists ? c->gvl[k] : c->gl[k], c->K,
                   k == 0 ? c->F.thr_w : c->F.thr_s, k == 0 ? c->F.uni_w : c->F.uni_s, k == 0 ? 0 : Wb.n, k,
-                  g->S, O));
+                  g->S, O, lists ? b.nl_same : Masks{}, lists ? (const int*)b.nl_cell : (const int*)nullptr));
     }
-    (void)Sb;
     return SPHG_OK;
 }
 
@@ -1998,7 +2015,22 @@ int forward(sphg_gnn* g, const sphg_gnn::Graph& G)
     A.G = gnn::Graph{G.n, G.nf, G.raw, G.nb, G.nd};
     A.W = g->W;
     A.out = g->y;
+    for (int o = 0; o < gnn::NO; ++o) A.inv_out[o] = 1.0f / g->S.out[o];
     return launch(g->c, "gnn_fwd", gnn::k_gnn<false>, (G.n + 31) / 32, 64, A);
+}
+
+// The rate-change features of graph G once the coarse step has run (the
+// state then holds the rates evaluated at the advance's start; gap: size in
+// dt of the step that produced the rates the graph build saw).
+int rate_features(sphg_gnn* g, sphg_gnn::Graph& G, int gap)
+{
+    sphg_ctx* c = g->c;
+    const double coef = 0.5 * g->K * (g->K - 1) * c->P.dt / gap;
+    for (int k = 0; k < 2; ++k) {
+        PhaseBuf& b = c->ph[k];
+        RC(launch(c, "gnn_delta", gnn::k_gnn_delta, nblk(b.n), TPB, b.d, (const int*)g->node_of_id[k], G.nf, coef));
+    }
+    return SPHG_OK;
 }
 
 }  // namespace
@@ -2023,13 +2055,15 @@ int sphg_gnn_create(sphg_ctx* c, int ksteps, int max_samples, uint64_t seed, sph
         (rc = galloc(g, &g->m, gnn::NPARAM)) || (rc = galloc(g, &g->v, gnn::NPARAM)) ||
         (rc = galloc(g, &g->loss, 1)) || 
         (rc = galloc(g, &g->y, (size_t)n * gnn::NO)) || (rc = galloc(g, &g->overflow, 1)) ||
-        (rc = galloc(g, &g->err, 1)) || (rc = galloc(g, &g->errsq, 2)) || (rc = graph_alloc(g, g->cur, n)))
+        (rc = galloc(g, &g->err, 1)) || (rc = galloc(g, &g->errsq, 2)) || (rc = graph_alloc(g, g->cur, n)) ||
+        (rc = galloc(g, &g->broken, 1)) || (rc = galloc(g, &g->fit, 2 * gnn::NO)))
         return fail(rc);
-    // default-priority stream and fork / join events of the forward pass
-    if (hipStreamCreateWithFlags(&g->fstream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&g->ev_f0, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&g->ev_f1, hipEventDisableTiming) != hipSuccess)
+    // events and pinned slots of the lagged sanity check
+    if (hipEventCreateWithFlags(&g->ev_ok[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->ev_ok[1], hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void**)&g->h_broken, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(SPHG_ERR_HIP);
+    g->h_broken[0] = g->h_broken[1] = 0;
     for (int k = 0; k < 2; ++k) {
         if ((rc = galloc(g, &g->node_of_id[k], c->ph[k].n)) || (rc = galloc(g, &g->slot_of_id[k], c->ph[k].n)))
             return fail(rc);
@@ -2046,7 +2080,7 @@ int sphg_gnn_create(sphg_ctx* c, int ksteps, int max_samples, uint64_t seed, sph
     g->S.inv_r = 100.0f;
     g->S.inv_s = (float)(1.0 / (c->K.s_rho0 * gabs * H));
     g->S.kdt = (float)kdt;
-    for (int o = 0; o < gnn::NO; ++o) g->S.out[o] = 1.0f;
+    for (int o = 0; o < gnn::NO; ++o) g->S.out[o] = g->S.gain[o] = 1.0f;
     // Xavier-uniform weights from SplitMix64(seed), zero biases
     std::vector<float> w(gnn::NPARAM, 0.0f);
     uint64_t st = seed;
@@ -2082,12 +2116,9 @@ void sphg_gnn_destroy(sphg_gnn* g)
 {
     if (!g) return;
     if (g->c) hipStreamSynchronize(g->c->stream);
-    if (g->fstream) {
-        hipStreamSynchronize(g->fstream);
-        hipStreamDestroy(g->fstream);
-    }
-    if (g->ev_f0) hipEventDestroy(g->ev_f0);
-    if (g->ev_f1) hipEventDestroy(g->ev_f1);
+    for (hipEvent_t e : g->ev_ok)
+        if (e) hipEventDestroy(e);
+    if (g->h_broken) hipHostFree(g->h_broken);
     for (void* p : g->allocs) hipFree(p);
     delete g;
 }
@@ -2114,8 +2145,10 @@ int sphg_gnn_record(sphg_gnn* g)
     RC(graph_alloc(g, G, c->ph[0].n + c->ph[1].n));
     const int step0 = c->itimestep;
     RC(snap_copy(g, 0, true));
+    const int gap = c->last_k;
     RC(build_graph(g, G));
     RC(coarse_step(c, g->K));           // coarse state -> snapshot 1
+    RC(rate_features(g, G, gap));
     RC(snap_copy(g, 1, true));
     RC(snap_copy(g, 0, false));
     c->itimestep = step0;
@@ -2150,6 +2183,37 @@ int sphg_gnn_record(sphg_gnn* g)
     g->samples.push_back(G);
     return check_flags(c);
 }
+
+}  // extern "C"
+
+namespace {
+
+// Gain of each correction channel: the least-squares factor of the network's
+// prediction against the recorded targets over every sample, within
+// [0, 1.5].  A channel the network does not predict gets a gain near zero,
+// so the correction never adds more error than it removes on the samples.
+int calibrate_gain(sphg_gnn* g)
+{
+    sphg_ctx* c = g->c;
+    RC(fill_dbl(c, g->fit, 2 * gnn::NO, 0.0));
+    for (const sphg_gnn::Graph& G : g->samples) {
+        RC(forward(g, G));
+        RC(launch(c, "gnn_fit", gnn::k_gnn_fit, nblk(G.n, 256), 256, (const float*)g->y, (const float*)G.tgt,
+                  (const float*)G.tw, G.n, g->S, g->fit));
+    }
+    double f[2 * gnn::NO] = {};
+    HIPCHK(c, hipMemcpyAsync(f, g->fit, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int o = 0; o < gnn::NO; ++o) {
+        const double gain = f[gnn::NO + o] > 0.0 ? f[o] / f[gnn::NO + o] : 0.0;
+        g->S.gain[o] = (float)std::min(std::max(gain, 0.0), 1.5);
+    }
+    return SPHG_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int sphg_gnn_train(sphg_gnn* g, int iters, float lr, float* loss_out)
 {
@@ -2188,8 +2252,7 @@ int sphg_gnn_train(sphg_gnn* g, int iters, float lr, float* loss_out)
         }
     }
     if (loss_out) *loss_out = (float)last;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return SPHG_OK;
+    return calibrate_gain(g);
 }
 
 int sphg_gnn_step(sphg_gnn* g, int use_net)
@@ -2197,47 +2260,53 @@ int sphg_gnn_step(sphg_gnn* g, int use_net)
     if (!g) return SPHG_ERR_INVALID;
     sphg_ctx* c = g->c;
     RC(check_fused_switches(c));
-    // the forward pass overlaps the coarse step unless kernels are being
-    // timed one by one
-    const bool ovl = use_net && side_on(c);
-    if (use_net) {
-        RC(build_graph(g, g->cur));
-        if (ovl) {
-            HIPCHK(c, hipEventRecord(g->ev_f0, c->stream));
-            HIPCHK(c, hipStreamWaitEvent(g->fstream, g->ev_f0, 0));
-            c->lstream = g->fstream;
-            const int rc = forward(g, g->cur);
-            c->lstream = nullptr;
-            RC(rc);
-            HIPCHK(c, hipEventRecord(g->ev_f1, g->fstream));
-        } else {
-            RC(forward(g, g->cur));
-        }
-    }
+    // graph and features of the state the advance starts from, the coarse
+    // step (its rhs evaluates the rates at that state), the rate-change
+    // features, the network, the correction
+    const int gap = c->last_k;
+    if (use_net) RC(build_graph(g, g->cur));
     RC(coarse_step(c, g->K));
     if (use_net) {
-        if (ovl) HIPCHK(c, hipStreamWaitEvent(c->stream, g->ev_f1, 0));
+        RC(rate_features(g, g->cur, gap));
+        RC(forward(g, g->cur));
         for (int k = 0; k < 2; ++k) {
             PhaseBuf& b = c->ph[k];
+            // (the moved positions reach the displacement check of the next
+            // step's pack; without lists the step left its cell lists invalid)
             RC(launch(c, "gnn_correct", gnn::k_gnn_correct, nblk(b.n), TPB, b.d, c->K,
                       (const int*)g->node_of_id[k], (const float*)g->y, g->S, k));
-            b.cells_valid = false;
         }
     }
     return SPHG_OK;
 }
 
-// Non-zero when the current state fails the sanity test of k_sanity.
-static int state_broken(sphg_gnn* g, int* broken)
+// Sanity test of the state after a learned advance, without a host round
+// trip: k_sanity raises the device flag, k_restore_if then copies the last
+// good state (snapshot 0) back on the device, and the flag is copied to the
+// pinned slot `slot` behind them.  The host reads it one advance later.
+static int sanity_async(sphg_gnn* g, int slot)
 {
     sphg_ctx* c = g->c;
-    RC(fill_int(c, g->overflow, 1, 0));
+    RC(fill_int(c, g->broken, 1, 0));
     for (int k = 0; k < 2; ++k) {
         PhaseBuf& b = c->ph[k];
-        RC(launch(c, "sanity", gnn::k_sanity, nblk(b.n), TPB, b.d, c->g, g->overflow));
+        RC(launch(c, "sanity", gnn::k_sanity, nblk(b.n), TPB, b.d, c->g, g->broken));
     }
-    HIPCHK(c, hipMemcpyAsync(broken, g->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 2; ++k) {
+        PhaseBuf& b = c->ph[k];
+        StateCopy& sc = g->snap[0][k];
+        double** f[N_STATE_FIELDS];
+        state_fields(b.d, f);
+        PermArgs A{};
+        A.nd = N_STATE_FIELDS;
+        A.ni = 3;
+        for (int i = 0; i < N_STATE_FIELDS; ++i) { A.sd[i] = sc.f[i]; A.dd[i] = *f[i]; }
+        A.si[0] = sc.id; A.si[1] = sc.itype; A.si[2] = sc.fail;
+        A.di[0] = b.d.id; A.di[1] = b.d.itype; A.di[2] = b.d.fail;
+        RC(launch(c, "sanity", gnn::k_restore_if, nblk(b.n), TPB, (const int*)g->broken, A, b.n));
+    }
+    HIPCHK(c, hipMemcpyAsync(&g->h_broken[slot], g->broken, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(g->ev_ok[slot], c->stream));
     return SPHG_OK;
 }
 
@@ -2250,35 +2319,57 @@ int sphg_gnn_rollout(sphg_gnn* g, int nlearned, int judge_every, double tol, int
     float worst = 0.0f;
     double sq = 0.0, cnt = 0.0;
     const int J = judge_every > 0 ? judge_every : nlearned + 1;
-    int good_step = c->itimestep;   // itimestep of the last known-good state (snapshot slot 0)
+    int good_step = c->itimestep;   // itimestep of the last known-good state (snapshot 0)
     int since_good = 0;             // learned advances since that state
+    int pending = -1;               // slot of the sanity flag not yet read
     RC(snap_copy(g, 0, true));
+    // fallback: back to the last good state, plain steps over the span of
+    // the advances since (the one that broke the state and any issued after)
+    auto fallback = [&]() -> int {
+        RC(snap_copy(g, 0, false));
+        state_changed(c);
+        RC(launch(c, "nlist", k_keep_bits, 1, 1, c->d_nlflag, 0));   // the broken span's displacement
+        c->h_nlstep[0] = c->h_nlstep[1] = -1;
+        c->itimestep = good_step;
+        RC(plain_steps(c, since_good * g->K));
+        fb += since_good;            // advances replaced by plain steps
+        worst = std::max(worst, 1.0f);
+        good_step = c->itimestep;
+        since_good = 0;
+        pending = -1;
+        return snap_copy(g, 0, true);
+    };
+    // the sanity flag of the advance before the last (or of the last, at a
+    // judge and at the end)
+    auto resolve = [&](int slot) -> int {
+        HIPCHK(c, hipEventSynchronize(g->ev_ok[slot]));
+        if (g->h_broken[slot]) return fallback();
+        return SPHG_OK;
+    };
     for (int it = 0; it < nlearned; ++it) {
         const bool judge = judge_every > 0 && it % J == J - 1;
         if (!judge) {
             RC(sphg_gnn_step(g, use_net));
             ++since_good;
-            int broken = 0;
-            RC(state_broken(g, &broken));
-            if (broken) {
-                // fallback: back to the last good state, plain steps over the span
-                RC(snap_copy(g, 0, false));
-                c->itimestep = good_step;
-                RC(plain_steps(c, since_good * g->K));
-                ++fb;
-                worst = std::max(worst, 1.0f);
-                good_step = c->itimestep;
-                since_good = 0;
-                RC(snap_copy(g, 0, true));
-            }
+            const int slot = it & 1;
+            RC(sanity_async(g, slot));
+            const int prev = pending;
+            pending = slot;
+            if (prev >= 0) RC(resolve(prev));
             continue;
         }
-        // judge: one learned advance against K plain steps from the same state
+        if (pending >= 0) {
+            const int p = pending;
+            pending = -1;
+            RC(resolve(p));
+        }
+        // judge: one learned advance against K plain steps from the same
+        // state.  The run goes on from the plain result (it re-anchors there);
+        // the learned one is measured, and one over tol counts as a fallback.
         const int step0 = c->itimestep;
         RC(snap_copy(g, 0, true));
         RC(sphg_gnn_step(g, use_net));
         RC(snap_copy(g, 1, true));
-        const int step1 = c->itimestep;
         RC(snap_copy(g, 0, false));
         c->itimestep = step0;
         RC(plain_steps(c, g->K));
@@ -2299,22 +2390,29 @@ int sphg_gnn_rollout(sphg_gnn* g, int nlearned, int judge_every, double tol, int
         cnt += es[1];
         float e;
         std::memcpy(&e, &eb, sizeof(e));
-        worst = std::max(worst, e);
-        if (e > tol) {
-            ++fb;                    // keep the plain solution
-        } else {
-            RC(snap_copy(g, 1, false));
-            c->itimestep = step1;
-        }
+        if (!(e <= tol)) ++fb;
+        worst = std::max(worst, e == e ? e : 1.0f);
         good_step = c->itimestep;
         since_good = 0;
         RC(snap_copy(g, 0, true));
+    }
+    if (pending >= 0) {
+        const int p = pending;
+        pending = -1;
+        RC(resolve(p));
     }
     if (nfallback) *nfallback = fb;
     if (max_err) *max_err = worst;
     if (rms_err) *rms_err = cnt > 0.0 ? std::sqrt(sq / cnt) : 0.0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return check_flags(c);
+}
+
+int sphg_gnn_gains(const sphg_gnn* g, float* gains)
+{
+    if (!g || !gains) return SPHG_ERR_INVALID;
+    for (int o = 0; o < gnn::NO; ++o) gains[o] = g->S.gain[o];
+    return SPHG_OK;
 }
 
 int sphg_gnn_sample_size(const sphg_gnn* g, int sample)
@@ -2351,6 +2449,7 @@ int sphg_gnn_debug(sphg_gnn* g, int sample, float* nf, float* raw, int* nb, int*
         return SPHG_OK;
     };
     RC(dl(nf, G.nf, n * gnn::FN * sizeof(float)));
+    const bool scale_nf = nf != nullptr;
     RC(dl(raw, G.raw, n * gnn::RAW * sizeof(float)));
     RC(dl(nd, G.nd, n * sizeof(int)));
     if (nb) {   // the device table is slot-major; the caller gets [n][DEG]
@@ -2367,6 +2466,9 @@ int sphg_gnn_debug(sphg_gnn* g, int sample, float* nf, float* raw, int* nb, int*
     RC(dl(loss, g->loss, sizeof(float)));
     if (out_scale) for (int o = 0; o < gnn::NO; ++o) out_scale[o] = g->S.out[o];
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (scale_nf)   // the rate-change features as the network reads them (target units)
+        for (size_t i = 0; i < n; ++i)
+            for (int f = 0; f < 3; ++f) nf[i * gnn::FN + gnn::FR + (size_t)f] *= 1.0f / g->S.out[f];
     return SPHG_OK;
 }
 

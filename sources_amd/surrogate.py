@@ -14,7 +14,7 @@ import numpy as np
 
 from . import abi
 
-FN, RAW, DEG, NO = 16, 12, 48, 4
+FN, RAW, DEG, NO = 32, 12, 48, 4
 
 
 class Surrogate:
@@ -60,6 +60,12 @@ class Surrogate:
         loss = C.c_float()
         self._check(self.lib.sphg_gnn_train(self._g, iters, C.c_float(lr), C.byref(loss)), "sphg_gnn_train")
         return loss.value
+
+    def gains(self):
+        """Fitted gains of the correction channels (vx, vy, rho, spare)."""
+        g = np.zeros(4, np.float32)
+        self._check(self.lib.sphg_gnn_gains(self._g, self._fp(g)), "sphg_gnn_gains")
+        return g
 
     def step(self, use_net=True):
         self._check(self.lib.sphg_gnn_step(self._g, 1 if use_net else 0), "sphg_gnn_step")

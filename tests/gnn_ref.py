@@ -10,7 +10,7 @@ exact float64 network, which autograd and finite differences pin.
 """
 import numpy as np
 
-FE, HE, FN, D, NO, NOP, DEG, RAW = 16, 32, 16, 64, 4, 32, 48, 12
+FE, HE, FN, D, NO, NOP, DEG, RAW = 16, 32, 32, 64, 4, 32, 48, 12
 FI = FN + HE
 EDGE_NORM = 1.0 / 16.0
 O_W1 = 0

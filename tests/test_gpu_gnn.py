@@ -102,8 +102,10 @@ def test_broken_state_falls_back_to_plain_steps():
     w = g.weights()
     w[G.O_B5] = 1.0e30
     g.set_weights(w)
+    it0 = s.itimestep
     nfb, err, _ = g.rollout(3, judge_every=0, tol=0.01, use_net=True)
     assert nfb == 3 and err >= 1.0        # every advance broke and was replaced
+    assert s.itimestep == it0 + 3 * 2     # plain steps covered the three advances
     s.sync_all()
     for ph in (sc.water, sc.soil):
         n = ph.n
