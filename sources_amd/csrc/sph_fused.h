@@ -316,14 +316,15 @@ __device__ __forceinline__ void track_displacement(const NRef& R, int t, double 
 }
 
 // q != nullptr selects the FP32 record of the mixed-precision form.
-__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs, NRef NR)
+// Pack of water particle i into record slot t from its current values x,
+// v, rho, vof (the state's, or those the final update of the step before has
+// just computed): first_half (do_half), the equation of state, the record.
+__device__ __forceinline__ void pack_water_at(const DPhase& P, const KParams& K, double* rec, int t, int i,
+                                              int do_half, float* q, const Grid& g, double cs, const NRef& NR,
+                                              double x, double y, double rho, double vof, double vx, double vy)
 {
 #pragma clang fp contract(off)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= P.n) return;
-    const int i = P.order[t];
-    double rho = P.rho[i], vof = P.vof[i], vx = P.vx[i], vy = P.vy[i];
-    track_displacement(NR, t, P.x[i], P.y[i], vx, vy);
+    track_displacement(NR, t, x, y, vx, vy);
     const int it = P.itype[i];
     if (do_half) {
         const double h = K.dt / 2.0;
@@ -359,14 +360,14 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
     if (q) {
         float4* Q = reinterpret_cast<float4*>(q) + t;
         const size_t pn = (size_t)P.n;
-        write_q01(Q, pn, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.w_rho0, m);
+        write_q01(Q, pn, g, cs, x, y, P.hsml[i], it < 0, vx, vy, rho - K.w_rho0, m);
         Q[2 * pn] = make_float4((float)vof, (float)irho, (float)(m * irho), (float)p);
         Q[3 * pn] = make_float4((float)cn, 0.0f, 0.0f, 0.0f);
         return;
     }
     double2* R = reinterpret_cast<double2*>(rec) + t;
     const size_t pn = (size_t)P.n;
-    R[0 * pn] = make_double2(P.x[i], P.y[i]);
+    R[0 * pn] = make_double2(x, y);
     R[1 * pn] = make_double2(vx, vy);
     R[2 * pn] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
     R[3 * pn] = make_double2(vof, p);
@@ -374,15 +375,50 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
     R[6 * pn] = make_double2(P.hsml[i], (double)it);
 }
 
-__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs,
-                            NRef NR)
+__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs, NRef NR)
 {
-#pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
-    double rho = P.rho[i], vx = P.vx[i], vy = P.vy[i];
-    track_displacement(NR, t, P.x[i], P.y[i], vx, vy);
+    pack_water_at(P, K, rec, t, i, do_half, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vof[i], P.vx[i], P.vy[i]);
+}
+
+// The final update of step s fused with the pack of step s + 1 (first_half,
+// full step): one pass over the state instead of two, with the same values
+// in memory afterwards as k_final_water followed by k_pack_water(do_half =
+// 1), apart from stores the pack would overwrite.  The record order of step
+// s + 1 must be the current one (neighbour lists kept, no cell-list rebuild).
+__global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int with_vof, int renorm, NRef NR)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const int i = P.order[t];
+    const WaterEnd e = water_end(P, K, i, first, with_vof, renorm);
+    const int it = P.itype[i];
+    // the final update's stores that the pack does not overwrite
+    if (K.summation_density && renorm) P.rho[i] = e.rho;
+    if (with_vof && !(K.volume_fraction && K.two_phase)) P.vof[i] = e.vof;
+    if (it >= 0) {
+        P.x[i] = e.x;
+        P.y[i] = e.y;
+        if (it == 0) {
+            P.vx[i] = e.vx;
+            P.vy[i] = e.vy;
+        }
+    }
+    pack_water_at(P, K, rec, t, i, 1, nullptr, Grid{}, 0.0, NR, e.x, e.y, e.rho, e.vof, e.vx, e.vy);
+}
+
+// Pack of soil particle i (values as for water, plus stress and pressure):
+// first_half (do_half), ndp Drucker-Prager projections (the last one
+// single_step's failure criterion), the record.
+__device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, double* rec, int t, int i,
+                                             int do_half, int ndp, float* q, const Grid& g, double cs,
+                                             const NRef& NR, double x, double y, double rho, double vx, double vy,
+                                             double p, double sxx, double sxy, double syy)
+{
+#pragma clang fp contract(off)
+    track_displacement(NR, t, x, y, vx, vy);
     const int it = P.itype[i];
     if (do_half) {
         const double h = K.dt / 2.0;
@@ -400,7 +436,6 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
             P.vy[i] = vy;
         }
     }
-    double p = P.p[i], sxx = P.sxx[i], sxy = P.sxy[i], syy = P.syy[i];
     if (do_half && K.stress_integration == 2) {   // first_half, declarations_sph.f90:664-672
         const double h = K.dt / 2.0;
         P.sxx_min[i] = sxx; P.sxy_min[i] = sxy; P.syy_min[i] = syy; P.p_min[i] = p;
@@ -430,14 +465,14 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
     if (q) {
         float4* Q = reinterpret_cast<float4*>(q) + t;
         const size_t pn = (size_t)P.n;
-        write_q01(Q, pn, g, cs, P.x[i], P.y[i], P.hsml[i], it < 0, vx, vy, rho - K.s_rho0, m);
+        write_q01(Q, pn, g, cs, x, y, P.hsml[i], it < 0, vx, vy, rho - K.s_rho0, m);
         Q[2 * pn] = make_float4((float)P.vof[i], (float)irho, (float)(m * irho), (float)p);
         Q[3 * pn] = make_float4((float)P.c[i], (float)sxx, (float)sxy, (float)syy);
         return;
     }
     double2* R = reinterpret_cast<double2*>(rec) + t;
     const size_t pn = (size_t)P.n;
-    R[0 * pn] = make_double2(P.x[i], P.y[i]);
+    R[0 * pn] = make_double2(x, y);
     R[1 * pn] = make_double2(vx, vy);
     R[2 * pn] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
     R[3 * pn] = make_double2(P.vof[i], p);
@@ -447,7 +482,33 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
     if (K.yield_criterion == 1) R[7 * pn] = make_double2(rho_pre, 0.0);
 }
 
+__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs,
+                            NRef NR)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const int i = P.order[t];
+    pack_soil_at(P, K, rec, t, i, do_half, ndp, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vx[i], P.vy[i], P.p[i],
+                 P.sxx[i], P.sxy[i], P.syy[i]);
+}
 
+// The soil final update of step s fused with its pack for step s + 1
+// (first_half, two projections), as for water.  The pack reads vof from
+// the state: the final update's vof is stored first.
+__global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, NRef NR)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const int i = P.order[t];
+    const SoilEnd e = soil_end(P, K, i, first);
+    if (K.volume_fraction && K.two_phase) P.vof[i] = e.vof;
+    if (P.itype[i] > 0) {
+        P.x[i] = e.x;
+        P.y[i] = e.y;
+    }
+    pack_soil_at(P, K, rec, t, i, 1, 2, nullptr, Grid{}, 0.0, NR, e.x, e.y, e.rho, e.vx, e.vy, e.p, e.sxx, e.sxy,
+                 e.syy);
+}
 
 // ------------------------------------------------------------------------
 // Partner masks of stage A, reused by stage B: per particle t a 128-bit mask

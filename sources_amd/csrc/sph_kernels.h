@@ -479,33 +479,53 @@ __global__ void k_first_half_soil(DPhase P, KParams K, int do_half, int ndp)
 // (declarations_sph.f90:1414-1420) before the update, rho <- rho / vof * vof2
 // with the half-step vof; second_half then recomputes vof (and, unless
 // summation_density is set, rho), so only a summation-density run keeps it.
-__global__ void k_final_water(DPhase P, KParams K, int first, int with_vof, int renorm)
+// The final update of a water particle (second_half of the leapfrog) as
+// values, without writing: x, v, rho, vof after the step.
+struct WaterEnd {
+    double x, y, vx, vy, rho, vof;
+};
+
+__device__ __forceinline__ WaterEnd water_end(const DPhase& P, const KParams& K, int i, int first, int with_vof,
+                                              int renorm)
 {
 #pragma clang fp contract(off)
+    const double dt = K.dt, h = dt / 2.0;
+    WaterEnd e;
+    e.rho = P.rho[i];
+    if (renorm) e.rho = e.rho / P.vof[i] * P.vof2[i];
+    if (!K.summation_density) e.rho = first ? e.rho + h * P.drho[i] : P.rho_min[i] + dt * P.drho[i];
+    e.vof = P.vof[i];
+    if (with_vof) e.vof = first ? e.vof + h * P.dvof[i] : P.vof_min[i] + dt * P.dvof[i];
+    e.x = P.x[i];
+    e.y = P.y[i];
+    e.vx = P.vx[i];
+    e.vy = P.vy[i];
+    if (P.itype[i] >= 0) {
+        if (first) {
+            e.vx = e.vx + h * P.ax[i] + P.avx[i];
+            e.vy = e.vy + h * P.ay[i] + P.avy[i];
+        } else {
+            e.vx = P.vx_min[i] + dt * P.ax[i] + P.avx[i];
+            e.vy = P.vy_min[i] + dt * P.ay[i] + P.avy[i];
+        }
+        e.x = e.x + dt * e.vx;
+        e.y = e.y + dt * e.vy;
+    }
+    return e;
+}
+
+__global__ void k_final_water(DPhase P, KParams K, int first, int with_vof, int renorm)
+{
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
-    const double dt = K.dt, h = dt / 2.0;
-    if (renorm) P.rho[i] = P.rho[i] / P.vof[i] * P.vof2[i];
-    if (first) {
-        if (!K.summation_density) P.rho[i] = P.rho[i] + h * P.drho[i];
-        if (with_vof) P.vof[i] = P.vof[i] + h * P.dvof[i];
-    } else {
-        if (!K.summation_density) P.rho[i] = P.rho_min[i] + dt * P.drho[i];
-        if (with_vof) P.vof[i] = P.vof_min[i] + dt * P.dvof[i];
-    }
+    const WaterEnd e = water_end(P, K, i, first, with_vof, renorm);
+    if (renorm || !K.summation_density) P.rho[i] = e.rho;
+    if (with_vof) P.vof[i] = e.vof;
     if (P.itype[i] < 0) return;
-    double vx, vy;
-    if (first) {
-        vx = P.vx[i] + h * P.ax[i] + P.avx[i];
-        vy = P.vy[i] + h * P.ay[i] + P.avy[i];
-    } else {
-        vx = P.vx_min[i] + dt * P.ax[i] + P.avx[i];
-        vy = P.vy_min[i] + dt * P.ay[i] + P.avy[i];
-    }
-    P.vx[i] = vx;
-    P.vy[i] = vy;
-    P.x[i] = P.x[i] + dt * vx;
-    P.y[i] = P.y[i] + dt * vy;
+    P.vx[i] = e.vx;
+    P.vy[i] = e.vy;
+    P.x[i] = e.x;
+    P.y[i] = e.y;
 }
 
 // The renormalisation as single_step itself leaves it (declarations_sph.f90:
@@ -563,17 +583,21 @@ __global__ void k_verlet_water(DPhase P, double* lvx, double* lvy, double* lrho,
 // volume_fraction_soil (m_particles.f90:3970-3988, applied by single_step
 // :1411 to the pre-update density), then first_step / second_half
 // (stress_integration = 1) and the closing Drucker-Prager projection (:583/:601).
-__global__ void k_final_soil(DPhase P, KParams K, int first)
+// The final update of a soil particle (second_half, plastic_or_not and
+// return_mapping, the closing Drucker-Prager projection) as values.
+struct SoilEnd {
+    double x, y, vx, vy, rho, vof, sxx, sxy, syy, p;
+    int fail;
+};
+
+__device__ __forceinline__ SoilEnd soil_end(const DPhase& P, const KParams& K, int i, int first)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
     const double dt = K.dt, h = dt / 2.0;
-    if (K.volume_fraction && K.two_phase) P.vof[i] = P.rho[i] / K.s_rho0;
-    if (!K.summation_density) {
-        if (first) P.rho[i] = P.rho[i] + h * P.drho[i];
-        else P.rho[i] = P.rho_min[i] + dt * P.drho[i];
-    }
+    SoilEnd e;
+    e.vof = (K.volume_fraction && K.two_phase) ? P.rho[i] / K.s_rho0 : P.vof[i];
+    e.rho = P.rho[i];
+    if (!K.summation_density) e.rho = first ? e.rho + h * P.drho[i] : P.rho_min[i] + dt * P.drho[i];
     double sxx, sxy, syy, p;
     if (K.stress_integration == 2) {   // :700-705, :744-749
         if (first) {
@@ -593,19 +617,20 @@ __global__ void k_final_soil(DPhase P, KParams K, int first)
         syy = P.syy[i] + dt * P.dsyy[i];
         p = P.p[i] + dt * P.dp[i];
     }
+    e.x = P.x[i];
+    e.y = P.y[i];
+    e.vx = P.vx[i];
+    e.vy = P.vy[i];
     if (P.itype[i] > 0) {
-        double vx, vy;
         if (first) {
-            vx = P.vx[i] + h * P.ax[i] + P.avx[i];
-            vy = P.vy[i] + h * P.ay[i] + P.avy[i];
+            e.vx = e.vx + h * P.ax[i] + P.avx[i];
+            e.vy = e.vy + h * P.ay[i] + P.avy[i];
         } else {
-            vx = P.vx_min[i] + dt * P.ax[i] + P.avx[i];
-            vy = P.vy_min[i] + dt * P.ay[i] + P.avy[i];
+            e.vx = P.vx_min[i] + dt * P.ax[i] + P.avx[i];
+            e.vy = P.vy_min[i] + dt * P.ay[i] + P.avy[i];
         }
-        P.vx[i] = vx;
-        P.vy[i] = vy;
-        P.x[i] = P.x[i] + dt * vx;
-        P.y[i] = P.y[i] + dt * vy;
+        e.x = e.x + dt * e.vx;
+        e.y = e.y + dt * e.vy;
     }
     if (K.plasticity == 2 && K.two_phase) {
         // plastic_or_not + return_mapping (time_integration :576-579, :594-597,
@@ -627,8 +652,29 @@ __global__ void k_final_soil(DPhase P, KParams K, int first)
             }
         }
     }
-    P.fail[i] = dp_project(K.s_alpha1, p, sxx, sxy, syy);
-    P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy; P.p[i] = p;
+    e.fail = dp_project(K.s_alpha1, p, sxx, sxy, syy);
+    e.sxx = sxx;
+    e.sxy = sxy;
+    e.syy = syy;
+    e.p = p;
+    return e;
+}
+
+__global__ void k_final_soil(DPhase P, KParams K, int first)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const SoilEnd e = soil_end(P, K, i, first);
+    if (K.volume_fraction && K.two_phase) P.vof[i] = e.vof;
+    if (!K.summation_density) P.rho[i] = e.rho;
+    if (P.itype[i] > 0) {
+        P.vx[i] = e.vx;
+        P.vy[i] = e.vy;
+        P.x[i] = e.x;
+        P.y[i] = e.y;
+    }
+    P.fail[i] = e.fail;
+    P.sxx[i] = e.sxx; P.sxy[i] = e.sxy; P.syy[i] = e.syy; P.p[i] = e.p;
 }
 
 // ------------------------------------------------------------------------

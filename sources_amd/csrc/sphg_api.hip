@@ -149,6 +149,9 @@ struct sphg_ctx {
     int64_t nl_off_spells = 0;
     bool capturing = false;     // steps are being captured into a graph (no host reads)
     int last_k = 1;             // the last step was of size last_k dt (a learned coarse step: K)
+    // the final kernels of the step before also packed this step (first_half
+    // applied to the state, records written): the step starts at its stages
+    bool packed = false;
     int* d_nlflag = nullptr;    // bit 0: rebuild due; bit 1: a particle moved past skin / 2
     int* h_nlflag = nullptr;    // pinned: two slots for the lagged check of sphg_time_integration
     int h_nlstep[2] = {-1, -1}; // nl_seq of the step whose flag each slot holds (-1: none)
@@ -308,6 +311,7 @@ void state_changed(sphg_ctx* c)
 {
     for (auto& pb : c->ph) pb.cells_valid = false;
     c->nl_valid = false;
+    c->packed = false;
 }
 
 // Permute the persistent state of phase k into the current cell order, so
@@ -1088,6 +1092,36 @@ int build_nlists(sphg_ctx* c)
     return SPHG_OK;
 }
 
+// The displacement reference of phase k for a pack (rebuild: record the
+// positions).  A rebuild comes past a quarter of the skin, or when four more
+// steps at the current speed would pass 0.45 skin; past half of it the run
+// fails.
+NRef nref(const sphg_ctx* c, int k, bool rebuild)
+{
+    const double hard = 0.5 * c->skin;
+    return NRef{c->ph[k].nl_x, c->ph[k].nl_y, c->d_nlflag, 0.25 * c->skin, 0.45 * c->skin, hard * hard,
+                NL_LOOK * c->K.dt, rebuild ? 1 : 0};
+}
+
+// The stage kernels of the fused two-phase step (records packed).
+int stages_of(sphg_ctx* c)
+{
+    if (c->P.precision == SPHG_MIXED) {
+        switch (c->P.skf) {
+        case 1: return stages_mixed<1>(c);
+        case 2: return stages_mixed<2>(c);
+        case 3: return stages_mixed<3>(c);
+        default: return stages_mixed<4>(c);
+        }
+    }
+    switch (c->P.skf) {
+    case 1: return stages_fused<1>(c);
+    case 2: return stages_fused<2>(c);
+    case 3: return stages_fused<3>(c);
+    default: return stages_fused<4>(c);
+    }
+}
+
 int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
 {
     PhaseBuf& W = c->ph[0];
@@ -1103,20 +1137,13 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
     // (cell lists invalidated since -- a restore, a correction, an upload --
     // mean a rebuild as well: the lists index their order)
     const bool rebuild = nl && (!c->nl_valid || !W.cells_valid || !S.cells_valid || !W.cells_nl || !S.cells_nl);
-    NRef nrW{}, nrS{};
-    if (nl) {
-        // rebuild past a quarter of the skin, or when four more steps at the
-        // current speed would pass 0.45 skin; fail past half of it
-        const double hard = 0.5 * c->skin, look = NL_LOOK * c->K.dt;
-        const NRef base{nullptr, nullptr, c->d_nlflag, 0.25 * c->skin, 0.45 * c->skin, hard * hard, look,
-                        rebuild ? 1 : 0};
-        nrW = base;
-        nrW.x = W.nl_x;
-        nrW.y = W.nl_y;
-        nrS = base;
-        nrS.x = S.nl_x;
-        nrS.y = S.nl_y;
+    if (c->packed) {
+        // the final kernels of the step before packed this one
+        c->packed = false;
+        if (!nl || rebuild || mixed || !water_half || !soil_half || soil_ndp != 2) return SPHG_ERR_INVALID;
+        return stages_of(c);
     }
+    const NRef nrW = nl ? nref(c, 0, rebuild) : NRef{}, nrS = nl ? nref(c, 1, rebuild) : NRef{};
     if (rebuild) {
         W.cells_valid = false;
         S.cells_valid = false;
@@ -1145,20 +1172,7 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
         else RC(soil_chain());
     }
     if (rebuild) RC(build_nlists(c));
-    if (mixed) {
-        switch (c->P.skf) {
-        case 1: return stages_mixed<1>(c);
-        case 2: return stages_mixed<2>(c);
-        case 3: return stages_mixed<3>(c);
-        default: return stages_mixed<4>(c);
-        }
-    }
-    switch (c->P.skf) {
-    case 1: return stages_fused<1>(c);
-    case 2: return stages_fused<2>(c);
-    case 3: return stages_fused<3>(c);
-    default: return stages_fused<4>(c);
-    }
+    return stages_of(c);
 }
 
 // single_step_for_water (declarations_sph.f90:1439-1526).
@@ -1255,7 +1269,9 @@ int nl_poll(sphg_ctx* c)
     return SPHG_OK;
 }
 
-int one_step(sphg_ctx* c)
+// ahead_ok: another step of the same run follows at once (nothing changes
+// the state in between), so its pack may join this step's final update.
+int one_step(sphg_ctx* c, bool ahead_ok = false)
 {
     c->last_k = 1;
     const int step = ++c->itimestep;
@@ -1311,18 +1327,30 @@ int one_step(sphg_ctx* c)
         else RC(rhs_two_phase_fused(c, 0, 0, 1));
     }
     const int vf = c->P.volume_fraction ? 1 : 0;
-    RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, vf,
-              renorm_step(c, step) ? 1 : 0));
-    RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, step == 1 ? 1 : 0));
-    if (nl) {
-        // the list mode keeps the rebuild's cell lists and record order
-        if (!c->capturing) RC(nl_poll(c));
+    const int first = step == 1 ? 1 : 0, renorm = renorm_step(c, step) ? 1 : 0;
+    // the list mode's displacement check (of the step before: it may ask
+    // for a rebuild at the next step, or report a particle past half the skin)
+    const int prc = nl && !c->capturing ? nl_poll(c) : SPHG_OK;
+    // when the next step keeps the lists (no rebuild, no spell without
+    // them), its pack joins this step's final update
+    const bool ahead = nl && ahead_ok && prc == SPHG_OK && c->nl_valid && !c->nl_due &&
+                       c->since_reorder + 1 < c->nl_every && c->itimestep + 1 >= c->nl_off_until;
+    if (ahead) {
+        RC(launch(c, "final_water", k_step_water, nblk(W.n), TPB, W.d, c->K, W.rec, first, vf, renorm,
+                  nref(c, 0, false)));
+        RC(launch(c, "final_soil", k_step_soil, nblk(S.n), TPB, S.d, c->K, S.rec, first, nref(c, 1, false)));
+        c->packed = true;
     } else {
-        // the positions moved: the next step builds its cell lists again
+        RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, first, vf, renorm));
+        RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, first));
+    }
+    if (!nl) {
+        // the positions moved: the next step builds its cell lists again (the
+        // list mode keeps the rebuild's cell lists and record order)
         W.cells_valid = false;
         S.cells_valid = false;
     }
-    return SPHG_OK;
+    return prc;
 }
 
 // time_integration_for_water_by_verlet (declarations_sph.f90:889-974): the
@@ -1706,7 +1734,7 @@ int sphg_time_integration(sphg_ctx* c, int nsteps, int sync)
     RC(check_fused_switches(c));
     if (c->P.single_phase && c->P.integrate_scheme == 2) RC(verlet_steps(c, nsteps));
     else
-        for (int it = 0; it < nsteps; ++it) RC(one_step(c));
+        for (int it = 0; it < nsteps; ++it) RC(one_step(c, it + 1 < nsteps));
     if (sync) {
         for (int k = 0; k < 2; ++k)
             if (c->ph[k].bound) RC(download(c, k, SPHG_F_ALL));

@@ -102,3 +102,41 @@ def test_fast_motion_turns_the_lists_off(monkeypatch):
     assert spells >= 1
     for ph in ("water", "soil"):
         assert_fields(getattr(sc, ph), getattr(ref, ph), 1e-10, label=ph)
+
+
+def _stress2():
+    sc = _column()
+    sc.params.stress_integration = 2
+    return sc
+
+
+def _renorm():
+    sc = _column()
+    sc.params.summation_density = 1
+    sc.params.volume_fraction_renorm = 1
+    return sc
+
+
+@pytest.mark.parametrize("make,nsteps", [(_column, 20), (_slope, 20), (_stress2, 20), (_renorm, 45)],
+                         ids=["column", "slope", "stress_integration_2", "summation_renorm"])
+def test_fused_final_and_pack_are_bit_identical(make, nsteps):
+    # Inside one time_integration call the final update of each step also
+    # packs the next step (one pass); a call's last step does not.  N calls
+    # of one step take the unfused path throughout: the states agree bit
+    # for bit.
+    print(f"case={make.__name__} steps={nsteps}", flush=True)
+    a, b = make(), make()
+    sa = Solver(a.params, a.water, a.soil)
+    sb = Solver(b.params, b.water, b.soil)
+    sa.time_integration(nsteps, sync=True)
+    for _ in range(nsteps):
+        sb.time_integration(1, sync=False)
+    sb.time_integration(0, sync=True)
+    for ph in ("water", "soil"):
+        pa, pb = getattr(a, ph), getattr(b, ph)
+        n = pa.n
+        for f in ("x", "vx", "rho", "p", "c", "vof", "sxx", "sxy", "syy", "drho", "dvx", "fail"):
+            if hasattr(pa, f):
+                assert np.array_equal(getattr(pa, f)[:n], getattr(pb, f)[:n]), (ph, f)
+    sa.close()
+    sb.close()
