@@ -105,6 +105,7 @@ struct sphg_ctx {
     // stages (fork and join events on the main stream)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_wx = nullptr, ev_sx = nullptr;   // cross kernels done (water on main, soil on side)
     // SPHG_SIDE_STREAM=0 keeps every kernel on the main stream; so does the
     // per-kernel profile mode, whose durations are then not shared
     bool use_side = true;
@@ -145,7 +146,10 @@ struct sphg_ctx {
     bool use_nl = true;         // SPHG_NLIST=0: scan the cell rows every step instead
     double nl_skin = NL_SKIN;   // skin / largest support radius (SPHG_NL_SKIN)
     int nl_every = NL_EVERY;    // steps between periodic rebuilds (SPHG_NL_EVERY)
-    int nl_off_until = 0;       // lists off (fast-moving particles) before this step
+    // steps [nl_off_from, nl_off_until) run without lists (fast-moving
+    // particles); a spell starts at the step after the one that decides it,
+    // so that a step's kernels all see one mode
+    int nl_off_from = 0, nl_off_until = 0;
     int64_t nl_off_spells = 0;
     bool capturing = false;     // steps are being captured into a graph (no host reads)
     int last_k = 1;             // the last step was of size last_k dt (a learned coarse step: K)
@@ -923,7 +927,12 @@ bool nl_config(const sphg_ctx* c)
     return c->use_nl && !c->legacy && !c->P.single_phase && c->P.precision != SPHG_MIXED;
 }
 
-bool nl_mode(const sphg_ctx* c) { return nl_config(c) && c->itimestep >= c->nl_off_until; }
+bool nl_at(const sphg_ctx* c, int step)
+{
+    return nl_config(c) && !(step >= c->nl_off_from && step < c->nl_off_until);
+}
+
+bool nl_mode(const sphg_ctx* c) { return nl_at(c, c->itimestep); }
 
 // single_step (declarations_sph.f90:1131-1435) after the soil Drucker-Prager
 // projection of :1244 has been applied (legacy unfused kernels).
@@ -942,13 +951,31 @@ int rhs_two_phase_legacy(sphg_ctx* c)
     return SPHG_OK;
 }
 
+// The displacement reference of phase k for a pack (rebuild: record the
+// positions).  A rebuild comes past a quarter of the skin, or when four more
+// steps at the current speed would pass 0.45 skin; past half of it the run
+// fails.
+NRef nref(const sphg_ctx* c, int k, bool rebuild)
+{
+    const double hard = 0.5 * c->skin;
+    return NRef{c->ph[k].nl_x, c->ph[k].nl_y, c->d_nlflag, 0.25 * c->skin, 0.45 * c->skin, hard * hard,
+                NL_LOOK * c->K.dt, rebuild ? 1 : 0};
+}
+
+// The final kernels a step's stages launch (FP64 list mode): second_half of
+// the leapfrog, fused with the next step's pack when ahead.
+struct FinalArgs {
+    int first, vf, renorm;
+    bool ahead;
+};
+
 // The fused step: pack (first_half + soil Drucker-Prager projections +
 // records), cross terms, partner masks, stage A, stage B.  The soil kernels
 // read only the packed records of both phases (the water pressure comes from
 // the pack): they run on the side stream while the water kernels run on the
 // main stream.
 template <int SKF>
-int stages_fused(sphg_ctx* c)
+int stages_fused(sphg_ctx* c, const FinalArgs* fin)
 {
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
@@ -969,6 +996,9 @@ int stages_fused(sphg_ctx* c)
     const bool ux = c->F.uni_x != 0;
     SideFork fork(c);
     RC(fork.rc);
+    // with the finals on both streams, each phase's final kernel waits only
+    // for the other phase's cross kernel (which reads its records)
+    const bool split = fin && fork.open;
     {
         SideScope side(c);
         if (cross) {
@@ -977,6 +1007,7 @@ int stages_fused(sphg_ctx* c)
             else RC(launch(c, "soil_x", k_soil_x4<SKF, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, rS, rW,
                            nS, gS, c->K, c->F));
         }
+        if (split) HIPCHK(c, hipEventRecord(c->ev_sx, c->side));
         if (us) RC(launch(c, "soil_ab", k_soil_ab4<SKF, true, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rS,
                           nS, gS, c->K, c->F, cross, mask_b(c, 1)));
         else RC(launch(c, "soil_ab", k_soil_ab4<SKF, false, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rS,
@@ -988,6 +1019,7 @@ int stages_fused(sphg_ctx* c)
         else RC(launch(c, "water_x", k_water_x4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, rW, rS,
                        nW, gW, c->K, c->F));
     }
+    if (split) HIPCHK(c, hipEventRecord(c->ev_wx, c->stream));
     if (uw) RC(launch(c, "water_a", k_water_a4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
                       nW, W.masks, gW, c->K, c->F, cross));
     else RC(launch(c, "water_a", k_water_a4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
@@ -998,6 +1030,34 @@ int stages_fused(sphg_ctx* c)
                       W.masks, gW, c->K, c->F, mask_b(c, 0)));
     else RC(launch(c, "water_b", k_water_b4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, nW,
                    W.masks, gW, c->K, c->F, mask_b(c, 0)));
+    if (fin) {
+        auto final_water = [&]() -> int {
+            if (fin->ahead)
+                return launch(c, "final_water", k_step_water, nblk(W.n), TPB, W.d, c->K, W.rec, fin->first, fin->vf,
+                              fin->renorm, nref(c, 0, false));
+            return launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, fin->first, fin->vf,
+                          fin->renorm);
+        };
+        auto final_soil = [&]() -> int {
+            if (fin->ahead)
+                return launch(c, "final_soil", k_step_soil, nblk(S.n), TPB, S.d, c->K, S.rec, fin->first,
+                              nref(c, 1, false));
+            return launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, fin->first);
+        };
+        if (split) {
+            {
+                SideScope side(c);
+                HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_wx, 0));
+                RC(final_soil());
+            }
+            HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_sx, 0));
+            RC(final_water());
+        } else {
+            RC(fork.join());
+            RC(final_water());
+            RC(final_soil());
+        }
+    }
     return fork.join();
 }
 
@@ -1092,21 +1152,12 @@ int build_nlists(sphg_ctx* c)
     return SPHG_OK;
 }
 
-// The displacement reference of phase k for a pack (rebuild: record the
-// positions).  A rebuild comes past a quarter of the skin, or when four more
-// steps at the current speed would pass 0.45 skin; past half of it the run
-// fails.
-NRef nref(const sphg_ctx* c, int k, bool rebuild)
-{
-    const double hard = 0.5 * c->skin;
-    return NRef{c->ph[k].nl_x, c->ph[k].nl_y, c->d_nlflag, 0.25 * c->skin, 0.45 * c->skin, hard * hard,
-                NL_LOOK * c->K.dt, rebuild ? 1 : 0};
-}
-
-// The stage kernels of the fused two-phase step (records packed).
-int stages_of(sphg_ctx* c)
+// The stage kernels of the fused two-phase step (records packed), and with
+// fin the step's final kernels (FP64 only).
+int stages_of(sphg_ctx* c, const FinalArgs* fin = nullptr)
 {
     if (c->P.precision == SPHG_MIXED) {
+        if (fin) return SPHG_ERR_INVALID;
         switch (c->P.skf) {
         case 1: return stages_mixed<1>(c);
         case 2: return stages_mixed<2>(c);
@@ -1115,14 +1166,16 @@ int stages_of(sphg_ctx* c)
         }
     }
     switch (c->P.skf) {
-    case 1: return stages_fused<1>(c);
-    case 2: return stages_fused<2>(c);
-    case 3: return stages_fused<3>(c);
-    default: return stages_fused<4>(c);
+    case 1: return stages_fused<1>(c, fin);
+    case 2: return stages_fused<2>(c, fin);
+    case 3: return stages_fused<3>(c, fin);
+    default: return stages_fused<4>(c, fin);
     }
 }
 
-int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
+// Everything of the fused two-phase right-hand side before its stages: cell
+// lists, reorder, packs and (list mode) the neighbour lists.
+int pack_two_phase(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
 {
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
@@ -1141,7 +1194,7 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
         // the final kernels of the step before packed this one
         c->packed = false;
         if (!nl || rebuild || mixed || !water_half || !soil_half || soil_ndp != 2) return SPHG_ERR_INVALID;
-        return stages_of(c);
+        return SPHG_OK;
     }
     const NRef nrW = nl ? nref(c, 0, rebuild) : NRef{}, nrS = nl ? nref(c, 1, rebuild) : NRef{};
     if (rebuild) {
@@ -1172,6 +1225,12 @@ int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp
         else RC(soil_chain());
     }
     if (rebuild) RC(build_nlists(c));
+    return SPHG_OK;
+}
+
+int rhs_two_phase_fused(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
+{
+    RC(pack_two_phase(c, water_half, soil_half, soil_ndp));
     return stages_of(c);
 }
 
@@ -1240,11 +1299,12 @@ bool renorm_active(const sphg_params& P)
 
 bool renorm_step(const sphg_ctx* c, int step) { return renorm_active(c->P) && step % 40 == 0; }
 
-// Lagged displacement check of the list mode, after each step
-// issued outside a stream capture: the flag word is copied to pinned memory behind the
-// step, and the copy of the step before is read (it has landed by the time
-// the next step is queued, so the queue keeps a step of work).  A particle
-// past the rebuild trigger schedules a rebuild for the next step; one past
+// Lagged displacement check of the list mode, in each step issued outside
+// a stream capture once its packs are queued: the flag word is copied to
+// pinned memory behind them, and the copy of the step before is read (it
+// has landed by the time the step's packs are queued, so the queue keeps
+// the step's stages and the rest of the step before).  A particle past the
+// rebuild trigger schedules a rebuild for the step after this one; one past
 // half the skin fails the run (its step may have missed pairs).
 int nl_poll(sphg_ctx* c)
 {
@@ -1262,6 +1322,7 @@ int nl_poll(sphg_ctx* c)
         // rebuilds this close together cost more than the row scans: lists
         // off for a while (never in a batch, whose captured period has them)
         if (s + 1 - c->nl_rebuilt_at < NL_MIN_RUN && !c->stable_ptrs) {
+            c->nl_off_from = c->itimestep + 1;
             c->nl_off_until = c->itimestep + 1 + NL_OFF;
             ++c->nl_off_spells;
         }
@@ -1313,8 +1374,19 @@ int one_step(sphg_ctx* c, bool ahead_ok = false)
         // the lists are rebuilt (with the reorder) every nl_every steps and
         // when the displacement check asked for it
         if (++c->since_reorder >= c->nl_every || c->nl_due) c->nl_valid = false;
-        if (step != 1) RC(rhs_two_phase_fused(c, 1, 1, 2));
-        else RC(rhs_two_phase_fused(c, 0, 0, 1));
+        if (step != 1) RC(pack_two_phase(c, 1, 1, 2));
+        else RC(pack_two_phase(c, 0, 0, 1));
+        // the displacement check of the step before (it may ask for a
+        // rebuild at the next step, or report a particle past half the skin)
+        const int prc = c->capturing ? SPHG_OK : nl_poll(c);
+        // when the next step keeps the lists (no rebuild, no spell without
+        // them), its pack joins this step's final update
+        const bool ahead = ahead_ok && prc == SPHG_OK && c->nl_valid && !c->nl_due &&
+                           c->since_reorder + 1 < c->nl_every && nl_at(c, c->itimestep + 1);
+        const FinalArgs fa{step == 1 ? 1 : 0, c->P.volume_fraction ? 1 : 0, renorm_step(c, step) ? 1 : 0, ahead};
+        RC(stages_of(c, &fa));
+        c->packed = ahead;
+        return prc;
     } else {
         if (++c->since_reorder >= c->reorder_every) {
             RC(build_cells(c, 0));
@@ -1327,30 +1399,13 @@ int one_step(sphg_ctx* c, bool ahead_ok = false)
         else RC(rhs_two_phase_fused(c, 0, 0, 1));
     }
     const int vf = c->P.volume_fraction ? 1 : 0;
-    const int first = step == 1 ? 1 : 0, renorm = renorm_step(c, step) ? 1 : 0;
-    // the list mode's displacement check (of the step before: it may ask
-    // for a rebuild at the next step, or report a particle past half the skin)
-    const int prc = nl && !c->capturing ? nl_poll(c) : SPHG_OK;
-    // when the next step keeps the lists (no rebuild, no spell without
-    // them), its pack joins this step's final update
-    const bool ahead = nl && ahead_ok && prc == SPHG_OK && c->nl_valid && !c->nl_due &&
-                       c->since_reorder + 1 < c->nl_every && c->itimestep + 1 >= c->nl_off_until;
-    if (ahead) {
-        RC(launch(c, "final_water", k_step_water, nblk(W.n), TPB, W.d, c->K, W.rec, first, vf, renorm,
-                  nref(c, 0, false)));
-        RC(launch(c, "final_soil", k_step_soil, nblk(S.n), TPB, S.d, c->K, S.rec, first, nref(c, 1, false)));
-        c->packed = true;
-    } else {
-        RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, first, vf, renorm));
-        RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, first));
-    }
-    if (!nl) {
-        // the positions moved: the next step builds its cell lists again (the
-        // list mode keeps the rebuild's cell lists and record order)
-        W.cells_valid = false;
-        S.cells_valid = false;
-    }
-    return prc;
+    RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, step == 1 ? 1 : 0, vf,
+              renorm_step(c, step) ? 1 : 0));
+    RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, step == 1 ? 1 : 0));
+    // the positions moved: the next step builds its cell lists again
+    W.cells_valid = false;
+    S.cells_valid = false;
+    return SPHG_OK;
 }
 
 // time_integration_for_water_by_verlet (declarations_sph.f90:889-974): the
@@ -1450,6 +1505,8 @@ int sphg_create(const sphg_params* params, sphg_ctx** out)
         hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_wx, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_sx, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_nl[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_nl[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&c->h_nlflag, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
@@ -1484,6 +1541,8 @@ void sphg_destroy(sphg_ctx* c)
     }
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->ev_wx) hipEventDestroy(c->ev_wx);
+    if (c->ev_sx) hipEventDestroy(c->ev_sx);
     for (hipEvent_t e : c->ev_nl)
         if (e) hipEventDestroy(e);
     if (c->h_nlflag) hipHostFree(c->h_nlflag);
