@@ -116,3 +116,78 @@ def test_broken_state_falls_back_to_plain_steps():
     assert nfb == 0
     g.close()
     s.close()
+
+
+def _check_graph(d, nodes, R_of_phase):
+    """Every same-phase pair within R (cubic spline: R = 2h) of a sampled node
+    is among its graph neighbours, and no neighbour lies beyond R; the band
+    |r - R| < 1e-4 R is left open (the debug positions are FP32)."""
+    from scipy.spatial import cKDTree
+    pos = d["raw"][:, 0:2].astype(np.float64)
+    phase = d["nf"][:, 1]
+    checked = 0
+    for ph in (0.0, 1.0):
+        idx = np.where(phase == ph)[0]
+        if idx.size == 0:
+            continue
+        R = R_of_phase(ph)
+        tree = cKDTree(pos[idx])
+        for i in nodes[phase[nodes] == ph]:
+            k = int(d["nd"][i])
+            assert k < G.DEG, (i, k)
+            got = set(int(j) for j in d["nb"][i, :k])
+            inner = set(int(idx[j]) for j in tree.query_ball_point(pos[i], R * (1.0 - 1e-4))) - {int(i)}
+            outer = set(int(idx[j]) for j in tree.query_ball_point(pos[i], R * (1.0 + 1e-4))) - {int(i)}
+            assert inner <= got <= outer, (int(i), sorted(inner - got), sorted(got - outer))
+            checked += 1
+    return checked
+
+
+def test_graph_from_neighbour_lists_is_the_radius_graph():
+    # after warm-up steps the FP64 step keeps neighbour lists and the graph
+    # is built from them: compare with a brute-force radius search
+    sc, s, g = setup(nwarm=10, nsamples=1)
+    d = g.debug(0)
+    h = {0.0: float(np.max(sc.water.hsml[:sc.water.n])), 1.0: float(np.max(sc.soil.hsml[:sc.soil.n]))}
+    n = len(d["nd"])
+    checked = _check_graph(d, np.arange(n), lambda ph: 2.0 * h[ph])
+    print("nodes checked", checked)
+    assert checked == n
+    g.close()
+    s.close()
+
+
+def test_c4_slope_250k_graph_forward_and_judged_rollout():
+    """Config 4's scale: the 250k slope.  The graph of a node sample against a
+    radius search, the network output of the sample against the CPU
+    reference (BF16 operands emulated), and a judged rollout (K = 4, a plain
+    FP64 check every 5 advances) inside the 1% density gate."""
+    sc = scenarios.slope(dx=0.001, scale=0.5, max_interaction=0)
+    s = Solver(sc.params, sc.water, sc.soil)
+    s.time_integration(5, sync=False)
+    g = Surrogate(s, ksteps=4, max_samples=4, seed=1)
+    g.record(2)
+    g.train(100, lr=2e-3)
+    d = g.debug(0)
+    n = len(d["nd"])
+    print("nodes", n, "edges", int(d["nd"].sum()), flush=True)
+    rng = np.random.default_rng(3)
+    real = np.where(d["tw"] > 0)[0]
+    sample = np.sort(rng.choice(real, 400, replace=False))
+    h = {0.0: float(np.max(sc.water.hsml[:sc.water.n])), 1.0: float(np.max(sc.soil.hsml[:sc.soil.n]))}
+    assert _check_graph(d, sample, lambda ph: 2.0 * h[ph]) == sample.size
+    ii = np.repeat(sample, d["nd"][sample])
+    jj = np.concatenate([d["nb"][i, :d["nd"][i]] for i in sample])
+    e = G.edge_features(d["raw"], ii, jj)
+    y, _ = G.forward(g.weights().astype(np.float64), d["nf"].astype(np.float64), e, ii, n, rb=True)
+    ey = normwise(d["y"][sample], y[sample, :G.NO])
+    print(f"forward on the sample: {ey:.2e}", flush=True)
+    assert ey <= 1e-2
+    nfb, jmax, jrms = g.rollout(10, judge_every=5, tol=0.01, use_net=True)
+    print(f"rollout: fallbacks {nfb}, judged rho error max {jmax:.2e} rms {jrms:.2e}", flush=True)
+    assert nfb == 0 and jmax < 0.01
+    s.sync_all()
+    for ph in (sc.water, sc.soil):
+        assert np.isfinite(ph.rho[:ph.n]).all() and np.isfinite(ph.x[:ph.n]).all()
+    g.close()
+    s.close()
