@@ -297,10 +297,13 @@ def sweep_section(args, hip):
     from sources_amd import abi
     cases = scenarios.sweep(args.sweep_cases, seed=11)
     for c in cases:
-        c.params.precision = abi.SPHG_MIXED
+        c.params.precision = abi.SPHG_MIXED if args.sweep_precision == "mixed" else abi.SPHG_FP64
     ntot = sum(c.water.n + c.soil.n for c in cases)
     ss = [Solver(c.params, c.water, c.soil) for c in cases]
-    steps = 32
+    # four captured periods (a period: the neighbour-list rebuild period of
+    # FP64 contexts, 64 steps; the reorder period of mixed ones, 16)
+    period = 64 if args.sweep_precision == "fp64" else 16
+    steps = 4 * period
     for s in ss:
         s.time_integration(2, sync=False)
     hip.sync()
@@ -311,9 +314,11 @@ def sweep_section(args, hip):
     ms_eager = (time.perf_counter() - t0) * 1e3 / steps
     b = Batch(ss)
     ns, _ = b.tune(trial_periods=1)
-    b.run(16)
+    b.run(period)
+    hip.sync()
     t0 = time.perf_counter()
     b.run(steps)
+    hip.sync()
     ms_graph = (time.perf_counter() - t0) * 1e3 / steps
     print(f"# sweep: {len(cases)} cases, {ntot} particles: eager {ms_eager:.2f} ms/step, "
           f"batched {ms_graph:.2f} ms/step (tuned: {ns or 'eager'} streams)", file=sys.stderr, flush=True)
@@ -327,7 +332,7 @@ def sweep_section(args, hip):
         "value_batched": ntot / (ms_graph * 1e-3), "unit": "particle-steps/s",
         "speedup_batched_vs_eager": ms_eager / ms_graph,
         "workload": "submerged column collapses, N ~ logU[5k,120k], aspect ~ U[0.5,2], "
-                    "phi ~ U[25,40] deg, k_s ~ logU[1e-4,1e-2], seed 11, mixed precision",
+                    f"phi ~ U[25,40] deg, k_s ~ logU[1e-4,1e-2], seed 11, {args.sweep_precision}",
     }}
 
 
@@ -346,6 +351,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the mixed and learned-surrogate section")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched scenario sweep section")
     ap.add_argument("--sweep-cases", type=int, default=64)
+    ap.add_argument("--sweep-precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--mixed-error-steps", type=int, default=1000,
                     help="mixed vs FP64 error is reported after 1 step and after this many (0: skip)")
     ap.add_argument("--no-pcie", action="store_true")

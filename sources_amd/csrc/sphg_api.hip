@@ -2665,7 +2665,9 @@ int batch_capture(sphg_batch* b)
         c->stream = b->streams[r % (size_t)b->nstreams];
         c->use_side = false;
         c->capturing = true;
-        for (int s = 0; s < b->period && rc == SPHG_OK; ++s) rc = one_step(c);
+        // (the last step of a period is followed by the next period's
+        // rebuild, so it does not pack ahead: every replay starts unpacked)
+        for (int s = 0; s < b->period && rc == SPHG_OK; ++s) rc = one_step(c, true);
         c->capturing = false;
         c->stream = own;
         c->use_side = own_side;
