@@ -316,6 +316,7 @@ void state_changed(sphg_ctx* c)
     for (auto& pb : c->ph) pb.cells_valid = false;
     c->nl_valid = false;
     c->packed = false;
+    c->last_k = 1;   // the rates the state holds count as those of a plain step
 }
 
 // Permute the persistent state of phase k into the current cell order, so
@@ -1912,6 +1913,7 @@ __global__ void k_fill_f32(float* a, int n, float v)
 
 // The FP64 fields of a phase that a later step reads; rho is entry 6.
 constexpr int N_STATE_FIELDS = 23;
+static_assert(N_STATE_FIELDS <= PERM_MAXD, "a snapshot copies its fields in one kernel");
 void state_fields(DPhase& d, double** out[N_STATE_FIELDS])
 {
     double** list[N_STATE_FIELDS] = {&d.x, &d.y, &d.vx, &d.vy, &d.mass, &d.hsml, &d.rho, &d.vof, &d.p, &d.c,
@@ -1996,32 +1998,31 @@ int snap_alloc(sphg_gnn* g, StateCopy& s, int n)
     return SPHG_OK;
 }
 
-// to_snap: device state -> snapshot, else snapshot -> device state
+// to_snap: device state -> snapshot, else snapshot -> device state; one
+// copy kernel per phase for all its fields
 int snap_copy(sphg_gnn* g, int slot, bool to_snap)
 {
     sphg_ctx* c = g->c;
     for (int k = 0; k < 2; ++k) {
         PhaseBuf& b = c->ph[k];
         if (!b.bound) continue;
-        StateCopy& s = g->snap[slot][k];
-        const size_t n = (size_t)b.n;
-        auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
-            HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
-            return SPHG_OK;
-        };
+        StateCopy& sc = g->snap[slot][k];
         double** f[N_STATE_FIELDS];
         state_fields(b.d, f);
-        if (to_snap) {
-            RC(cp(s.id, b.d.id, n * sizeof(int)));
-            RC(cp(s.itype, b.d.itype, n * sizeof(int)));
-            RC(cp(s.fail, b.d.fail, n * sizeof(int)));
-            for (int i = 0; i < N_STATE_FIELDS; ++i) RC(cp(s.f[i], *f[i], n * sizeof(double)));
-        } else {
-            RC(cp(b.d.id, s.id, n * sizeof(int)));
-            RC(cp(b.d.itype, s.itype, n * sizeof(int)));
-            RC(cp(b.d.fail, s.fail, n * sizeof(int)));
-            for (int i = 0; i < N_STATE_FIELDS; ++i) RC(cp(*f[i], s.f[i], n * sizeof(double)));
+        int* st[3] = {b.d.id, b.d.itype, b.d.fail};
+        int* sn[3] = {sc.id, sc.itype, sc.fail};
+        PermArgs A{};
+        A.nd = N_STATE_FIELDS;
+        A.ni = 3;
+        for (int i = 0; i < N_STATE_FIELDS; ++i) {
+            A.sd[i] = to_snap ? *f[i] : sc.f[i];
+            A.dd[i] = to_snap ? sc.f[i] : *f[i];
         }
+        for (int i = 0; i < 3; ++i) {
+            A.si[i] = to_snap ? st[i] : sn[i];
+            A.di[i] = to_snap ? sn[i] : st[i];
+        }
+        RC(launch(c, "snapshot", k_permute_multi, nblk(b.n), TPB, A, (const int*)nullptr, b.n));
     }
     if (to_snap) {
         g->snap_rebuilds[slot] = c->nl_rebuilds;
