@@ -308,9 +308,9 @@ def sweep_section(args, hip):
         c.params.precision = abi.SPHG_MIXED if args.sweep_precision == "mixed" else abi.SPHG_FP64
     ntot = sum(c.water.n + c.soil.n for c in cases)
     ss = [Solver(c.params, c.water, c.soil) for c in cases]
-    # four captured periods (a period: the neighbour-list rebuild period of
-    # FP64 contexts, 64 steps; the reorder period of mixed ones, 16)
-    period = 64 if args.sweep_precision == "fp64" else 16
+    # four captured periods (a period: the neighbour-list rebuild period, 64
+    # steps)
+    period = 64
     steps = 4 * period
     for s in ss:
         s.time_integration(2, sync=False)

@@ -265,13 +265,21 @@ __device__ __forceinline__ int tag_cy(int tag) { return tag & 0x7fff; }
 // FP32 neighbour record of the mixed-precision form (the layout is
 // described in sph_mixed.h): chunks q0 and q1 of the record at R (plane 0),
 // planes pn elements apart.
+// key >= 0: the record's cell is the row-major cell `key` (the cell of the
+// particle at the neighbour-list rebuild, which it may have left since);
+// otherwise the cell of its position.
 __device__ __forceinline__ void write_q01(float4* R, size_t pn, const Grid& g, double cs, double x, double y, double h,
-                                          bool virt, double vx, double vy, double drho, double m)
+                                          bool virt, double vx, double vy, double drho, double m, int key)
 {
     int cx, cy;
-    grid_cell(g, x, y, cx, cy);
-    cx = min(max(cx, 0), g.ncx - 1);
-    cy = min(max(cy, 0), g.ncy - 1);
+    if (key >= 0) {
+        cy = key / g.ncx;
+        cx = key - cy * g.ncx;
+    } else {
+        grid_cell(g, x, y, cx, cy);
+        cx = min(max(cx, 0), g.ncx - 1);
+        cy = min(max(cy, 0), g.ncy - 1);
+    }
     const double xc = g.x0 + ((double)cx + 0.5) * cs, yc = g.y0 + ((double)cy + 0.5) * cs;
     R[0] = make_float4((float)(x - xc), (float)(y - yc), __int_as_float(cell_tag(cx, cy, virt)), (float)h);
     R[pn] = make_float4((float)vx, (float)vy, (float)drho, (float)m);
@@ -294,7 +302,10 @@ struct NRef {
     int* flag;
     double trig, near, hard2, look;
     int rebuild;
+    const int* cell;   // cell key per record slot since the rebuild (FP32 records)
 };
+
+__device__ __forceinline__ int record_cell(const NRef& R, int t) { return R.cell && !R.rebuild ? R.cell[t] : -1; }
 
 __device__ __forceinline__ void track_displacement(const NRef& R, int t, double x, double y, double vx, double vy)
 {
@@ -360,7 +371,7 @@ __device__ __forceinline__ void pack_water_at(const DPhase& P, const KParams& K,
     if (q) {
         float4* Q = reinterpret_cast<float4*>(q) + t;
         const size_t pn = (size_t)P.n;
-        write_q01(Q, pn, g, cs, x, y, P.hsml[i], it < 0, vx, vy, rho - K.w_rho0, m);
+        write_q01(Q, pn, g, cs, x, y, P.hsml[i], it < 0, vx, vy, rho - K.w_rho0, m, record_cell(NR, t));
         Q[2 * pn] = make_float4((float)vof, (float)irho, (float)(m * irho), (float)p);
         Q[3 * pn] = make_float4((float)cn, 0.0f, 0.0f, 0.0f);
         return;
@@ -388,7 +399,8 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
 // in memory afterwards as k_final_water followed by k_pack_water(do_half =
 // 1), apart from stores the pack would overwrite.  The record order of step
 // s + 1 must be the current one (neighbour lists kept, no cell-list rebuild).
-__global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int with_vof, int renorm, NRef NR)
+__global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int with_vof, int renorm, float* q, Grid g,
+                             double cs, NRef NR)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
@@ -406,7 +418,7 @@ __global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int wi
             P.vy[i] = e.vy;
         }
     }
-    pack_water_at(P, K, rec, t, i, 1, nullptr, Grid{}, 0.0, NR, e.x, e.y, e.rho, e.vof, e.vx, e.vy);
+    pack_water_at(P, K, rec, t, i, 1, q, g, cs, NR, e.x, e.y, e.rho, e.vof, e.vx, e.vy);
 }
 
 // Pack of soil particle i (values as for water, plus stress and pressure):
@@ -465,7 +477,7 @@ __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, 
     if (q) {
         float4* Q = reinterpret_cast<float4*>(q) + t;
         const size_t pn = (size_t)P.n;
-        write_q01(Q, pn, g, cs, x, y, P.hsml[i], it < 0, vx, vy, rho - K.s_rho0, m);
+        write_q01(Q, pn, g, cs, x, y, P.hsml[i], it < 0, vx, vy, rho - K.s_rho0, m, record_cell(NR, t));
         Q[2 * pn] = make_float4((float)P.vof[i], (float)irho, (float)(m * irho), (float)p);
         Q[3 * pn] = make_float4((float)P.c[i], (float)sxx, (float)sxy, (float)syy);
         return;
@@ -495,7 +507,7 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
 // The soil final update of step s fused with its pack for step s + 1
 // (first_half, two projections), as for water.  The pack reads vof from
 // the state: the final update's vof is stored first.
-__global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, NRef NR)
+__global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, float* q, Grid g, double cs, NRef NR)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
@@ -506,8 +518,7 @@ __global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, NRef NR
         P.x[i] = e.x;
         P.y[i] = e.y;
     }
-    pack_soil_at(P, K, rec, t, i, 1, 2, nullptr, Grid{}, 0.0, NR, e.x, e.y, e.rho, e.vx, e.vy, e.p, e.sxx, e.sxy,
-                 e.syy);
+    pack_soil_at(P, K, rec, t, i, 1, 2, q, g, cs, NR, e.x, e.y, e.rho, e.vx, e.vy, e.p, e.sxx, e.sxy, e.syy);
 }
 
 // ------------------------------------------------------------------------
@@ -879,26 +890,42 @@ __device__ __forceinline__ CatMask cand_mask(const LaneRows& L, const Masks& M, 
     return scan_mask(L, self, rec, acc);
 }
 
-// Neighbour lists of phase A (records recA) against the records of phase B
-// (cell list startB on grid g): candidates with dr < thr (thr = (R + skin)^2
-// of the pair type), the partner itself excluded when same.  Also records
-// each particle's cell key.
-__global__ __launch_bounds__(256) void k_nlist(const double* recA, int nA, const double* recB, int nB,
+// Neighbour lists of phase A against phase B (cell list startB on grid g,
+// candidate k at state slot ordB[k]): candidates with dr < thr (thr =
+// (R + skin)^2 of the pair type), the particle itself excluded when same.
+// Also records each particle's cell key.  Positions come from the state, so
+// the FP64 and the FP32 records share the lists.
+__global__ __launch_bounds__(256) void k_nlist(const double* xA, const double* yA, const int* ordA, int nA,
+                                               const double* xB, const double* yB, const int* ordB,
                                                const int* startB, Grid g, double thr, int same, Masks VM, int* cell)
 {
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= nA) return;
-    const RecV RA = recv(recA, nA), RB = recv(recB, nB);
-    const double2 p = RA(t, 0);
+    const int ia = ordA[t];
+    const double px = xA[ia], py = yA[ia];
     int cx, cy;
-    clamped_cell(g, p.x, p.y, cx, cy);
+    clamped_cell(g, px, py, cx, cy);
     if (cell) cell[t] = cy * g.ncx + cx;
     const LaneRows L = lane_rows(block_rows_at(g, startB, cx, cy));
-    auto acc = [&](int, double2 b) {
-        const double dx = p.x - b.x, dy = p.y - b.y;
+    auto acc = [&](int k) {
+        const int ib = ordB[k];
+        const double dx = px - xB[ib], dy = py - yB[ib];
         return dx * dx + dy * dy < thr;
     };
-    const CatMask cm = scan_mask(L, same ? t : -1, RB, acc);
+    const int self = same ? t : -1;
+    uint64_t m[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int k0 = r == 0 ? L.k0[0] : (r == 1 ? L.k0[1] : L.k0[2]);
+        const int k1 = r == 0 ? L.k1[0] : (r == 1 ? L.k1[1] : L.k1[2]);
+        uint64_t mm = 0;
+        if (k1 - k0 <= 64) {
+#pragma unroll 4
+            for (int k = k0; k < k1; ++k) mm |= (uint64_t)(k != self && acc(k)) << (k - k0);
+        }
+        m[r] = mm;
+    }
+    const CatMask cm = cat_masks(L, m[0], m[1], m[2]);
     VM.m[t] = cm.lo;
     VM.m[nA + t] = cm.hi;
     VM.ovf[t] = cm.ovf;

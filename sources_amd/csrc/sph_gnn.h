@@ -571,8 +571,7 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
         }
     }
     if (cnt > DEG) atomicOr(O.overflow, 1);
-    O.nd[node] = min(cnt, DEG);
-    for (int s = min(cnt, DEG); s < DEG; ++s) nb[(size_t)s * ns] = -1;
+    O.nd[node] = min(cnt, DEG);   // (slots past the degree are never read)
 }
 
 // Rate-change features: after the coarse step P holds the rates evaluated at

@@ -182,13 +182,71 @@ __device__ __forceinline__ void visit_f(const CatMask& cm, const LaneRows& L, in
 #pragma unroll 1
     for (int k = it.next(i); k >= 0; k = it.next(i)) body(k);
     if (cm.ovf) {
+        // one pass over the concatenated rows (row bounds as scalars: an
+        // indexed LaneRows would be kept in scratch memory)
+        const int s1 = L.k1[0] - L.k0[0], s2 = s1 + L.k1[1] - L.k0[1], tot = s2 + L.k1[2] - L.k0[2];
+        const int e0 = L.k0[0], e1 = L.k0[1] - s1, e2 = L.k0[2] - s2;
 #pragma unroll 1
-        for (int r = 0; r < 3; ++r) {
-#pragma unroll 1
-            for (int k = L.k0[r]; k < L.k1[r]; ++k)
-                if (k != self && accept(k)) body(k);
+        for (int c = 0; c < tot; ++c) {
+            const int k = c + (c < s1 ? e0 : (c < s2 ? e1 : e2));
+            if (k != self && accept(k)) body(k);
         }
     }
+}
+
+// The exact FP32 partner mask from a neighbour-list mask (as filter_mask of
+// the FP64 records): the listed candidates whose records pass accept(q0).
+template <class A>
+__device__ __forceinline__ CatMask filter_mask_f(const LaneRows& L, const CatMask& vm, const RecQ& rec, A&& acc)
+{
+    if (vm.ovf) return vm;
+    const int s1 = L.k1[0] - L.k0[0], s2 = s1 + L.k1[1] - L.k0[1];
+    const int e0 = L.k0[0], e1 = L.k0[1] - s1, e2 = L.k0[2] - s2;
+    uint64_t lo = vm.lo, hi = vm.hi, olo = 0, ohi = 0;
+#pragma unroll 1
+    for (;;) {
+        int c[4], k[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool in_lo = lo != 0;
+            const uint64_t m = in_lo ? lo : hi;
+            c[j] = m ? __builtin_ctzll(m) + (in_lo ? 0 : 64) : -1;
+            const uint64_t rest = m & (m - 1);
+            lo = in_lo ? rest : lo;
+            hi = in_lo ? hi : rest;
+            k[j] = c[j] < 0 ? -1 : c[j] + (c[j] < s1 ? e0 : (c[j] < s2 ? e1 : e2));
+        }
+        if (k[0] < 0) break;
+        float4 q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = rec(k[j] >= 0 ? k[j] : k[0], 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (k[j] >= 0 && acc(q[j])) {
+                if (c[j] < 64) olo |= 1ull << c[j];
+                else ohi |= 1ull << (c[j] - 64);
+            }
+        }
+        if (k[3] < 0) break;
+    }
+    CatMask out;
+    out.lo = olo;
+    out.hi = ohi;
+    out.ovf = 0;
+    return out;
+}
+
+// Partner mask over the rows L: the list M filtered (neighbour lists on), or
+// the scan of the three rows.
+template <class A>
+__device__ __forceinline__ CatMask cand_mask_f(const LaneRows& L, const Masks& M, int n, int tt, int self,
+                                               const RecQ& rec, A&& acc)
+{
+    if (M.m) return filter_mask_f(L, mask_of(M, n, tt), rec, acc);
+    const uint64_t m0 = scan_row_f(rec, L.k0[0], L.k1[0], self, acc);
+    const uint64_t m1 = scan_row_f(rec, L.k0[1], L.k1[1], self, acc);
+    const uint64_t m2 = scan_row_f(rec, L.k0[2], L.k1[2], self, acc);
+    return cat_masks(L, m0, m1, m2);
 }
 
 // ------------------------------------------------------------------------
@@ -196,7 +254,8 @@ __device__ __forceinline__ void visit_f(const CatMask& cm, const LaneRows& L, in
 // ------------------------------------------------------------------------
 template <int SKF>
 __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS,
-                                                   Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
+                                                   NList NL, Masks MW, Grid g, KParams K, FusedK F, MixK X,
+                                                   int cross)
 {
     const RecQ QWv = recqv(qW, W.n), QSv = recqv(qS, S.n);
     const WaveCtx wc = wave_ctx(W.n);
@@ -215,10 +274,7 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
         sep(me, b0, X.cs, dx, dy);
         return pred_f(uni, X.thr_w, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
     };
-    const uint64_t m0 = scan_row_f(QWv, L.k0[0], L.k1[0], t, acc_q);
-    const uint64_t m1 = scan_row_f(QWv, L.k0[1], L.k1[1], t, acc_q);
-    const uint64_t m2 = scan_row_f(QWv, L.k0[2], L.k1[2], t, acc_q);
-    const CatMask cm = cat_masks(L, m0, m1, m2);
+    const CatMask cm = cand_mask_f(L, NL.same, W.n, wc.tt, t, QWv, acc_q);
     if (wc.valid) {
         MW.m[t] = cm.lo;
         MW.m[W.n + t] = cm.hi;
@@ -268,10 +324,7 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
             sep(me, b0, X.cs, dx, dy);
             return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
         };
-        const uint64_t c0 = scan_row_f(QSv, LX.k0[0], LX.k1[0], -1, accx_q);
-        const uint64_t c1 = scan_row_f(QSv, LX.k0[1], LX.k1[1], -1, accx_q);
-        const uint64_t c2 = scan_row_f(QSv, LX.k0[2], LX.k1[2], -1, accx_q);
-        const CatMask cx3 = cat_masks(LX, c0, c1, c2);
+        const CatMask cx3 = cand_mask_f(LX, NL.cross, W.n, wc.tt, -1, QSv, accx_q);
         const double vofa_d = W.vof[a], rhoa_d = W.rho[a];
         const float cfa = (float)(vofa_d * rhoa_d * (-K.numeric_gravity) / K.s_perm);
         const float irhoa = a2.y;
@@ -415,8 +468,8 @@ __global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, 
 // share one scan and one partner loop; the terms are those of k_soil_ab4.
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* qS, const float* qW, Grid g,
-                                           KParams K, FusedK F, MixK X, int cross, int mask)
+__global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* qS, const float* qW, NList NL,
+                                           Grid g, KParams K, FusedK F, MixK X, int cross, int mask)
 {
     const RecQ QSv = recqv(qS, S.n), QWv = recqv(qW, W.n);
     const WaveCtx wc = wave_ctx(S.n);
@@ -447,10 +500,7 @@ __global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* q
             sep(me, b0, X.cs, dx, dy);
             return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
         };
-        const uint64_t c0 = scan_row_f(QWv, LX.k0[0], LX.k1[0], -1, accx_q);
-        const uint64_t c1 = scan_row_f(QWv, LX.k0[1], LX.k1[1], -1, accx_q);
-        const uint64_t c2 = scan_row_f(QWv, LX.k0[2], LX.k1[2], -1, accx_q);
-        const CatMask cx3 = cat_masks(LX, c0, c1, c2);
+        const CatMask cx3 = cand_mask_f(LX, NL.cross, S.n, wc.tt, -1, QWv, accx_q);
         const float cfs = (float)(vofa_d * (-K.numeric_gravity) / K.s_perm);
         const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)QWv(0, 0).w + (double)me.h) : (double)me.h);
         visit_f(cx3, LX, -1, [&](int k) { return accx_q(QWv(k, 0)); }, [&](int k) {
@@ -483,10 +533,7 @@ __global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* q
         sep(me, b0, X.cs, dx, dy);
         return pred_f(uni, X.thr_s, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
     };
-    const uint64_t m0 = scan_row_f(QSv, L.k0[0], L.k1[0], t, acc_q);
-    const uint64_t m1 = scan_row_f(QSv, L.k0[1], L.k1[1], t, acc_q);
-    const uint64_t m2 = scan_row_f(QSv, L.k0[2], L.k1[2], t, acc_q);
-    const CatMask cm = cat_masks(L, m0, m1, m2);
+    const CatMask cm = cand_mask_f(L, NL.same, S.n, wc.tt, t, QSv, acc_q);
     const float et2 = (float)(K.etq * K.etq);
     const float beta = (float)K.beta, alpha = (float)K.alpha;
     Kahan con, drd, txx, txy, tyy, vcc, wxy, avx, avy;

@@ -123,6 +123,7 @@ struct sphg_ctx {
     FusedK F{};
     MixK X{};                   // mixed-precision constants
     double cs = 0.0;            // cell size
+    double csv = 0.0;           // cell size of the neighbour-list grid
     bool legacy = false;        // SPHG_LEGACY=1: unfused reference-order kernels
     int reorder_every = 16;     // steps between permutations of the state into cell order
     int since_reorder = 0;
@@ -479,6 +480,7 @@ int setup_grid(sphg_ctx* c)
         v.ncx = (int)(std::floor((P.x_maxgeom - P.x_mingeom) / csv) + 1.0);
         v.ncy = (int)(std::floor((P.y_maxgeom - P.y_mingeom) / csv) + 1.0);
         c->gv = v;
+        c->csv = csv;
         c->ncell_v = v.ncx * v.ncy;
     }
     // per phase: reference link_list sorting grid (init_grid, m_particles.f90:1069-1113)
@@ -921,11 +923,11 @@ int mask_b(const sphg_ctx* c, int phase)
     return m;
 }
 
-// FP64 fused two-phase steps keep neighbour lists with a skin (nl_config),
+// Fused two-phase steps keep neighbour lists with a skin (nl_config),
 // except during a spell of fast motion (nl_mode).
 bool nl_config(const sphg_ctx* c)
 {
-    return c->use_nl && !c->legacy && !c->P.single_phase && c->P.precision != SPHG_MIXED;
+    return c->use_nl && !c->legacy && !c->P.single_phase;
 }
 
 bool nl_at(const sphg_ctx* c, int step)
@@ -960,7 +962,7 @@ NRef nref(const sphg_ctx* c, int k, bool rebuild)
 {
     const double hard = 0.5 * c->skin;
     return NRef{c->ph[k].nl_x, c->ph[k].nl_y, c->d_nlflag, 0.25 * c->skin, 0.45 * c->skin, hard * hard,
-                NL_LOOK * c->K.dt, rebuild ? 1 : 0};
+                NL_LOOK * c->K.dt, rebuild ? 1 : 0, c->ph[k].nl_cell};
 }
 
 // The final kernels a step's stages launch (FP64 list mode): second_half of
@@ -1035,14 +1037,14 @@ int stages_fused(sphg_ctx* c, const FinalArgs* fin)
         auto final_water = [&]() -> int {
             if (fin->ahead)
                 return launch(c, "final_water", k_step_water, nblk(W.n), TPB, W.d, c->K, W.rec, fin->first, fin->vf,
-                              fin->renorm, nref(c, 0, false));
+                              fin->renorm, (float*)nullptr, c->gvl[0], c->csv, nref(c, 0, false));
             return launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, fin->first, fin->vf,
                           fin->renorm);
         };
         auto final_soil = [&]() -> int {
             if (fin->ahead)
                 return launch(c, "final_soil", k_step_soil, nblk(S.n), TPB, S.d, c->K, S.rec, fin->first,
-                              nref(c, 1, false));
+                              (float*)nullptr, c->gvl[1], c->csv, nref(c, 1, false));
             return launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, fin->first);
         };
         if (split) {
@@ -1105,26 +1107,49 @@ bool soil_only(const sphg_ctx* c) { return c->P.single_phase && !c->ph[0].bound 
 
 // Mixed-precision stages: FP32 records and pair sums, FP64 per-particle tails.
 template <int SKF>
-int stages_mixed(sphg_ctx* c)
+int stages_mixed(sphg_ctx* c, const FinalArgs* fin)
 {
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
     float* qW = reinterpret_cast<float*>(W.rec);
     float* qS = reinterpret_cast<float*>(S.rec);
+    // with neighbour lists: the list grid (records hold positions relative
+    // to their list cells) and the lists
+    const bool nl = nl_mode(c);
+    const NList nW = nl ? NList{W.nl_same, W.nl_cross, W.nl_cell} : NList{};
+    const NList nS = nl ? NList{S.nl_same, S.nl_cross, S.nl_cell} : NList{};
+    const Grid& gW = nl ? c->gvl[0] : c->gl[0];
+    const Grid& gS = nl ? c->gvl[1] : c->gl[1];
+    MixK X = c->X;
+    if (nl) X.cs = (float)c->csv;
     SideFork fork(c);   // soil stage beside the water stages
     RC(fork.rc);
     {
         SideScope side(c);
         RC(launch(c, "soil_ab", k_soil_ab_mx<SKF>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, (const float*)qS,
-                  (const float*)qW, c->gl[1], c->K, c->F, c->X, 1, mask_b(c, 1)));
+                  (const float*)qW, nS, gS, c->K, c->F, X, 1, mask_b(c, 1)));
     }
     RC(launch(c, "water_a", k_water_a_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, (const float*)qW, qW,
-              (const float*)qS, W.masks, c->gl[0], c->K, c->F, c->X, 1));
+              (const float*)qS, nW, W.masks, gW, c->K, c->F, X, 1));
     swap_c(W);
     swap_c(S);
     RC(launch(c, "water_b", k_water_b_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, (const float*)qW, W.masks,
-              c->gl[0], c->K, c->F, c->X, mask_b(c, 0)));
-    return fork.join();
+              gW, c->K, c->F, X, mask_b(c, 0)));
+    RC(fork.join());
+    if (fin) {
+        // (the soil stage reads the water records and the water stages the
+        // soil records: both finals after the join)
+        if (fin->ahead) {
+            RC(launch(c, "final_water", k_step_water, nblk(W.n), TPB, W.d, c->K, W.rec, fin->first, fin->vf,
+                      fin->renorm, qW, gW, c->csv, nref(c, 0, false)));
+            RC(launch(c, "final_soil", k_step_soil, nblk(S.n), TPB, S.d, c->K, S.rec, fin->first, qS, gS, c->csv,
+                      nref(c, 1, false)));
+        } else {
+            RC(launch(c, "final_water", k_final_water, nblk(W.n), TPB, W.d, c->K, fin->first, fin->vf, fin->renorm));
+            RC(launch(c, "final_soil", k_final_soil, nblk(S.n), TPB, S.d, c->K, fin->first));
+        }
+    }
+    return SPHG_OK;
 }
 
 // Neighbour lists of the fused step at the rebuild: records written in the
@@ -1134,18 +1159,22 @@ int build_nlists(sphg_ctx* c)
 {
     PhaseBuf& W = c->ph[0];
     PhaseBuf& S = c->ph[1];
-    const double* rW = W.rec;
-    const double* rS = S.rec;
+    const DPhase& w = W.d;
+    const DPhase& sd = S.d;
     // the displacement trigger restarts (a past-the-skin bit stays for the check)
     RC(launch(c, "nlist", k_keep_bits, 1, 1, c->d_nlflag, 2));
-    RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, rW, W.n, rW, W.n, (const int*)W.d.cell_start, c->gv,
-              c->nl_thr[0], 1, W.nl_same, W.nl_cell));
-    RC(launch(c, "nlist", k_nlist, nblk(S.n, 256), 256, rS, S.n, rS, S.n, (const int*)S.d.cell_start, c->gv,
-              c->nl_thr[1], 1, S.nl_same, S.nl_cell));
-    RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, rW, W.n, rS, S.n, (const int*)S.d.cell_start, c->gv,
-              c->nl_thr[2], 0, W.nl_cross, (int*)nullptr));
-    RC(launch(c, "nlist", k_nlist, nblk(S.n, 256), 256, rS, S.n, rW, W.n, (const int*)W.d.cell_start, c->gv,
-              c->nl_thr[2], 0, S.nl_cross, (int*)nullptr));
+    RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, (const double*)w.x, (const double*)w.y,
+              (const int*)w.order, W.n, (const double*)w.x, (const double*)w.y, (const int*)w.order,
+              (const int*)w.cell_start, c->gv, c->nl_thr[0], 1, W.nl_same, W.nl_cell));
+    RC(launch(c, "nlist", k_nlist, nblk(S.n, 256), 256, (const double*)sd.x, (const double*)sd.y,
+              (const int*)sd.order, S.n, (const double*)sd.x, (const double*)sd.y, (const int*)sd.order,
+              (const int*)sd.cell_start, c->gv, c->nl_thr[1], 1, S.nl_same, S.nl_cell));
+    RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, (const double*)w.x, (const double*)w.y,
+              (const int*)w.order, W.n, (const double*)sd.x, (const double*)sd.y, (const int*)sd.order,
+              (const int*)sd.cell_start, c->gv, c->nl_thr[2], 0, W.nl_cross, (int*)nullptr));
+    RC(launch(c, "nlist", k_nlist, nblk(S.n, 256), 256, (const double*)sd.x, (const double*)sd.y,
+              (const int*)sd.order, S.n, (const double*)w.x, (const double*)w.y, (const int*)w.order,
+              (const int*)w.cell_start, c->gv, c->nl_thr[2], 0, S.nl_cross, (int*)nullptr));
     c->nl_valid = true;
     c->nl_due = false;
     c->nl_rebuilt_at = c->nl_seq + 1;   // the step being issued
@@ -1158,12 +1187,11 @@ int build_nlists(sphg_ctx* c)
 int stages_of(sphg_ctx* c, const FinalArgs* fin = nullptr)
 {
     if (c->P.precision == SPHG_MIXED) {
-        if (fin) return SPHG_ERR_INVALID;
         switch (c->P.skf) {
-        case 1: return stages_mixed<1>(c);
-        case 2: return stages_mixed<2>(c);
-        case 3: return stages_mixed<3>(c);
-        default: return stages_mixed<4>(c);
+        case 1: return stages_mixed<1>(c, fin);
+        case 2: return stages_mixed<2>(c, fin);
+        case 3: return stages_mixed<3>(c, fin);
+        default: return stages_mixed<4>(c, fin);
         }
     }
     switch (c->P.skf) {
@@ -1194,7 +1222,7 @@ int pack_two_phase(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
     if (c->packed) {
         // the final kernels of the step before packed this one
         c->packed = false;
-        if (!nl || rebuild || mixed || !water_half || !soil_half || soil_ndp != 2) return SPHG_ERR_INVALID;
+        if (!nl || rebuild || !water_half || !soil_half || soil_ndp != 2) return SPHG_ERR_INVALID;
         return SPHG_OK;
     }
     const NRef nrW = nl ? nref(c, 0, rebuild) : NRef{}, nrS = nl ? nref(c, 1, rebuild) : NRef{};
@@ -1206,8 +1234,8 @@ int pack_two_phase(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
     auto soil_chain = [&]() -> int {
         RC(build_cells(c, 1, nl));
         if (rebuild) RC(reorder(c, 1));
-        return launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS, c->gl[1],
-                      c->cs, nrS);
+        return launch(c, "pack", k_pack_soil, nblk(S.n), TPB, S.d, c->K, S.rec, soil_half, soil_ndp, qS,
+                      nl ? c->gvl[1] : c->gl[1], nl ? c->csv : c->cs, nrS);
     };
     // The two phases' cell lists and packs are independent: the soil chain
     // runs on the side stream beside the water chain (the stages after them
@@ -1221,7 +1249,8 @@ int pack_two_phase(sphg_ctx* c, int water_half, int soil_half, int soil_ndp)
         }
         RC(build_cells(c, 0, nl));
         if (rebuild) RC(reorder(c, 0));
-        RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, c->gl[0], c->cs, nrW));
+        RC(launch(c, "pack", k_pack_water, nblk(W.n), TPB, W.d, c->K, W.rec, water_half, qW, nl ? c->gvl[0] : c->gl[0],
+                  nl ? c->csv : c->cs, nrW));
         if (fork.open) RC(fork.join());
         else RC(soil_chain());
     }
@@ -2541,11 +2570,14 @@ int sphg_gnn_debug(sphg_gnn* g, int sample, float* nf, float* raw, int* nb, int*
     RC(dl(raw, G.raw, n * gnn::RAW * sizeof(float)));
     RC(dl(nd, G.nd, n * sizeof(int)));
     if (nb) {   // the device table is slot-major; the caller gets [n][DEG]
-        std::vector<int> t(n * gnn::DEG);
+        // (slots past a node's degree are not written on the device: -1 here)
+        std::vector<int> t(n * gnn::DEG), d(n);
         HIPCHK(c, hipMemcpyAsync(t.data(), G.nb, t.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(d.data(), G.nd, d.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < n; ++i)
-            for (int s = 0; s < gnn::DEG; ++s) nb[i * gnn::DEG + (size_t)s] = t[(size_t)s * n + i];
+            for (int s = 0; s < gnn::DEG; ++s)
+                nb[i * gnn::DEG + (size_t)s] = s < d[i] ? t[(size_t)s * n + i] : -1;
     }
     RC(dl(tgt, G.tgt, n * gnn::NO * sizeof(float)));
     RC(dl(tw, G.tw, n * sizeof(float)));
