@@ -118,10 +118,11 @@ def cpu_child(args):
 
 
 def learned_section(args, hip):
-    """The learned K-step surrogate (coarse FP64 step of K dt + the network's
-    correction, the FP64 plain step as judge every J advances), its ablation
-    (the same rollout with the network off: the coarse step alone) and FP64
-    plain steps, all from the same state of the 250k slope."""
+    """The learned K-step surrogate (coarse mixed-precision step of K dt + the
+    network's correction, the FP64 plain step as judge every J advances), its
+    ablation (the same rollout with the network off: the coarse step alone)
+    and FP64 plain steps, all from the same state of the 250k slope on one
+    context."""
     import numpy as np
     from sources_amd import abi, scenarios
     from sources_amd.sph import Solver
@@ -130,7 +131,7 @@ def learned_section(args, hip):
     K, S, M, J = args.learned_k, args.learned_samples, args.learned_advances, args.learned_judge
     # the slope at half the linear size: ~250k particles
     sc = scenarios.slope(dx=args.dx, scale=args.scale * args.learned_scale, max_interaction=0)
-    sc.params.precision = abi.SPHG_FP64
+    sc.params.precision = abi.SPHG_MIXED   # the coarse steps; judge, targets and reference run FP64
     ntot = sc.water.n + sc.soil.n
     b = Solver(sc.params, sc.water, sc.soil)
     b.time_integration(args.warmup, sync=False)
@@ -166,8 +167,13 @@ def learned_section(args, hip):
         hip.record(e1, b.stream)
         return hip.elapsed_ms(e0, e1), r
 
+    def precision(p):
+        sc.params.precision = p
+        b.set_params(sc.params)
+
     # the plain FP64 reference: the same context and state, M K steps
     reset()
+    precision(abi.SPHG_FP64)
     ms_plain0, _ = timed(lambda: b.time_integration(M * K, sync=False))
     b.sync_all()
     ref = sc.copy()
@@ -187,8 +193,10 @@ def learned_section(args, hip):
         # GPU's speed drifts between runs: the plain time is taken again
         # right before the leg)
         reset()
+        precision(abi.SPHG_FP64)
         ms_plain, _ = timed(lambda: b.time_integration(M * K, sync=False))
         reset()
+        precision(abi.SPHG_MIXED)
         ms, (nfb, jmax, jrms) = timed(lambda: g.rollout(M, judge_every=J, tol=0.01, use_net=use_net))
         b.sync_all()
         emax, erms = rho_err()
@@ -218,7 +226,7 @@ def learned_section(args, hip):
                                        net["rho_rel_err_rms_vs_fp64"] < off["rho_rel_err_rms_vs_fp64"]),
             "train": {"samples": S, "iters": args.learned_iters, "loss_before": l0, "loss_after": l1,
                       "seconds": t_train, "gains": gains},
-            "rates_precision": "fp64",
+            "rates_precision": "mixed",
         },
     }
 
