@@ -891,7 +891,8 @@ __device__ __forceinline__ CatMask cand_mask(const LaneRows& L, const Masks& M, 
 }
 
 // Neighbour lists of phase A against phase B (cell list startB on grid g,
-// candidate k at state slot ordB[k]): candidates with dr < thr (thr =
+// candidate k at state slot ordB[k], or k when ordB is null: the state in
+// cell order, as a rebuild leaves it): candidates with dr < thr (thr =
 // (R + skin)^2 of the pair type), the particle itself excluded when same.
 // Also records each particle's cell key.  Positions come from the state, so
 // the FP64 and the FP32 records share the lists.
@@ -901,14 +902,14 @@ __global__ __launch_bounds__(256) void k_nlist(const double* xA, const double* y
 {
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= nA) return;
-    const int ia = ordA[t];
+    const int ia = ordA ? ordA[t] : t;
     const double px = xA[ia], py = yA[ia];
     int cx, cy;
     clamped_cell(g, px, py, cx, cy);
     if (cell) cell[t] = cy * g.ncx + cx;
     const LaneRows L = lane_rows(block_rows_at(g, startB, cx, cy));
     auto acc = [&](int k) {
-        const int ib = ordB[k];
+        const int ib = ordB ? ordB[k] : k;
         const double dx = px - xB[ib], dy = py - yB[ib];
         return dx * dx + dy * dy < thr;
     };

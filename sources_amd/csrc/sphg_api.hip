@@ -1161,19 +1161,20 @@ int build_nlists(sphg_ctx* c)
     PhaseBuf& S = c->ph[1];
     const DPhase& w = W.d;
     const DPhase& sd = S.d;
-    // the displacement trigger restarts (a past-the-skin bit stays for the check)
+    // the displacement trigger restarts (a past-the-skin bit stays for the check);
+    // the rebuild has just permuted the state into cell order (no order lookups)
     RC(launch(c, "nlist", k_keep_bits, 1, 1, c->d_nlflag, 2));
     RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, (const double*)w.x, (const double*)w.y,
-              (const int*)w.order, W.n, (const double*)w.x, (const double*)w.y, (const int*)w.order,
+              (const int*)nullptr, W.n, (const double*)w.x, (const double*)w.y, (const int*)nullptr,
               (const int*)w.cell_start, c->gv, c->nl_thr[0], 1, W.nl_same, W.nl_cell));
     RC(launch(c, "nlist", k_nlist, nblk(S.n, 256), 256, (const double*)sd.x, (const double*)sd.y,
-              (const int*)sd.order, S.n, (const double*)sd.x, (const double*)sd.y, (const int*)sd.order,
+              (const int*)nullptr, S.n, (const double*)sd.x, (const double*)sd.y, (const int*)nullptr,
               (const int*)sd.cell_start, c->gv, c->nl_thr[1], 1, S.nl_same, S.nl_cell));
     RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, (const double*)w.x, (const double*)w.y,
-              (const int*)w.order, W.n, (const double*)sd.x, (const double*)sd.y, (const int*)sd.order,
+              (const int*)nullptr, W.n, (const double*)sd.x, (const double*)sd.y, (const int*)nullptr,
               (const int*)sd.cell_start, c->gv, c->nl_thr[2], 0, W.nl_cross, (int*)nullptr));
     RC(launch(c, "nlist", k_nlist, nblk(S.n, 256), 256, (const double*)sd.x, (const double*)sd.y,
-              (const int*)sd.order, S.n, (const double*)w.x, (const double*)w.y, (const int*)w.order,
+              (const int*)nullptr, S.n, (const double*)w.x, (const double*)w.y, (const int*)nullptr,
               (const int*)w.cell_start, c->gv, c->nl_thr[2], 0, S.nl_cross, (int*)nullptr));
     c->nl_valid = true;
     c->nl_due = false;
