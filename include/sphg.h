@@ -291,7 +291,10 @@ int sphg_gnn_gains(const sphg_gnn *gnn, float *gains);
 int sphg_gnn_step(sphg_gnn *gnn, int use_net);
 /* nlearned learned advances with the plain step as judge and fallback:
    nfallback: judged advances over tol plus advances replaced by plain steps
-   after a failed sanity test; max_err / rms_err:
+   after a failed sanity test or a failed check of the steps themselves (a
+   particle past half the neighbour-list skin or out of the domain: such a
+   span goes back to the last good state, as after a failed sanity test);
+   max_err / rms_err:
    largest and root-mean-square relative density deviation of the judged
    learned advances from the plain steps (real particles). */
 int sphg_gnn_rollout(sphg_gnn *gnn, int nlearned, int judge_every, double tol, int use_net, int *nfallback,

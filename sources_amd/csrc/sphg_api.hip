@@ -1984,8 +1984,10 @@ struct sphg_gnn {
     gnn::Scales S{};
     double tsum[gnn::NO] = {0, 0, 0, 0}, tsq[gnn::NO] = {0, 0, 0, 0};
     double tcount = 0.0;
-    StateCopy snap[2][2];           // [slot][phase]
-    int64_t snap_rebuilds[2] = {-1, -1};   // list rebuilds when each slot was taken
+    // [slot][phase]: 0 the last good state, 1 a judged advance's learned
+    // result, 2 the state a judge starts from
+    StateCopy snap[3][2];
+    int64_t snap_rebuilds[3] = {-1, -1, -1};   // list rebuilds when each slot was taken
     // lagged sanity check of the learned advances: device flag, its pinned
     // copies (two slots) and their events; gain fit sums
     int* broken = nullptr;
@@ -2185,7 +2187,7 @@ int sphg_gnn_create(sphg_ctx* c, int ksteps, int max_samples, uint64_t seed, sph
     for (int k = 0; k < 2; ++k) {
         if ((rc = galloc(g, &g->node_of_id[k], c->ph[k].n)) || (rc = galloc(g, &g->slot_of_id[k], c->ph[k].n)))
             return fail(rc);
-        for (int sl = 0; sl < 2; ++sl)
+        for (int sl = 0; sl < 3; ++sl)
             if ((rc = snap_alloc(g, g->snap[sl][k], c->ph[k].n))) return fail(rc);
     }
     // scales: velocity sqrt(|g| H), density deviation in percent, lithostatic stress
@@ -2464,11 +2466,19 @@ int sphg_gnn_rollout(sphg_gnn* g, int nlearned, int judge_every, double tol, int
         if (g->h_broken[slot]) return fallback();
         return SPHG_OK;
     };
+    // the step's own checks (a particle past half the list skin, or out of
+    // the domain) reject a learned state like the sanity test does
+    auto broken = [](int rc) { return rc == SPHG_ERR_MOVED_TOO_FAR || rc == SPHG_ERR_OUT_OF_RANGE; };
     for (int it = 0; it < nlearned; ++it) {
         const bool judge = judge_every > 0 && it % J == J - 1;
         if (!judge) {
-            RC(sphg_gnn_step(g, use_net));
+            const int src = sphg_gnn_step(g, use_net);
             ++since_good;
+            if (broken(src)) {
+                RC(fallback());
+                continue;
+            }
+            RC(src);
             const int slot = it & 1;
             RC(sanity_async(g, slot));
             const int prev = pending;
@@ -2484,11 +2494,22 @@ int sphg_gnn_rollout(sphg_gnn* g, int nlearned, int judge_every, double tol, int
         // judge: one learned advance against K plain steps from the same
         // state.  The run goes on from the plain result (it re-anchors there);
         // the learned one is measured, and one over tol counts as a fallback.
+        // The last good state (slot 0) stays until the learned step's pack
+        // has checked the displacement of the state the judge starts from
+        // (slot 2): past the skin there, or in an advance before, the span
+        // since slot 0 is replaced by plain steps.
         const int step0 = c->itimestep;
-        RC(snap_copy(g, 0, true));
-        RC(sphg_gnn_step(g, use_net));
+        RC(snap_copy(g, 2, true));
+        const int lrc = sphg_gnn_step(g, use_net);
+        ++since_good;
+        const int frc = broken(lrc) ? lrc : (lrc != SPHG_OK ? lrc : check_flags(c));
+        if (broken(frc)) {
+            RC(fallback());
+            continue;
+        }
+        RC(frc);
         RC(snap_copy(g, 1, true));
-        RC(snap_copy(g, 0, false));
+        RC(snap_copy(g, 2, false));
         c->itimestep = step0;
         RC(plain_steps(c, g->K));
         RC(fill_int(c, reinterpret_cast<int*>(g->err), 1, 0));
@@ -2518,6 +2539,12 @@ int sphg_gnn_rollout(sphg_gnn* g, int nlearned, int judge_every, double tol, int
         const int p = pending;
         pending = -1;
         RC(resolve(p));
+    }
+    {
+        // the displacement checks of the last advances' packs
+        const int frc = check_flags(c);
+        if (broken(frc) && since_good > 0) RC(fallback());
+        else RC(frc);
     }
     if (nfallback) *nfallback = fb;
     if (max_err) *max_err = worst;

@@ -118,6 +118,43 @@ def test_broken_state_falls_back_to_plain_steps():
     s.close()
 
 
+def test_advance_past_the_list_skin_falls_back_to_plain_steps():
+    # A velocity correction that also shifts every real particle by 0.75 of
+    # the neighbour-list skin (the correction moves x by K dt / 2 times the
+    # velocity it adds): the state stays finite and inside the grid, so the
+    # sanity test passes, but the next pack's displacement check flags it and
+    # the lagged check reports it one advance later.  The rollout replaces
+    # the span since the last good state by plain steps instead of failing
+    # (whichever check fires, every advance ends up replaced).
+    sc, s, g = setup(nwarm=10, nsamples=1)
+    rebuilds, _, skin = s.nlist_stats()
+    assert rebuilds > 0 and skin > 0.0            # the step runs on neighbour lists
+    out = g.debug(0)["out_scale"]
+    gain = g.gains()
+    assert out[0] > 0.0 and gain[0] > 0.0
+    kdt = 2 * sc.params.dt
+    w = g.weights()
+    w[G.O_B5] = 0.75 * skin / (0.5 * kdt) / (float(out[0]) * float(gain[0]))
+    g.set_weights(w)
+    nfb, err, _ = g.rollout(6, judge_every=0, tol=0.01, use_net=True)
+    assert nfb == 6 and err >= 1.0
+    steps = s.itimestep
+    s.sync_all()
+    # the result is the plain run from the same start (the lists rebuilt at
+    # other steps only reorder partner sums)
+    tw = make_case()
+    t = Solver(tw.params, tw.water, tw.soil)
+    t.time_integration(steps)
+    for a, b in ((sc.water, tw.water), (sc.soil, tw.soil)):
+        n = a.n
+        assert np.isfinite(a.x[:n]).all() and np.isfinite(a.rho[:n]).all()
+        assert normwise(a.rho[:n], b.rho[:n]) < 1e-9
+        assert normwise(a.x[:n], b.x[:n]) < 1e-9
+    t.close()
+    g.close()
+    s.close()
+
+
 def _check_graph(d, nodes, R_of_phase):
     """Every same-phase pair within R (cubic spline: R = 2h) of a sampled node
     is among its graph neighbours, and no neighbour lies beyond R; the band
