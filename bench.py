@@ -305,7 +305,8 @@ def pcie_section(args, hip):
 
 def sweep_section(args, hip):
     """64 independent submerged-column cases of uneven size (one seed):
-    eager per-context launches against the batched hipGraph replays."""
+    eager per-context launches against the batched hipGraph replays (staged:
+    one launch per kernel for all cases after each period's rebuild step)."""
     from sources_amd import scenarios
     from sources_amd.batch import Batch
     from sources_amd.sph import Solver
@@ -336,15 +337,19 @@ def sweep_section(args, hip):
     b.run(steps)
     hip.sync()
     ms_graph = (time.perf_counter() - t0) * 1e3 / steps
+    staged = b.staged
     print(f"# sweep: {len(cases)} cases, {ntot} particles: eager {ms_eager:.2f} ms/step, "
-          f"batched {ms_graph:.2f} ms/step (tuned: {ns or 'eager'} streams)", file=sys.stderr, flush=True)
+          f"batched {ms_graph:.2f} ms/step (tuned: {ns or 'eager'} streams, staged {staged})", file=sys.stderr,
+          flush=True)
     b.close()
     for s in ss:
         s.close()
     return {"sweep": {
         "cases": len(cases), "particles": ntot, "steps": steps, "streams": ns,
         "ms_per_step_eager": ms_eager, "ms_per_step_batched": ms_graph,
-        "batched_mode": "eager" if ns == 0 else f"hipGraph, {ns} streams",
+        "batched_mode": "eager" if ns == 0 else (
+            f"hipGraph, rebuild steps over {ns} streams, later steps one launch per kernel for all cases"
+            if staged else f"hipGraph, {ns} streams"),
         "value_batched": ntot / (ms_graph * 1e-3), "unit": "particle-steps/s",
         "speedup_batched_vs_eager": ms_eager / ms_graph,
         "workload": "submerged column collapses, N ~ logU[5k,120k], aspect ~ U[0.5,2], "

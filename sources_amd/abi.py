@@ -185,6 +185,7 @@ def load_library(path=None):
     lib.sphg_batch_run.argtypes = [vp, C.c_int]
     lib.sphg_batch_set_streams.argtypes = [vp, C.c_int]
     lib.sphg_batch_tune.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_double)]
+    lib.sphg_batch_staged.argtypes = [vp]
     lib.sphg_profile_enable.argtypes = [vp, C.c_int]
     lib.sphg_profile_reset.argtypes = [vp]
     lib.sphg_profile_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
@@ -213,6 +214,7 @@ EXPORTED_SYMBOLS = (
     "sphg_gnn_record", "sphg_gnn_train", "sphg_gnn_step", "sphg_gnn_rollout", "sphg_gnn_sample_size",
     "sphg_gnn_debug", "sphg_gnn_gains",
     "sphg_batch_create", "sphg_batch_destroy", "sphg_batch_run", "sphg_batch_set_streams", "sphg_batch_tune",
+    "sphg_batch_staged",
     "sphg_direct_find_noalloc",
 )
 

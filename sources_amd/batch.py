@@ -1,6 +1,7 @@
 """Batched scenarios (the sphg_batch_* entry points): many independent
 two-phase solvers advanced together by replays of one captured hipGraph per
-reorder period, bit-identical to stepping each solver alone."""
+period (in the staged form, one launch per kernel for all solvers in the
+steps after a period's first), bit-identical to stepping each solver alone."""
 import ctypes as C
 
 from . import abi
@@ -31,6 +32,12 @@ class Batch:
         ms = C.c_double()
         self._check(self.lib.sphg_batch_tune(self._b, trial_periods, C.byref(ns), C.byref(ms)), "sphg_batch_tune")
         return ns.value, ms.value
+
+    @property
+    def staged(self):
+        """True when the captured steps after a period's first are one launch
+        per kernel for all solvers."""
+        return self.lib.sphg_batch_staged(self._b) == 1
 
     def close(self):
         if self._b:

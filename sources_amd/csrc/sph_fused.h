@@ -399,10 +399,10 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
 // in memory afterwards as k_final_water followed by k_pack_water(do_half =
 // 1), apart from stores the pack would overwrite.  The record order of step
 // s + 1 must be the current one (neighbour lists kept, no cell-list rebuild).
-__global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int with_vof, int renorm, float* q, Grid g,
-                             double cs, NRef NR)
+__device__ __forceinline__ void step_water(int blk, DPhase P, KParams K, double* rec, int first, int with_vof,
+                                           int renorm, float* q, Grid g, double cs, NRef NR)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
     const WaterEnd e = water_end(P, K, i, first, with_vof, renorm);
@@ -419,6 +419,12 @@ __global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int wi
         }
     }
     pack_water_at(P, K, rec, t, i, 1, q, g, cs, NR, e.x, e.y, e.rho, e.vof, e.vx, e.vy);
+}
+
+__global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int with_vof, int renorm, float* q, Grid g,
+                             double cs, NRef NR)
+{
+    step_water(blockIdx.x, P, K, rec, first, with_vof, renorm, q, g, cs, NR);
 }
 
 // Pack of soil particle i (values as for water, plus stress and pressure):
@@ -507,9 +513,10 @@ __global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int n
 // The soil final update of step s fused with its pack for step s + 1
 // (first_half, two projections), as for water.  The pack reads vof from
 // the state: the final update's vof is stored first.
-__global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, float* q, Grid g, double cs, NRef NR)
+__device__ __forceinline__ void step_soil(int blk, DPhase P, KParams K, double* rec, int first, float* q, Grid g,
+                                          double cs, NRef NR)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
     const SoilEnd e = soil_end(P, K, i, first);
@@ -519,6 +526,11 @@ __global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, float* 
         P.y[i] = e.y;
     }
     pack_soil_at(P, K, rec, t, i, 1, 2, q, g, cs, NR, e.x, e.y, e.rho, e.vx, e.vy, e.p, e.sxx, e.sxy, e.syy);
+}
+
+__global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, float* q, Grid g, double cs, NRef NR)
+{
+    step_soil(blockIdx.x, P, K, rec, first, q, g, cs, NR);
 }
 
 // ------------------------------------------------------------------------
@@ -786,17 +798,21 @@ struct WaveCtx {
     bool valid;
 };
 
-__device__ __forceinline__ WaveCtx wave_ctx(int n)
+// blk: the workgroup's block of the phase (the XCD-aware block of a solo
+// launch, the tile's block of a batched one).
+__device__ __forceinline__ WaveCtx wave_ctx(int n, int blk)
 {
     WaveCtx c;
     c.lane = threadIdx.x & 63;
-    const int t0 = (xcd_block() * SPHG_WPB + (int)(threadIdx.x >> 6)) * 64;
+    const int t0 = (blk * SPHG_WPB + (int)(threadIdx.x >> 6)) * 64;
     c.t = t0 + c.lane;
     c.valid = c.t < n;
     c.last = min(63, n - 1 - t0);
     c.tt = c.valid ? c.t : n - 1;
     return c;
 }
+
+__device__ __forceinline__ WaveCtx wave_ctx(int n) { return wave_ctx(n, xcd_block()); }
 
 // Mask of a particle over its candidate rows in the records rec: scan of the
 // three rows with the predicate P (self: own record index, or -1).
@@ -947,11 +963,11 @@ constexpr int NEED_B = 0x3F | NEED_IV;               // x,y v rho,m vof,p s3,c s
 // twice the occupancy of the same-phase stage.
 // ------------------------------------------------------------------------
 template <int SKF, bool U>
-__global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, const double* recS, NList NL,
-                                      Grid g, KParams K, FusedK F)
+__device__ __forceinline__ void water_x4(int blk, DPhase W, DPhase S, const double* recW, const double* recS,
+                                         NList NL, Grid g, KParams K, FusedK F)
 {
     const RecV RW = recv(recW, W.n), RS = recv(recS, S.n);
-    const WaveCtx wc = wave_ctx(W.n);
+    const WaveCtx wc = wave_ctx(W.n, blk);
     const double2 xy = RW(wc.tt, 0), va = RW(wc.tt, 1), rm = RW(wc.tt, 2);
     const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, vofa = RW(wc.tt, 3).x;
     const double ha = U ? K.h1_water : RW(wc.tt, 6).x;
@@ -993,17 +1009,24 @@ __global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, co
     W.countiac[a] = cntx;
 }
 
+template <int SKF, bool U>
+__global__ STAGE_ATTR void k_water_x4(DPhase W, DPhase S, const double* recW, const double* recS, NList NL, Grid g,
+                                      KParams K, FusedK F)
+{
+    water_x4<SKF, U>(xcd_block(), W, S, recW, recS, NL, g, K, F);
+}
+
 // ------------------------------------------------------------------------
 // Cross terms, soil side: darcy_law's drag on the soil and
 // pore_water_pressure (m_particles.f90:3832-3909) from the water partners
 // (whose pressure is the new one the water pack evaluated).
 // ------------------------------------------------------------------------
 template <int SKF, bool U>
-__global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, const double* recW, NList NL,
-                                     Grid g, KParams K, FusedK F)
+__device__ __forceinline__ void soil_x4(int blk, DPhase S, DPhase W, const double* recS, const double* recW, NList NL,
+                                        Grid g, KParams K, FusedK F)
 {
     const RecV RS = recv(recS, S.n), RW = recv(recW, W.n);
-    const WaveCtx wc = wave_ctx(S.n);
+    const WaveCtx wc = wave_ctx(S.n, blk);
     const double2 xy = RS(wc.tt, 0), va = RS(wc.tt, 1), rm = RS(wc.tt, 2);
     const double xa = xy.x, ya = xy.y, vxa = va.x, vya = va.y, rhoa = rm.x, vofa = RS(wc.tt, 3).x;
     const double ha = U ? K.h1_soil : RS(wc.tt, 6).x;
@@ -1045,6 +1068,13 @@ __global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, con
     S.countiac[a] = cntx;
 }
 
+template <int SKF, bool U>
+__global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, const double* recW, NList NL, Grid g,
+                                     KParams K, FusedK F)
+{
+    soil_x4<SKF, U>(xcd_block(), S, W, recS, recW, NL, g, K, F);
+}
+
 // ------------------------------------------------------------------------
 // Stage A, water: same-phase terms (con_density, delta_sph(rho),
 // shear_strain_rate, av_vel, repulsive_force), then pressure() per particle.
@@ -1053,11 +1083,11 @@ __global__ STAGE_ATTR void k_soil_x4(DPhase S, DPhase W, const double* recS, con
 // soil partners, which k_water_x4 writes).
 // ------------------------------------------------------------------------
 template <int SKF, bool U>
-__global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recWw, NList NL, Masks MW, Grid g,
-                                      KParams K, FusedK F, int cross)
+__device__ __forceinline__ void water_a4(int blk, DPhase W, const double* recW, double* recWw, NList NL, Masks MW,
+                                         Grid g, KParams K, FusedK F, int cross)
 {
     const RecV RW = recv(recW, W.n);
-    const WaveCtx wc = wave_ctx(W.n);
+    const WaveCtx wc = wave_ctx(W.n, blk);
     const int t = wc.t;
     auto RA = [&](int c) { return RW(wc.tt, c); };
     const double2 xy = RA(0), va = RA(1), rm = RA(2), hi = RA(6);
@@ -1130,17 +1160,24 @@ __global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recW
     WP[5 * nw] = make_double2(vofa * (mu * txx), vofa * (mu * txy));
 }
 
+template <int SKF, bool U>
+__global__ STAGE_ATTR void k_water_a4(DPhase W, const double* recW, double* recWw, NList NL, Masks MW, Grid g,
+                                      KParams K, FusedK F, int cross)
+{
+    water_a4<SKF, U>(xcd_block(), W, recW, recWw, NL, MW, g, K, F, cross);
+}
+
 // ------------------------------------------------------------------------
 // Stage B, water: -vof*df(p) + df(vof*sigma) (declarations_sph.f90:1257-1265),
 // art_visc, delta_sph(vof, dvof); /rho, + repulsion, + gravity, - Darcy drag.
 // The partner masks come from stage A.
 // ------------------------------------------------------------------------
 template <int SKF, bool U>
-__global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, NList NL, Masks MW, Grid g, KParams K, FusedK F,
-                                      int mask)
+__device__ __forceinline__ void water_b4(int blk, DPhase W, const double* recW, NList NL, Masks MW, Grid g, KParams K,
+                                         FusedK F, int mask)
 {
     const RecV RW = recv(recW, W.n);
-    const WaveCtx wc = wave_ctx(W.n);
+    const WaveCtx wc = wave_ctx(W.n, blk);
     const int t = wc.t;
     auto RA = [&](int c) { return RW(wc.tt, c); };
     const double2 axy = RA(0), av = RA(1), arm = RA(2), a3 = RA(3), apc = RA(4), a12 = RA(5);
@@ -1217,6 +1254,13 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, NList NL, Ma
     W.dvof[a] = dvof;
 }
 
+template <int SKF, bool U>
+__global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, NList NL, Masks MW, Grid g, KParams K, FusedK F,
+                                      int mask)
+{
+    water_b4<SKF, U>(xcd_block(), W, recW, NL, MW, g, K, F, mask);
+}
+
 // ------------------------------------------------------------------------
 // Soil, stages A and B in one kernel (SOLO: single-phase soil,
 // single_step_for_soil, declarations_sph.f90:1530-1655: the df form of the
@@ -1233,11 +1277,11 @@ __global__ STAGE_ATTR void k_water_b4(DPhase W, const double* recW, NList NL, Ma
 //               (the cross terms k_soil_x4 wrote; cross: two-phase run).
 // ------------------------------------------------------------------------
 template <int SKF, bool U, bool SOLO>
-__global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, const double* recS, NList NL, Grid g, KParams K, FusedK F,
+__device__ __forceinline__ void soil_ab4(int blk, DPhase S, const double* recS, NList NL, Grid g, KParams K, FusedK F,
                                          int cross, int mask)
 {
     const RecV RS = recv(recS, S.n);
-    const WaveCtx wc = wave_ctx(S.n);
+    const WaveCtx wc = wave_ctx(S.n, blk);
     const int t = wc.t;
     auto RA = [&](int c) { return RS(wc.tt, c); };
     const double2 xy = RA(0), va = RA(1), rm = RA(2), a3 = RA(3), a4 = RA(4), a12 = RA(5), hi = RA(6);
@@ -1425,4 +1469,11 @@ __global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, const double* recS, NList NL,
     S.dsxx[a] = dsxx; S.dsxy[a] = dsxy; S.dsyy[a] = dsyy; S.dp[a] = dpv;
     S.ax[a] = ax;
     S.ay[a] = ay;
+}
+
+template <int SKF, bool U, bool SOLO>
+__global__ STAGE_ATTR_AB void k_soil_ab4(DPhase S, const double* recS, NList NL, Grid g, KParams K, FusedK F,
+                                         int cross, int mask)
+{
+    soil_ab4<SKF, U, SOLO>(xcd_block(), S, recS, NL, g, K, F, cross, mask);
 }

@@ -514,9 +514,9 @@ __device__ __forceinline__ WaterEnd water_end(const DPhase& P, const KParams& K,
     return e;
 }
 
-__global__ void k_final_water(DPhase P, KParams K, int first, int with_vof, int renorm)
+__device__ __forceinline__ void final_water(int blk, DPhase P, KParams K, int first, int with_vof, int renorm)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const WaterEnd e = water_end(P, K, i, first, with_vof, renorm);
     if (renorm || !K.summation_density) P.rho[i] = e.rho;
@@ -526,6 +526,11 @@ __global__ void k_final_water(DPhase P, KParams K, int first, int with_vof, int 
     P.vy[i] = e.vy;
     P.x[i] = e.x;
     P.y[i] = e.y;
+}
+
+__global__ void k_final_water(DPhase P, KParams K, int first, int with_vof, int renorm)
+{
+    final_water(blockIdx.x, P, K, first, with_vof, renorm);
 }
 
 // The renormalisation as single_step itself leaves it (declarations_sph.f90:
@@ -660,9 +665,9 @@ __device__ __forceinline__ SoilEnd soil_end(const DPhase& P, const KParams& K, i
     return e;
 }
 
-__global__ void k_final_soil(DPhase P, KParams K, int first)
+__device__ __forceinline__ void final_soil(int blk, DPhase P, KParams K, int first)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const SoilEnd e = soil_end(P, K, i, first);
     if (K.volume_fraction && K.two_phase) P.vof[i] = e.vof;
@@ -675,6 +680,11 @@ __global__ void k_final_soil(DPhase P, KParams K, int first)
     }
     P.fail[i] = e.fail;
     P.sxx[i] = e.sxx; P.sxy[i] = e.sxy; P.syy[i] = e.syy; P.p[i] = e.p;
+}
+
+__global__ void k_final_soil(DPhase P, KParams K, int first)
+{
+    final_soil(blockIdx.x, P, K, first);
 }
 
 // ------------------------------------------------------------------------

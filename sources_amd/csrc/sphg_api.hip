@@ -18,10 +18,22 @@
 #include "sph_fused.h"
 #include "sph_mixed.h"
 #include "sph_gnn.h"
+#include "sph_batch.h"
+
+#include <new>
+#include <unordered_map>
 
 namespace {
 
 constexpr int TPB = 256;
+
+// A launch held back while a batch records a step: the kernel, its grid and
+// block, and its arguments as the bytes of an ArgTup of its parameters.
+struct LaunchRec {
+    const void* kern;
+    int grid, block;
+    std::vector<unsigned char> args;
+};
 
 // Skin of the neighbour lists as a fraction of the largest support radius R:
 // lists hold candidates within R + skin, a particle that has moved skin / 4
@@ -153,6 +165,8 @@ struct sphg_ctx {
     int nl_off_from = 0, nl_off_until = 0;
     int64_t nl_off_spells = 0;
     bool capturing = false;     // steps are being captured into a graph (no host reads)
+    // while set, launch() records the launches here instead of issuing them
+    std::vector<LaunchRec>* recording = nullptr;
     int last_k = 1;             // the last step was of size last_k dt (a learned coarse step: K)
     // the final kernels of the step before also packed this step (first_half
     // applied to the state, records written): the step starts at its stages
@@ -212,6 +226,14 @@ void drain_events(sphg_ctx* c)
 template <typename... KArgs, typename... Args>
 int launch(sphg_ctx* c, const char* name, void (*kern)(KArgs...), int grid, int block, Args... args)
 {
+    if (c->recording) {
+        // (empty grids too: every case of a batch records the same sequence)
+        LaunchRec r{reinterpret_cast<const void*>(kern), grid, block, {}};
+        r.args.assign(sizeof(ArgTup<KArgs...>), 0);
+        new (r.args.data()) ArgTup<KArgs...>(args...);
+        c->recording->push_back(std::move(r));
+        return SPHG_OK;
+    }
     if (grid <= 0) return SPHG_OK;
     hipEvent_t ea = nullptr, eb = nullptr;
     hipStream_t st = c->lstream ? c->lstream : c->stream;
@@ -2628,6 +2650,11 @@ int sphg_gnn_debug(sphg_gnn* g, int sample, float* nf, float* raw, int* nb, int*
 // ---------------------------------------------------------------------------
 struct sphg_batch {
     std::vector<sphg_ctx*> ctx;
+    // One launch per stage for all cases in the steps of a period after its
+    // first (rebuild) step; off when a capture in that form failed.
+    bool staged = true;
+    void* arena = nullptr;              // device: the batched launches' arguments and tile tables
+    size_t arena_cap = 0;
     std::vector<hipStream_t> streams;   // contexts are dealt round-robin, largest first
     std::vector<int> order;             // contexts by decreasing particle count
     hipGraphExec_t exec = nullptr;
@@ -2640,6 +2667,102 @@ struct sphg_batch {
 };
 
 namespace {
+
+// Batched form of a solo kernel: the size of its ArgTup and a launcher of
+// k_batched over a tile table.
+struct Batched {
+    size_t argsz;
+    hipError_t (*go)(hipStream_t, const void*, const int2*, int);
+};
+
+template <class F, class... T>
+hipError_t go_batched(hipStream_t s, const void* a, const int2* t, int nb)
+{
+    hipLaunchKernelGGL((k_batched<F, T...>), dim3(nb), dim3(256), 0, s, static_cast<const ArgTup<T...>*>(a), t);
+    return hipGetLastError();
+}
+
+template <class F, class... T>
+void reg_batched(std::map<const void*, Batched>& m, void (*solo)(T...))
+{
+    m[reinterpret_cast<const void*>(solo)] = Batched{sizeof(ArgTup<T...>), &go_batched<F, T...>};
+}
+
+template <int SKF>
+void reg_batched_skf(std::map<const void*, Batched>& m)
+{
+    reg_batched<BodyWaterX<SKF, true>>(m, k_water_x4<SKF, true>);
+    reg_batched<BodyWaterX<SKF, false>>(m, k_water_x4<SKF, false>);
+    reg_batched<BodySoilX<SKF, true>>(m, k_soil_x4<SKF, true>);
+    reg_batched<BodySoilX<SKF, false>>(m, k_soil_x4<SKF, false>);
+    reg_batched<BodyWaterA<SKF, true>>(m, k_water_a4<SKF, true>);
+    reg_batched<BodyWaterA<SKF, false>>(m, k_water_a4<SKF, false>);
+    reg_batched<BodyWaterB<SKF, true>>(m, k_water_b4<SKF, true>);
+    reg_batched<BodyWaterB<SKF, false>>(m, k_water_b4<SKF, false>);
+    reg_batched<BodySoilAB<SKF, true, false>>(m, k_soil_ab4<SKF, true, false>);
+    reg_batched<BodySoilAB<SKF, false, false>>(m, k_soil_ab4<SKF, false, false>);
+}
+
+// The kernels of a list-mode FP64 step between rebuilds (stages, finals).
+const std::map<const void*, Batched>& batched_registry()
+{
+    static const std::map<const void*, Batched> m = [] {
+        std::map<const void*, Batched> r;
+        reg_batched_skf<1>(r);
+        reg_batched_skf<2>(r);
+        reg_batched_skf<3>(r);
+        reg_batched_skf<4>(r);
+        reg_batched<BodyStepWater>(r, k_step_water);
+        reg_batched<BodyStepSoil>(r, k_step_soil);
+        reg_batched<BodyFinalWater>(r, k_final_water);
+        reg_batched<BodyFinalSoil>(r, k_final_soil);
+        return r;
+    }();
+    return m;
+}
+
+const Batched* batched_of(const void* kern)
+{
+    const auto& m = batched_registry();
+    const auto it = m.find(kern);
+    return it == m.end() ? nullptr : &it->second;
+}
+
+size_t batched_max_argsz()
+{
+    size_t mx = 0;
+    for (const auto& kv : batched_registry()) mx = std::max(mx, kv.second.argsz);
+    return mx;
+}
+
+// Host image of the batch arena: argument arrays and tile tables, each kept
+// once (the arguments of a stage recur every other step: the two sound-speed
+// buffers alternate), 256-byte aligned.
+struct ArenaImage {
+    std::vector<unsigned char> bytes;
+    size_t cap = 0;
+    std::unordered_map<uint64_t, std::vector<size_t>> index;   // content hash -> offsets
+    static uint64_t hash(const unsigned char* p, size_t n)
+    {
+        uint64_t h = 1469598103934665603ull ^ n;   // FNV-1a
+        for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
+        return h;
+    }
+    // offset of the blob, or SIZE_MAX when the arena is full
+    size_t put(const unsigned char* p, size_t n)
+    {
+        const uint64_t h = hash(p, n);
+        auto& offs = index[h];
+        for (size_t o : offs)
+            if (std::memcmp(bytes.data() + o, p, n) == 0) return o;
+        const size_t off = (bytes.size() + 255) & ~size_t(255);
+        if (off + n > cap) return SIZE_MAX;
+        bytes.resize(off + n);
+        std::memcpy(bytes.data() + off, p, n);
+        offs.push_back(off);
+        return off;
+    }
+};
 
 void batch_release_graph(sphg_batch* b)
 {
@@ -2703,8 +2826,131 @@ int batch_align(sphg_ctx* c, int* left)
     return SPHG_OK;
 }
 
-// Capture `period` steps of every context, one stream per group of contexts.
-// While captured, a context's main stream is a batch stream and its side
+// Capture `period` steps of every context in the staged form: the period's
+// first step (the rebuild: cell lists, reorder, packs, neighbour lists) runs
+// context by context over the batch streams; each later step is recorded for
+// every context and issued as one batched launch per kernel of the step on
+// the first stream, over all contexts.  SPHG_ERR_UNSUPPORTED: the contexts'
+// steps do not batch (different kernel sequences or instantiations, a kernel
+// without a batched form, a full arena); the host steps of the period have
+// still run, as after any capture.
+int batch_capture_staged(sphg_batch* b)
+{
+    const size_t n = b->ctx.size();
+    if (!b->arena) {
+        // room for 64 distinct argument arrays and 16 tile tables
+        size_t blocks = 0;
+        for (sphg_ctx* c : b->ctx) blocks += (size_t)nblk(c->ph[0].n, 256) + (size_t)nblk(c->ph[1].n, 256);
+        const size_t cap = 64 * (n * batched_max_argsz() + 256) + 16 * (blocks * sizeof(int2) + 256);
+        if (hipMalloc(&b->arena, cap) != hipSuccess) {
+            b->arena = nullptr;
+            return SPHG_ERR_HIP;
+        }
+        b->arena_cap = cap;
+    }
+    ArenaImage img;
+    img.cap = b->arena_cap;
+    struct Saved {
+        hipStream_t stream;
+        bool side;
+        int it0;
+    };
+    std::vector<Saved> saved(n);
+    for (size_t i = 0; i < n; ++i) saved[i] = {b->ctx[i]->stream, b->ctx[i]->use_side, b->ctx[i]->itimestep};
+    hipStream_t origin = b->streams[0];
+    if (hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPHG_ERR_HIP;
+    int rc = SPHG_OK;
+    bool staged_ok = true;
+    if (hipEventRecord(b->fork, origin) != hipSuccess) rc = SPHG_ERR_HIP;
+    for (int i = 1; i < b->nstreams && rc == SPHG_OK; ++i)
+        if (hipStreamWaitEvent(b->streams[i], b->fork, 0) != hipSuccess) rc = SPHG_ERR_HIP;
+    for (size_t r = 0; r < b->order.size() && rc == SPHG_OK; ++r) {
+        sphg_ctx* c = b->ctx[(size_t)b->order[r]];
+        c->stream = b->streams[r % (size_t)b->nstreams];
+        c->use_side = false;
+        c->capturing = true;
+        rc = one_step(c, true);
+    }
+    for (int i = 1; i < b->nstreams && rc == SPHG_OK; ++i) {
+        if (hipEventRecord(b->join[(size_t)i], b->streams[(size_t)i]) != hipSuccess ||
+            hipStreamWaitEvent(origin, b->join[(size_t)i], 0) != hipSuccess)
+            rc = SPHG_ERR_HIP;
+    }
+    std::vector<std::vector<LaunchRec>> recs(n);
+    std::vector<unsigned char> args;
+    std::vector<int2> tiles;
+    unsigned char* base = static_cast<unsigned char*>(b->arena);
+    for (int s = 1; s < b->period && rc == SPHG_OK; ++s) {
+        // (the host steps run even once the form has failed: the period ends
+        // where a capture's period ends)
+        for (size_t i = 0; i < n && rc == SPHG_OK; ++i) {
+            recs[i].clear();
+            b->ctx[i]->recording = &recs[i];
+            rc = one_step(b->ctx[i], true);
+            b->ctx[i]->recording = nullptr;
+        }
+        if (rc != SPHG_OK || !staged_ok) continue;
+        const size_t nk = recs[0].size();
+        for (size_t i = 1; i < n && staged_ok; ++i) {
+            staged_ok = recs[i].size() == nk;
+            for (size_t p = 0; p < nk && staged_ok; ++p)
+                staged_ok = recs[i][p].kern == recs[0][p].kern && recs[i][p].block == recs[0][p].block;
+        }
+        for (size_t p = 0; p < nk && staged_ok && rc == SPHG_OK; ++p) {
+            const Batched* bt = batched_of(recs[0][p].kern);
+            staged_ok = bt && recs[0][p].block == 256 && recs[0][p].args.size() == bt->argsz;
+            if (!staged_ok) break;
+            args.clear();
+            tiles.clear();
+            for (size_t i = 0; i < n; ++i) {
+                const LaunchRec& lr = recs[i][p];
+                args.insert(args.end(), lr.args.begin(), lr.args.end());
+                for (int k = 0; k < lr.grid; ++k) tiles.push_back(make_int2((int)i, k));
+            }
+            if (tiles.empty()) continue;
+            const size_t oa = img.put(args.data(), args.size());
+            const size_t ot = img.put(reinterpret_cast<const unsigned char*>(tiles.data()), tiles.size() * sizeof(int2));
+            staged_ok = oa != SIZE_MAX && ot != SIZE_MAX;
+            if (staged_ok && bt->go(origin, base + oa, reinterpret_cast<const int2*>(base + ot), (int)tiles.size()) !=
+                                 hipSuccess)
+                rc = SPHG_ERR_HIP;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        sphg_ctx* c = b->ctx[i];
+        c->capturing = false;
+        c->recording = nullptr;
+        c->stream = saved[i].stream;
+        c->use_side = saved[i].side;
+        c->itimestep = saved[i].it0;   // the capture issued no work: the step count advances on replay
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(origin, &g);
+    if (rc != SPHG_OK || !staged_ok || ec != hipSuccess || !g || batch_capturing(b)) {
+        if (g) hipGraphDestroy(g);
+        if (batch_capturing(b)) {
+            const int rs = batch_set_streams(b, b->nstreams);
+            if (rs != SPHG_OK) return rs;
+        }
+        if (rc != SPHG_OK) return rc;
+        return staged_ok ? SPHG_ERR_HIP : SPHG_ERR_UNSUPPORTED;
+    }
+    // the arguments and tile tables the batched launches read (no replay of
+    // an earlier graph is in flight: runs end synchronised)
+    for (hipStream_t st : b->streams) hipStreamSynchronize(st);
+    if (hipMemcpy(b->arena, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipGraphDestroy(g);
+        return SPHG_ERR_HIP;
+    }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) return SPHG_ERR_HIP;
+    return SPHG_OK;
+}
+
+// Capture `period` steps of every context: staged (one launch per kernel for
+// all contexts after the period's first step) while the batch's steps allow
+// it, else one stream per group of contexts.
+// In the second form a context's main stream is a batch stream and its side
 // stream is off: the graph overlaps whole contexts across the streams.  (With
 // each context's side stream in the graph as well, the autotuner's best for
 // the 64-case sweep was 8.6 ms per step, on one stream, against 3.8 ms
@@ -2712,6 +2958,11 @@ int batch_align(sphg_ctx* c, int* left)
 int batch_capture(sphg_batch* b)
 {
     batch_release_graph(b);
+    if (b->staged) {
+        const int rs = batch_capture_staged(b);
+        if (rs != SPHG_ERR_UNSUPPORTED) return rs;
+        b->staged = false;   // this batch's steps do not batch: the second form from here on
+    }
     hipStream_t origin = b->streams[0];
     if (hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPHG_ERR_HIP;
     int rc = SPHG_OK;
@@ -2798,6 +3049,9 @@ int sphg_batch_create(sphg_ctx** ctxs, int n, sphg_batch** out)
         }
     }
     for (sphg_ctx* c : b->ctx) c->stable_ptrs = true;
+    // SPHG_BATCH_STAGED=0: per-context launches in the captured steps too
+    const char* st = std::getenv("SPHG_BATCH_STAGED");
+    b->staged = !(st && std::atoi(st) == 0);
     if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
         delete b;
         return SPHG_ERR_HIP;
@@ -2815,10 +3069,13 @@ void sphg_batch_destroy(sphg_batch* b)
 {
     if (!b) return;
     batch_free_streams(b);
+    if (b->arena) hipFree(b->arena);
     if (b->fork) hipEventDestroy(b->fork);
     for (sphg_ctx* c : b->ctx) c->stable_ptrs = false;
     delete b;
 }
+
+int sphg_batch_staged(const sphg_batch* b) { return b ? (b->staged ? 1 : 0) : -1; }
 
 int sphg_batch_set_streams(sphg_batch* b, int nstreams)
 {

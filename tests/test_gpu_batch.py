@@ -1,6 +1,8 @@
 """Batched scenarios: contexts advanced by replays of one captured hipGraph
-(per period: the neighbour-list rebuild period of FP64 contexts, several
-streams) give bit-identical states to the same contexts stepped alone."""
+(per period: the neighbour-list rebuild period of FP64 contexts; staged: the
+period's first step context by context over several streams, every later
+step one launch per kernel for all contexts; or every step context by
+context) give bit-identical states to the same contexts stepped alone."""
 import numpy as np
 import pytest
 
@@ -17,10 +19,12 @@ def cases():
     return scenarios.sweep(4, seed=5, nmin=2000, nmax=8000)
 
 
-@pytest.mark.parametrize("nsteps", [40, 140])
-def test_batch_is_bit_identical_to_solo(nsteps):
+@pytest.mark.parametrize("nsteps,staged", [(40, "1"), (140, "1"), (140, "0")])
+def test_batch_is_bit_identical_to_solo(nsteps, staged, monkeypatch):
     # 40 steps: eager only; 140: eager to step 64, one replayed period of 64
-    # steps (65-128, starting with the list rebuild), eager remainder
+    # steps (65-128, starting with the list rebuild), eager remainder; the
+    # captured period staged or context by context
+    monkeypatch.setenv("SPHG_BATCH_STAGED", staged)
     solo = cases()
     bat = cases()
     print("particles", [c.water.n + c.soil.n for c in solo], flush=True)
@@ -30,6 +34,7 @@ def test_batch_is_bit_identical_to_solo(nsteps):
     sb = [Solver(c.params, c.water, c.soil) for c in bat]
     b = Batch(sb)
     b.run(nsteps)
+    assert b.staged == (staged == "1")
     for s in sb:
         s.sync_all()
     for i, (a, c) in enumerate(zip(solo, bat)):
@@ -50,8 +55,9 @@ def test_batch_is_bit_identical_to_solo(nsteps):
 def test_full_sweep_is_bit_identical_to_solo(monkeypatch):
     """The whole 64-case sweep (N ~ logU[5k, 120k], 2.41M particles, seed 11):
     33 steps batched (eager to the period boundary, two captured periods of
-    16 steps replayed, eager remainder) equal the same 33 steps of each case
-    alone (both with the neighbour lists rebuilt every 16 steps)."""
+    16 steps replayed in the staged form, eager remainder) equal the same 33
+    steps of each case alone (both with the neighbour lists rebuilt every 16
+    steps)."""
     monkeypatch.setenv("SPHG_NL_EVERY", "16")
     nsteps = 33
     solo = scenarios.sweep(64, seed=11)
@@ -64,6 +70,7 @@ def test_full_sweep_is_bit_identical_to_solo(monkeypatch):
     sb = [Solver(c.params, c.water, c.soil) for c in bat]
     b = Batch(sb)
     b.run(nsteps)
+    assert b.staged
     for s in sb:
         s.sync_all()
     for i, (a, c) in enumerate(zip(solo, bat)):
