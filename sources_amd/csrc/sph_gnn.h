@@ -81,6 +81,30 @@ __device__ __forceinline__ f32x16 mma(FA&& a, FB&& b, int K, f32x16 acc)
     return acc;
 }
 
+// Weights of the node MLP as BF16 B operands for inference: for each matrix,
+// n-tile and 16-row k-step, 64 lanes x 8 values (lane l: column l & 31 of
+// the tile, rows k0 + (l >> 5) * 8 + 0..7) -- the values mma() rounds from the
+// FP32 weights, one 16-byte load per lane and step.
+constexpr int P_W2 = 0;                // HE x HE: 1 tile x 2 steps
+constexpr int P_W3 = P_W2 + 2 * 64;    // FI x D: 2 tiles x 4 steps
+constexpr int P_W4 = P_W3 + 8 * 64;    // D x D: 2 tiles x 4 steps
+constexpr int P_W5 = P_W4 + 8 * 64;    // D x NOP: 1 tile x 4 steps
+constexpr int NPACK = P_W5 + 4 * 64;
+
+// acc += A (32 x K) Bp, Bp: the packed operands of one n-tile (K / 16 steps).
+template <class FA>
+__device__ __forceinline__ f32x16 mma_p(FA&& a, const bf16x8* bp, int K, f32x16 acc)
+{
+    const int l = lane_id(), r = l & 31, h = l >> 5;
+    for (int k0 = 0, ks = 0; k0 < K; k0 += 16, ++ks) {
+        bf16x8 av;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) av[i] = (__bf16)a(r, k0 + h * 8 + i);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bp[ks * 64 + l], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 // f(row, col, value&) over the 16 results a lane holds.
 template <class F>
 __device__ __forceinline__ void tile_each(f32x16& c, F&& f)
@@ -147,6 +171,7 @@ __device__ __forceinline__ void edge_features(const float* ri, const float* rj, 
 struct GnnArgs {
     Graph G;
     const float* W;      // [NPARAM]
+    const bf16x8* Wp;    // [NPACK] packed node-MLP weights (inference)
     float* out;          // [n][NO] (forward) or nullptr
     // training
     const float* tgt;    // [n][NO] targets (normalised)
@@ -189,6 +214,12 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
     const int i = base + p;
     const bool valid = i < A.G.n;
     const float* W = A.W;
+    // acc = A x (n-tile nt of the weight matrix at o, ld columns): packed
+    // operands for inference, FP32 weights rounded in mma() for training
+    auto mmw = [&](auto&& a, int o, int ld, int pbase, int nt, int K) {
+        if constexpr (!TRAIN) return mma_p(a, A.Wp + pbase + nt * (K / 16) * 64, K, zero16());
+        else return mma(a, [&](int k, int c) { return W[o + k * ld + nt * 32 + c]; }, K, zero16());
+    };
     if (h == 0) {
 #pragma unroll
         for (int f = 0; f < FN; ++f) {
@@ -209,9 +240,23 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
 
     // neighbour index of slot s (slot-major table), -1 past the degree
     auto nbr = [&](int s) { return (s < deg) ? A.G.nb[(size_t)s * A.G.n + i] : -1; };
+    // The two half-waves build different features of the same edge: h = 0
+    // (features 0-7) reads raw 0-4 of the neighbour (position and three field
+    // values), h = 1 (features 8-15) raw 5, 6 and 8.  Each half loads one
+    // aligned float4 (raw 0-3, or 4-7) and one float (raw 4, or 8); the
+    // values the other half's features would read stay zero.
     auto load_raw = [&](int j, float* rj) {
 #pragma unroll
-        for (int f = 0; f < RAW; ++f) rj[f] = (j >= 0) ? A.G.raw[(size_t)j * RAW + f] : 0.0f;
+        for (int f = 0; f < RAW; ++f) rj[f] = 0.0f;
+        if (j < 0) return;
+        const float* row = A.G.raw + (size_t)j * RAW;
+        const float4 q = reinterpret_cast<const float4*>(row)[h];
+        const float w = row[4 + 4 * h];
+        if (h == 0) {
+            rj[0] = q.x; rj[1] = q.y; rj[2] = q.z; rj[3] = q.w; rj[4] = w;
+        } else {
+            rj[4] = q.x; rj[5] = q.y; rj[6] = q.z; rj[7] = q.w; rj[8] = w;
+        }
     };
     auto edge_from = [&](int j, const float* rj) {
         float e[FE];
@@ -243,16 +288,14 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
     __syncthreads();
     // ---- message
     {
-        f32x16 m = mma([&](int r, int k) { return HS[r][k]; }, [&](int k, int c) { return W[O_W2 + k * HE + c]; },
-                       HE, zero16());
+        f32x16 m = mmw([&](int r, int k) { return HS[r][k]; }, O_W2, HE, P_W2, 0, HE);
         tile_each(m, [&](int row, int col, float& v) { X0[row][FN + col] = v; });
     }
     __syncthreads();
     // ---- node MLP
 #pragma unroll 1
     for (int nt = 0; nt < 2; ++nt) {
-        f32x16 z = mma([&](int r, int k) { return X0[r][k]; },
-                       [&](int k, int c) { return W[O_W3 + k * D + nt * 32 + c]; }, FI, zero16());
+        f32x16 z = mmw([&](int r, int k) { return X0[r][k]; }, O_W3, D, P_W3, nt, FI);
         tile_each(z, [&](int row, int col, float& v) {
             H3[row][nt * 32 + col] = fmaxf(v + W[O_B3 + nt * 32 + col], 0.0f);
         });
@@ -260,15 +303,13 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
     __syncthreads();
 #pragma unroll 1
     for (int nt = 0; nt < 2; ++nt) {
-        f32x16 z = mma([&](int r, int k) { return H3[r][k]; },
-                       [&](int k, int c) { return W[O_W4 + k * D + nt * 32 + c]; }, D, zero16());
+        f32x16 z = mmw([&](int r, int k) { return H3[r][k]; }, O_W4, D, P_W4, nt, D);
         tile_each(z, [&](int row, int col, float& v) {
             H4[row][nt * 32 + col] = fmaxf(v + W[O_B4 + nt * 32 + col], 0.0f);
         });
     }
     __syncthreads();
-    f32x16 y = mma([&](int r, int k) { return H4[r][k]; }, [&](int k, int c) { return W[O_W5 + k * NOP + c]; }, D,
-                   zero16());
+    f32x16 y = mmw([&](int r, int k) { return H4[r][k]; }, O_W5, NOP, P_W5, 0, D);
     if constexpr (!TRAIN) {
         tile_each(y, [&](int row, int col, float& v) {
             if (col < NO && base + row < A.G.n) A.out[(size_t)(base + row) * NO + col] = v + W[O_B5 + col];
@@ -408,6 +449,24 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master weights.
+// Packed weights (see P_W2): one entry of 8 BF16 values per thread.
+__global__ void k_gnn_pack(const float* W, bf16x8* Wp)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NPACK) return;
+    int o, ld, base, steps;
+    if (e < P_W3) { o = O_W2; ld = HE; base = P_W2; steps = 2; }
+    else if (e < P_W4) { o = O_W3; ld = D; base = P_W3; steps = 4; }
+    else if (e < P_W5) { o = O_W4; ld = D; base = P_W4; steps = 4; }
+    else { o = O_W5; ld = NOP; base = P_W5; steps = 4; }
+    const int q = e - base, l = q & 63, ks = (q >> 6) % steps, nt = (q >> 6) / steps;
+    const int r = l & 31, h = l >> 5;
+    bf16x8 v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (__bf16)W[o + (ks * 16 + h * 8 + i) * ld + nt * 32 + r];
+    Wp[e] = v;
+}
+
 __global__ void k_adam(float* W, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                        float c1, float c2, float clip)
 {

@@ -1989,6 +1989,8 @@ struct sphg_gnn {
     int maxs = 0;
     std::vector<void*> allocs;
     float *W = nullptr, *gW = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;
+    gnn::bf16x8* Wp = nullptr;      // node-MLP weights packed for inference
+    bool wp_stale = true;           // W changed since Wp was packed
     int adam_t = 0;
     struct Graph {
         int n = 0, nw = 0;
@@ -2153,9 +2155,14 @@ int build_graph(sphg_gnn* g, sphg_gnn::Graph& G)
 
 int forward(sphg_gnn* g, const sphg_gnn::Graph& G)
 {
+    if (g->wp_stale) {
+        RC(launch(g->c, "gnn_pack", gnn::k_gnn_pack, nblk(gnn::NPACK), TPB, (const float*)g->W, g->Wp));
+        g->wp_stale = false;
+    }
     gnn::GnnArgs A{};
     A.G = gnn::Graph{G.n, G.nf, G.raw, G.nb, G.nd};
     A.W = g->W;
+    A.Wp = g->Wp;
     A.out = g->y;
     for (int o = 0; o < gnn::NO; ++o) A.inv_out[o] = 1.0f / g->S.out[o];
     return launch(g->c, "gnn_fwd", gnn::k_gnn<false>, (G.n + 31) / 32, 64, A);
@@ -2195,7 +2202,7 @@ int sphg_gnn_create(sphg_ctx* c, int ksteps, int max_samples, uint64_t seed, sph
     int rc;
     if ((rc = galloc(g, &g->W, gnn::NPARAM)) || (rc = galloc(g, &g->gW, gnn::NPARAM)) ||
         (rc = galloc(g, &g->m, gnn::NPARAM)) || (rc = galloc(g, &g->v, gnn::NPARAM)) ||
-        (rc = galloc(g, &g->loss, 1)) || 
+        (rc = galloc(g, &g->loss, 1)) || (rc = galloc(g, &g->Wp, gnn::NPACK)) ||
         (rc = galloc(g, &g->y, (size_t)n * gnn::NO)) || (rc = galloc(g, &g->overflow, 1)) ||
         (rc = galloc(g, &g->err, 1)) || (rc = galloc(g, &g->errsq, 2)) || (rc = graph_alloc(g, g->cur, n)) ||
         (rc = galloc(g, &g->broken, 1)) || (rc = galloc(g, &g->fit, 2 * gnn::NO)))
@@ -2271,7 +2278,10 @@ int sphg_gnn_weights(sphg_gnn* g, float* host, int to_host)
     sphg_ctx* c = g->c;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (to_host) HIPCHK(c, hipMemcpy(host, g->W, sizeof(float) * gnn::NPARAM, hipMemcpyDeviceToHost));
-    else HIPCHK(c, hipMemcpy(g->W, host, sizeof(float) * gnn::NPARAM, hipMemcpyHostToDevice));
+    else {
+        HIPCHK(c, hipMemcpy(g->W, host, sizeof(float) * gnn::NPARAM, hipMemcpyHostToDevice));
+        g->wp_stale = true;
+    }
     return SPHG_OK;
 }
 
@@ -2384,6 +2394,7 @@ int sphg_gnn_train(sphg_gnn* g, int iters, float lr, float* loss_out)
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f / (1.0f - std::pow(b1, (float)g->adam_t));
         const float c2 = 1.0f / (1.0f - std::pow(b2, (float)g->adam_t));
+        g->wp_stale = true;
         RC(launch(c, "adam", gnn::k_adam, nblk(gnn::NPARAM), TPB, g->W, (const float*)g->gW, g->m, g->v,
                   gnn::NPARAM, lr, b1, b2, 1e-8f, c1, c2, 10.0f));
         if (it + ns >= iters) {   // mean loss over the last pass through the samples
