@@ -96,3 +96,23 @@ struct BodyFinalSoil {
     template <class... A>
     __device__ __forceinline__ void operator()(int blk, const A&... a) const { final_soil(blk, a...); }
 };
+
+// ... and of the kernels of its rebuild step (cell lists, reorder, packs,
+// neighbour lists).
+#define SPHG_BATCH_BODY(NAME, FN)                                                        \
+    struct NAME {                                                                        \
+        template <class... A>                                                            \
+        __device__ __forceinline__ void operator()(int blk, const A&... a) const { FN(blk, a...); } \
+    };
+SPHG_BATCH_BODY(BodyFillInt, fill_int_body)
+SPHG_BATCH_BODY(BodyKeepBits, keep_bits_body)
+SPHG_BATCH_BODY(BodyIota, iota_body)
+SPHG_BATCH_BODY(BodyPermute, permute_multi_body)
+SPHG_BATCH_BODY(BodyScanTile, scan_tile_body)
+SPHG_BATCH_BODY(BodyCellKey, cell_key_body)
+SPHG_BATCH_BODY(BodyCellScatter, cell_scatter_t_body)
+SPHG_BATCH_BODY(BodyCellSort, cell_sort_t_body)
+SPHG_BATCH_BODY(BodyPackWater, pack_water_body)
+SPHG_BATCH_BODY(BodyPackSoil, pack_soil_body)
+SPHG_BATCH_BODY(BodyNlist, nlist_body)
+#undef SPHG_BATCH_BODY

@@ -386,12 +386,18 @@ __device__ __forceinline__ void pack_water_at(const DPhase& P, const KParams& K,
     R[6 * pn] = make_double2(P.hsml[i], (double)it);
 }
 
-__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs, NRef NR)
+__device__ __forceinline__ void pack_water_body(int blk, DPhase P, KParams K, double* rec, int do_half, float* q,
+                                                Grid g, double cs, NRef NR)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
     pack_water_at(P, K, rec, t, i, do_half, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vof[i], P.vx[i], P.vy[i]);
+}
+
+__global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs, NRef NR)
+{
+    pack_water_body(blockIdx.x, P, K, rec, do_half, q, g, cs, NR);
 }
 
 // The final update of step s fused with the pack of step s + 1 (first_half,
@@ -500,14 +506,20 @@ __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, 
     if (K.yield_criterion == 1) R[7 * pn] = make_double2(rho_pre, 0.0);
 }
 
-__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs,
-                            NRef NR)
+__device__ __forceinline__ void pack_soil_body(int blk, DPhase P, KParams K, double* rec, int do_half, int ndp,
+                                               float* q, Grid g, double cs, NRef NR)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
     pack_soil_at(P, K, rec, t, i, do_half, ndp, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vx[i], P.vy[i], P.p[i],
                  P.sxx[i], P.sxy[i], P.syy[i]);
+}
+
+__global__ void k_pack_soil(DPhase P, KParams K, double* rec, int do_half, int ndp, float* q, Grid g, double cs,
+                            NRef NR)
+{
+    pack_soil_body(blockIdx.x, P, K, rec, do_half, ndp, q, g, cs, NR);
 }
 
 // The soil final update of step s fused with its pack for step s + 1
@@ -912,11 +924,11 @@ __device__ __forceinline__ CatMask cand_mask(const LaneRows& L, const Masks& M, 
 // (R + skin)^2 of the pair type), the particle itself excluded when same.
 // Also records each particle's cell key.  Positions come from the state, so
 // the FP64 and the FP32 records share the lists.
-__global__ __launch_bounds__(256) void k_nlist(const double* xA, const double* yA, const int* ordA, int nA,
-                                               const double* xB, const double* yB, const int* ordB,
-                                               const int* startB, Grid g, double thr, int same, Masks VM, int* cell)
+__device__ __forceinline__ void nlist_body(int blk, const double* xA, const double* yA, const int* ordA, int nA,
+                                           const double* xB, const double* yB, const int* ordB, const int* startB,
+                                           Grid g, double thr, int same, Masks VM, int* cell)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= nA) return;
     const int ia = ordA ? ordA[t] : t;
     const double px = xA[ia], py = yA[ia];
@@ -946,6 +958,13 @@ __global__ __launch_bounds__(256) void k_nlist(const double* xA, const double* y
     VM.m[t] = cm.lo;
     VM.m[nA + t] = cm.hi;
     VM.ovf[t] = cm.ovf;
+}
+
+__global__ __launch_bounds__(256) void k_nlist(const double* xA, const double* yA, const int* ordA, int nA,
+                                               const double* xB, const double* yB, const int* ordB, const int* startB,
+                                               Grid g, double thr, int same, Masks VM, int* cell)
+{
+    nlist_body(xcd_block(), xA, yA, ordA, nA, xB, yB, ordB, startB, g, thr, same, VM, cell);
 }
 
 // Chunks read per partner by each loop (h only when it is not uniform).

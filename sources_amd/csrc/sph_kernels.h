@@ -84,16 +84,26 @@ __device__ __forceinline__ int xcd_block()
 // ------------------------------------------------------------------------
 // utility kernels
 // ------------------------------------------------------------------------
-__global__ void k_fill_int(int* a, int n, int v)
+__device__ __forceinline__ void fill_int_body(int blk, int* a, int n, int v)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
 }
 
+__global__ void k_fill_int(int* a, int n, int v)
+{
+    fill_int_body(blockIdx.x, a, n, v);
+}
+
 // a[0] &= keep (one thread).
+__device__ __forceinline__ void keep_bits_body(int blk, int* a, int keep)
+{
+    if (blk == 0 && threadIdx.x == 0) a[0] &= keep;
+}
+
 __global__ void k_keep_bits(int* a, int keep)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) a[0] &= keep;
+    keep_bits_body(blockIdx.x, a, keep);
 }
 
 __global__ void k_fill_dbl(double* a, int n, double v)
@@ -102,10 +112,15 @@ __global__ void k_fill_dbl(double* a, int n, double v)
     if (i < n) a[i] = v;
 }
 
+__device__ __forceinline__ void iota_body(int blk, int* a, int n)
+{
+    int i = blk * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
 __global__ void k_iota(int* a, int n)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = i;
+    iota_body(blockIdx.x, a, n);
 }
 
 // dst = src (element-wise copy).
@@ -136,9 +151,9 @@ struct PermArgs {
     int nd, ni;
 };
 
-__global__ void k_permute_multi(PermArgs A, const int* idx, int n)
+__device__ __forceinline__ void permute_multi_body(int blk, PermArgs A, const int* idx, int n)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int s = idx ? idx[t] : t;
 #pragma unroll
@@ -147,6 +162,11 @@ __global__ void k_permute_multi(PermArgs A, const int* idx, int n)
 #pragma unroll
     for (int a = 0; a < PERM_MAXI; ++a)
         if (a < A.ni) A.di[a][t] = A.si[a][s];
+}
+
+__global__ void k_permute_multi(PermArgs A, const int* idx, int n)
+{
+    permute_multi_body(blockIdx.x, A, idx, n);
 }
 
 // dst[idx[t]] = src[t]: device -> host order.
@@ -179,10 +199,10 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total)
 
 // clear (optional, the same array as in): zeroed after it is read, so that a
 // counting pass can start from zeros without a fill.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile(const int* in, int* out, int n, int* tile_sums, int* clear)
+__device__ __forceinline__ void scan_tile_body(int blk, const int* in, int* out, int n, int* tile_sums, int* clear)
 {
     __shared__ int lds[SCAN_BLOCK];
-    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const int base = blk * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
@@ -201,7 +221,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile(const int* in, int* ou
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i)
         if (base + i < n) out[base + i] = pre + v[i];
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+    if (threadIdx.x == 0) tile_sums[blk] = total;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile(const int* in, int* out, int n, int* tile_sums, int* clear)
+{
+    scan_tile_body(blockIdx.x, in, out, n, tile_sums, clear);
 }
 
 // Single block: exclusive scan of the tile sums (ntiles <= SCAN_TILE), total at [ntiles].
@@ -235,9 +260,9 @@ __global__ void k_scan_add(int* out, int n, const int* tile_sums, int ntiles)
 // ------------------------------------------------------------------------
 // Cell list: key + rank (counting sort), scatter, per-cell ordering.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_cell_key(DPhase P, Grid g, int* err)
+__device__ __forceinline__ void cell_key_body(int blk, DPhase P, Grid g, int* err)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * blockDim.x + threadIdx.x;
     const bool valid = i < P.n;
     int key = -1;
     if (valid) {
@@ -270,6 +295,11 @@ __global__ __launch_bounds__(256) void k_cell_key(DPhase P, Grid g, int* err)
     if (valid) P.slot[i] = base + (lane - start);
 }
 
+__global__ __launch_bounds__(256) void k_cell_key(DPhase P, Grid g, int* err)
+{
+    cell_key_body(blockIdx.x, P, g, err);
+}
+
 // Exclusive prefix of the tile sums of k_scan_tile (ntiles <= SCAN_TILE)
 // into LDS tp[0..ntiles], by the whole workgroup (SCAN_BLOCK threads).
 __device__ __forceinline__ void tile_prefix(const int* tile_sums, int ntiles, int* tp, int* lds)
@@ -295,29 +325,35 @@ __device__ __forceinline__ void tile_prefix(const int* tile_sums, int ntiles, in
 // Cell-list scatter on the tile-local prefix of the counts (part, from
 // k_scan_tile) plus the prefix of the tile sums, computed per workgroup:
 // the three-pass scan becomes one pass.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_cell_scatter_t(DPhase P, const int* part, const int* tile_sums,
-                                                               int ntiles)
+__device__ __forceinline__ void cell_scatter_t_body(int blk, DPhase P, const int* part, const int* tile_sums,
+                                                    int ntiles)
 {
     __shared__ int tp[SCAN_TILE + 1];
     __shared__ int lds[SCAN_BLOCK];
     tile_prefix(tile_sums, ntiles, tp, lds);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const int key = P.key[i];
     P.order[part[key] + tp[key / SCAN_TILE] + P.slot[i]] = i;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_cell_scatter_t(DPhase P, const int* part, const int* tile_sums,
+                                                               int ntiles)
+{
+    cell_scatter_t_body(blockIdx.x, P, part, tile_sums, ntiles);
 }
 
 // Ascending particle number inside each cell (insertion sort; the ranks of
 // k_cell_key depend on atomic arrival order), which makes the neighbour
 // visiting order and so every floating-point sum deterministic; and the
 // final cell_start from the tile-local prefix and the tile sums.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_cell_sort_t(DPhase P, const int* part, const int* tile_sums,
-                                                            int ntiles, int ncell)
+__device__ __forceinline__ void cell_sort_t_body(int blk, DPhase P, const int* part, const int* tile_sums, int ntiles,
+                                                 int ncell)
 {
     __shared__ int tp[SCAN_TILE + 1];
     __shared__ int lds[SCAN_BLOCK];
     tile_prefix(tile_sums, ntiles, tp, lds);
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blk * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const int s = part[c] + tp[c / SCAN_TILE];
     const int e = c + 1 < ncell ? part[c + 1] + tp[(c + 1) / SCAN_TILE] : tp[ntiles];
@@ -353,6 +389,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_cell_sort_t(DPhase P, const int*
         }
         P.order[m + 1] = v;
     }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_cell_sort_t(DPhase P, const int* part, const int* tile_sums,
+                                                            int ntiles, int ncell)
+{
+    cell_sort_t_body(blockIdx.x, P, part, tile_sums, ntiles, ncell);
 }
 
 

@@ -28,12 +28,30 @@ namespace {
 constexpr int TPB = 256;
 
 // A launch held back while a batch records a step: the kernel, its grid and
-// block, and its arguments as the bytes of an ArgTup of its parameters.
+// block, its arguments as the bytes of an ArgTup of its parameters, and a
+// launcher of the kernel itself from those bytes.
 struct LaunchRec {
     const void* kern;
     int grid, block;
     std::vector<unsigned char> args;
+    hipError_t (*solo)(const void*, int, int, hipStream_t, const unsigned char*);
 };
+
+template <class... T, size_t... I>
+hipError_t solo_launch_at(void (*kern)(T...), int grid, int block, hipStream_t s, const ArgTup<T...>& a,
+                          std::index_sequence<I...>)
+{
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, tup_get<I>(a)...);
+    return hipGetLastError();
+}
+
+template <class... T>
+hipError_t solo_launch(const void* kern, int grid, int block, hipStream_t s, const unsigned char* args)
+{
+    if (grid <= 0) return hipSuccess;
+    return solo_launch_at(reinterpret_cast<void (*)(T...)>(const_cast<void*>(kern)), grid, block, s,
+                          *reinterpret_cast<const ArgTup<T...>*>(args), std::index_sequence_for<T...>{});
+}
 
 // Skin of the neighbour lists as a fraction of the largest support radius R:
 // lists hold candidates within R + skin, a particle that has moved skin / 4
@@ -228,7 +246,7 @@ int launch(sphg_ctx* c, const char* name, void (*kern)(KArgs...), int grid, int 
 {
     if (c->recording) {
         // (empty grids too: every case of a batch records the same sequence)
-        LaunchRec r{reinterpret_cast<const void*>(kern), grid, block, {}};
+        LaunchRec r{reinterpret_cast<const void*>(kern), grid, block, {}, &solo_launch<KArgs...>};
         r.args.assign(sizeof(ArgTup<KArgs...>), 0);
         new (r.args.data()) ArgTup<KArgs...>(args...);
         c->recording->push_back(std::move(r));
@@ -1185,7 +1203,7 @@ int build_nlists(sphg_ctx* c)
     const DPhase& sd = S.d;
     // the displacement trigger restarts (a past-the-skin bit stays for the check);
     // the rebuild has just permuted the state into cell order (no order lookups)
-    RC(launch(c, "nlist", k_keep_bits, 1, 1, c->d_nlflag, 2));
+    RC(launch(c, "nlist", k_keep_bits, 1, TPB, c->d_nlflag, 2));
     RC(launch(c, "nlist", k_nlist, nblk(W.n, 256), 256, (const double*)w.x, (const double*)w.y,
               (const int*)nullptr, W.n, (const double*)w.x, (const double*)w.y, (const int*)nullptr,
               (const int*)w.cell_start, c->gv, c->nl_thr[0], 1, W.nl_same, W.nl_cell));
@@ -2727,6 +2745,17 @@ const std::map<const void*, Batched>& batched_registry()
         reg_batched<BodyStepSoil>(r, k_step_soil);
         reg_batched<BodyFinalWater>(r, k_final_water);
         reg_batched<BodyFinalSoil>(r, k_final_soil);
+        reg_batched<BodyFillInt>(r, k_fill_int);
+        reg_batched<BodyKeepBits>(r, k_keep_bits);
+        reg_batched<BodyIota>(r, k_iota);
+        reg_batched<BodyPermute>(r, k_permute_multi);
+        reg_batched<BodyScanTile>(r, k_scan_tile);
+        reg_batched<BodyCellKey>(r, k_cell_key);
+        reg_batched<BodyCellScatter>(r, k_cell_scatter_t);
+        reg_batched<BodyCellSort>(r, k_cell_sort_t);
+        reg_batched<BodyPackWater>(r, k_pack_water);
+        reg_batched<BodyPackSoil>(r, k_pack_soil);
+        reg_batched<BodyNlist>(r, k_nlist);
         return r;
     }();
     return m;
@@ -2837,22 +2866,25 @@ int batch_align(sphg_ctx* c, int* left)
     return SPHG_OK;
 }
 
-// Capture `period` steps of every context in the staged form: the period's
-// first step (the rebuild: cell lists, reorder, packs, neighbour lists) runs
-// context by context over the batch streams; each later step is recorded for
-// every context and issued as one batched launch per kernel of the step on
-// the first stream, over all contexts.  SPHG_ERR_UNSUPPORTED: the contexts'
-// steps do not batch (different kernel sequences or instantiations, a kernel
-// without a batched form, a full arena); the host steps of the period have
+// Capture `period` steps of every context in the staged form: each step is
+// recorded for every context (the host runs the step as usual, the launches
+// are held back) and issued as one batched launch per kernel of the step on
+// the first stream, over all contexts.  A kernel without a batched form runs
+// context by context there; a step whose kernel sequence differs between
+// contexts is replayed context by context over the batch streams.
+// SPHG_ERR_UNSUPPORTED: the arena is full; the host steps of the period have
 // still run, as after any capture.
 int batch_capture_staged(sphg_batch* b)
 {
     const size_t n = b->ctx.size();
     if (!b->arena) {
-        // room for 64 distinct argument arrays and 16 tile tables
+        // room for 96 distinct argument arrays and 32 tile tables
         size_t blocks = 0;
-        for (sphg_ctx* c : b->ctx) blocks += (size_t)nblk(c->ph[0].n, 256) + (size_t)nblk(c->ph[1].n, 256);
-        const size_t cap = 64 * (n * batched_max_argsz() + 256) + 16 * (blocks * sizeof(int2) + 256);
+        for (sphg_ctx* c : b->ctx) {
+            blocks += (size_t)nblk(c->ph[0].n, 256) + (size_t)nblk(c->ph[1].n, 256);
+            blocks += 2 * (size_t)nblk(std::max(c->ncell, c->ncell_v), 256);
+        }
+        const size_t cap = 96 * (n * batched_max_argsz() + 256) + 32 * (blocks * sizeof(int2) + 256);
         if (hipMalloc(&b->arena, cap) != hipSuccess) {
             b->arena = nullptr;
             return SPHG_ERR_HIP;
@@ -2871,28 +2903,17 @@ int batch_capture_staged(sphg_batch* b)
     hipStream_t origin = b->streams[0];
     if (hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPHG_ERR_HIP;
     int rc = SPHG_OK;
-    bool staged_ok = true;
-    if (hipEventRecord(b->fork, origin) != hipSuccess) rc = SPHG_ERR_HIP;
-    for (int i = 1; i < b->nstreams && rc == SPHG_OK; ++i)
-        if (hipStreamWaitEvent(b->streams[i], b->fork, 0) != hipSuccess) rc = SPHG_ERR_HIP;
-    for (size_t r = 0; r < b->order.size() && rc == SPHG_OK; ++r) {
-        sphg_ctx* c = b->ctx[(size_t)b->order[r]];
-        c->stream = b->streams[r % (size_t)b->nstreams];
+    bool room = true;
+    for (sphg_ctx* c : b->ctx) {
         c->use_side = false;
         c->capturing = true;
-        rc = one_step(c, true);
-    }
-    for (int i = 1; i < b->nstreams && rc == SPHG_OK; ++i) {
-        if (hipEventRecord(b->join[(size_t)i], b->streams[(size_t)i]) != hipSuccess ||
-            hipStreamWaitEvent(origin, b->join[(size_t)i], 0) != hipSuccess)
-            rc = SPHG_ERR_HIP;
     }
     std::vector<std::vector<LaunchRec>> recs(n);
     std::vector<unsigned char> args;
     std::vector<int2> tiles;
     unsigned char* base = static_cast<unsigned char*>(b->arena);
-    for (int s = 1; s < b->period && rc == SPHG_OK; ++s) {
-        // (the host steps run even once the form has failed: the period ends
+    for (int s = 0; s < b->period && rc == SPHG_OK; ++s) {
+        // (the host steps run even once the arena is full: the period ends
         // where a capture's period ends)
         for (size_t i = 0; i < n && rc == SPHG_OK; ++i) {
             recs[i].clear();
@@ -2900,17 +2921,42 @@ int batch_capture_staged(sphg_batch* b)
             rc = one_step(b->ctx[i], true);
             b->ctx[i]->recording = nullptr;
         }
-        if (rc != SPHG_OK || !staged_ok) continue;
+        if (rc != SPHG_OK || !room) continue;
         const size_t nk = recs[0].size();
-        for (size_t i = 1; i < n && staged_ok; ++i) {
-            staged_ok = recs[i].size() == nk;
-            for (size_t p = 0; p < nk && staged_ok; ++p)
-                staged_ok = recs[i][p].kern == recs[0][p].kern && recs[i][p].block == recs[0][p].block;
+        bool same = true;
+        for (size_t i = 1; i < n && same; ++i) {
+            same = recs[i].size() == nk;
+            for (size_t p = 0; p < nk && same; ++p)
+                same = recs[i][p].kern == recs[0][p].kern && recs[i][p].block == recs[0][p].block;
         }
-        for (size_t p = 0; p < nk && staged_ok && rc == SPHG_OK; ++p) {
+        if (!same) {
+            // context by context, over the streams
+            if (hipEventRecord(b->fork, origin) != hipSuccess) rc = SPHG_ERR_HIP;
+            for (int k = 1; k < b->nstreams && rc == SPHG_OK; ++k)
+                if (hipStreamWaitEvent(b->streams[(size_t)k], b->fork, 0) != hipSuccess) rc = SPHG_ERR_HIP;
+            for (size_t r = 0; r < n && rc == SPHG_OK; ++r) {
+                const size_t i = (size_t)b->order[r];
+                hipStream_t st = b->streams[r % (size_t)b->nstreams];
+                for (const LaunchRec& lr : recs[i])
+                    if (rc == SPHG_OK && lr.solo(lr.kern, lr.grid, lr.block, st, lr.args.data()) != hipSuccess)
+                        rc = SPHG_ERR_HIP;
+            }
+            for (int k = 1; k < b->nstreams && rc == SPHG_OK; ++k) {
+                if (hipEventRecord(b->join[(size_t)k], b->streams[(size_t)k]) != hipSuccess ||
+                    hipStreamWaitEvent(origin, b->join[(size_t)k], 0) != hipSuccess)
+                    rc = SPHG_ERR_HIP;
+            }
+            continue;
+        }
+        for (size_t p = 0; p < nk && room && rc == SPHG_OK; ++p) {
             const Batched* bt = batched_of(recs[0][p].kern);
-            staged_ok = bt && recs[0][p].block == 256 && recs[0][p].args.size() == bt->argsz;
-            if (!staged_ok) break;
+            if (!bt || recs[0][p].block != 256 || recs[0][p].args.size() != bt->argsz) {
+                for (size_t i = 0; i < n && rc == SPHG_OK; ++i) {
+                    const LaunchRec& lr = recs[i][p];
+                    if (lr.solo(lr.kern, lr.grid, lr.block, origin, lr.args.data()) != hipSuccess) rc = SPHG_ERR_HIP;
+                }
+                continue;
+            }
             args.clear();
             tiles.clear();
             for (size_t i = 0; i < n; ++i) {
@@ -2921,9 +2967,9 @@ int batch_capture_staged(sphg_batch* b)
             if (tiles.empty()) continue;
             const size_t oa = img.put(args.data(), args.size());
             const size_t ot = img.put(reinterpret_cast<const unsigned char*>(tiles.data()), tiles.size() * sizeof(int2));
-            staged_ok = oa != SIZE_MAX && ot != SIZE_MAX;
-            if (staged_ok && bt->go(origin, base + oa, reinterpret_cast<const int2*>(base + ot), (int)tiles.size()) !=
-                                 hipSuccess)
+            room = oa != SIZE_MAX && ot != SIZE_MAX;
+            if (room && bt->go(origin, base + oa, reinterpret_cast<const int2*>(base + ot), (int)tiles.size()) !=
+                            hipSuccess)
                 rc = SPHG_ERR_HIP;
         }
     }
@@ -2937,14 +2983,14 @@ int batch_capture_staged(sphg_batch* b)
     }
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(origin, &g);
-    if (rc != SPHG_OK || !staged_ok || ec != hipSuccess || !g || batch_capturing(b)) {
+    if (rc != SPHG_OK || !room || ec != hipSuccess || !g || batch_capturing(b)) {
         if (g) hipGraphDestroy(g);
         if (batch_capturing(b)) {
             const int rs = batch_set_streams(b, b->nstreams);
             if (rs != SPHG_OK) return rs;
         }
         if (rc != SPHG_OK) return rc;
-        return staged_ok ? SPHG_ERR_HIP : SPHG_ERR_UNSUPPORTED;
+        return room ? SPHG_ERR_HIP : SPHG_ERR_UNSUPPORTED;
     }
     // the arguments and tile tables the batched launches read (no replay of
     // an earlier graph is in flight: runs end synchronised)
