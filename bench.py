@@ -348,8 +348,7 @@ def sweep_section(args, hip):
         "cases": len(cases), "particles": ntot, "steps": steps, "streams": ns,
         "ms_per_step_eager": ms_eager, "ms_per_step_batched": ms_graph,
         "batched_mode": "eager" if ns == 0 else (
-            f"hipGraph, rebuild steps over {ns} streams, later steps one launch per kernel for all cases"
-            if staged else f"hipGraph, {ns} streams"),
+            "hipGraph, every step one launch per kernel for all cases" if staged else f"hipGraph, {ns} streams"),
         "value_batched": ntot / (ms_graph * 1e-3), "unit": "particle-steps/s",
         "speedup_batched_vs_eager": ms_eager / ms_graph,
         "workload": "submerged column collapses, N ~ logU[5k,120k], aspect ~ U[0.5,2], "

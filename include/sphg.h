@@ -310,17 +310,17 @@ int sphg_gnn_debug(sphg_gnn *gnn, int sample, float *nf, float *raw, int *nb, in
    No counterpart in the reference (one case per process there).  Many
    independent two-phase contexts advance together by replays of one hipGraph
    per period (the neighbour-list rebuild period, 64 steps; without lists the
-   reorder period, 16).  Staged form (the default): the period's first step
-   (the rebuild) runs context by context, dealt largest-first over a few
-   streams, and every later step is one launch per kernel for all contexts (a
-   tile table maps each workgroup to a context and a block of it).  When the
-   contexts' steps do not batch that way (different kernel instantiations,
-   e.g. smoothing lengths uniform in some and not in others), every step runs
-   context by context over the streams.  Results are bit-identical to
+   reorder period, 16).  Staged form (the default): every step of the period,
+   the rebuild included, is one launch per kernel for all contexts (a tile
+   table maps each workgroup to a context and a block of it); a kernel without
+   a batched form runs context by context, and a step whose kernels differ
+   between contexts (e.g. smoothing lengths uniform in some and not in others)
+   runs context by context over a few streams.  Results are bit-identical to
    advancing each context alone.  While batched, a context keeps its device
    buffers in place (graph mode); use it only through the batch until
    sphg_batch_destroy.  Per-launch profiling must be off.  SPHG_BATCH_STAGED=0
-   in the environment at sphg_batch_create turns the staged form off. */
+   in the environment at sphg_batch_create captures every step context by
+   context over the streams instead. */
 typedef struct sphg_batch sphg_batch;
 int sphg_batch_create(sphg_ctx **ctxs, int n, sphg_batch **out);
 void sphg_batch_destroy(sphg_batch *batch);
@@ -334,8 +334,8 @@ int sphg_batch_set_streams(sphg_batch *batch, int nstreams);
    streams, and keeps the fastest (best_streams = 0: eager).  The contexts
    advance during tuning; numerics do not depend on the choice. */
 int sphg_batch_tune(sphg_batch *batch, int trial_periods, int *best_streams, double *best_ms_per_step);
-/* 1: the batch's captured steps after a period's first are batched launches
-   (staged form); 0: context by context; -1: batch is NULL. */
+/* 1: the batch's captured steps are batched launches (staged form); 0:
+   context by context over the streams; -1: batch is NULL. */
 int sphg_batch_staged(const sphg_batch *batch);
 
 /* ---- pair search without allocation -------------------------------------

@@ -1,7 +1,7 @@
 """Batched scenarios (the sphg_batch_* entry points): many independent
 two-phase solvers advanced together by replays of one captured hipGraph per
-period (in the staged form, one launch per kernel for all solvers in the
-steps after a period's first), bit-identical to stepping each solver alone."""
+period (in the staged form, one launch per kernel for all solvers in every
+step), bit-identical to stepping each solver alone."""
 import ctypes as C
 
 from . import abi
@@ -35,8 +35,8 @@ class Batch:
 
     @property
     def staged(self):
-        """True when the captured steps after a period's first are one launch
-        per kernel for all solvers."""
+        """True when the captured steps are one launch per kernel for all
+        solvers."""
         return self.lib.sphg_batch_staged(self._b) == 1
 
     def close(self):

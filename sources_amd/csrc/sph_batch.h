@@ -1,10 +1,9 @@
 // One launch per stage for a batch of independent cases (no counterpart in
 // the reference, whose host runs one case per process).
 //
-// The host records the kernel launches of one step of every case (the same
-// kernel sequence for each case), stores each case's arguments as an ArgTup
-// in device memory, and launches the batched form of each kernel once over
-// all cases: a tile table maps each workgroup of the batched grid to (case,
+// The host records the kernel launches of one step of every case, stores
+// each case's arguments as an ArgTup in device memory, and launches the
+// batched form of each kernel of the step once over all cases: a tile table maps each workgroup of the batched grid to (case,
 // block of the case).  The batched kernel calls the same device body as the
 // solo kernel with that block, so every particle sees exactly the arithmetic
 // of a solo launch.

@@ -2679,8 +2679,8 @@ int sphg_gnn_debug(sphg_gnn* g, int sample, float* nf, float* raw, int* nb, int*
 // ---------------------------------------------------------------------------
 struct sphg_batch {
     std::vector<sphg_ctx*> ctx;
-    // One launch per stage for all cases in the steps of a period after its
-    // first (rebuild) step; off when a capture in that form failed.
+    // One launch per kernel for all cases in every captured step (staged
+    // form); off when a capture in that form found no room in the arena.
     bool staged = true;
     void* arena = nullptr;              // device: the batched launches' arguments and tile tables
     size_t arena_cap = 0;
@@ -3005,8 +3005,7 @@ int batch_capture_staged(sphg_batch* b)
 }
 
 // Capture `period` steps of every context: staged (one launch per kernel for
-// all contexts after the period's first step) while the batch's steps allow
-// it, else one stream per group of contexts.
+// all contexts), else one stream per group of contexts.
 // In the second form a context's main stream is a batch stream and its side
 // stream is off: the graph overlaps whole contexts across the streams.  (With
 // each context's side stream in the graph as well, the autotuner's best for
@@ -3018,7 +3017,7 @@ int batch_capture(sphg_batch* b)
     if (b->staged) {
         const int rs = batch_capture_staged(b);
         if (rs != SPHG_ERR_UNSUPPORTED) return rs;
-        b->staged = false;   // this batch's steps do not batch: the second form from here on
+        b->staged = false;   // the arena was too small: the second form from here on
     }
     hipStream_t origin = b->streams[0];
     if (hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPHG_ERR_HIP;
