@@ -114,4 +114,21 @@ SPHG_BATCH_BODY(BodyCellSort, cell_sort_t_body)
 SPHG_BATCH_BODY(BodyPackWater, pack_water_body)
 SPHG_BATCH_BODY(BodyPackSoil, pack_soil_body)
 SPHG_BATCH_BODY(BodyNlist, nlist_body)
+
+// ... and of the mixed-precision stage kernels.
+template <int SKF>
+struct BodyWaterAMx {
+    template <class... A>
+    __device__ __forceinline__ void operator()(int blk, const A&... a) const { water_a_mx<SKF>(blk, a...); }
+};
+template <int SKF>
+struct BodyWaterBMx {
+    template <class... A>
+    __device__ __forceinline__ void operator()(int blk, const A&... a) const { water_b_mx<SKF>(blk, a...); }
+};
+template <int SKF>
+struct BodySoilABMx {
+    template <class... A>
+    __device__ __forceinline__ void operator()(int blk, const A&... a) const { soil_ab_mx<SKF>(blk, a...); }
+};
 #undef SPHG_BATCH_BODY

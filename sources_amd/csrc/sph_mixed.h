@@ -253,12 +253,11 @@ __device__ __forceinline__ CatMask cand_mask_f(const LaneRows& L, const Masks& M
 // Stage A, water (mixed): the terms of k_water_a4 with FP32 pair sums.
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS,
-                                                   NList NL, Masks MW, Grid g, KParams K, FusedK F, MixK X,
-                                                   int cross)
+__device__ __forceinline__ void water_a_mx(int blk, DPhase W, DPhase S, const float* qW, float* qWw, const float* qS,
+                                           NList NL, Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
 {
     const RecQ QWv = recqv(qW, W.n), QSv = recqv(qS, S.n);
-    const WaveCtx wc = wave_ctx(W.n);
+    const WaveCtx wc = wave_ctx(W.n, blk);
     const int t = wc.t;
     auto QA = [&](int c) { return QWv(wc.tt, c); };
     const float4 a0 = QA(0), a1 = QA(1), a2 = QA(2);
@@ -375,15 +374,22 @@ __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, 
     Q3[3] = (float)(vofa_d * (mu * tyy_d));
 }
 
+template <int SKF>
+__global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS, NList NL,
+                                           Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
+{
+    water_a_mx<SKF>(xcd_block(), W, S, qW, qWw, qS, NL, MW, g, K, F, X, cross);
+}
+
 // ------------------------------------------------------------------------
 // Stage B, water (mixed): momentum, art_visc, delta_sph(vof).
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, Grid g, KParams K, FusedK F,
-                                                   MixK X, int mask)
+__device__ __forceinline__ void water_b_mx(int blk, DPhase W, const float* qW, Masks MW, Grid g, KParams K, FusedK F,
+                                           MixK X, int mask)
 {
     const RecQ QWv = recqv(qW, W.n);
-    const WaveCtx wc = wave_ctx(W.n);
+    const WaveCtx wc = wave_ctx(W.n, blk);
     const int t = wc.t;
     auto QA = [&](int c) { return QWv(wc.tt, c); };
     const float4 a0 = QA(0), a1 = QA(1), a2 = QA(2), a3 = QA(3);
@@ -461,6 +467,13 @@ __global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, 
     W.dvof[a] = dvof;
 }
 
+template <int SKF>
+__global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, Grid g, KParams K, FusedK F, MixK X,
+                                           int mask)
+{
+    water_b_mx<SKF>(xcd_block(), W, qW, MW, g, K, F, X, mask);
+}
+
 // ------------------------------------------------------------------------
 // Soil, stages A and B in one kernel (mixed): soil stage B needs nothing
 // from the stage A of other soil particles (its partners' p, sigma, vof come
@@ -468,11 +481,11 @@ __global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, 
 // share one scan and one partner loop; the terms are those of k_soil_ab4.
 // ------------------------------------------------------------------------
 template <int SKF>
-__global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* qS, const float* qW, NList NL,
+__device__ __forceinline__ void soil_ab_mx(int blk, DPhase S, DPhase W, const float* qS, const float* qW, NList NL,
                                            Grid g, KParams K, FusedK F, MixK X, int cross, int mask)
 {
     const RecQ QSv = recqv(qS, S.n), QWv = recqv(qW, W.n);
-    const WaveCtx wc = wave_ctx(S.n);
+    const WaveCtx wc = wave_ctx(S.n, blk);
     const int t = wc.t;
     auto QA = [&](int c) { return QSv(wc.tt, c); };
     const float4 a0 = QA(0), a1 = QA(1), a2 = QA(2), a3 = QA(3);
@@ -666,4 +679,11 @@ __global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* q
     S.dsxx[a] = dsxx; S.dsxy[a] = dsxy; S.dsyy[a] = dsyy; S.dp[a] = dpv;
     S.ax[a] = ax;
     S.ay[a] = ay;
+}
+
+template <int SKF>
+__global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* qS, const float* qW, NList NL, Grid g,
+                                              KParams K, FusedK F, MixK X, int cross, int mask)
+{
+    soil_ab_mx<SKF>(xcd_block(), S, W, qS, qW, NL, g, K, F, X, cross, mask);
 }

@@ -2730,6 +2730,9 @@ void reg_batched_skf(std::map<const void*, Batched>& m)
     reg_batched<BodyWaterB<SKF, false>>(m, k_water_b4<SKF, false>);
     reg_batched<BodySoilAB<SKF, true, false>>(m, k_soil_ab4<SKF, true, false>);
     reg_batched<BodySoilAB<SKF, false, false>>(m, k_soil_ab4<SKF, false, false>);
+    reg_batched<BodyWaterAMx<SKF>>(m, k_water_a_mx<SKF>);
+    reg_batched<BodyWaterBMx<SKF>>(m, k_water_b_mx<SKF>);
+    reg_batched<BodySoilABMx<SKF>>(m, k_soil_ab_mx<SKF>);
 }
 
 // The kernels of a list-mode FP64 step between rebuilds (stages, finals).
