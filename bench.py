@@ -27,19 +27,18 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at 1M two-phase SPH; learned-vs-plain speedup at ≤1% ρ err"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per particle and step, the two-neighbour-stage model
-# of the survey of the reference's loop (state read once per stage, final
-# state written once, stage-A -> stage-B intermediates written and read once;
-# neighbour re-reads on chip): water 368, soil 456, cell list 28 (both
-# phases).  The roofline of the dominant kernel (the soil stage pass,
-# soil_ab) takes the soil figure per soil particle of its launch, as if that
-# pass moved all of the soil's bytes; "frac_split_model" beside it takes the
-# share this build's stage split assigns to that pass: 124 state + 48
-# intermediates + 112 outputs = 284.  Mixed precision (integrated fields x,
-# v, rho, vof, stress, p in FP64, the rest FP32): water 260, soil 344 (split
-# share 216).
-SURVEY_BYTES = {"fp64": {"water": 368, "soil": 456, "cell": 28},
-                "mixed": {"water": 260, "soil": 344, "cell": 28}}
+# Algorithmic HBM bytes per particle and step of the reference loop with two
+# neighbour stages (state read once per stage, final state written once,
+# stage-A -> stage-B intermediates written and read once; neighbour re-reads
+# on chip): water 368, soil 456, cell list 28 (both phases).  The roofline of
+# the dominant kernel (the soil stage pass, soil_ab) takes the soil figure per
+# soil particle of its launch, as if that pass moved all of the soil's bytes;
+# "frac_split_model" beside it takes the share the library's stage split
+# assigns to that pass: 124 state + 48 intermediates + 112 outputs = 284.
+# Mixed precision (integrated fields x, v, rho, vof, stress, p in FP64, the
+# rest FP32): water 260, soil 344 (split share 216).
+ALGO_BYTES = {"fp64": {"water": 368, "soil": 456, "cell": 28},
+              "mixed": {"water": 260, "soil": 344, "cell": 28}}
 SPLIT_BYTES = {"fp64": {"soil_ab": 284}, "mixed": {"soil_ab": 216}}
 
 
@@ -403,7 +402,7 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the library runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the slab decomposition over several GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
@@ -455,7 +454,7 @@ def main():
     s.profile(False)
     pairs_w, pairs_s, pairs_x = s.pair_counts()
     nl_rebuilds, nl_off, nl_skin = s.nlist_stats()
-    sb = SURVEY_BYTES[args.precision]
+    sb = ALGO_BYTES[args.precision]
     dom = "soil_ab"
     algo = sb["soil"] * ns
     algo_split = SPLIT_BYTES[args.precision][dom] * ns
