@@ -52,6 +52,7 @@ constexpr int NPARAM = O_B5 + NOP;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -272,17 +273,57 @@ __global__ __launch_bounds__(64) void k_gnn(GnnArgs A)
         return edge_from(j, rj);
     };
 
-    // ---- edge layer and sum over neighbours
+    // ---- edge layer and sum over neighbours.  Two slots per pass: half-wave
+    // h builds all 16 features of slot s + h of its node, then lanes r and
+    // r + 32 exchange the feature half the other slot's MFMA operand needs
+    // (v_permlane32_swap), so each edge's features are computed once.  The
+    // sum takes the slots in order, as one slot per pass would.
     f32x16 hsum = zero16();
-    int jn = nbr(0);
+    int jn = nbr(h);
 #pragma unroll 1
-    for (int s = 0; s < maxdeg; ++s) {
+    for (int s = 0; s < maxdeg; s += 2) {
         const int j = jn;
-        jn = nbr(s + 1);   // next slot's index in flight during this slot
-        const bf16x8 av = edge_operand(j);
-        f32x16 z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, w1, zero16(), 0, 0, 0);
+        jn = nbr(s + 2 + h);   // the next pair's index in flight during this one
+        float rj[RAW];
+#pragma unroll
+        for (int f = 0; f < RAW; ++f) rj[f] = 0.0f;
+        if (j >= 0) {
+            const float* row = A.G.raw + (size_t)j * RAW;
+            const float4 q0 = reinterpret_cast<const float4*>(row)[0];
+            const float4 q1 = reinterpret_cast<const float4*>(row)[1];
+            rj[0] = q0.x; rj[1] = q0.y; rj[2] = q0.z; rj[3] = q0.w;
+            rj[4] = q1.x; rj[5] = q1.y; rj[6] = q1.z; rj[7] = q1.w;
+            rj[8] = row[8];
+        }
+        float e[FE];
+        edge_features(ri, rj, j >= 0, e);
+        bf16x8 flo, fhi;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            flo[q] = (__bf16)e[q];
+            fhi[q] = (__bf16)e[8 + q];
+        }
+        const u32x4 lo = __builtin_bit_cast(u32x4, flo), hi = __builtin_bit_cast(u32x4, fhi);
+        u32x4 o1, o2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            // lane r offers features 8-15 of slot s, lane r + 32 features 0-7
+            // of slot s + 1; each takes the other's (first result: the lower
+            // lanes' values in both halves, second: the upper lanes')
+            const unsigned x = h ? lo[q] : hi[q];
+            const auto sw = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+            const unsigned y = h ? sw[0] : sw[1];
+            o1[q] = h ? y : lo[q];   // slot s: features 0-7 | 8-15
+            o2[q] = h ? hi[q] : y;   // slot s + 1
+        }
+        f32x16 z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, o1), w1, zero16(), 0, 0, 0);
 #pragma unroll
         for (int v = 0; v < 16; ++v) hsum[v] += fmaxf(z[v], 0.0f);
+        if (s + 1 < maxdeg) {
+            z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, o2), w1, zero16(), 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) hsum[v] += fmaxf(z[v], 0.0f);
+        }
     }
     tile_each(hsum, [&](int row, int col, float& v) { HS[row][col] = v * EDGE_NORM; });
     __syncthreads();
