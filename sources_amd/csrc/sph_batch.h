@@ -132,3 +132,26 @@ struct BodySoilABMx {
     __device__ __forceinline__ void operator()(int blk, const A&... a) const { soil_ab_mx<SKF>(blk, a...); }
 };
 #undef SPHG_BATCH_BODY
+
+// Two argument sets of one body in one launch (the two phases of a context):
+// blocks [0, nb0) run the first set, the rest the second.
+template <class F, class... T>
+__global__ __launch_bounds__(256) void k_pair(ArgTup<T...> a0, ArgTup<T...> a1, int nb0)
+{
+    const int b = blockIdx.x;
+    if (b < nb0) tup_call<F>(b, a0, std::index_sequence_for<T...>{});
+    else tup_call<F>(b - nb0, a1, std::index_sequence_for<T...>{});
+}
+
+// The surrogate's per-phase kernels.
+#define SPHG_PAIR_BODY(NAME, FN)                                                         \
+    struct NAME {                                                                        \
+        template <class... A>                                                            \
+        __device__ __forceinline__ void operator()(int blk, const A&... a) const { FN(blk, a...); } \
+    };
+SPHG_PAIR_BODY(BodyGnnGather, gnn::gnn_gather_body)
+SPHG_PAIR_BODY(BodyGnnDelta, gnn::gnn_delta_body)
+SPHG_PAIR_BODY(BodyGnnCorrect, gnn::gnn_correct_body)
+SPHG_PAIR_BODY(BodySanity, gnn::sanity_body)
+SPHG_PAIR_BODY(BodyRestoreIf, gnn::restore_if_body)
+#undef SPHG_PAIR_BODY

@@ -543,10 +543,10 @@ struct GatherOut {
 // their grid): the candidates are the particle's listed same-phase
 // candidates, tested with the current positions; a list that overflowed
 // walks its rows.
-__global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, int off, int phase, Scales S,
-                             GatherOut O, Masks NLM, const int* nl_cell)
+__device__ __forceinline__ void gnn_gather_body(int blk, DPhase P, Grid g, KParams K, double thr, int uni, int off,
+                                                int phase, Scales S, GatherOut O, Masks NLM, const int* nl_cell)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int a = P.order[t];
     const int node = off + t;
@@ -674,6 +674,12 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
     O.nd[node] = min(cnt, DEG);   // (slots past the degree are never read)
 }
 
+__global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, int off, int phase, Scales S,
+                             GatherOut O, Masks NLM, const int* nl_cell)
+{
+    gnn_gather_body(blockIdx.x, P, g, K, thr, uni, off, phase, S, O, NLM, nl_cell);
+}
+
 // Rate-change features: after the coarse step P holds the rates evaluated at
 // the state the advance starts from, and features FR..FR+2 hold the rates of
 // the evaluation `gap` steps of dt before it.  Over K steps of dt a rate r
@@ -681,14 +687,19 @@ __global__ void k_gnn_gather(DPhase P, Grid g, KParams K, double thr, int uni, i
 // K (K - 1) / 2 dt^2 dr/dt: with dr/dt ~ (r_new - r_old) / (gap dt) the
 // features become coef (r_new - r_old), coef = K (K - 1) / 2 dt / gap, in the
 // units of the correction (velocity, density).
-__global__ void k_gnn_delta(DPhase P, const int* node_of_id, float* nf, double coef)
+__device__ __forceinline__ void gnn_delta_body(int blk, DPhase P, const int* node_of_id, float* nf, double coef)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blk * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
     float* f = nf + (size_t)node_of_id[P.id[a]] * FN + FR;
     f[0] = (float)((P.ax[a] - (double)f[0]) * coef);
     f[1] = (float)((P.ay[a] - (double)f[1]) * coef);
     f[2] = (float)((P.drho[a] - (double)f[2]) * coef);
+}
+
+__global__ void k_gnn_delta(DPhase P, const int* node_of_id, float* nf, double coef)
+{
+    gnn_delta_body(blockIdx.x, P, node_of_id, nf, coef);
 }
 
 // Targets: fine state (K plain steps, current) minus coarse state (one step
@@ -708,9 +719,10 @@ __global__ void k_gnn_target(DPhase P, const int* node_of_id, DPhase C, const in
 }
 
 // The network's correction of the coarse step (real particles).
-__global__ void k_gnn_correct(DPhase P, KParams K, const int* node_of_id, const float* y, Scales S, int phase)
+__device__ __forceinline__ void gnn_correct_body(int blk, DPhase P, KParams K, const int* node_of_id, const float* y,
+                                                 Scales S, int phase)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blk * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
     if (P.itype[a] <= 0) return;
     const int node = node_of_id[P.id[a]];
@@ -724,6 +736,11 @@ __global__ void k_gnn_correct(DPhase P, KParams K, const int* node_of_id, const 
     P.y[a] = P.y[a] + 0.5 * (double)S.kdt * cvy;
     P.rho[a] = P.rho[a] + crh;
     if (phase == 1) P.vof[a] = P.rho[a] / K.s_rho0;
+}
+
+__global__ void k_gnn_correct(DPhase P, KParams K, const int* node_of_id, const float* y, Scales S, int phase)
+{
+    gnn_correct_body(blockIdx.x, P, K, node_of_id, y, S, phase);
 }
 
 // out[0]: max (as float bits: non-negative floats order as their bits);
@@ -771,14 +788,19 @@ __global__ __launch_bounds__(256) void k_rho_err(DPhase A, DPhase B, const int* 
 
 // Flags a state a learned advance has broken: a non-finite or non-positive
 // density, a non-finite velocity, or a particle outside the cell grid.
-__global__ void k_sanity(DPhase P, Grid g, int* flag)
+__device__ __forceinline__ void sanity_body(int blk, DPhase P, Grid g, int* flag)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blk * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
     const double x = P.x[a], y = P.y[a], r = P.rho[a];
     const bool ok = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1 && r > 0.0 && r < 1.0e300 &&
                     P.vx[a] == P.vx[a] && P.vy[a] == P.vy[a];
     if (!ok) atomicOr(flag, 1);
+}
+
+__global__ void k_sanity(DPhase P, Grid g, int* flag)
+{
+    sanity_body(blockIdx.x, P, g, flag);
 }
 
 // Least-squares gain of each output channel over one sample (real nodes):
@@ -815,13 +837,24 @@ __global__ __launch_bounds__(256) void k_gnn_fit(const float* y, const float* tg
 // The state of a phase copied back from a snapshot (A.s* -> A.d*) when the
 // sanity flag is set: a learned advance that broke the state is undone on
 // the device, before any later kernel reads it.
-__global__ void k_restore_if(const int* flag, PermArgs A, int n)
+__device__ __forceinline__ void restore_if_body(int blk, const int* flag, PermArgs A, int n)
 {
     if (*flag == 0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    for (int q = 0; q < A.nd; ++q) A.dd[q][i] = A.sd[q][i];
-    for (int q = 0; q < A.ni; ++q) A.di[q][i] = A.si[q][i];
+    // (constant indices into the argument arrays: a runtime-indexed loop
+    // would copy them to scratch memory)
+#pragma unroll
+    for (int q = 0; q < PERM_MAXD; ++q)
+        if (q < A.nd) A.dd[q][i] = A.sd[q][i];
+#pragma unroll
+    for (int q = 0; q < PERM_MAXI; ++q)
+        if (q < A.ni) A.di[q][i] = A.si[q][i];
+}
+
+__global__ void k_restore_if(const int* flag, PermArgs A, int n)
+{
+    restore_if_body(blockIdx.x, flag, A, n);
 }
 
 __global__ void k_slot_of_id(DPhase P, int* slot_of_id)

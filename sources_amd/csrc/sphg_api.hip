@@ -274,6 +274,21 @@ int launch(sphg_ctx* c, const char* name, void (*kern)(KArgs...), int grid, int 
     return SPHG_OK;
 }
 
+// The argument tuple of kernel kern (arguments converted to its parameters).
+template <class... K, class... A>
+ArgTup<K...> args_of(void (*)(K...), const A&... a)
+{
+    return ArgTup<K...>(a...);
+}
+
+// Kernel body F over two argument sets in one launch (the two phases): nb0
+// blocks of the first set, then nb1 of the second.
+template <class F, class... K>
+int launch_pair(sphg_ctx* c, const char* name, int nb0, const ArgTup<K...>& a0, int nb1, const ArgTup<K...>& a1)
+{
+    return launch(c, name, k_pair<F, K...>, nb0 + nb1, TPB, a0, a1, nb0);
+}
+
 template <class T>
 int dalloc(sphg_ctx* c, PhaseBuf& b, T** p, size_t count)
 {
@@ -2077,15 +2092,16 @@ int snap_alloc(sphg_gnn* g, StateCopy& s, int n)
 int snap_copy(sphg_gnn* g, int slot, bool to_snap)
 {
     sphg_ctx* c = g->c;
-    for (int k = 0; k < 2; ++k) {
+    // both phases in one launch (an unbound phase takes no blocks)
+    auto copy = [&](int k) {
         PhaseBuf& b = c->ph[k];
-        if (!b.bound) continue;
+        PermArgs A{};
+        if (!b.bound) return args_of(k_permute_multi, A, (const int*)nullptr, 0);
         StateCopy& sc = g->snap[slot][k];
         double** f[N_STATE_FIELDS];
         state_fields(b.d, f);
         int* st[3] = {b.d.id, b.d.itype, b.d.fail};
         int* sn[3] = {sc.id, sc.itype, sc.fail};
-        PermArgs A{};
         A.nd = N_STATE_FIELDS;
         A.ni = 3;
         for (int i = 0; i < N_STATE_FIELDS; ++i) {
@@ -2096,8 +2112,10 @@ int snap_copy(sphg_gnn* g, int slot, bool to_snap)
             A.si[i] = to_snap ? st[i] : sn[i];
             A.di[i] = to_snap ? sn[i] : st[i];
         }
-        RC(launch(c, "snapshot", k_permute_multi, nblk(b.n), TPB, A, (const int*)nullptr, b.n));
-    }
+        return args_of(k_permute_multi, A, (const int*)nullptr, b.n);
+    };
+    auto nb = [&](int k) { return c->ph[k].bound ? nblk(c->ph[k].n) : 0; };
+    RC(launch_pair<BodyPermute>(c, "snapshot", nb(0), copy(0), nb(1), copy(1)));
     if (to_snap) {
         g->snap_rebuilds[slot] = c->nl_rebuilds;
     } else if (!(nl_mode(c) && c->nl_valid && c->nl_rebuilds == g->snap_rebuilds[slot])) {
@@ -2161,14 +2179,14 @@ int build_graph(sphg_gnn* g, sphg_gnn::Graph& G)
     }
     G.nw = Wb.n;
     RC(fill_int(c, g->overflow, 1, 0));
-    for (int k = 0; k < 2; ++k) {
+    auto args = [&](int k) {
         PhaseBuf& b = c->ph[k];
         gnn::GatherOut O{G.nf, G.raw, G.nb, G.nd, G.n, G.tw, g->node_of_id[k], g->overflow};
-        RC(launch(c, "gnn_gather", gnn::k_gnn_gather, nblk(b.n), TPB, b.d, lists ? c->gvl[k] : c->gl[k], c->K,
-                  k == 0 ? c->F.thr_w : c->F.thr_s, k == 0 ? c->F.uni_w : c->F.uni_s, k == 0 ? 0 : Wb.n, k,
-                  g->S, O, lists ? b.nl_same : Masks{}, lists ? (const int*)b.nl_cell : (const int*)nullptr));
-    }
-    return SPHG_OK;
+        return args_of(gnn::k_gnn_gather, b.d, lists ? c->gvl[k] : c->gl[k], c->K, k == 0 ? c->F.thr_w : c->F.thr_s,
+                       k == 0 ? c->F.uni_w : c->F.uni_s, k == 0 ? 0 : Wb.n, k, g->S, O,
+                       lists ? b.nl_same : Masks{}, lists ? (const int*)b.nl_cell : (const int*)nullptr);
+    };
+    return launch_pair<BodyGnnGather>(c, "gnn_gather", nblk(Wb.n), args(0), nblk(Sb.n), args(1));
 }
 
 int forward(sphg_gnn* g, const sphg_gnn::Graph& G)
@@ -2193,11 +2211,10 @@ int rate_features(sphg_gnn* g, sphg_gnn::Graph& G, int gap)
 {
     sphg_ctx* c = g->c;
     const double coef = 0.5 * g->K * (g->K - 1) * c->P.dt / gap;
-    for (int k = 0; k < 2; ++k) {
-        PhaseBuf& b = c->ph[k];
-        RC(launch(c, "gnn_delta", gnn::k_gnn_delta, nblk(b.n), TPB, b.d, (const int*)g->node_of_id[k], G.nf, coef));
-    }
-    return SPHG_OK;
+    auto args = [&](int k) {
+        return args_of(gnn::k_gnn_delta, c->ph[k].d, (const int*)g->node_of_id[k], G.nf, coef);
+    };
+    return launch_pair<BodyGnnDelta>(c, "gnn_delta", nblk(c->ph[0].n), args(0), nblk(c->ph[1].n), args(1));
 }
 
 }  // namespace
@@ -2440,13 +2457,13 @@ int sphg_gnn_step(sphg_gnn* g, int use_net)
     if (use_net) {
         RC(rate_features(g, g->cur, gap));
         RC(forward(g, g->cur));
-        for (int k = 0; k < 2; ++k) {
-            PhaseBuf& b = c->ph[k];
-            // (the moved positions reach the displacement check of the next
-            // step's pack; without lists the step left its cell lists invalid)
-            RC(launch(c, "gnn_correct", gnn::k_gnn_correct, nblk(b.n), TPB, b.d, c->K,
-                      (const int*)g->node_of_id[k], (const float*)g->y, g->S, k));
-        }
+        // (the moved positions reach the displacement check of the next
+        // step's pack; without lists the step left its cell lists invalid)
+        auto args = [&](int k) {
+            return args_of(gnn::k_gnn_correct, c->ph[k].d, c->K, (const int*)g->node_of_id[k], (const float*)g->y,
+                           g->S, k);
+        };
+        RC(launch_pair<BodyGnnCorrect>(c, "gnn_correct", nblk(c->ph[0].n), args(0), nblk(c->ph[1].n), args(1)));
     }
     return SPHG_OK;
 }
@@ -2459,11 +2476,10 @@ static int sanity_async(sphg_gnn* g, int slot)
 {
     sphg_ctx* c = g->c;
     RC(fill_int(c, g->broken, 1, 0));
-    for (int k = 0; k < 2; ++k) {
-        PhaseBuf& b = c->ph[k];
-        RC(launch(c, "sanity", gnn::k_sanity, nblk(b.n), TPB, b.d, c->g, g->broken));
-    }
-    for (int k = 0; k < 2; ++k) {
+    const int nb0 = nblk(c->ph[0].n), nb1 = nblk(c->ph[1].n);
+    auto check = [&](int k) { return args_of(gnn::k_sanity, c->ph[k].d, c->g, g->broken); };
+    RC(launch_pair<BodySanity>(c, "sanity", nb0, check(0), nb1, check(1)));
+    auto restore = [&](int k) {
         PhaseBuf& b = c->ph[k];
         StateCopy& sc = g->snap[0][k];
         double** f[N_STATE_FIELDS];
@@ -2474,8 +2490,9 @@ static int sanity_async(sphg_gnn* g, int slot)
         for (int i = 0; i < N_STATE_FIELDS; ++i) { A.sd[i] = sc.f[i]; A.dd[i] = *f[i]; }
         A.si[0] = sc.id; A.si[1] = sc.itype; A.si[2] = sc.fail;
         A.di[0] = b.d.id; A.di[1] = b.d.itype; A.di[2] = b.d.fail;
-        RC(launch(c, "sanity", gnn::k_restore_if, nblk(b.n), TPB, (const int*)g->broken, A, b.n));
-    }
+        return args_of(gnn::k_restore_if, (const int*)g->broken, A, b.n);
+    };
+    RC(launch_pair<BodyRestoreIf>(c, "sanity", nb0, restore(0), nb1, restore(1)));
     HIPCHK(c, hipMemcpyAsync(&g->h_broken[slot], g->broken, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(g->ev_ok[slot], c->stream));
     return SPHG_OK;
