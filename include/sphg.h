@@ -59,7 +59,7 @@ extern "C" {
 
 /* ---- precision modes --------------------------------------------------- */
 #define SPHG_FP64  0   /* every operation in FP64 (the reference's real(dp)) */
-#define SPHG_MIXED 1   /* pair sums in FP32 with Kahan compensation; FP64 leapfrog,
+#define SPHG_MIXED 1   /* pair terms in FP32, summed in FP64; FP64 leapfrog,
                           density and Drucker-Prager projection */
 
 /* ---- field masks for sphg_sync --------------------------------------- */

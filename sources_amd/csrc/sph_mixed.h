@@ -2,9 +2,9 @@
 //
 // The state, the leapfrog update, the Drucker-Prager projections, the EOS,
 // the flow rule and the Jaumann rate stay FP64 (pack and final kernels and
-// the per-particle tails of the stage kernels).  The pair sums run in FP32
-// with compensated (Kahan) accumulation over FP32 neighbour records of four
-// float4 chunks, stored as chunk planes like the FP64 records:
+// the per-particle tails of the stage kernels).  The pair terms are FP32,
+// summed in FP64, over FP32 neighbour records of four float4 chunks, stored
+// as chunk planes like the FP64 records:
 //   q0: xl, yl, tag, h         xl, yl: position relative to the centre of the
 //                              particle's cell; tag: cx << 15 | cy, bit 31 set
 //                              for a virtual particle (int bits)
@@ -35,17 +35,13 @@
 
 
 
-// Compensated FP32 sum (Kahan): value() = s - c.
-struct Kahan {
-    float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void add(float v)
-    {
-        const float y = v - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+// FP32 pair terms summed in FP64: a conversion and an add per term (with
+// packed FP32 off, FP32 and FP64 adds issue at the same rate, so this costs
+// half of a compensated FP32 sum and rounds only once per term, in FP64).
+struct Sum64 {
+    double s = 0.0;
+    __device__ __forceinline__ void add(float v) { s += (double)v; }
+    __device__ __forceinline__ double value() const { return s; }
 };
 
 struct KCf {
@@ -281,7 +277,7 @@ __device__ __forceinline__ void water_a_mx(int blk, DPhase W, DPhase S, const fl
     }
     const bool real_a = W.itype[a] > 0;
     const float dspp = (float)W.dspp;
-    Kahan con, drd, txx, txy, tyy, avx, avy;
+    Sum64 con, drd, txx, txy, tyy, avx, avy;
     float repx = 0.0f, repy = 0.0f;
     int cnt = 0;
     visit_f(cm, L, t, [&](int k) { return acc_q(QWv(k, 0)); }, [&](int k) {
@@ -314,7 +310,7 @@ __device__ __forceinline__ void water_a_mx(int blk, DPhase W, DPhase S, const fl
         }
     });
 
-    Kahan dg_x, dg_y, dvof_x, vof2;
+    Sum64 dg_x, dg_y, dvof_x, vof2;
     int cntx = 0;
     if (cross) {
         const LaneRows LX = rows_of_cell(g, S.cell_start, me.cx, me.cy);
@@ -413,7 +409,7 @@ __device__ __forceinline__ void water_b_mx(int blk, DPhase W, const float* qW, M
     }
     const float et2 = (float)(K.etq * K.etq);
     const float beta = (float)K.beta, alpha = (float)K.alpha;
-    Kahan Px, Py, Sxx, Sxyy, Sxyx, Syy, artx, arty, dvd;
+    Sum64 Px, Py, Sxx, Sxyy, Sxyx, Syy, artx, arty, dvd;
     visit_f(cm, L, t, [&](int k) { return acc_q(QWv(k, 0)); }, [&](int k) {
         auto Q = [&](int c) { return QWv(k, c); };
         const float4 b0 = Q(0), b1 = Q(1), b2 = Q(2), b3 = Q(3);
@@ -503,7 +499,7 @@ __device__ __forceinline__ void soil_ab_mx(int blk, DPhase S, DPhase W, const fl
     const bool real_a = S.itype[a] > 0;
     const float dspp = (float)S.dspp;
 
-    Kahan dg_x, dg_y, po_x, po_y;
+    Sum64 dg_x, dg_y, po_x, po_y;
     int cntx = 0;
     const double vofa_d = S.vof[a];
     if (cross) {
@@ -549,8 +545,8 @@ __device__ __forceinline__ void soil_ab_mx(int blk, DPhase S, DPhase W, const fl
     const CatMask cm = cand_mask_f(L, NL.same, S.n, wc.tt, t, QSv, acc_q);
     const float et2 = (float)(K.etq * K.etq);
     const float beta = (float)K.beta, alpha = (float)K.alpha;
-    Kahan con, drd, txx, txy, tyy, vcc, wxy, avx, avy;
-    Kahan Fx, Fy, artx, arty, dP, dXX, dXY, dYY;
+    Sum64 con, drd, txx, txy, tyy, vcc, wxy, avx, avy;
+    Sum64 Fx, Fy, artx, arty, dP, dXX, dXY, dYY;
     float repx = 0.0f, repy = 0.0f;
     int cnt = 0;
     visit_f(cm, L, t, [&](int k) { return acc_q(QSv(k, 0)); }, [&](int k) {

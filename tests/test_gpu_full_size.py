@@ -150,7 +150,7 @@ def _mixed_tol(f):
 
 
 def test_full_size_mixed_slope_on_windows():
-    """The 1M slope in its own precision (FP32 Kahan pair sums, FP64
+    """The 1M slope in its own precision (FP32 pair terms summed in FP64, FP64
     integration and projection): one step against the FP64 oracle."""
     sc = _slope()
     inp = sc.copy()

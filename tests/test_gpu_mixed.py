@@ -1,5 +1,5 @@
-"""Mixed-precision form (precision = SPHG_MIXED): FP32 pair sums with Kahan
-compensation, FP64 leapfrog / Drucker-Prager / EOS.  Its error is measured
+"""Mixed-precision form (precision = SPHG_MIXED): FP32 pair terms summed in
+FP64, FP64 leapfrog / Drucker-Prager / EOS.  Its error is measured
 against the FP64 path, which the parity tests pin to the oracle, and after
 one step also against the oracle directly."""
 import numpy as np
