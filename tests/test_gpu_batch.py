@@ -19,17 +19,24 @@ def cases():
     return scenarios.sweep(4, seed=5, nmin=2000, nmax=8000)
 
 
-@pytest.mark.parametrize("nsteps,staged,prec", [(40, "1", "fp64"), (140, "1", "fp64"), (140, "0", "fp64"),
-                                                (140, "1", "mixed")])
-def test_batch_is_bit_identical_to_solo(nsteps, staged, prec, monkeypatch):
+@pytest.mark.parametrize("nsteps,staged,prec,vary_h", [(40, "1", "fp64", False), (140, "1", "fp64", False),
+                                                       (140, "0", "fp64", False), (140, "1", "mixed", False),
+                                                       (140, "1", "fp64", True)])
+def test_batch_is_bit_identical_to_solo(nsteps, staged, prec, vary_h, monkeypatch):
     # 40 steps: eager only; 140: eager to step 64, one replayed period of 64
     # steps (65-128, starting with the list rebuild), eager remainder; the
-    # captured period staged or context by context; FP64 or mixed precision
+    # captured period staged or context by context; FP64 or mixed precision;
+    # vary_h: one case's water smoothing lengths not uniform, so its water
+    # kernels are other instantiations than the other cases' and the staged
+    # capture replays those steps case by case over the streams
     monkeypatch.setenv("SPHG_BATCH_STAGED", staged)
     solo = cases()
     bat = cases()
     for c in solo + bat:
         c.params.precision = abi.SPHG_MIXED if prec == "mixed" else abi.SPHG_FP64
+    if vary_h:
+        for c in (solo[1], bat[1]):
+            c.water.hsml[0] *= 1.001
     print("particles", [c.water.n + c.soil.n for c in solo], flush=True)
     ss = [Solver(c.params, c.water, c.soil) for c in solo]
     for s in ss:
