@@ -119,7 +119,12 @@ SPHG_BATCH_BODY(BodyNlist, nlist_body)
 template <int SKF>
 struct BodyWaterAMx {
     template <class... A>
-    __device__ __forceinline__ void operator()(int blk, const A&... a) const { water_a_mx<SKF>(blk, a...); }
+    __device__ __forceinline__ void operator()(int blk, const A&... a) const { water_a_mx<SKF, 1>(blk, a...); }
+};
+template <int SKF>
+struct BodyWaterXMx {
+    template <class... A>
+    __device__ __forceinline__ void operator()(int blk, const A&... a) const { water_a_mx<SKF, 2>(blk, a...); }
 };
 template <int SKF>
 struct BodyWaterBMx {
@@ -129,7 +134,12 @@ struct BodyWaterBMx {
 template <int SKF>
 struct BodySoilABMx {
     template <class... A>
-    __device__ __forceinline__ void operator()(int blk, const A&... a) const { soil_ab_mx<SKF>(blk, a...); }
+    __device__ __forceinline__ void operator()(int blk, const A&... a) const { soil_ab_mx<SKF, 1>(blk, a...); }
+};
+template <int SKF>
+struct BodySoilXMx {
+    template <class... A>
+    __device__ __forceinline__ void operator()(int blk, const A&... a) const { soil_ab_mx<SKF, 2>(blk, a...); }
 };
 #undef SPHG_BATCH_BODY
 

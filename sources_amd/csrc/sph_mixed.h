@@ -248,7 +248,9 @@ __device__ __forceinline__ CatMask cand_mask_f(const LaneRows& L, const Masks& M
 // ------------------------------------------------------------------------
 // Stage A, water (mixed): the terms of k_water_a4 with FP32 pair sums.
 // ------------------------------------------------------------------------
-template <int SKF>
+// XM = 1: the same-phase terms (the cross terms come from the XM = 2 pass,
+// which runs first and stores them); XM = 2: the cross terms alone.
+template <int SKF, int XM>
 __device__ __forceinline__ void water_a_mx(int blk, DPhase W, DPhase S, const float* qW, float* qWw, const float* qS,
                                            NList NL, Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
 {
@@ -263,6 +265,52 @@ __device__ __forceinline__ void water_a_mx(int blk, DPhase W, DPhase S, const fl
     const int a = W.order[wc.tt];
     const bool uni = F.uni_w;
     const KCf kc = make_kcf(SKF, me.h);
+    if constexpr (XM == 2) {
+        Sum64 dg_x, dg_y, dvof_x, vof2;
+        int cntx = 0;
+        if (cross) {
+            const LaneRows LX = rows_of_cell(g, S.cell_start, me.cx, me.cy);
+            auto accx_q = [&](float4 b0) {
+                float dx, dy;
+                sep(me, b0, X.cs, dx, dy);
+                return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
+            };
+            const CatMask cx3 = cand_mask_f(LX, NL.cross, W.n, wc.tt, -1, QSv, accx_q);
+            const double vofa_d = W.vof[a], rhoa_d = W.rho[a];
+            const float cfa = (float)(vofa_d * rhoa_d * (-K.numeric_gravity) / K.s_perm);
+            const float irhoa = a2.y;
+            const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)me.h + (double)QSv(0, 0).w) : (double)me.h);
+            visit_f(cx3, LX, -1, [&](int k) { return accx_q(QSv(k, 0)); }, [&](int k) {
+                auto Q = [&](int c) { return QSv(k, c); };
+                const float4 b0 = Q(0), b1 = Q(1), b2 = Q(2);
+                float dx, dy;
+                sep(me, b0, X.cs, dx, dy);
+                const float dr = dx * dx + dy * dy;
+                const float ir = dr > 0.0f ? __builtin_amdgcn_rsqf(dr) : 0.0f;
+                const float r = dr * ir;
+                float ww, gx, gy;
+                if (F.uni_x) kern_f<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
+                else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
+                ++cntx;
+                const float mb = b1.w;
+                const float dvx = vxa - b1.x, dvy = vya - b1.y;
+                const float sc = mb * (cfa * b2.x) * (ww * irhoa * b2.y);
+                dg_x.add(sc * dvx);
+                dg_y.add(sc * dvy);
+                dvof_x.add(mb * (dvx * gx + dvy * gy));
+                vof2.add(mb * ww);
+            });
+        }
+        if (!wc.valid) return;
+        if (cross) {
+            W.drgx[a] = dg_x.value();
+            W.drgy[a] = dg_y.value();
+            W.dvof_x[a] = dvof_x.value() * (1.0 / K.s_rho0);
+            W.vof2[a] = 1.0 - vof2.value() / K.s_rho0;
+            W.countiac[a] = cntx;
+        }
+        return;
+    }
     const LaneRows L = rows_of_cell(g, W.cell_start, me.cx, me.cy);
     auto acc_q = [&](float4 b0) {
         float dx, dy;
@@ -310,41 +358,6 @@ __device__ __forceinline__ void water_a_mx(int blk, DPhase W, DPhase S, const fl
         }
     });
 
-    Sum64 dg_x, dg_y, dvof_x, vof2;
-    int cntx = 0;
-    if (cross) {
-        const LaneRows LX = rows_of_cell(g, S.cell_start, me.cx, me.cy);
-        auto accx_q = [&](float4 b0) {
-            float dx, dy;
-            sep(me, b0, X.cs, dx, dy);
-            return pred_f(F.uni_x, X.thr_x, dx * dx + dy * dy, me.h, b0.w, X.scale_k);
-        };
-        const CatMask cx3 = cand_mask_f(LX, NL.cross, W.n, wc.tt, -1, QSv, accx_q);
-        const double vofa_d = W.vof[a], rhoa_d = W.rho[a];
-        const float cfa = (float)(vofa_d * rhoa_d * (-K.numeric_gravity) / K.s_perm);
-        const float irhoa = a2.y;
-        const KCf kx = make_kcf(SKF, F.uni_x ? 0.5 * ((double)me.h + (double)QSv(0, 0).w) : (double)me.h);
-        visit_f(cx3, LX, -1, [&](int k) { return accx_q(QSv(k, 0)); }, [&](int k) {
-            auto Q = [&](int c) { return QSv(k, c); };
-            const float4 b0 = Q(0), b1 = Q(1), b2 = Q(2);
-            float dx, dy;
-            sep(me, b0, X.cs, dx, dy);
-            const float dr = dx * dx + dy * dy;
-            const float ir = dr > 0.0f ? __builtin_amdgcn_rsqf(dr) : 0.0f;
-            const float r = dr * ir;
-            float ww, gx, gy;
-            if (F.uni_x) kern_f<SKF, true>(kx, r, ir, dx, dy, ww, gx, gy);
-            else kern_f<SKF, true>(make_kcf(SKF, 0.5 * ((double)me.h + (double)b0.w)), r, ir, dx, dy, ww, gx, gy);
-            ++cntx;
-            const float mb = b1.w;
-            const float dvx = vxa - b1.x, dvy = vya - b1.y;
-            const float sc = mb * (cfa * b2.x) * (ww * irhoa * b2.y);
-            dg_x.add(sc * dvx);
-            dg_y.add(sc * dvy);
-            dvof_x.add(mb * (dvx * gx + dvy * gy));
-            vof2.add(mb * ww);
-        });
-    }
     if (!wc.valid) return;
     const double vofa_d = W.vof[a], ha_d = W.hsml[a], ca_d = W.c[a];
     const double mu = K.w_mu;
@@ -355,13 +368,7 @@ __device__ __forceinline__ void water_a_mx(int blk, DPhase W, DPhase S, const fl
     if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
     if (K.average_velocity) { W.avx[a] = avx_d; W.avy[a] = avy_d; }
     W.repx[a] = repx; W.repy[a] = repy;
-    if (cross) {
-        W.drgx[a] = dg_x.value(); W.drgy[a] = dg_y.value(); W.dvof_x[a] = dvof_x.value() * (1.0 / K.s_rho0);
-        W.vof2[a] = 1.0 - vof2.value() / K.s_rho0;
-        W.countiac[a] = cntx;
-    } else {
-        W.countiac[a] = cnt;
-    }
+    if (!cross) W.countiac[a] = cnt;   // (with cross terms: the XM = 2 pass's count)
     W.nsame[a] = cnt;
     // p (q2.w) and the new c (q3.x) come from the pack
     float* Q3 = reinterpret_cast<float*>(reinterpret_cast<float4*>(qWw) + 3 * (size_t)W.n + t);
@@ -374,7 +381,15 @@ template <int SKF>
 __global__ STAGE_ATTR_MX void k_water_a_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS, NList NL,
                                            Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
 {
-    water_a_mx<SKF>(xcd_block(), W, S, qW, qWw, qS, NL, MW, g, K, F, X, cross);
+    water_a_mx<SKF, 1>(xcd_block(), W, S, qW, qWw, qS, NL, MW, g, K, F, X, cross);
+}
+
+// The water side's cross terms (soil partners), before k_water_a_mx.
+template <int SKF>
+__global__ STAGE_ATTR void k_water_x_mx(DPhase W, DPhase S, const float* qW, float* qWw, const float* qS, NList NL,
+                                        Masks MW, Grid g, KParams K, FusedK F, MixK X, int cross)
+{
+    water_a_mx<SKF, 2>(xcd_block(), W, S, qW, qWw, qS, NL, MW, g, K, F, X, cross);
 }
 
 // ------------------------------------------------------------------------
@@ -476,7 +491,9 @@ __global__ STAGE_ATTR_MX void k_water_b_mx(DPhase W, const float* qW, Masks MW, 
 // from the pack and the new sound speed is the constant s_c), so both stages
 // share one scan and one partner loop; the terms are those of k_soil_ab4.
 // ------------------------------------------------------------------------
-template <int SKF>
+// XM = 1: the same-phase terms and the tail (the cross terms from the XM = 2
+// pass, which runs first and stores them); XM = 2: the cross terms alone.
+template <int SKF, int XM>
 __device__ __forceinline__ void soil_ab_mx(int blk, DPhase S, DPhase W, const float* qS, const float* qW, NList NL,
                                            Grid g, KParams K, FusedK F, MixK X, int cross, int mask)
 {
@@ -502,7 +519,7 @@ __device__ __forceinline__ void soil_ab_mx(int blk, DPhase S, DPhase W, const fl
     Sum64 dg_x, dg_y, po_x, po_y;
     int cntx = 0;
     const double vofa_d = S.vof[a];
-    if (cross) {
+    if (XM == 2 && cross) {
         const LaneRows LX = rows_of_cell(g, W.cell_start, me.cx, me.cy);
         auto accx_q = [&](float4 b0) {
             float dx, dy;
@@ -534,6 +551,18 @@ __device__ __forceinline__ void soil_ab_mx(int blk, DPhase S, DPhase W, const fl
             po_x.add(mprr * gx);
             po_y.add(mprr * gy);
         });
+    }
+
+    if constexpr (XM == 2) {
+        if (!wc.valid) return;
+        if (cross) {
+            S.drgx[a] = dg_x.value();
+            S.drgy[a] = dg_y.value();
+            S.porx[a] = po_x.value() * vofa_d;
+            S.pory[a] = po_y.value() * vofa_d;
+            S.countiac[a] = cntx;
+        }
+        return;
     }
 
     const LaneRows L = rows_of_cell(g, S.cell_start, me.cx, me.cy);
@@ -613,14 +642,10 @@ __device__ __forceinline__ void soil_ab_mx(int blk, DPhase S, DPhase W, const fl
     if (real_a) { avx_d *= K.epsilon; avy_d *= K.epsilon; }
     if (K.average_velocity) { S.avx[a] = avx_d; S.avy[a] = avy_d; }
     S.repx[a] = repx; S.repy[a] = repy;
-    const double dgx_d = dg_x.value(), dgy_d = dg_y.value();
-    const double pox_d = po_x.value() * vofa_d, poy_d = po_y.value() * vofa_d;
-    if (cross) {
-        S.drgx[a] = dgx_d; S.drgy[a] = dgy_d; S.porx[a] = pox_d; S.pory[a] = poy_d;
-        S.countiac[a] = cntx;
-    } else {
-        S.countiac[a] = cnt;
-    }
+    // the cross terms the XM = 2 pass stored
+    const double dgx_d = cross ? S.drgx[a] : 0.0, dgy_d = cross ? S.drgy[a] : 0.0;
+    const double pox_d = cross ? S.porx[a] : 0.0, poy_d = cross ? S.pory[a] : 0.0;
+    if (!cross) S.countiac[a] = cnt;
     S.nsame[a] = cnt;
     S.c_next[a] = K.s_c;
     const double p = S.p[a], sxx = S.sxx[a], sxy = S.sxy[a], syy = S.syy[a];
@@ -681,5 +706,13 @@ template <int SKF>
 __global__ STAGE_ATTR_MX_AB void k_soil_ab_mx(DPhase S, DPhase W, const float* qS, const float* qW, NList NL, Grid g,
                                               KParams K, FusedK F, MixK X, int cross, int mask)
 {
-    soil_ab_mx<SKF>(xcd_block(), S, W, qS, qW, NL, g, K, F, X, cross, mask);
+    soil_ab_mx<SKF, 1>(xcd_block(), S, W, qS, qW, NL, g, K, F, X, cross, mask);
+}
+
+// The soil side's cross terms (water partners), before k_soil_ab_mx.
+template <int SKF>
+__global__ STAGE_ATTR void k_soil_x_mx(DPhase S, DPhase W, const float* qS, const float* qW, NList NL, Grid g,
+                                       KParams K, FusedK F, MixK X, int cross, int mask)
+{
+    soil_ab_mx<SKF, 2>(xcd_block(), S, W, qS, qW, NL, g, K, F, X, cross, mask);
 }

@@ -1180,10 +1180,15 @@ int stages_mixed(sphg_ctx* c, const FinalArgs* fin)
     SideFork fork(c);   // soil stage beside the water stages
     RC(fork.rc);
     {
+        // the cross terms in a pass of their own before the same-phase pass
         SideScope side(c);
+        RC(launch(c, "soil_x", k_soil_x_mx<SKF>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, (const float*)qS,
+                  (const float*)qW, nS, gS, c->K, c->F, X, 1, mask_b(c, 1)));
         RC(launch(c, "soil_ab", k_soil_ab_mx<SKF>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, (const float*)qS,
                   (const float*)qW, nS, gS, c->K, c->F, X, 1, mask_b(c, 1)));
     }
+    RC(launch(c, "water_x", k_water_x_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, (const float*)qW, qW,
+              (const float*)qS, nW, W.masks, gW, c->K, c->F, X, 1));
     RC(launch(c, "water_a", k_water_a_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, (const float*)qW, qW,
               (const float*)qS, nW, W.masks, gW, c->K, c->F, X, 1));
     swap_c(W);
@@ -2750,6 +2755,8 @@ void reg_batched_skf(std::map<const void*, Batched>& m)
     reg_batched<BodyWaterAMx<SKF>>(m, k_water_a_mx<SKF>);
     reg_batched<BodyWaterBMx<SKF>>(m, k_water_b_mx<SKF>);
     reg_batched<BodySoilABMx<SKF>>(m, k_soil_ab_mx<SKF>);
+    reg_batched<BodyWaterXMx<SKF>>(m, k_water_x_mx<SKF>);
+    reg_batched<BodySoilXMx<SKF>>(m, k_soil_x_mx<SKF>);
 }
 
 // The kernels of a list-mode FP64 step between rebuilds (stages, finals).
