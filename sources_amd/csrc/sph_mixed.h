@@ -19,11 +19,18 @@
 #pragma once
 #include "sph_fused.h"
 
-// The FP32 water stage kernels at six waves per SIMD (80 VGPRs, a few bytes
-// of spills): alone each is a little slower than at five, but beside the soil
-// stage on the side stream the step is about 1% faster (measured; five beat
-// the compiler's default of four).
-#define STAGE_ATTR_MX __launch_bounds__(STAGE_BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8)))
+// Waves per SIMD asked of the FP32 water stage kernels (0: the compiler's
+// choice).  With the cross terms in passes of their own, the compiler's
+// choice (no spills) is about 1.5% faster on the step than six waves with
+// 12-32 bytes of spills, and level with five (measured on the 1M slope).
+#ifndef SPHG_MX_WAVES
+#define SPHG_MX_WAVES 0
+#endif
+#if SPHG_MX_WAVES > 0
+#define STAGE_ATTR_MX __launch_bounds__(STAGE_BLOCK) __attribute__((amdgpu_waves_per_eu(SPHG_MX_WAVES, 8)))
+#else
+#define STAGE_ATTR_MX __launch_bounds__(STAGE_BLOCK)
+#endif
 #ifndef SPHG_MX_AB_WAVES
 #define SPHG_MX_AB_WAVES 0   // the one-pass soil kernel: compiler's choice (no spills) beats 4 waves with spills
 #endif
