@@ -1071,6 +1071,13 @@ int stages_fused(sphg_ctx* c, const FinalArgs* fin)
         else RC(launch(c, "soil_ab", k_soil_ab4<SKF, false, false>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, rS,
                        nS, gS, c->K, c->F, cross, mask_b(c, 1)));
     }
+    // water stage A, then the water cross terms (before stage B, whose tail
+    // adds them): beside the soil side's cross pass and then its stage,
+    // each pair of concurrent kernels reads the same phase's records
+    if (uw) RC(launch(c, "water_a", k_water_a4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
+                      nW, W.masks, gW, c->K, c->F, cross));
+    else RC(launch(c, "water_a", k_water_a4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
+                   nW, W.masks, gW, c->K, c->F, cross));
     if (cross) {
         if (ux) RC(launch(c, "water_x", k_water_x4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, rW, rS,
                           nW, gW, c->K, c->F));
@@ -1078,10 +1085,6 @@ int stages_fused(sphg_ctx* c, const FinalArgs* fin)
                        nW, gW, c->K, c->F));
     }
     if (split) HIPCHK(c, hipEventRecord(c->ev_wx, c->stream));
-    if (uw) RC(launch(c, "water_a", k_water_a4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
-                      nW, W.masks, gW, c->K, c->F, cross));
-    else RC(launch(c, "water_a", k_water_a4<SKF, false>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, W.rec,
-                   nW, W.masks, gW, c->K, c->F, cross));
     swap_c(W);
     swap_c(S);
     if (uw) RC(launch(c, "water_b", k_water_b4<SKF, true>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, rW, nW,
@@ -1187,9 +1190,11 @@ int stages_mixed(sphg_ctx* c, const FinalArgs* fin)
         RC(launch(c, "soil_ab", k_soil_ab_mx<SKF>, nblk(S.n, STAGE_BLOCK), STAGE_BLOCK, S.d, W.d, (const float*)qS,
                   (const float*)qW, nS, gS, c->K, c->F, X, 1, mask_b(c, 1)));
     }
-    RC(launch(c, "water_x", k_water_x_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, (const float*)qW, qW,
-              (const float*)qS, nW, W.masks, gW, c->K, c->F, X, 1));
+    // (stage A before the cross pass: beside the soil side's cross pass and
+    // then its stage, each pair of concurrent kernels reads one phase's records)
     RC(launch(c, "water_a", k_water_a_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, (const float*)qW, qW,
+              (const float*)qS, nW, W.masks, gW, c->K, c->F, X, 1));
+    RC(launch(c, "water_x", k_water_x_mx<SKF>, nblk(W.n, STAGE_BLOCK), STAGE_BLOCK, W.d, S.d, (const float*)qW, qW,
               (const float*)qS, nW, W.masks, gW, c->K, c->F, X, 1));
     swap_c(W);
     swap_c(S);
