@@ -582,7 +582,7 @@ struct NList {
 // overflow flag and that particle walks its rows directly.
 // ------------------------------------------------------------------------
 struct LaneRows {
-    int k0[3], k1[3], base[3];   // base: index offset of a staged copy (0: the records themselves)
+    int k0[3], k1[3];
 };
 
 __device__ __forceinline__ LaneRows lane_rows(const Rows& R)
@@ -592,7 +592,6 @@ __device__ __forceinline__ LaneRows lane_rows(const Rows& R)
     for (int r = 0; r < 3; ++r) {
         L.k0[r] = R.k0[r];
         L.k1[r] = R.k1[r];
-        L.base[r] = 0;
     }
     return L;
 }
@@ -624,9 +623,8 @@ __device__ __forceinline__ int ovf_of(const LaneRows& L)
 template <class IT, class LD, class B, class P>
 __device__ __forceinline__ void visit(IT it, const LaneRows& L, int ovf, int self, const LD& load, P&& accept, B&& body)
 {
-    int i = 0;
 #pragma unroll 1
-    for (int k = it.next(i); k >= 0; k = it.next(i)) body(load(k));
+    for (int k = it.next(); k >= 0; k = it.next()) body(load(k));
     if (ovf) {
         // one pass over the concatenated rows (row bounds as scalars: an
         // indexed LaneRows would be kept in scratch memory)
@@ -683,8 +681,7 @@ struct CatIter {
     uint64_t lo, hi;
     int s1, s2;        // start of row 1 and row 2 in the concatenated list
     int e0, e1, e2;    // record index - list index, per row
-    int d0, d1, d2;    // staged index - record index, per row
-    __device__ __forceinline__ int next(int& i)
+    __device__ __forceinline__ int next()
     {
         const bool in_lo = lo != 0;
         const uint64_t m = pick(in_lo, lo, hi);
@@ -694,9 +691,7 @@ struct CatIter {
         lo = pick(in_lo, rest, lo);
         hi = pick(in_lo, hi, rest);
         const bool a = c < s1, b = c < s2;
-        const int k = c + pick(a, e0, pick(b, e1, e2));
-        i = k + pick(a, d0, pick(b, d1, d2));
-        return k;
+        return c + pick(a, e0, pick(b, e1, e2));
     }
 };
 
@@ -711,9 +706,6 @@ __device__ __forceinline__ CatIter cat_iter(const CatMask& c, const LaneRows& L)
     it.e0 = L.k0[0];
     it.e1 = L.k0[1] - l0;
     it.e2 = L.k0[2] - l0 - l1;
-    it.d0 = L.base[0];
-    it.d1 = L.base[1];
-    it.d2 = L.base[2];
     return it;
 }
 

@@ -131,7 +131,6 @@ __device__ __forceinline__ LaneRows rows_of_cell(const Grid& g, const int* start
         const bool ok = row >= 0 && row < g.ncy;
         L.k0[r] = ok ? start[row * g.ncx + c0] : 0;
         L.k1[r] = ok ? start[row * g.ncx + c1 + 1] : 0;
-        L.base[r] = 0;
     }
     return L;
 }
@@ -181,9 +180,8 @@ template <class B, class P>
 __device__ __forceinline__ void visit_f(const CatMask& cm, const LaneRows& L, int self, P&& accept, B&& body)
 {
     CatIter it = cat_iter(cm, L);
-    int i = 0;
 #pragma unroll 1
-    for (int k = it.next(i); k >= 0; k = it.next(i)) body(k);
+    for (int k = it.next(); k >= 0; k = it.next()) body(k);
     if (cm.ovf) {
         // one pass over the concatenated rows (row bounds as scalars: an
         // indexed LaneRows would be kept in scratch memory)
