@@ -330,8 +330,11 @@ __device__ __forceinline__ void track_displacement(const NRef& R, int t, double 
 // Pack of water particle i into record slot t from its current values x,
 // v, rho, vof (the state's, or those the final update of the step before has
 // just computed): first_half (do_half), the equation of state, the record.
+// lean: the half-step levels that nothing reads before the next final update
+// (which starts from the saved full-step levels) stay out of the state; only
+// for a pack that the next step's stages read from FP64 records alone.
 __device__ __forceinline__ void pack_water_at(const DPhase& P, const KParams& K, double* rec, int t, int i,
-                                              int do_half, float* q, const Grid& g, double cs, const NRef& NR,
+                                              int do_half, bool lean, float* q, const Grid& g, double cs, const NRef& NR,
                                               double x, double y, double rho, double vof, double vx, double vy)
 {
 #pragma clang fp contract(off)
@@ -342,20 +345,22 @@ __device__ __forceinline__ void pack_water_at(const DPhase& P, const KParams& K,
         if (!K.summation_density) {   // first_half's rho update (declarations_sph.f90:647-650)
             P.rho_min[i] = rho;
             rho = rho + h * P.drho[i];
-            P.rho[i] = rho;
+            if (!lean) P.rho[i] = rho;
         }
         if (K.volume_fraction && K.two_phase) {
             P.vof_min[i] = vof;
             vof = vof + h * P.dvof[i];
-            P.vof[i] = vof;
+            if (!lean || K.summation_density) P.vof[i] = vof;   // the renormalisation reads it then
         }
         if (it > 0) {
             P.vx_min[i] = vx;
             P.vy_min[i] = vy;
             vx = vx + h * P.ax[i];
             vy = vy + h * P.ay[i];
-            P.vx[i] = vx;
-            P.vy[i] = vy;
+            if (!lean) {
+                P.vx[i] = vx;
+                P.vy[i] = vy;
+            }
         }
     }
     const double m = P.mass[i];
@@ -392,7 +397,7 @@ __device__ __forceinline__ void pack_water_body(int blk, DPhase P, KParams K, do
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
-    pack_water_at(P, K, rec, t, i, do_half, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vof[i], P.vx[i], P.vy[i]);
+    pack_water_at(P, K, rec, t, i, do_half, false, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vof[i], P.vx[i], P.vy[i]);
 }
 
 __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs, NRef NR)
@@ -403,8 +408,11 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
 // The final update of step s fused with the pack of step s + 1 (first_half,
 // full step): one pass over the state instead of two, with the same values
 // in memory afterwards as k_final_water followed by k_pack_water(do_half =
-// 1), apart from stores the pack would overwrite.  The record order of step
-// s + 1 must be the current one (neighbour lists kept, no cell-list rebuild).
+// 1), apart from stores the pack would overwrite and, with FP64 records, the
+// half-step levels the pack leaves out (lean).  The record order of step
+// s + 1 must be the current one (neighbour lists kept, no cell-list rebuild),
+// and step s + 1 must follow at once: its final update is the next reader of
+// the state.
 __device__ __forceinline__ void step_water(int blk, DPhase P, KParams K, double* rec, int first, int with_vof,
                                            int renorm, float* q, Grid g, double cs, NRef NR)
 {
@@ -424,7 +432,7 @@ __device__ __forceinline__ void step_water(int blk, DPhase P, KParams K, double*
             P.vy[i] = e.vy;
         }
     }
-    pack_water_at(P, K, rec, t, i, 1, q, g, cs, NR, e.x, e.y, e.rho, e.vof, e.vx, e.vy);
+    pack_water_at(P, K, rec, t, i, 1, q == nullptr, q, g, cs, NR, e.x, e.y, e.rho, e.vof, e.vx, e.vy);
 }
 
 __global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int with_vof, int renorm, float* q, Grid g,
@@ -435,9 +443,12 @@ __global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int wi
 
 // Pack of soil particle i (values as for water, plus stress and pressure):
 // first_half (do_half), ndp Drucker-Prager projections (the last one
-// single_step's failure criterion), the record.
+// single_step's failure criterion), the record.  lean as for water: the
+// soil density stays (the final update's vof reads it), and the stress
+// levels stay unless the final update starts from the saved ones
+// (stress_integration = 2).
 __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, double* rec, int t, int i,
-                                             int do_half, int ndp, float* q, const Grid& g, double cs,
+                                             int do_half, bool lean, int ndp, float* q, const Grid& g, double cs,
                                              const NRef& NR, double x, double y, double rho, double vx, double vy,
                                              double p, double sxx, double sxy, double syy)
 {
@@ -456,10 +467,13 @@ __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, 
             P.vy_min[i] = vy;
             vx = vx + h * P.ax[i];
             vy = vy + h * P.ay[i];
-            P.vx[i] = vx;
-            P.vy[i] = vy;
+            if (!lean) {
+                P.vx[i] = vx;
+                P.vy[i] = vy;
+            }
         }
     }
+    const bool keep_stress = !lean || K.stress_integration != 2;
     if (do_half && K.stress_integration == 2) {   // first_half, declarations_sph.f90:664-672
         const double h = K.dt / 2.0;
         P.sxx_min[i] = sxx; P.sxy_min[i] = sxy; P.syy_min[i] = syy; P.p_min[i] = p;
@@ -467,7 +481,9 @@ __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, 
         sxy = sxy + h * P.dsxy[i];
         syy = syy + h * P.dsyy[i];
         p = p + h * P.dp[i];
-        P.p[i] = p; P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy;
+        if (keep_stress) {
+            P.p[i] = p; P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy;
+        }
     }
     // the last projection is single_step's failure criterion: Mohr-Coulomb
     // when yield_criterion = 1, which may also reset rho (the record then
@@ -480,7 +496,9 @@ __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, 
             if (k == ndp - 1 && K.yield_criterion == 1) f = mc_project(K, p, sxx, sxy, syy, rho);
             else f = dp_project(K.s_alpha1, p, sxx, sxy, syy);
         }
-        P.p[i] = p; P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy;
+        if (keep_stress) {
+            P.p[i] = p; P.sxx[i] = sxx; P.sxy[i] = sxy; P.syy[i] = syy;
+        }
         P.fail[i] = f;
         if (rho != rho_pre) P.rho[i] = rho;
     }
@@ -512,7 +530,7 @@ __device__ __forceinline__ void pack_soil_body(int blk, DPhase P, KParams K, dou
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
-    pack_soil_at(P, K, rec, t, i, do_half, ndp, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vx[i], P.vy[i], P.p[i],
+    pack_soil_at(P, K, rec, t, i, do_half, false, ndp, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vx[i], P.vy[i], P.p[i],
                  P.sxx[i], P.sxy[i], P.syy[i]);
 }
 
@@ -537,7 +555,7 @@ __device__ __forceinline__ void step_soil(int blk, DPhase P, KParams K, double* 
         P.x[i] = e.x;
         P.y[i] = e.y;
     }
-    pack_soil_at(P, K, rec, t, i, 1, 2, q, g, cs, NR, e.x, e.y, e.rho, e.vx, e.vy, e.p, e.sxx, e.sxy, e.syy);
+    pack_soil_at(P, K, rec, t, i, 1, q == nullptr, 2, q, g, cs, NR, e.x, e.y, e.rho, e.vx, e.vy, e.p, e.sxx, e.sxy, e.syy);
 }
 
 __global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, float* q, Grid g, double cs, NRef NR)
