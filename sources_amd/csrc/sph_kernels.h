@@ -533,23 +533,36 @@ __device__ __forceinline__ WaterEnd water_end(const DPhase& P, const KParams& K,
 #pragma clang fp contract(off)
     const double dt = K.dt, h = dt / 2.0;
     WaterEnd e;
-    e.rho = P.rho[i];
-    if (renorm) e.rho = e.rho / P.vof[i] * P.vof2[i];
-    if (!K.summation_density) e.rho = first ? e.rho + h * P.drho[i] : P.rho_min[i] + dt * P.drho[i];
-    e.vof = P.vof[i];
-    if (with_vof) e.vof = first ? e.vof + h * P.dvof[i] : P.vof_min[i] + dt * P.dvof[i];
+    // (a level that the update replaces by the saved one is not read: after
+    // a fused final/pack the state does not hold it)
+    if (!K.summation_density && !first) {
+        e.rho = P.rho_min[i] + dt * P.drho[i];
+    } else {
+        e.rho = P.rho[i];
+        if (renorm) e.rho = e.rho / P.vof[i] * P.vof2[i];
+        if (!K.summation_density) e.rho = e.rho + h * P.drho[i];
+    }
+    if (with_vof && !first) {
+        e.vof = P.vof_min[i] + dt * P.dvof[i];
+    } else {
+        e.vof = P.vof[i];
+        if (with_vof) e.vof = e.vof + h * P.dvof[i];
+    }
     e.x = P.x[i];
     e.y = P.y[i];
-    e.vx = P.vx[i];
-    e.vy = P.vy[i];
-    if (P.itype[i] >= 0) {
-        if (first) {
+    const int it = P.itype[i];
+    if (it >= 0 && !first) {
+        e.vx = P.vx_min[i] + dt * P.ax[i] + P.avx[i];
+        e.vy = P.vy_min[i] + dt * P.ay[i] + P.avy[i];
+    } else {
+        e.vx = P.vx[i];
+        e.vy = P.vy[i];
+        if (it >= 0) {
             e.vx = e.vx + h * P.ax[i] + P.avx[i];
             e.vy = e.vy + h * P.ay[i] + P.avy[i];
-        } else {
-            e.vx = P.vx_min[i] + dt * P.ax[i] + P.avx[i];
-            e.vy = P.vy_min[i] + dt * P.ay[i] + P.avy[i];
         }
+    }
+    if (it >= 0) {
         e.x = e.x + dt * e.vx;
         e.y = e.y + dt * e.vy;
     }
@@ -666,16 +679,19 @@ __device__ __forceinline__ SoilEnd soil_end(const DPhase& P, const KParams& K, i
     }
     e.x = P.x[i];
     e.y = P.y[i];
-    e.vx = P.vx[i];
-    e.vy = P.vy[i];
-    if (P.itype[i] > 0) {
-        if (first) {
+    const int it = P.itype[i];
+    if (it > 0 && !first) {   // (the current level is not read, as for water)
+        e.vx = P.vx_min[i] + dt * P.ax[i] + P.avx[i];
+        e.vy = P.vy_min[i] + dt * P.ay[i] + P.avy[i];
+    } else {
+        e.vx = P.vx[i];
+        e.vy = P.vy[i];
+        if (it > 0) {
             e.vx = e.vx + h * P.ax[i] + P.avx[i];
             e.vy = e.vy + h * P.ay[i] + P.avy[i];
-        } else {
-            e.vx = P.vx_min[i] + dt * P.ax[i] + P.avx[i];
-            e.vy = P.vy_min[i] + dt * P.ay[i] + P.avy[i];
         }
+    }
+    if (it > 0) {
         e.x = e.x + dt * e.vx;
         e.y = e.y + dt * e.vy;
     }
