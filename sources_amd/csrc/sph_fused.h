@@ -333,9 +333,11 @@ __device__ __forceinline__ void track_displacement(const NRef& R, int t, double 
 // lean: the half-step levels that nothing reads before the next final update
 // (which starts from the saved full-step levels) stay out of the state; only
 // for a pack that the next step's stages read from FP64 records alone.
+// lean_v: the same for the velocity alone (no stage reads the state's).
 __device__ __forceinline__ void pack_water_at(const DPhase& P, const KParams& K, double* rec, int t, int i,
-                                              int do_half, bool lean, float* q, const Grid& g, double cs, const NRef& NR,
-                                              double x, double y, double rho, double vof, double vx, double vy)
+                                              int do_half, bool lean, bool lean_v, float* q, const Grid& g,
+                                              double cs, const NRef& NR, double x, double y, double rho, double vof,
+                                              double vx, double vy)
 {
 #pragma clang fp contract(off)
     track_displacement(NR, t, x, y, vx, vy);
@@ -357,7 +359,7 @@ __device__ __forceinline__ void pack_water_at(const DPhase& P, const KParams& K,
             P.vy_min[i] = vy;
             vx = vx + h * P.ax[i];
             vy = vy + h * P.ay[i];
-            if (!lean) {
+            if (!lean_v) {
                 P.vx[i] = vx;
                 P.vy[i] = vy;
             }
@@ -397,7 +399,7 @@ __device__ __forceinline__ void pack_water_body(int blk, DPhase P, KParams K, do
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
-    pack_water_at(P, K, rec, t, i, do_half, false, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vof[i], P.vx[i], P.vy[i]);
+    pack_water_at(P, K, rec, t, i, do_half, false, false, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vof[i], P.vx[i], P.vy[i]);
 }
 
 __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, float* q, Grid g, double cs, NRef NR)
@@ -408,11 +410,11 @@ __global__ void k_pack_water(DPhase P, KParams K, double* rec, int do_half, floa
 // The final update of step s fused with the pack of step s + 1 (first_half,
 // full step): one pass over the state instead of two, with the same values
 // in memory afterwards as k_final_water followed by k_pack_water(do_half =
-// 1), apart from stores the pack would overwrite and, with FP64 records, the
-// half-step levels the pack leaves out (lean).  The record order of step
-// s + 1 must be the current one (neighbour lists kept, no cell-list rebuild),
-// and step s + 1 must follow at once: its final update is the next reader of
-// the state.
+// 1), apart from stores the pack would overwrite and the half-step levels
+// the pack leaves out (lean with FP64 records, lean_v always).  The record
+// order of step s + 1 must be the current one (neighbour lists kept, no
+// cell-list rebuild), and step s + 1 must follow at once: its final update
+// is the next reader of the state.
 __device__ __forceinline__ void step_water(int blk, DPhase P, KParams K, double* rec, int first, int with_vof,
                                            int renorm, float* q, Grid g, double cs, NRef NR)
 {
@@ -432,7 +434,7 @@ __device__ __forceinline__ void step_water(int blk, DPhase P, KParams K, double*
             P.vy[i] = e.vy;
         }
     }
-    pack_water_at(P, K, rec, t, i, 1, q == nullptr, q, g, cs, NR, e.x, e.y, e.rho, e.vof, e.vx, e.vy);
+    pack_water_at(P, K, rec, t, i, 1, q == nullptr, true, q, g, cs, NR, e.x, e.y, e.rho, e.vof, e.vx, e.vy);
 }
 
 __global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int with_vof, int renorm, float* q, Grid g,
@@ -448,9 +450,9 @@ __global__ void k_step_water(DPhase P, KParams K, double* rec, int first, int wi
 // levels stay unless the final update starts from the saved ones
 // (stress_integration = 2).
 __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, double* rec, int t, int i,
-                                             int do_half, bool lean, int ndp, float* q, const Grid& g, double cs,
-                                             const NRef& NR, double x, double y, double rho, double vx, double vy,
-                                             double p, double sxx, double sxy, double syy)
+                                             int do_half, bool lean, bool lean_v, int ndp, float* q, const Grid& g,
+                                             double cs, const NRef& NR, double x, double y, double rho, double vx,
+                                             double vy, double p, double sxx, double sxy, double syy)
 {
 #pragma clang fp contract(off)
     track_displacement(NR, t, x, y, vx, vy);
@@ -467,7 +469,7 @@ __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, 
             P.vy_min[i] = vy;
             vx = vx + h * P.ax[i];
             vy = vy + h * P.ay[i];
-            if (!lean) {
+            if (!lean_v) {
                 P.vx[i] = vx;
                 P.vy[i] = vy;
             }
@@ -530,7 +532,7 @@ __device__ __forceinline__ void pack_soil_body(int blk, DPhase P, KParams K, dou
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const int i = P.order[t];
-    pack_soil_at(P, K, rec, t, i, do_half, false, ndp, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vx[i], P.vy[i], P.p[i],
+    pack_soil_at(P, K, rec, t, i, do_half, false, false, ndp, q, g, cs, NR, P.x[i], P.y[i], P.rho[i], P.vx[i], P.vy[i], P.p[i],
                  P.sxx[i], P.sxy[i], P.syy[i]);
 }
 
@@ -555,7 +557,7 @@ __device__ __forceinline__ void step_soil(int blk, DPhase P, KParams K, double* 
         P.x[i] = e.x;
         P.y[i] = e.y;
     }
-    pack_soil_at(P, K, rec, t, i, 1, q == nullptr, 2, q, g, cs, NR, e.x, e.y, e.rho, e.vx, e.vy, e.p, e.sxx, e.sxy, e.syy);
+    pack_soil_at(P, K, rec, t, i, 1, q == nullptr, true, 2, q, g, cs, NR, e.x, e.y, e.rho, e.vx, e.vy, e.p, e.sxx, e.sxy, e.syy);
 }
 
 __global__ void k_step_soil(DPhase P, KParams K, double* rec, int first, float* q, Grid g, double cs, NRef NR)
