@@ -390,7 +390,9 @@ __device__ __forceinline__ void pack_water_at(const DPhase& P, const KParams& K,
     R[2 * pn] = make_double2(rho, it < 0 ? -m : m);   // the sign of m marks a virtual particle
     R[3 * pn] = make_double2(vof, p);
     R[4 * pn] = make_double2(0.0, cn);
-    R[6 * pn] = make_double2(P.hsml[i], (double)it);
+    // (h and itype stay between list rebuilds, in the same slot: a lean pack
+    // finds chunk 6 as the call's first pack or the rebuild's wrote it)
+    if (!lean) R[6 * pn] = make_double2(P.hsml[i], (double)it);
 }
 
 __device__ __forceinline__ void pack_water_body(int blk, DPhase P, KParams K, double* rec, int do_half, float* q,
@@ -522,7 +524,7 @@ __device__ __forceinline__ void pack_soil_at(const DPhase& P, const KParams& K, 
     R[3 * pn] = make_double2(P.vof[i], p);
     R[4 * pn] = make_double2(syy, P.c[i]);
     R[5 * pn] = make_double2(sxx, sxy);
-    R[6 * pn] = make_double2(P.hsml[i], (double)it);
+    if (!lean) R[6 * pn] = make_double2(P.hsml[i], (double)it);   // (as for water)
     if (K.yield_criterion == 1) R[7 * pn] = make_double2(rho_pre, 0.0);
 }
 
